@@ -1,0 +1,56 @@
+// hmx_device.h -- host/device interface between the plan builder and the
+// gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "hmx_internal.h"
+
+// Device-side view of a plan (all pointers are device pointers).
+struct DevPlan {
+    int kid;       // enum hm_kernel
+    int r;         // Chebyshev rank
+    int64_t nx, ny;
+    const double *x, *y;     // points (non-increasing)
+    const double *lam;       // r barycentric weights
+    const DCluster *rcl;     // row clusters
+    const DCluster *ccl;     // col clusters
+    int32_t nrc, ncc;
+    const double *rnodes;    // nrc*r mapped nodes (x side)
+    const double *cnodes;    // ncc*r mapped nodes (y side)
+    const DFarBlk *far;
+    int32_t nfar;
+    const int32_t *far_by_rc;  // far ids grouped by row cluster (CSR in DCluster.far_*)
+    const int32_t *far_by_cc;  // far ids grouped by col cluster
+    const DDense *dense;
+    int32_t ndense;
+    const DChunk *rch, *cch;   // row chunks, col chunks
+    int32_t nrch, ncch;
+    const DNear *rnear, *cnear;
+    const DFar *rfar, *cfar;
+    const DTask *rtask, *ctask;  // upward-pass tasks over row / col clusters
+    int32_t nrtask, nctask;
+    int64_t nslots_r, nslots_c;
+    double *A, *U, *V, *F;     // stored factors (null when matrix-free)
+};
+
+struct MvArgs {
+    double *y;
+    const double *x;
+    int64_t incy, incx;
+    int beta;
+    double *partial;  // >= max(nslots) * r
+    double *w;        // >= max(nrc, ncc) * r
+    double *z;        // >= max(nrc, ncc) * r
+};
+
+// compression: dense blocks, U, V, F (bit-exact operation order)
+hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
+// one product y = beta*y + op(H) x for a single right-hand side; when ev is
+// not null, ev[0..4] are recorded before up, reduce, translate, down and after down
+hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, bool matrix_free,
+                             hipStream_t s, const hipEvent_t *ev = nullptr);
+// entries: kind[t] = 0 none, 1 dense (id), 2 far (id); rows/cols 0-based
+hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,
+                              const int32_t *kind, const int32_t *id, double *out, hipStream_t s);
+// number of non-finite values among generated factors (after compression)
+hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long long *count,
+                                      hipStream_t s);

@@ -1,0 +1,76 @@
+// hmx_internal.h -- device layout of a flattened KernelMatrix tree.
+//
+// The reference keeps a tree of 2x2 nodes with an `assigned` code matrix per
+// node (src/hierarchical.jl:49-52) and walks it depth first for every product
+// (src/KernelMatrix.jl:17-45). Here the tree is flattened once, at plan time,
+// into level-ordered tables in HBM:
+//
+//   row clusters   one per distinct (depth, row range) that is the row side of
+//                  a far block; owns U (len x r, column-major) and its mapped
+//                  Chebyshev nodes. Far blocks that share a row cluster share
+//                  one U (the reference stores identical copies per block).
+//   col clusters   the same for the column side and V.
+//   far blocks     (row cluster, col cluster, F offset); F is r x r column-major.
+//   dense blocks   (row0, col0, m, n, offset); A is m x n column-major.
+//   row chunks     <= 64 consecutive rows that every dense block and row
+//                  cluster either covers completely or misses; each lists its
+//                  dense blocks (near) and row clusters (far). One wavefront
+//                  owns one chunk, so the output needs no atomics.
+//   col chunks     the same over columns.
+//   tasks          (cluster, range of <= HM_TASK_POINTS points): one workgroup
+//                  of the upward pass reduces B[range, :]^T x[range] into one
+//                  partial slot; a cluster's slots are consecutive.
+#pragma once
+#include <stdint.h>
+
+#define HM_CHUNK 64          // rows per chunk = wavefront width
+#define HM_TASK_POINTS 8192  // points per upward-pass task
+
+struct DChunk {
+    int64_t p0;        // first row (or column)
+    int32_t np;        // number of rows in the chunk (<= HM_CHUNK)
+    int32_t near_beg;  // CSR range into DNear
+    int32_t near_end;
+    int32_t far_beg;   // CSR range into DFar
+    int32_t far_end;
+    int32_t pad;
+};
+
+struct DNear {
+    int32_t blk;  // dense block id
+    int32_t off;  // offset of the chunk inside the block (rows, or columns when transposed)
+};
+
+struct DFar {
+    int32_t cl;   // cluster id
+    int32_t off;  // offset of the chunk inside the cluster
+};
+
+struct DTask {
+    int32_t cl;    // cluster id
+    int32_t off;   // first point of the task inside the cluster
+    int32_t len;   // number of points
+    int32_t pad;
+    int64_t slot;  // partial-sum slot
+};
+
+struct DDense {
+    int64_t row0, col0;
+    int32_t m, n;
+    int64_t aoff;  // offset of A in the dense store
+};
+
+struct DCluster {
+    int64_t first;  // first point index
+    int64_t boff;   // offset of the basis (U or V) in the basis store
+    int64_t pbase;  // first partial-sum slot (its tasks' slots are consecutive)
+    int32_t len;    // number of points
+    int32_t nslots; // number of tasks of this cluster
+    int32_t far_beg, far_end;  // CSR range into the far-block list grouped by this cluster
+    int32_t depth, pad;
+};
+
+struct DFarBlk {
+    int32_t rc, cc;  // row cluster, col cluster
+    int64_t foff;    // offset of F
+};

@@ -1,0 +1,802 @@
+// plan.cpp -- host side of libhmx: the KernelMatrix block tree, its flattened
+// device layout, and the C ABI declared in include/hmx.h.
+#include "hmx.h"
+#include "hmx_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HM_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail(HM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// sin(pi q) evaluated in long double and rounded once, for the Chebyshev
+// points and weights (the reference calls Julia's sinpi).
+double sinpi_ld(double q) {
+    const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
+    return (double)sinl(PI_L * (long double)q);
+}
+
+// src/BarycentricMatrix.jl:92-111
+void cheb_points(double *x, int n, int kind) {
+    std::fill(x, x + n, 0.0);
+    const int nd2 = n / 2;
+    const double den = kind == 1 ? (double)(2 * n) : (double)(2 * (n - 1));
+    for (int k = 1; k <= nd2; ++k) x[k - 1] = sinpi_ld(((double)(n - 2 * k) + 1.0) / den);
+    for (int k = 1; k <= nd2; ++k) x[n - k] = -x[k - 1];
+}
+
+// src/BarycentricMatrix.jl:114-136
+void cheb_weights(double *l, int n, int kind) {
+    std::fill(l, l + n, 0.0);
+    const int nd2 = n / 2;
+    if (kind == 1) {
+        for (int k = 1; k <= nd2 + 1 && k <= n; ++k)
+            l[k - 1] = sinpi_ld(((double)(2 * k) - 1.0) / (double)(2 * n));
+        for (int k = 1; k <= nd2; ++k) l[n - k] = l[k - 1];
+    } else {
+        for (int k = 0; k < n; ++k) l[k] = 1.0;
+    }
+    for (int k = 2; k <= n; k += 2) l[k - 1] = -l[k - 1];
+    if (kind != 1) {
+        l[0] *= 0.5;
+        l[n - 1] *= 0.5;
+    }
+}
+
+bool non_increasing(const double *p, int64_t n, int64_t *bad) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(p[i]) || (i + 1 < n && !(p[i] >= p[i + 1]))) {
+            *bad = i;
+            return false;
+        }
+    }
+    return true;
+}
+
+struct Leaf {
+    int code, depth;
+    int64_t row0, m, col0, n;
+    double a, b, c, d;
+};
+
+struct HCluster {
+    int depth;
+    int64_t first, len;
+    double lo, hi;  // interval as passed to the block: (a, b) with a the upper end
+};
+
+// The tree rule of KernelMatrix / KernelMatrix1 / KernelMatrix2
+// (src/KernelMatrix.jl:49-116), recorded as flat tables.
+struct Builder {
+    const double *x, *y;
+    int64_t nx, ny;
+    int bs, r;
+    std::vector<Leaf> leaves;
+    std::vector<HCluster> rcl, ccl;
+    std::unordered_map<uint64_t, int32_t> rmap, cmap;
+    std::vector<DFarBlk> far;
+    std::vector<int> far_depth;
+    std::vector<DDense> dense;
+    int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
+    int maxdepth = 0;
+    int err = 0;
+    std::string msg;
+
+    // indsplit (src/BarycentricMatrix.jl:299-307) on a validated non-increasing
+    // range: the linear scan stops at the first point below the midpoint, which
+    // a binary search finds directly.
+    int64_t split(const double *p, int64_t np, int64_t first, int64_t len, double a, double b) {
+        const double ab2 = 0.5 * (a + b);
+        if (len == 0) {
+            if (first >= np || p[first] >= ab2) {
+                err = HM_ERR_DEGENERATE;
+                char buf[160];
+                snprintf(buf, sizeof(buf),
+                         "empty cluster at index %lld reached indsplit: the reference would "
+                         "read out of its range", (long long)first);
+                msg = buf;
+            }
+            return 0;
+        }
+        const double *lo = p + first, *hi = p + first + len;
+        const double *it = std::partition_point(lo, hi, [ab2](double v) { return v >= ab2; });
+        return (int64_t)(it - lo);
+    }
+
+    int32_t cluster(std::vector<HCluster> &cl, std::unordered_map<uint64_t, int32_t> &mp, int depth,
+                    int64_t first, int64_t len, double lo, double hi) {
+        const uint64_t key = ((uint64_t)depth << 42) | (uint64_t)first;
+        auto it = mp.find(key);
+        if (it != mp.end()) {
+            const HCluster &h = cl[it->second];
+            if (h.len != len || h.lo != lo || h.hi != hi) {
+                err = HM_ERR_INVALID;
+                msg = "internal: inconsistent cluster";
+            }
+            return it->second;
+        }
+        const int32_t id = (int32_t)cl.size();
+        cl.push_back({depth, first, len, lo, hi});
+        mp.emplace(key, id);
+        return id;
+    }
+
+    void put_dense(int depth, int64_t i0, int64_t m, int64_t j0, int64_t n) {
+        leaves.push_back({3, depth, i0, m, j0, n, 0, 0, 0, 0});
+        if (m > 0 && n > 0) dense.push_back({i0, j0, (int32_t)m, (int32_t)n, 0});
+        dense_entries += m * n;
+    }
+
+    // BarycentricMatrix2D(T, f, a, b, c, d, x, y, ir, jr), src/BarycentricMatrix.jl:236
+    void put_far(int depth, double a, double b, double c, double d, int64_t i0, int64_t m,
+                 int64_t j0, int64_t n) {
+        leaves.push_back({2, depth, i0, m, j0, n, a, b, c, d});
+        far_rows += m;
+        far_cols += n;
+        if (m > 0 && n > 0) {
+            const int32_t rc = cluster(rcl, rmap, depth, i0, m, a, b);
+            const int32_t cc = cluster(ccl, cmap, depth, j0, n, c, d);
+            far.push_back({rc, cc, 0});
+            far_depth.push_back(depth);
+        }
+    }
+
+    void node(int type, int depth, int64_t i0, int64_t ni, int64_t j0, int64_t nj, double a,
+              double b, double c, double d) {
+        if (err) return;
+        maxdepth = std::max(maxdepth, depth);
+        if (depth > 200) {
+            err = HM_ERR_DEGENERATE;
+            msg = "block tree deeper than 200 levels";
+            return;
+        }
+        if (type == 0) ++nd; else if (type == 1) ++nk1; else ++nk2;
+        const int64_t ni1 = split(x, nx, i0, ni, a, b), ni2 = ni - ni1;
+        const int64_t nj1 = split(y, ny, j0, nj, c, d), nj2 = nj - nj1;
+        if (err) return;
+        const int64_t i2 = i0 + ni1, j2 = j0 + nj1;
+        const double ab2 = 0.5 * (a + b), cd2 = 0.5 * (c + d);
+        const bool leaf = ni1 < bs && ni2 < bs && nj1 < bs && nj2 < bs;
+        const int cd_ = depth + 1;
+        if (type == 0) {  // KernelMatrix, src/KernelMatrix.jl:49-70
+            if (leaf) {
+                put_dense(cd_, i0, ni1, j0, nj1);
+                put_dense(cd_, i0, ni1, j2, nj2);
+                put_dense(cd_, i2, ni2, j0, nj1);
+                put_dense(cd_, i2, ni2, j2, nj2);
+            } else {
+                node(0, cd_, i0, ni1, j0, nj1, a, ab2, c, cd2);
+                node(1, cd_, i0, ni1, j2, nj2, a, ab2, cd2, d);
+                node(2, cd_, i2, ni2, j0, nj1, ab2, b, c, cd2);
+                node(0, cd_, i2, ni2, j2, nj2, ab2, b, cd2, d);
+            }
+        } else if (type == 1) {  // KernelMatrix1, src/KernelMatrix.jl:72-93
+            put_far(cd_, a, ab2, c, cd2, i0, ni1, j0, nj1);
+            put_far(cd_, a, ab2, cd2, d, i0, ni1, j2, nj2);
+            if (leaf) put_dense(cd_, i2, ni2, j0, nj1);
+            else node(1, cd_, i2, ni2, j0, nj1, ab2, b, c, cd2);
+            put_far(cd_, ab2, b, cd2, d, i2, ni2, j2, nj2);
+        } else {  // KernelMatrix2, src/KernelMatrix.jl:95-116
+            put_far(cd_, a, ab2, c, cd2, i0, ni1, j0, nj1);
+            if (leaf) put_dense(cd_, i0, ni1, j2, nj2);
+            else node(2, cd_, i0, ni1, j2, nj2, a, ab2, cd2, d);
+            put_far(cd_, ab2, b, c, cd2, i2, ni2, j0, nj1);
+            put_far(cd_, ab2, b, cd2, d, i2, ni2, j2, nj2);
+        }
+    }
+};
+
+// Chunks over one side: boundaries of every block and cluster, split to
+// HM_CHUNK; then CSR lists of the dense blocks and clusters covering each.
+struct Side {
+    std::vector<DChunk> ch;
+    std::vector<DNear> near;
+    std::vector<DFar> farl;
+    std::vector<DTask> tasks;
+    int64_t nslots = 0;
+};
+
+void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
+                std::vector<DCluster> &cl, Side &S) {
+    std::vector<int64_t> bnd;
+    bnd.reserve(2 * (dense.size() + cl.size()) + 2);
+    bnd.push_back(0);
+    bnd.push_back(n);
+    for (const DDense &D : dense) {
+        const int64_t s = rows ? D.row0 : D.col0, l = rows ? D.m : D.n;
+        bnd.push_back(s);
+        bnd.push_back(s + l);
+    }
+    for (const DCluster &C : cl) {
+        bnd.push_back(C.first);
+        bnd.push_back(C.first + C.len);
+    }
+    std::sort(bnd.begin(), bnd.end());
+    bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
+    std::vector<int64_t> p0;
+    for (size_t k = 0; k + 1 < bnd.size(); ++k)
+        for (int64_t s = bnd[k]; s < bnd[k + 1]; s += HM_CHUNK) p0.push_back(s);
+    const int64_t nch = (int64_t)p0.size();
+    S.ch.assign(nch, DChunk{});
+    for (int64_t c = 0; c < nch; ++c) {
+        S.ch[c].p0 = p0[c];
+        const int64_t e = c + 1 < nch ? p0[c + 1] : n;
+        S.ch[c].np = (int32_t)(e - p0[c]);
+    }
+    auto range = [&](int64_t s, int64_t l, int64_t &c0, int64_t &c1) {
+        c0 = std::lower_bound(p0.begin(), p0.end(), s) - p0.begin();
+        c1 = std::lower_bound(p0.begin(), p0.end(), s + l) - p0.begin();
+    };
+    std::vector<int32_t> ncount(nch + 1, 0), fcount(nch + 1, 0);
+    for (const DDense &D : dense) {
+        int64_t c0, c1;
+        range(rows ? D.row0 : D.col0, rows ? D.m : D.n, c0, c1);
+        for (int64_t c = c0; c < c1; ++c) ncount[c + 1]++;
+    }
+    // clusters in order of depth so every chunk lists coarse levels first
+    std::vector<int32_t> order(cl.size());
+    for (size_t k = 0; k < cl.size(); ++k) order[k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return cl[a].depth < cl[b].depth; });
+    for (int32_t id : order) {
+        const DCluster &C = cl[id];
+        int64_t c0, c1;
+        range(C.first, C.len, c0, c1);
+        for (int64_t c = c0; c < c1; ++c) fcount[c + 1]++;
+    }
+    for (int64_t c = 0; c < nch; ++c) {
+        ncount[c + 1] += ncount[c];
+        fcount[c + 1] += fcount[c];
+    }
+    S.near.assign(ncount[nch], DNear{});
+    S.farl.assign(fcount[nch], DFar{});
+    std::vector<int32_t> nfill(nch, 0), ffill(nch, 0);
+    for (size_t b = 0; b < dense.size(); ++b) {
+        const DDense &D = dense[b];
+        const int64_t s = rows ? D.row0 : D.col0;
+        int64_t c0, c1;
+        range(s, rows ? D.m : D.n, c0, c1);
+        for (int64_t c = c0; c < c1; ++c)
+            S.near[ncount[c] + nfill[c]++] = DNear{(int32_t)b, (int32_t)(p0[c] - s)};
+    }
+    for (int32_t id : order) {
+        const DCluster &C = cl[id];
+        int64_t c0, c1;
+        range(C.first, C.len, c0, c1);
+        for (int64_t c = c0; c < c1; ++c)
+            S.farl[fcount[c] + ffill[c]++] = DFar{id, (int32_t)(p0[c] - C.first)};
+    }
+    // upward-pass tasks: each cluster split into ranges of <= HM_TASK_POINTS
+    // points with consecutive slots; launched largest first
+    int64_t slot = 0;
+    S.tasks.clear();
+    for (size_t id = 0; id < cl.size(); ++id) {
+        DCluster &C = cl[id];
+        C.pbase = slot;
+        C.nslots = 0;
+        for (int64_t o = 0; o < C.len; o += HM_TASK_POINTS) {
+            const int32_t len = (int32_t)std::min<int64_t>(HM_TASK_POINTS, C.len - o);
+            S.tasks.push_back(DTask{(int32_t)id, (int32_t)o, len, 0, slot++});
+            C.nslots++;
+        }
+    }
+    S.nslots = slot;
+    std::stable_sort(S.tasks.begin(), S.tasks.end(),
+                     [](const DTask &u, const DTask &v) { return u.len > v.len; });
+    for (int64_t c = 0; c < nch; ++c) {
+        S.ch[c].near_beg = ncount[c];
+        S.ch[c].near_end = ncount[c + 1];
+        S.ch[c].far_beg = fcount[c];
+        S.ch[c].far_end = fcount[c + 1];
+    }
+}
+
+template <class T>
+int upload(T **dst, const std::vector<T> &src) {
+    *dst = nullptr;
+    if (src.empty()) return HM_OK;
+    HM_HIP(hipMalloc((void **)dst, sizeof(T) * src.size()));
+    HM_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return HM_OK;
+}
+
+int dalloc(double **dst, int64_t n) {
+    *dst = nullptr;
+    if (n <= 0) return HM_OK;
+    hipError_t e = hipMalloc((void **)dst, sizeof(double) * (size_t)n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *dst = nullptr;
+        return fail(HM_ERR_OOM, "hipMalloc of %lld doubles failed: %s", (long long)n,
+                    hipGetErrorString(e));
+    }
+    return HM_OK;
+}
+
+}  // namespace
+
+struct hm_plan {
+    int kid = 0, r = 0, bs = 0;
+    unsigned flags = 0;
+    int64_t nx = 0, ny = 0;
+    // host copies
+    std::vector<Leaf> leaves;
+    std::vector<DDense> dense;
+    std::vector<DCluster> rcl, ccl;
+    std::vector<DFarBlk> far;
+    std::vector<int32_t> far_by_rc_h;
+    Side rows, cols;
+    int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
+    int maxdepth = 0;
+    int64_t nonfinite = 0;
+    // device
+    DevPlan dp{};
+    double *d_x = nullptr, *d_y = nullptr, *d_lam = nullptr, *d_rnodes = nullptr,
+           *d_cnodes = nullptr;
+    DCluster *d_rcl = nullptr, *d_ccl = nullptr;
+    DFarBlk *d_far = nullptr;
+    int32_t *d_far_by_rc = nullptr, *d_far_by_cc = nullptr;
+    DDense *d_dense = nullptr;
+    DChunk *d_rch = nullptr, *d_cch = nullptr;
+    DNear *d_rnear = nullptr, *d_cnear = nullptr;
+    DFar *d_rfar = nullptr, *d_cfar = nullptr;
+    DTask *d_rtask = nullptr, *d_ctask = nullptr;
+    double *d_A = nullptr, *d_U = nullptr, *d_V = nullptr, *d_F = nullptr;
+    double *d_partial = nullptr, *d_w = nullptr, *d_z = nullptr;
+    bool profile = false;
+    std::vector<hipEvent_t> prof_events;  // 5 per product
+
+    void drop_events() {
+        for (hipEvent_t e : prof_events) (void)hipEventDestroy(e);
+        prof_events.clear();
+    }
+
+    ~hm_plan() {
+        drop_events();
+        void *bufs[] = {d_x, d_y, d_lam, d_rnodes, d_cnodes, d_rcl, d_ccl, d_far, d_far_by_rc,
+                        d_far_by_cc, d_dense, d_rch, d_cch, d_rnear, d_cnear, d_rfar, d_cfar,
+                        d_rtask, d_ctask,
+                        d_A, d_U, d_V, d_F, d_partial, d_w, d_z};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+    }
+
+    int64_t basis_rows(const std::vector<DCluster> &cl) const {
+        int64_t s = 0;
+        for (const DCluster &C : cl) s += C.len;
+        return s;
+    }
+};
+
+extern "C" {
+
+const char *hm_last_error(void) { return g_err.c_str(); }
+
+int hm_version(void) { return 100; }
+
+int hm_blockrank(int mant_bits) {
+    if (mant_bits < 1 || mant_bits > 4096) return -1;
+    const double v = 0.5 * (std::log(std::ldexp(1.0, mant_bits)) / std::log(3.0 + std::sqrt(8.0)));
+    return 2 * (int)std::lround(v);
+}
+
+int hm_chebyshevpoints(double *out, int n, int kind) {
+    if (!out || n < 1 || (kind != 1 && kind != 2) || (kind == 2 && n < 2))
+        return fail(HM_ERR_INVALID, "chebyshevpoints: bad arguments");
+    cheb_points(out, n, kind);
+    return HM_OK;
+}
+
+int hm_chebyshevbarycentricweights(double *out, int n, int kind) {
+    if (!out || n < 1 || (kind != 1 && kind != 2))
+        return fail(HM_ERR_INVALID, "chebyshevbarycentricweights: bad arguments");
+    cheb_weights(out, n, kind);
+    return HM_OK;
+}
+
+int hm_indsplit(const double *x, int64_t nx, int64_t first, int64_t len, double a, double b,
+                int64_t out[4]) {
+    if (!x || !out || len < 0 || first < 1) return fail(HM_ERR_INVALID, "indsplit: bad arguments");
+    if (len > 0 && first + len - 1 > nx) return fail(HM_ERR_INVALID, "indsplit: range out of bounds");
+    int64_t bad;
+    if (len > 0 && !non_increasing(x + first - 1, len, &bad))
+        return fail(HM_ERR_NOT_DESCENDING, "indsplit: x not non-increasing at %lld",
+                    (long long)(first + bad));
+    Builder B{};
+    const int64_t l1 = B.split(x, nx, first - 1, len, a, b);
+    if (B.err) return fail(B.err, "%s", B.msg.c_str());
+    out[0] = first;
+    out[1] = l1;
+    out[2] = first + l1;
+    out[3] = len - l1;
+    return HM_OK;
+}
+
+int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double *y, int64_t ny,
+                        double a, double b, double c, double d, int blocksize, int rank,
+                        unsigned flags, void *stream, hm_plan_t *out) {
+    if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
+    *out = nullptr;
+    if (kernel_id < HM_CAUCHY || kernel_id > HM_LOG) return fail(HM_ERR_INVALID, "unknown kernel id %d", kernel_id);
+    if (!x || !y || nx < 1 || ny < 1)
+        return fail(HM_ERR_DEGENERATE, "KernelMatrix needs non-empty x and y (the reference indexes x[1])");
+    if (nx >= (1LL << 40) || ny >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many points");
+    if (blocksize < 1) return fail(HM_ERR_INVALID, "blocksize must be >= 1");
+    if (rank < 1 || rank > 64) return fail(HM_ERR_UNSUPPORTED, "rank %d outside 1..64", rank);
+    if (flags & ~7u) return fail(HM_ERR_INVALID, "unknown plan flags 0x%x", flags);
+    if (flags & HM_PLAN_FAR_F32) return fail(HM_ERR_UNSUPPORTED, "FP32 far-field plans are not built by this entry point");
+    int64_t bad;
+    if (!non_increasing(x, nx, &bad))
+        return fail(HM_ERR_NOT_DESCENDING, "x must be finite and non-increasing (index %lld)", (long long)bad);
+    if (!non_increasing(y, ny, &bad))
+        return fail(HM_ERR_NOT_DESCENDING, "y must be finite and non-increasing (index %lld)", (long long)bad);
+
+    Builder B{};
+    B.x = x; B.y = y; B.nx = nx; B.ny = ny; B.bs = blocksize; B.r = rank;
+    B.node(0, 0, 0, nx, 0, ny, a, b, c, d);
+    if (B.err) return fail(B.err, "%s", B.msg.c_str());
+
+    hm_plan *P = new hm_plan();
+    P->kid = kernel_id; P->r = rank; P->bs = blocksize; P->flags = flags;
+    P->nx = nx; P->ny = ny;
+    P->nd = B.nd; P->nk1 = B.nk1; P->nk2 = B.nk2; P->maxdepth = B.maxdepth;
+    P->far_rows = B.far_rows; P->far_cols = B.far_cols; P->dense_entries = B.dense_entries;
+    P->leaves = std::move(B.leaves);
+    P->dense = std::move(B.dense);
+    P->far = std::move(B.far);
+    const int r = rank;
+
+    // storage offsets
+    int64_t off = 0;
+    for (DDense &D : P->dense) { D.aoff = off; off += (int64_t)D.m * D.n; }
+    const int64_t nA = off;
+    auto mk = [&](const std::vector<HCluster> &hc, std::vector<DCluster> &dc) {
+        dc.resize(hc.size());
+        int64_t o = 0;
+        for (size_t k = 0; k < hc.size(); ++k) {
+            dc[k] = DCluster{};
+            dc[k].first = hc[k].first;
+            dc[k].len = (int32_t)hc[k].len;
+            dc[k].boff = o;
+            dc[k].depth = hc[k].depth;
+            o += hc[k].len * r;
+        }
+        return o;
+    };
+    const int64_t nU = mk(B.rcl, P->rcl), nV = mk(B.ccl, P->ccl);
+    for (size_t k = 0; k < P->far.size(); ++k) P->far[k].foff = (int64_t)k * r * r;
+    const int64_t nF = (int64_t)P->far.size() * r * r;
+
+    // far blocks grouped by row / column cluster
+    std::vector<int32_t> by_rc(P->far.size()), by_cc(P->far.size());
+    for (size_t k = 0; k < P->far.size(); ++k) by_rc[k] = by_cc[k] = (int32_t)k;
+    std::stable_sort(by_rc.begin(), by_rc.end(), [&](int32_t u, int32_t v) { return P->far[u].rc < P->far[v].rc; });
+    std::stable_sort(by_cc.begin(), by_cc.end(), [&](int32_t u, int32_t v) { return P->far[u].cc < P->far[v].cc; });
+    for (auto &C : P->rcl) C.far_beg = C.far_end = 0;
+    for (auto &C : P->ccl) C.far_beg = C.far_end = 0;
+    for (size_t k = 0; k < by_rc.size(); ++k) {
+        DCluster &C = P->rcl[P->far[by_rc[k]].rc];
+        if (C.far_end == 0) C.far_beg = (int32_t)k;
+        C.far_end = (int32_t)k + 1;
+    }
+    for (size_t k = 0; k < by_cc.size(); ++k) {
+        DCluster &C = P->ccl[P->far[by_cc[k]].cc];
+        if (C.far_end == 0) C.far_beg = (int32_t)k;
+        C.far_end = (int32_t)k + 1;
+    }
+
+    build_side(nx, P->dense, true, P->rcl, P->rows);
+    build_side(ny, P->dense, false, P->ccl, P->cols);
+
+    // mapped nodes t = (a+b)/2 + (b-a)/2 * xhat, src/BarycentricMatrix.jl:156-166
+    std::vector<double> xhat(r), lam(r);
+    cheb_points(xhat.data(), r, 1);
+    cheb_weights(lam.data(), r, 1);
+    auto nodes = [&](const std::vector<HCluster> &hc) {
+        std::vector<double> t(hc.size() * r);
+        for (size_t k = 0; k < hc.size(); ++k) {
+            const double ab2 = 0.5 * (hc[k].lo + hc[k].hi), ba2 = 0.5 * (hc[k].hi - hc[k].lo);
+            for (int p = 0; p < r; ++p) t[k * r + p] = ab2 + ba2 * xhat[p];
+        }
+        return t;
+    };
+    const std::vector<double> rn = nodes(B.rcl), cn = nodes(B.ccl);
+
+    if (flags & HM_PLAN_TREE_ONLY) {
+        P->far_by_rc_h = std::move(by_rc);
+        *out = P;
+        return HM_OK;
+    }
+    std::vector<double> xv(x, x + nx), yv(y, y + ny);
+    int rc_;
+#define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
+    HM_TRY(upload(&P->d_x, xv));
+    HM_TRY(upload(&P->d_y, yv));
+    HM_TRY(upload(&P->d_lam, lam));
+    HM_TRY(upload(&P->d_rnodes, rn));
+    HM_TRY(upload(&P->d_cnodes, cn));
+    HM_TRY(upload(&P->d_rcl, P->rcl));
+    HM_TRY(upload(&P->d_ccl, P->ccl));
+    HM_TRY(upload(&P->d_far, P->far));
+    HM_TRY(upload(&P->d_far_by_rc, by_rc));
+    P->far_by_rc_h = by_rc;
+    HM_TRY(upload(&P->d_far_by_cc, by_cc));
+    HM_TRY(upload(&P->d_dense, P->dense));
+    HM_TRY(upload(&P->d_rch, P->rows.ch));
+    HM_TRY(upload(&P->d_cch, P->cols.ch));
+    HM_TRY(upload(&P->d_rnear, P->rows.near));
+    HM_TRY(upload(&P->d_cnear, P->cols.near));
+    HM_TRY(upload(&P->d_rfar, P->rows.farl));
+    HM_TRY(upload(&P->d_cfar, P->cols.farl));
+    HM_TRY(upload(&P->d_rtask, P->rows.tasks));
+    HM_TRY(upload(&P->d_ctask, P->cols.tasks));
+    const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
+    const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
+    HM_TRY(dalloc(&P->d_partial, nslots * r));
+    HM_TRY(dalloc(&P->d_w, ncl * r));
+    HM_TRY(dalloc(&P->d_z, ncl * r));
+    const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
+    if (stored) {
+        HM_TRY(dalloc(&P->d_A, nA));
+        HM_TRY(dalloc(&P->d_U, nU));
+        HM_TRY(dalloc(&P->d_V, nV));
+        HM_TRY(dalloc(&P->d_F, nF));
+    }
+
+    DevPlan &dp = P->dp;
+    dp.kid = kernel_id; dp.r = r; dp.nx = nx; dp.ny = ny;
+    dp.x = P->d_x; dp.y = P->d_y; dp.lam = P->d_lam;
+    dp.rcl = P->d_rcl; dp.ccl = P->d_ccl;
+    dp.nrc = (int32_t)P->rcl.size(); dp.ncc = (int32_t)P->ccl.size();
+    dp.rnodes = P->d_rnodes; dp.cnodes = P->d_cnodes;
+    dp.far = P->d_far; dp.nfar = (int32_t)P->far.size();
+    dp.far_by_rc = P->d_far_by_rc; dp.far_by_cc = P->d_far_by_cc;
+    dp.dense = P->d_dense; dp.ndense = (int32_t)P->dense.size();
+    dp.rch = P->d_rch; dp.cch = P->d_cch;
+    dp.nrch = (int32_t)P->rows.ch.size(); dp.ncch = (int32_t)P->cols.ch.size();
+    dp.rnear = P->d_rnear; dp.cnear = P->d_cnear;
+    dp.rfar = P->d_rfar; dp.cfar = P->d_cfar;
+    dp.rtask = P->d_rtask; dp.ctask = P->d_ctask;
+    dp.nrtask = (int32_t)P->rows.tasks.size(); dp.nctask = (int32_t)P->cols.tasks.size();
+    dp.nslots_r = P->rows.nslots; dp.nslots_c = P->cols.nslots;
+    dp.A = P->d_A; dp.U = P->d_U; dp.V = P->d_V; dp.F = P->d_F;
+
+    hipStream_t s = (hipStream_t)stream;
+    if (stored) {
+        hipError_t e = hmx_launch_compress(dp, s);
+        if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression launch: %s", hipGetErrorString(e)); }
+        // count non-finite factors (coincident points give Inf, as in the reference)
+        unsigned long long *d_cnt = nullptr;
+        e = hipMalloc((void **)&d_cnt, 1024 * sizeof(unsigned long long));
+        if (e == hipSuccess) {
+            std::vector<unsigned long long> h(1024);
+            int64_t total = 0;
+            const double *arrs[] = {P->d_A, P->d_U, P->d_V, P->d_F};
+            const int64_t lens[] = {nA, nU, nV, nF};
+            for (int q = 0; q < 4 && e == hipSuccess; ++q) {
+                if (lens[q] <= 0) continue;
+                e = hipMemsetAsync(d_cnt, 0, 1024 * sizeof(unsigned long long), s);
+                if (e == hipSuccess) e = hmx_launch_count_nonfinite(arrs[q], lens[q], d_cnt, s);
+                if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_cnt, 1024 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                for (auto v : h) total += (int64_t)v;
+            }
+            (void)hipFree(d_cnt);
+            P->nonfinite = total;
+        }
+        if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression: %s", hipGetErrorString(e)); }
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression sync: %s", hipGetErrorString(e)); }
+#undef HM_TRY
+    *out = P;
+    return HM_OK;
+}
+
+int hm_plan_destroy(hm_plan_t plan) {
+    if (!plan) return HM_OK;
+    if (!(plan->flags & HM_PLAN_TREE_ONLY)) (void)hipDeviceSynchronize();
+    delete plan;
+    return HM_OK;
+}
+
+int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
+    if (!P || !info || n < 0) return fail(HM_ERR_INVALID, "plan_info: bad arguments");
+    int64_t v[HM_INFO_COUNT] = {0};
+    const int64_t r = P->r;
+    const int64_t bu = P->basis_rows(P->rcl), bv = P->basis_rows(P->ccl);
+    int64_t dense_live = 0;
+    for (const DDense &D : P->dense) dense_live += (int64_t)D.m * D.n;
+    const int64_t vec = 8 * (P->nx + P->ny);
+    v[HM_INFO_NROWS] = P->nx;
+    v[HM_INFO_NCOLS] = P->ny;
+    v[HM_INFO_RANK] = P->r;
+    v[HM_INFO_BLOCKSIZE] = P->bs;
+    v[HM_INFO_KERNEL] = P->kid;
+    v[HM_INFO_N_D] = P->nd;
+    v[HM_INFO_N_K1] = P->nk1;
+    v[HM_INFO_N_K2] = P->nk2;
+    int64_t ndense = 0, nfar = 0;
+    for (const Leaf &L : P->leaves) (L.code == 3 ? ndense : nfar)++;
+    v[HM_INFO_N_DENSE] = ndense;
+    v[HM_INFO_N_FAR] = nfar;
+    v[HM_INFO_N_ROWCLUSTERS] = (int64_t)P->rcl.size();
+    v[HM_INFO_N_COLCLUSTERS] = (int64_t)P->ccl.size();
+    v[HM_INFO_DEPTH] = P->maxdepth;
+    v[HM_INFO_DENSE_ENTRIES] = P->dense_entries;
+    v[HM_INFO_BASIS_ROWS_U] = bu;
+    v[HM_INFO_BASIS_ROWS_V] = bv;
+    v[HM_INFO_BYTES_STORED] = 8 * (dense_live + r * (bu + bv) + (int64_t)P->far.size() * r * r) + vec;
+    v[HM_INFO_BYTES_REF_LAYOUT] = 8 * (P->dense_entries + r * (P->far_rows + P->far_cols) + nfar * r * r) + vec;
+    v[HM_INFO_FLAGS] = P->flags;
+    v[HM_INFO_N_ROWCHUNKS] = (int64_t)P->rows.ch.size();
+    v[HM_INFO_N_COLCHUNKS] = (int64_t)P->cols.ch.size();
+    v[HM_INFO_NONFINITE] = P->nonfinite;
+    for (int k = 0; k < n && k < HM_INFO_COUNT; ++k) info[k] = v[k];
+    for (int k = HM_INFO_COUNT; k < n; ++k) info[k] = 0;
+    return HM_OK;
+}
+
+int hm_plan_blocks(hm_plan_t P, int64_t *ints, double *dbls, int64_t cap, int64_t *count) {
+    if (!P) return fail(HM_ERR_INVALID, "plan_blocks: NULL plan");
+    if (count) *count = (int64_t)P->leaves.size();
+    for (int64_t k = 0; k < cap && k < (int64_t)P->leaves.size(); ++k) {
+        const Leaf &L = P->leaves[k];
+        if (ints) {
+            int64_t *o = ints + 6 * k;
+            o[0] = L.code; o[1] = L.depth; o[2] = L.row0; o[3] = L.col0; o[4] = L.m; o[5] = L.n;
+        }
+        if (dbls) {
+            double *g = dbls + 4 * k;
+            g[0] = L.a; g[1] = L.b; g[2] = L.c; g[3] = L.d;
+        }
+    }
+    return HM_OK;
+}
+
+static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
+                         int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                         unsigned flags, void *stream, bool tr) {
+    if (!P) return fail(HM_ERR_INVALID, "matvec: NULL plan");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec: tree-only plan has no device data");
+    if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
+        return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
+    if (flags & ~1u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (k == 0) return HM_OK;
+    if (!y || !x) return fail(HM_ERR_INVALID, "matvec: NULL vector");
+    if (k > 1 && (ldx < 1 || ldy < 1)) return fail(HM_ERR_INVALID, "matvec: bad ld");
+    const bool free_ = (P->flags & HM_PLAN_MATRIX_FREE) || (flags & HM_MV_MATRIX_FREE);
+    hipStream_t s = (hipStream_t)stream;
+    for (int64_t q = 0; q < k; ++q) {
+        MvArgs a{};
+        a.y = y + ioff + q * ldy;
+        a.x = x + joff + q * ldx;
+        a.incy = incy;
+        a.incx = incx;
+        a.beta = beta;
+        a.partial = P->d_partial;
+        a.w = P->d_w;
+        a.z = P->d_z;
+        const hipEvent_t *ev = nullptr;
+        if (P->profile) {
+            const size_t at = P->prof_events.size();
+            for (int q = 0; q < 5; ++q) {
+                hipEvent_t h;
+                HM_HIP(hipEventCreate(&h));
+                P->prof_events.push_back(h);
+            }
+            ev = P->prof_events.data() + at;
+        }
+        hipError_t e = hmx_launch_matvec(P->dp, a, tr, free_, s, ev);
+        if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
+    }
+    return HM_OK;
+}
+
+int hm_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff, int64_t incy,
+              int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta, unsigned flags,
+              void *stream) {
+    return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, false);
+}
+
+int hm_matvec_t(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
+                int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                unsigned flags, void *stream) {
+    return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, true);
+}
+
+int hm_plan_profile(hm_plan_t P, int enable) {
+    if (!P) return fail(HM_ERR_INVALID, "profile: NULL plan");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "profile: tree-only plan");
+    HM_HIP(hipDeviceSynchronize());
+    P->drop_events();
+    P->profile = enable != 0;
+    return HM_OK;
+}
+
+int hm_plan_profile_read(hm_plan_t P, double ms[4], int64_t *count) {
+    if (!P || !ms) return fail(HM_ERR_INVALID, "profile_read: bad arguments");
+    for (int q = 0; q < 4; ++q) ms[q] = 0.0;
+    const size_t nprod = P->prof_events.size() / 5;
+    if (nprod) HM_HIP(hipEventSynchronize(P->prof_events.back()));
+    for (size_t t = 0; t < nprod; ++t) {
+        for (int q = 0; q < 4; ++q) {
+            float f = 0.f;
+            HM_HIP(hipEventElapsedTime(&f, P->prof_events[5 * t + q], P->prof_events[5 * t + q + 1]));
+            ms[q] += f;
+        }
+    }
+    if (count) *count = (int64_t)nprod;
+    return HM_OK;
+}
+
+int hm_entries(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, double *out,
+               void *stream) {
+    if (!P || n < 0) return fail(HM_ERR_INVALID, "entries: bad arguments");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "entries: tree-only plan has no device data");
+    if (n == 0) return HM_OK;
+    if (!i || !j || !out) return fail(HM_ERR_INVALID, "entries: NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int64_t> hi(n), hj(n);
+    HM_HIP(hipMemcpyAsync(hi.data(), i, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HM_HIP(hipMemcpyAsync(hj.data(), j, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HM_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> kind(n, 0), id(n, -1);
+    const Side &R = P->rows;
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t ii = hi[t], jj = hj[t];
+        if (ii < 0 || ii >= P->nx || jj < 0 || jj >= P->ny)
+            return fail(HM_ERR_INVALID, "entries: index (%lld, %lld) out of range", (long long)ii, (long long)jj);
+        auto it = std::upper_bound(R.ch.begin(), R.ch.end(), ii,
+                                   [](int64_t v, const DChunk &c) { return v < c.p0; });
+        const DChunk &c = *(it - 1);
+        for (int e = c.near_beg; e < c.near_end && !kind[t]; ++e) {
+            const DDense &D = P->dense[R.near[e].blk];
+            if (jj >= D.col0 && jj < D.col0 + D.n) { kind[t] = 1; id[t] = R.near[e].blk; }
+        }
+        for (int e = c.far_beg; e < c.far_end && !kind[t]; ++e) {
+            const DCluster &C = P->rcl[R.farl[e].cl];
+            for (int32_t q = C.far_beg; q < C.far_end && !kind[t]; ++q) {
+                const int32_t f = P->far_by_rc_h[q];
+                const DCluster &Cc = P->ccl[P->far[f].cc];
+                if (jj >= Cc.first && jj < Cc.first + Cc.len) { kind[t] = 2; id[t] = f; }
+            }
+        }
+    }
+    int32_t *d_kind = nullptr, *d_id = nullptr;
+    HM_HIP(hipMalloc((void **)&d_kind, n * sizeof(int32_t)));
+    hipError_t e = hipMalloc((void **)&d_id, n * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_kind, kind.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_id, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hmx_launch_entries(P->dp, n, i, j, d_kind, d_id, out, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_kind);
+    if (d_id) (void)hipFree(d_id);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "entries: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,18 @@
+"""hmx -- MI355X (gfx950) KernelMatrix compression and hierarchical products.
+
+Python mirror of the KernelMatrix API of HierarchicalMatrices.jl over the C ABI
+of libhmx.so (include/hmx.h). Importing fails if libhmx.so is not built.
+"""
+from ._lib import HMError, lib  # noqa: F401  (raises ImportError if libhmx.so is missing)
+from .kernelmatrix import (  # noqa: F401
+    BLOCKRANK, BLOCKSIZE, Block, KernelMatrix, KERNELS, NamedKernel, blocksize,
+    cauchykernel, chebyshevbarycentricweights, chebyshevpoints, coulombkernel,
+    coulombprimekernel, indsplit, logkernel, mul_, size,
+)
+from . import workloads  # noqa: F401
+
+__all__ = [
+    "BLOCKRANK", "BLOCKSIZE", "Block", "KernelMatrix", "HMError", "blocksize", "size",
+    "cauchykernel", "coulombkernel", "coulombprimekernel", "logkernel",
+    "chebyshevpoints", "chebyshevbarycentricweights", "indsplit", "mul_", "workloads",
+]

@@ -1,0 +1,94 @@
+"""ctypes binding of libhmx.so (the C ABI declared in include/hmx.h).
+
+The library is loaded from this package directory. There is no fallback: if
+libhmx.so is missing or fails to load, importing the package raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhmx.so")
+
+HM_OK = 0
+HM_ERR_INVALID = 1
+HM_ERR_NOT_DESCENDING = 2
+HM_ERR_DEGENERATE = 3
+HM_ERR_OOM = 4
+HM_ERR_HIP = 5
+HM_ERR_UNSUPPORTED = 6
+
+HM_PLAN_STORED = 0
+HM_PLAN_MATRIX_FREE = 1
+HM_PLAN_FAR_F32 = 2
+HM_PLAN_TREE_ONLY = 4
+HM_MV_MATRIX_FREE = 1
+
+INFO_KEYS = [
+    "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
+    "n_rowclusters", "n_colclusters", "depth", "dense_entries", "basis_rows_u", "basis_rows_v",
+    "bytes_stored", "bytes_ref_layout", "flags", "n_rowchunks", "n_colchunks", "nonfinite",
+]
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+
+
+class HMError(RuntimeError):
+    """A libhmx call returned a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libhmx error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libhmx.so not found at {LIB_PATH}; build it with `python -c 'import "
+            f"__graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.hm_last_error.restype = ctypes.c_char_p
+    L.hm_last_error.argtypes = []
+    L.hm_version.restype = ctypes.c_int
+    L.hm_blockrank.restype = ctypes.c_int
+    L.hm_blockrank.argtypes = [ctypes.c_int]
+    for name in ("hm_chebyshevpoints", "hm_chebyshevbarycentricweights"):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+    L.hm_indsplit.restype = ctypes.c_int
+    L.hm_indsplit.argtypes = [_dp, _i64, _i64, _i64, ctypes.c_double, ctypes.c_double, _ip]
+    L.hm_plan_from_points.restype = ctypes.c_int
+    L.hm_plan_from_points.argtypes = [ctypes.c_int, _dp, _i64, _dp, _i64, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_uint, _vp,
+                                      ctypes.POINTER(_vp)]
+    L.hm_plan_destroy.restype = ctypes.c_int
+    L.hm_plan_destroy.argtypes = [_vp]
+    L.hm_plan_info.restype = ctypes.c_int
+    L.hm_plan_info.argtypes = [_vp, _ip, ctypes.c_int]
+    L.hm_plan_blocks.restype = ctypes.c_int
+    L.hm_plan_blocks.argtypes = [_vp, _ip, _dp, _i64, _ip]
+    for name in ("hm_matvec", "hm_matvec_t"):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, ctypes.c_int,
+                       ctypes.c_uint, _vp]
+    L.hm_plan_profile.restype = ctypes.c_int
+    L.hm_plan_profile.argtypes = [_vp, ctypes.c_int]
+    L.hm_plan_profile_read.restype = ctypes.c_int
+    L.hm_plan_profile_read.argtypes = [_vp, _dp, _ip]
+    L.hm_entries.restype = ctypes.c_int
+    L.hm_entries.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp]
+    return L
+
+
+lib = _load()
+
+
+def check(code):
+    if code != HM_OK:
+        raise HMError(code, lib.hm_last_error().decode(errors="replace"))
+    return code

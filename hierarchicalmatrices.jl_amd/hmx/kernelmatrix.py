@@ -1,0 +1,323 @@
+"""Host-side mirror of HierarchicalMatrices.jl's KernelMatrix API, backed by libhmx.
+
+Names, argument meaning and accumulate semantics follow the reference:
+
+  KernelMatrix(f, x, y, a, b, c, d)        src/KernelMatrix.jl:47
+  mul_(u, H, v, istart, jstart)            mul!, src/KernelMatrix.jl:14-45 (u += H v)
+  H * v, H @ v                             src/KernelMatrix.jl:5-12 (zeros, then mul!)
+  BLOCKRANK(T), BLOCKSIZE(T)               src/HierarchicalMatrices.jl:5-7
+  chebyshevpoints, chebyshevbarycentricweights, indsplit
+                                           src/BarycentricMatrix.jl:92-136,299-307
+  blocksize(H), size(H), H[i, j]           src/hierarchical.jl:31-47,120-147
+
+Differences a caller sees: the kernel f is one of the named kernels of
+examples/Kernel.jl:34-37 (cauchykernel, coulombkernel, coulombprimekernel,
+logkernel) instead of an arbitrary Julia function; element access H[i, j] uses
+0-based Python indices; vectors may be float64 torch tensors on the GPU (used
+in place, stream-ordered on torch's current stream) or numpy arrays (copied to
+and from the device).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+class NamedKernel:
+    """A kernel of examples/Kernel.jl:34-37, known to the device code by id."""
+
+    def __init__(self, name, kid, fn):
+        self.name, self.kid, self._fn = name, kid, fn
+
+    def __call__(self, x, y):
+        return self._fn(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64))
+
+    def __repr__(self):
+        return self.name
+
+
+cauchykernel = NamedKernel("cauchykernel", 0, lambda x, y: 1.0 / (x - y))
+coulombkernel = NamedKernel("coulombkernel", 1, lambda x, y: 1.0 / ((x - y) * (x - y)))
+coulombprimekernel = NamedKernel("coulombprimekernel", 2,
+                                 lambda x, y: 1.0 / ((x - y) * (x - y) * (x - y)))
+logkernel = NamedKernel("logkernel", 3, lambda x, y: np.log(np.abs(x - y)))
+KERNELS = {k.name: k for k in (cauchykernel, coulombkernel, coulombprimekernel, logkernel)}
+
+_MANT = {np.dtype(np.float16): 10, np.dtype(np.float32): 23, np.dtype(np.float64): 52}
+
+
+def BLOCKRANK(T=np.float64):
+    """2*round(0.5*log_{3+sqrt 8}(1/eps(T))), src/HierarchicalMatrices.jl:5."""
+    return lib.hm_blockrank(_MANT[np.dtype(T)])
+
+
+def BLOCKSIZE(T=np.float64):
+    """4*BLOCKRANK(T), src/HierarchicalMatrices.jl:7."""
+    return 4 * BLOCKRANK(T)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def chebyshevpoints(n, kind=1):
+    out = np.zeros(n)
+    check(lib.hm_chebyshevpoints(_dptr(out), n, kind))
+    return out
+
+
+def chebyshevbarycentricweights(n, kind=1):
+    out = np.zeros(n)
+    check(lib.hm_chebyshevbarycentricweights(_dptr(out), n, kind))
+    return out
+
+
+def indsplit(x, ir, a, b):
+    """indsplit(x, ir, a, b): ir is a range of 1-based indices (Julia's
+    first:last); returns the two halves as ranges of 1-based indices."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(4, dtype=np.int64)
+    check(lib.hm_indsplit(_dptr(x), len(x), ir.start, len(ir), a, b,
+                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    f1, l1, f2, l2 = (int(v) for v in out)
+    return range(f1, f1 + l1), range(f2, f2 + l2)
+
+
+class Block:
+    """Block index, src/block.jl:8-10."""
+
+    def __init__(self, K):
+        self.K = int(K)
+
+    def __eq__(self, o):
+        return isinstance(o, Block) and o.K == self.K
+
+    def __repr__(self):
+        return f"Block({self.K})"
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class _Vec:
+    """A float64 vector or (n, k) matrix on the device, with its strides."""
+
+    def __init__(self, a, writable):
+        torch = _torch()
+        self.host = None
+        if isinstance(a, torch.Tensor):
+            if a.device.type != "cuda" or a.dtype != torch.float64:
+                raise TypeError("device vectors must be float64 CUDA tensors")
+            self.t = a
+        else:
+            arr = np.asarray(a)
+            if arr.dtype != np.float64:
+                raise TypeError("vectors must be float64 (the eltype of H, src/KernelMatrix.jl:17)")
+            self.host = arr
+            self.t = torch.from_numpy(np.array(arr, copy=True)).cuda()
+        self.writable = writable
+        t = self.t
+        if t.dim() == 1:
+            self.n, self.k, self.inc, self.ld = t.shape[0], 1, t.stride(0), 0
+        elif t.dim() == 2:
+            self.n, self.k = t.shape
+            self.inc, self.ld = t.stride(0), t.stride(1)
+        else:
+            raise ValueError("vectors must be 1-D or 2-D")
+
+    def ptr(self):
+        return ctypes.c_void_p(self.t.data_ptr())
+
+    def writeback(self):
+        if self.host is not None and self.writable:
+            torch = _torch()
+            torch.cuda.synchronize()
+            self.host[...] = self.t.cpu().numpy()
+
+
+class KernelMatrix:
+    """KernelMatrix(f, x, y, a, b, c, d), src/KernelMatrix.jl:47-116.
+
+    x and y must be non-increasing (the reference's indsplit assumes it,
+    src/BarycentricMatrix.jl:299-307); (a, b), (c, d) are their intervals as
+    (upper, lower). blocksize and rank default to BLOCKSIZE(Float64) = 80 and
+    BLOCKRANK(Float64) = 20. matrix_free=True keeps only the points and
+    regenerates every factor in each product.
+    """
+
+    def __init__(self, f, x, y, a, b, c, d, *, blocksize=None, rank=None, matrix_free=False,
+                 tree_only=False, stream=None):
+        if isinstance(f, str):
+            f = KERNELS[f]
+        if not isinstance(f, NamedKernel):
+            raise TypeError("f must be one of cauchykernel, coulombkernel, coulombprimekernel, "
+                            "logkernel")
+        self.f = f
+        self.rank = BLOCKRANK(np.float64) if rank is None else int(rank)
+        self.blocksize_ = 4 * self.rank if blocksize is None else int(blocksize)
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.intervals = (float(a), float(b), float(c), float(d))
+        flags = 0
+        if matrix_free:
+            flags |= _lib.HM_PLAN_MATRIX_FREE
+        if tree_only:
+            flags |= _lib.HM_PLAN_TREE_ONLY
+        self.flags = flags
+        h = ctypes.c_void_p()
+        sp = ctypes.c_void_p(0) if tree_only else _stream_ptr(stream)
+        check(lib.hm_plan_from_points(f.kid, _dptr(self.x), len(self.x), _dptr(self.y),
+                                      len(self.y), float(a), float(b), float(c), float(d),
+                                      self.blocksize_, self.rank, flags, sp, ctypes.byref(h)))
+        self._h = h
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.hm_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- structure (src/hierarchical.jl)
+    @property
+    def info(self):
+        out = np.zeros(len(_lib.INFO_KEYS), dtype=np.int64)
+        check(lib.hm_plan_info(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                               len(out)))
+        return dict(zip(_lib.INFO_KEYS, (int(v) for v in out)))
+
+    @property
+    def shape(self):
+        return (len(self.x), len(self.y))
+
+    def size(self, k=None):
+        return self.shape if k is None else self.shape[k - 1]
+
+    def blocksize(self):
+        """blocksize(H) = size(H.assigned) of the root node (2, 2)."""
+        return (2, 2)
+
+    def blocks(self):
+        """Leaf blocks in the reference's traversal order: (ints (nb, 6) =
+        code, depth, row0, col0, m, n; dbls (nb, 4) = a, b, c, d)."""
+        cnt = ctypes.c_int64(0)
+        check(lib.hm_plan_blocks(self._h, None, None, 0, ctypes.byref(cnt)))
+        nb = cnt.value
+        ints = np.zeros((max(nb, 1), 6), dtype=np.int64)
+        dbls = np.zeros((max(nb, 1), 4))
+        check(lib.hm_plan_blocks(self._h, ints.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 _dptr(dbls), nb, ctypes.byref(cnt)))
+        return ints[:nb], dbls[:nb]
+
+    # -- products
+    def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream):
+        U = _Vec(u, True)
+        V = _Vec(v, False)
+        if U.k != V.k:
+            raise ValueError("u and v hold different numbers of right-hand sides")
+        m, n = self.shape
+        if transpose:
+            m, n = n, m
+        ioff, joff = istart - 1, jstart - 1
+        incy = U.inc * incy
+        incx = V.inc * incx
+        if ioff < 0 or joff < 0:
+            raise ValueError("istart and jstart are 1-based")
+        if ioff + (m - 1) * incy >= U.n * U.inc or joff + (n - 1) * incx >= V.n * V.inc:
+            raise ValueError("vector too short for H at the given offset and stride")
+        fn = lib.hm_matvec_t if transpose else lib.hm_matvec
+        flags = _lib.HM_MV_MATRIX_FREE if matrix_free else 0
+        check(fn(self._h, U.ptr(), V.ptr(), ioff * U.inc, joff * V.inc, incy, incx, U.k,
+                 U.ld, V.ld, beta, flags, _stream_ptr(stream)))
+        U.writeback()
+        return u
+
+    def mul(self, u, v, istart=1, jstart=1, INCX=1, INCY=1, *, matrix_free=False, stream=None):
+        """u[istart + (i-1)INCY] += (H v)_i: the reference's accumulating mul!."""
+        return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, False, stream)
+
+    def tmul(self, u, v, istart=1, jstart=1, INCX=1, INCY=1, *, matrix_free=False, stream=None):
+        """u += H^T v (accumulating)."""
+        return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, True, stream)
+
+    def matvec(self, v, *, matrix_free=False, transpose=False, stream=None):
+        """H v (or H^T v) into a new vector on the device of v."""
+        torch = _torch()
+        ishost = not isinstance(v, torch.Tensor)
+        vt = torch.from_numpy(np.asarray(v, dtype=np.float64)).cuda() if ishost else v
+        m = self.shape[1] if transpose else self.shape[0]
+        out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=torch.float64, device=vt.device)
+        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream)
+        if ishost:
+            torch.cuda.synchronize()
+            return out.cpu().numpy()
+        return out
+
+    def __matmul__(self, v):
+        return self.matvec(v)
+
+    __mul__ = __matmul__
+
+    @property
+    def T(self):
+        return _Transposed(self)
+
+    def __getitem__(self, ij):
+        """H[i, j] with 0-based indices (getindex, src/hierarchical.jl:120-147)."""
+        torch = _torch()
+        i, j = ij
+        ii = torch.as_tensor(np.atleast_1d(np.asarray(i, dtype=np.int64))).cuda()
+        jj = torch.as_tensor(np.atleast_1d(np.asarray(j, dtype=np.int64))).cuda()
+        ii, jj = torch.broadcast_tensors(ii, jj)
+        ii, jj = ii.contiguous(), jj.contiguous()
+        out = torch.empty(ii.shape, dtype=torch.float64, device=ii.device)
+        check(lib.hm_entries(self._h, ii.numel(), ctypes.c_void_p(ii.data_ptr()),
+                             ctypes.c_void_p(jj.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                             _stream_ptr(None)))
+        res = out.cpu().numpy()
+        return float(res.reshape(-1)[0]) if np.ndim(i) == 0 and np.ndim(j) == 0 else res
+
+
+class _Transposed:
+    def __init__(self, H):
+        self.H = H
+
+    def __matmul__(self, v):
+        return self.H.matvec(v, transpose=True)
+
+    @property
+    def shape(self):
+        return self.H.shape[::-1]
+
+
+def mul_(u, H, v, istart=1, jstart=1, INCX=1, INCY=1, **kw):
+    """mul!(u, H, v[, istart, jstart[, INCX, INCY]]): u += H v in place,
+    src/KernelMatrix.jl:14-45 (offsets 1-based, strides as in
+    src/HierarchicalMatrix.jl:24-52)."""
+    return H.mul(u, v, istart, jstart, INCX, INCY, **kw)
+
+
+def blocksize(H):
+    return H.blocksize()
+
+
+def size(H, k=None):
+    return H.size(k)

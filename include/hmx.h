@@ -1,0 +1,153 @@
+/*
+ * hmx.h -- C ABI of libhmx, the MI355X (gfx950) implementation of the
+ * KernelMatrix compression and hierarchical matrix-vector product of
+ * HierarchicalMatrices.jl.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (HM_OK == 0) and never aborts; hm_last_error() gives a thread-local message
+ * for the last failing call. Vector arguments of the products are DEVICE
+ * pointers; point arrays of the constructors are HOST pointers (the block tree
+ * is built on the host from them, as the reference builds it). `stream` is a
+ * hipStream_t (NULL = the null stream); all work is stream-ordered.
+ *
+ * Index conventions: offsets are 0-based (the reference's 1-based istart/jstart
+ * minus one); strides inc >= 1 step between consecutive entries of one vector;
+ * ld steps between right-hand sides.
+ */
+#ifndef HMX_H
+#define HMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hm_plan *hm_plan_t;
+
+enum hm_status {
+    HM_OK = 0,
+    HM_ERR_INVALID = 1,        /* bad argument */
+    HM_ERR_NOT_DESCENDING = 2, /* points not non-increasing: the reference's
+                                  indsplit (src/BarycentricMatrix.jl:299-307)
+                                  recurses without bound on such input */
+    HM_ERR_DEGENERATE = 3,     /* an empty cluster reached indsplit; the
+                                  reference then forms an out-of-range block */
+    HM_ERR_OOM = 4,
+    HM_ERR_HIP = 5,
+    HM_ERR_UNSUPPORTED = 6
+};
+
+/* kernels of examples/Kernel.jl:34-37 */
+enum hm_kernel { HM_CAUCHY = 0, HM_COULOMB = 1, HM_COULOMB3 = 2, HM_LOG = 3 };
+
+/* plan flags (hm_plan_from_points) */
+enum hm_plan_flags {
+    HM_PLAN_STORED = 0,          /* generate and keep dense blocks, U, V, F in HBM */
+    HM_PLAN_MATRIX_FREE = 1,     /* keep only points and nodes; regenerate in every product */
+    HM_PLAN_FAR_F32 = 2,         /* stored far-field factors in FP32 (mixed precision) */
+    HM_PLAN_TREE_ONLY = 4        /* build the block tree and its tables on the host only:
+                                    no device memory, no products (for inspection) */
+};
+
+/* product flags (hm_matvec) */
+enum hm_mv_flags {
+    HM_MV_DEFAULT = 0,
+    HM_MV_MATRIX_FREE = 1        /* regenerate factors even if the plan stores them */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hm_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int hm_version(void);
+
+/* BLOCKRANK for an IEEE type with `mant_bits` stored mantissa bits.
+ * Replaces BLOCKRANK(T), src/HierarchicalMatrices.jl:5. BLOCKSIZE(T) = 4*rank
+ * (src/HierarchicalMatrices.jl:7). */
+int hm_blockrank(int mant_bits);
+
+/* Replaces chebyshevpoints(T, n; kind), src/BarycentricMatrix.jl:92-111. */
+int hm_chebyshevpoints(double *out, int n, int kind);
+
+/* Replaces chebyshevbarycentricweights(T, n; kind), src/BarycentricMatrix.jl:114-136. */
+int hm_chebyshevbarycentricweights(double *out, int n, int kind);
+
+/* Replaces indsplit(x, ir, a, b), src/BarycentricMatrix.jl:299-307.
+ * ir = first:first+len-1 (1-based first, as in Julia); out = {first1, len1,
+ * first2, len2}. Returns HM_ERR_NOT_DESCENDING if x[ir] is not non-increasing
+ * and HM_ERR_DEGENERATE where the reference would read out of range. */
+int hm_indsplit(const double *x, int64_t nx, int64_t first, int64_t len, double a, double b,
+                int64_t out[4]);
+
+/* Replaces KernelMatrix(f, x, y, a, b, c, d), src/KernelMatrix.jl:47-116, with
+ * f one of enum hm_kernel, and BLOCKSIZE/BLOCKRANK given explicitly
+ * (blocksize = leaf threshold, rank = Chebyshev rank). Builds the block tree on
+ * the host with the reference's rule, lays it out level by level in HBM and
+ * (unless HM_PLAN_MATRIX_FREE) runs the compression kernels on `stream`:
+ * dense blocks (src/KernelMatrix.jl:57-60,82,104), F (src/BarycentricMatrix.jl:168-175)
+ * and U, V (update!, src/BarycentricMatrix.jl:248-297), one U per row cluster
+ * and one V per column cluster. x (length nx) and y (length ny) are host arrays
+ * in non-increasing order; (a, b) and (c, d) are their intervals given as
+ * (upper, lower), as in examples/Kernel.jl:66. */
+int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double *y, int64_t ny,
+                        double a, double b, double c, double d, int blocksize, int rank,
+                        unsigned flags, void *stream, hm_plan_t *plan);
+
+/* Frees every device buffer of the plan (synchronises the device first). */
+int hm_plan_destroy(hm_plan_t plan);
+
+/* Plan facts, written to info[0..n): see enum hm_info. Replaces size(H) and
+ * blocksize(H) of src/hierarchical.jl:31-47 for the root. */
+enum hm_info {
+    HM_INFO_NROWS = 0, HM_INFO_NCOLS, HM_INFO_RANK, HM_INFO_BLOCKSIZE, HM_INFO_KERNEL,
+    HM_INFO_N_D, HM_INFO_N_K1, HM_INFO_N_K2, HM_INFO_N_DENSE, HM_INFO_N_FAR,
+    HM_INFO_N_ROWCLUSTERS, HM_INFO_N_COLCLUSTERS, HM_INFO_DEPTH,
+    HM_INFO_DENSE_ENTRIES, HM_INFO_BASIS_ROWS_U, HM_INFO_BASIS_ROWS_V,
+    HM_INFO_BYTES_STORED,      /* bytes of every stored array read by one product */
+    HM_INFO_BYTES_REF_LAYOUT,  /* same arrays in the reference's per-block layout */
+    HM_INFO_FLAGS, HM_INFO_N_ROWCHUNKS, HM_INFO_N_COLCHUNKS, HM_INFO_NONFINITE,
+    HM_INFO_COUNT
+};
+int hm_plan_info(hm_plan_t plan, int64_t *info, int n);
+
+/* Leaf blocks in the order the reference's traversal mul! visits them
+ * (src/KernelMatrix.jl:17-45). ints: 6 per block = code (2 far, 3 dense),
+ * depth, row0, col0, m, n; dbls: 4 per block = a, b, c, d (far blocks).
+ * *count receives the total; at most cap records are written. */
+int hm_plan_blocks(hm_plan_t plan, int64_t *ints, double *dbls, int64_t cap, int64_t *count);
+
+/* y[ioff + i*incy + q*ldy] = beta*y[...] + sum_j H[i,j] x[joff + j*incx + q*ldx]
+ * for q < k right-hand sides, beta in {0, 1}. beta = 1 is the reference's
+ * accumulating mul!(u, H, v, istart, jstart) (src/KernelMatrix.jl:14-45, the
+ * leaf loops src/algebra.jl:34-48 and :243-277); beta = 0 is H*v
+ * (src/KernelMatrix.jl:5-8). */
+int hm_matvec(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_t joff,
+              int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+              unsigned flags, void *stream);
+
+/* Transpose product y = beta*y + H^T x (same argument meaning, y indexed by
+ * columns of H). No counterpart for the tree in the reference; its leaf forms
+ * are src/algebra.jl:50-82 (dense) and :133-159 (low rank). */
+int hm_matvec_t(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_t joff,
+                int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                unsigned flags, void *stream);
+
+/* Per-pass timing of the products of this plan (bench support, no counterpart
+ * in the reference). enable=1 starts recording HIP events around the four
+ * passes of every later product (up, reduce, translate, down) on its stream;
+ * enable=0 stops and discards. hm_plan_profile_read synchronises, returns the
+ * summed milliseconds per pass in ms[0..3] and the number of products. */
+int hm_plan_profile(hm_plan_t plan, int enable);
+int hm_plan_profile_read(hm_plan_t plan, double ms[4], int64_t *count);
+
+/* out[t] = H[i[t], j[t]] (0-based) for t < n; i, j, out are device pointers.
+ * Replaces getindex(H, i, j), src/hierarchical.jl:120-147 with
+ * src/BarycentricMatrix.jl:222-234 at far blocks. */
+int hm_entries(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j, double *out,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,248 @@
+"""ctypes front-end of the CPU oracle (oracle/hmx_oracle.c).
+
+TEST INFRASTRUCTURE ONLY. Imported by tests/, by the smoke check and by the
+cpu_baseline leg of bench.py, always as the checker. The product package never
+imports this module.
+
+Index conventions follow the reference (src/KernelMatrix.jl, src/algebra.jl):
+leaf-kernel offsets are 1-based, block records use 0-based row/col offsets.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+CAUCHY, COULOMB, COULOMB3, LOG = 0, 1, 2, 3
+CODE_KM, CODE_BARY, CODE_DENSE = 1, 2, 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_kernel.restype = ctypes.c_double
+        L.or_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.or_blockrank.restype = ctypes.c_int
+        L.or_blockrank.argtypes = [ctypes.c_int]
+        L.or_sinpi.restype = ctypes.c_double
+        L.or_sinpi.argtypes = [ctypes.c_double]
+        L.or_chebyshevpoints.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.or_chebyshevbarycentricweights.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.or_indsplit.restype = ctypes.c_int
+        L.or_indsplit.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                  ctypes.c_double, ctypes.c_double, _ip]
+        L.or_tree_build.restype = ctypes.c_void_p
+        L.or_tree_build.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)]
+        L.or_tree_free.argtypes = [ctypes.c_void_p]
+        L.or_tree_stats.argtypes = [ctypes.c_void_p, _ip]
+        L.or_tree_blocks.restype = ctypes.c_int64
+        L.or_tree_blocks.argtypes = [ctypes.c_void_p, _ip, _dp, ctypes.c_int64]
+        L.or_tree_matvec.restype = ctypes.c_int
+        L.or_tree_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.or_tree_matvec_omp.restype = ctypes.c_int
+        L.or_tree_matvec_omp.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.or_tree_matvec_rows.restype = ctypes.c_int
+        L.or_tree_matvec_rows.argtypes = [ctypes.c_void_p, _dp, _ip, ctypes.c_int64, _dp,
+                                          ctypes.c_int]
+        L.or_tree_getindex.restype = ctypes.c_double
+        L.or_tree_getindex.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
+        L.or_tree_block_data.restype = ctypes.c_int
+        L.or_tree_block_data.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]
+        for name in ("or_dense_mul", "or_dense_tmul"):
+            getattr(L, name).argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _dp,
+                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64]
+        for name in ("or_lowrank_mul", "or_lowrank_adjmul"):
+            getattr(L, name).argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64, ctypes.c_int64]
+        L.or_bary_mul.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                                  ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64]
+        L.or_dense_matvec_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _ip,
+                                           ctypes.c_int64, _dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def kernel(kid, x, y):
+    return lib().or_kernel(kid, float(x), float(y))
+
+
+def blockrank(mant_bits):
+    return lib().or_blockrank(mant_bits)
+
+
+def chebyshevpoints(n, kind=1):
+    out = np.zeros(n)
+    lib().or_chebyshevpoints(_d(out), n, kind)
+    return out
+
+
+def chebyshevbarycentricweights(n, kind=1):
+    out = np.zeros(n)
+    lib().or_chebyshevbarycentricweights(_d(out), n, kind)
+    return out
+
+
+def indsplit(x, first, length, a, b):
+    """Returns ((first1, len1), (first2, len2)) with 1-based firsts."""
+    x = f64(x)
+    out = np.zeros(4, dtype=np.int64)
+    rc = lib().or_indsplit(_d(x), len(x), first, length, a, b, _i(out))
+    if rc:
+        raise IndexError("BoundsError in indsplit")
+    return (int(out[0]), int(out[1])), (int(out[2]), int(out[3]))
+
+
+class Tree:
+    """KernelMatrix tree built by the oracle (src/KernelMatrix.jl:47-116)."""
+
+    def __init__(self, kid, x, y, a, b, c, d, blocksize, rank, store=True):
+        self.x = f64(x)
+        self.y = f64(y)
+        self.kid, self.rank, self.blocksize = kid, rank, blocksize
+        quirk = ctypes.c_int(0)
+        h = lib().or_tree_build(kid, _d(self.x), _d(self.y), len(self.x), len(self.y), a, b, c, d,
+                                blocksize, rank, 1 if store else 0, ctypes.byref(quirk))
+        if not h:
+            raise ValueError("oracle tree build failed (bounds error or unbounded recursion)")
+        self.h = h
+        self.quirk = quirk.value
+        self.store = store
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_tree_free(self.h)
+            self.h = None
+
+    def stats(self):
+        out = np.zeros(9, dtype=np.int64)
+        lib().or_tree_stats(self.h, _i(out))
+        keys = ["d", "k1", "k2", "dense", "far", "dense_entries", "far_rows", "far_cols", "depth"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def blocks(self):
+        """Leaf blocks in traversal order: ints (nb,6) = code, depth, row0,
+        col0, m, n; dbls (nb,4) = a, b, c, d."""
+        nb = lib().or_tree_blocks(self.h, None, None, 0)
+        ints = np.zeros((max(nb, 1), 6), dtype=np.int64)
+        dbls = np.zeros((max(nb, 1), 4))
+        lib().or_tree_blocks(self.h, _i(ints), _d(dbls), nb)
+        return ints[:nb], dbls[:nb]
+
+    def block_data(self, k):
+        ints, _ = self.blocks()
+        code, _, _, _, m, n = ints[k]
+        r = self.rank
+        if code == CODE_DENSE:
+            A = np.zeros(m * n)
+            lib().or_tree_block_data(self.h, k, _d(A), None, None, None)
+            return {"A": A.reshape((m, n), order="F")}
+        U, V, F = np.zeros(m * r), np.zeros(n * r), np.zeros(r * r)
+        lib().or_tree_block_data(self.h, k, None, _d(U), _d(V), _d(F))
+        return {"U": U.reshape((m, r), order="F"), "V": V.reshape((n, r), order="F"),
+                "F": F.reshape((r, r), order="F")}
+
+    def matvec(self, v, u=None):
+        """Faithful mul!: returns u + H*v (u defaults to zeros)."""
+        v = f64(v)
+        u = np.zeros(len(self.x)) if u is None else f64(u).copy()
+        if lib().or_tree_matvec(self.h, _d(u), _d(v)):
+            raise ValueError("tree has no stored factors")
+        return u
+
+    def matvec_omp(self, v, nthreads=0, u=None):
+        v = f64(v)
+        u = np.zeros(len(self.x)) if u is None else f64(u).copy()
+        if lib().or_tree_matvec_omp(self.h, _d(u), _d(v), nthreads):
+            raise ValueError("tree has no stored factors")
+        return u
+
+    def matvec_rows(self, v, rows, nthreads=0):
+        v = f64(v)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.zeros(len(rows))
+        lib().or_tree_matvec_rows(self.h, _d(v), _i(rows), len(rows), _d(out), nthreads)
+        return out
+
+    def getindex(self, i, j):
+        return lib().or_tree_getindex(self.h, i, j)
+
+
+def dense_matvec(kid, x, y, v, rows=None, nthreads=0):
+    """sum_j f(x_i, y_j) v_j in long double, for rows (0-based) or all rows."""
+    x, y, v = f64(x), f64(y), f64(v)
+    if rows is None:
+        out = np.zeros(len(x))
+        lib().or_dense_matvec_rows(kid, _d(x), _d(y), len(y), _d(v), None, len(x), _d(out),
+                                   nthreads)
+        return out
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.zeros(len(rows))
+    lib().or_dense_matvec_rows(kid, _d(x), _d(y), len(y), _d(v), _i(rows), len(rows), _d(out),
+                               nthreads)
+    return out
+
+
+def dense_mul(y, A, x, istart, jstart, incx=1, incy=1):
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    lib().or_dense_mul(_d(y), A.ctypes.data_as(_dp), m, n, _d(x), istart, jstart, incx, incy)
+    return y
+
+
+def dense_tmul(y, A, x, istart, jstart, incx=1, incy=1):
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    lib().or_dense_tmul(_d(y), A.ctypes.data_as(_dp), m, n, _d(x), istart, jstart, incx, incy)
+    return y
+
+
+def lowrank_mul(y, U, S, V, x, istart, jstart, incx=1, incy=1, adjoint=False):
+    U = np.asfortranarray(U, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.float64)
+    S = f64(S)
+    fn = lib().or_lowrank_adjmul if adjoint else lib().or_lowrank_mul
+    fn(_d(y), U.ctypes.data_as(_dp), _d(S), V.ctypes.data_as(_dp), U.shape[0], V.shape[0],
+       len(S), _d(x), istart, jstart, incx, incy)
+    return y
+
+
+def bary_mul(u, U, F, V, v, istart, jstart):
+    U = np.asfortranarray(U, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.float64)
+    F = np.asfortranarray(F, dtype=np.float64)
+    lib().or_bary_mul(_d(u), U.ctypes.data_as(_dp), F.ctypes.data_as(_dp),
+                      V.ctypes.data_as(_dp), U.shape[0], V.shape[0], F.shape[0], _d(v),
+                      istart, jstart)
+    return u

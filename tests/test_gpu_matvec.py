@@ -1,0 +1,181 @@
+"""GPU parity of the KernelMatrix compression and product against the oracle.
+
+The oracle (oracle/hmx_oracle.c) restates src/KernelMatrix.jl, src/BarycentricMatrix.jl
+and src/algebra.jl loop for loop; its traversal mul! is "the reference's mul!"
+here. Bars: stored factors bit-exact (IEEE kernels); products within 1e-12
+relative 2-norm of the oracle in FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+import hmx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KID = {"cauchykernel": O.CAUCHY, "coulombkernel": O.COULOMB,
+       "coulombprimekernel": O.COULOMB3, "logkernel": O.LOG}
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def plain(n=4096, bs=64, r=16):
+    return hmx.workloads.config_plain(n=n, blocksize=bs, rank=r)
+
+
+@pytest.mark.parametrize("free", [False, True], ids=["stored", "free"])
+@pytest.mark.parametrize("kernel", list(KID))
+def test_plain_vs_oracle(cuda, kernel, free):
+    w = plain()
+    print(f"case: plain n=4096 kernel={kernel} free={free}", flush=True)
+    ref = O.Tree(KID[kernel], w["x"], w["y"], *w["intervals"], 64, 16).matvec(w["b"])
+    H = hmx.KernelMatrix(kernel, w["x"], w["y"], *w["intervals"], blocksize=64, rank=16,
+                         matrix_free=free)
+    assert H.info["nonfinite"] == 0
+    u = (H @ torch.from_numpy(w["b"]).to(cuda)).cpu().numpy()
+    assert rel(u, ref) <= TOL
+
+
+@pytest.mark.parametrize("kernel", ["cauchykernel", "coulombkernel", "coulombprimekernel"])
+def test_compression_bit_exact(cuda, kernel):
+    """Entries of the stored plan (dense blocks and U F V' at far blocks,
+    src/BarycentricMatrix.jl:222-234) equal the oracle's bit for bit."""
+    w = plain(n=2048)
+    t = O.Tree(KID[kernel], w["x"], w["y"], *w["intervals"], 64, 16)
+    H = hmx.KernelMatrix(kernel, w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
+    rng = np.random.default_rng(1)
+    ii = rng.integers(0, 2048, 3000)
+    jj = rng.integers(0, 2048, 3000)
+    got = H[ii, jj]
+    want = np.array([t.getindex(i, j) for i, j in zip(ii, jj)])
+    assert np.array_equal(got, want)
+
+
+def test_log_entries_close(cuda):
+    w = plain(n=2048)
+    t = O.Tree(O.LOG, w["x"], w["y"], *w["intervals"], 64, 16)
+    H = hmx.KernelMatrix("logkernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
+    ii = np.arange(0, 2048, 7)
+    jj = (ii * 13) % 2048
+    got = H[ii, jj]
+    want = np.array([t.getindex(i, j) for i, j in zip(ii, jj)])
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)) <= 4e-15
+
+
+@pytest.mark.parametrize("free", [False, True], ids=["stored", "free"])
+def test_headline_size_sampled_rows(cuda, free):
+    """n = 2^20, leaf 256, rank 24: sampled rows against the oracle's
+    traversal mul! (same per-row summation order as the full product)."""
+    w = hmx.workloads.config_headline()
+    print(f"case: headline n=2^20 free={free}", flush=True)
+    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=256,
+                         rank=24, matrix_free=free)
+    u = (H @ torch.from_numpy(w["b"]).to(cuda)).cpu().numpy()
+    H.close()
+    t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], 256, 24, store=False)
+    rng = np.random.default_rng(2)
+    rows = np.unique(np.concatenate([rng.integers(0, 1 << 20, 192), [0, 1, 127, 128, (1 << 20) - 1]]))
+    ref = t.matvec_rows(w["b"], rows)
+    assert rel(u[rows], ref) <= TOL
+    # properties over the whole vector: finite, and matches the dense product on a sample
+    assert np.all(np.isfinite(u))
+    dense = O.dense_matvec(O.CAUCHY, w["x"], w["y"], w["b"], rows=rows[:32])
+    assert rel(u[rows[:32]], dense) <= 1e-11
+
+
+def test_stored_equals_free_full_vector(cuda):
+    w = hmx.workloads.config_plain(n=1 << 16, blocksize=256, rank=24)
+    b = torch.from_numpy(w["b"]).to(cuda)
+    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=256, rank=24)
+    u1 = H @ b
+    u2 = H.matvec(b, matrix_free=True)
+    assert rel(u2.cpu().numpy(), u1.cpu().numpy()) <= 1e-14
+    # linearity
+    b2 = torch.from_numpy(hmx.workloads.normal(5, 1 << 16)).to(cuda)
+    u3 = H @ (2.5 * b + b2)
+    assert rel(u3.cpu().numpy(), (2.5 * u1 + H @ b2).cpu().numpy()) <= 1e-13
+
+
+def test_accumulate_offsets_strides(cuda):
+    """mul!(u, H, v, istart, jstart, INCX, INCY) accumulates (src/KernelMatrix.jl:14,
+    src/HierarchicalMatrix.jl:24-52); offsets 1-based."""
+    w = plain(n=1024)
+    n = 1024
+    t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], 64, 16)
+    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
+    v = hmx.workloads.normal(3, 2 * n + 10)
+    u0 = hmx.workloads.normal(4, 3 * n + 10)
+    istart, jstart, incx, incy = 5, 3, 2, 3
+    vs = v[jstart - 1: jstart - 1 + incx * n: incx]
+    want = u0.copy()
+    want[istart - 1: istart - 1 + incy * n: incy] += t.matvec(vs)
+    ud = torch.from_numpy(u0.copy()).to(cuda)
+    hmx.mul_(ud, H, torch.from_numpy(v).to(cuda), istart, jstart, incx, incy)
+    got = ud.cpu().numpy()
+    mask = np.zeros(len(u0), bool)
+    mask[istart - 1: istart - 1 + incy * n: incy] = True
+    assert np.array_equal(got[~mask], u0[~mask])
+    assert rel(got[mask], want[mask]) <= TOL
+    # host arrays are updated in place
+    uh = u0.copy()
+    hmx.mul_(uh, H, v, istart, jstart, incx, incy)
+    assert rel(uh[mask], want[mask]) <= TOL
+
+
+def test_multiple_rhs_columns(cuda):
+    w = plain(n=2048)
+    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
+    B = torch.from_numpy(hmx.workloads.normal(9, 2048 * 5).reshape(2048, 5)).to(cuda)
+    Y = H @ B
+    for q in range(5):
+        assert torch.equal(Y[:, q], H @ B[:, q].contiguous())
+
+
+def test_transpose(cuda):
+    w = plain(n=4096)
+    x, y = w["x"], w["y"]
+    H = hmx.KernelMatrix("cauchykernel", x, y, *w["intervals"], blocksize=64, rank=16)
+    r = torch.from_numpy(hmx.workloads.normal(21, 4096)).to(cuda)
+    b = torch.from_numpy(w["b"]).to(cuda)
+    for free in (False, True):
+        g = H.matvec(r, transpose=True, matrix_free=free)
+        # adjoint identity <H^T r, b> = <r, H b>
+        lhs = float(torch.dot(g, b))
+        rhs = float(torch.dot(r, H @ b))
+        assert abs(lhs - rhs) <= 1e-13 * float(torch.linalg.norm(g) * torch.linalg.norm(b))
+        # Cauchy on equal intervals: H(x, y)^T = -H(y, x) block for block
+        Hs = hmx.KernelMatrix("cauchykernel", y, x, *w["intervals"], blocksize=64, rank=16)
+        want = -(Hs @ r)
+        assert rel(g.cpu().numpy(), want.cpu().numpy()) <= 1e-14
+        # and the dense transpose within the interpolation error
+        K = 1.0 / (x[:, None] - y[None, :])
+        assert rel(g.cpu().numpy(), K.T @ r.cpu().numpy()) <= 1e-11
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "quadratic"])
+@pytest.mark.parametrize("kernel", list(KID))
+@pytest.mark.parametrize("N", [1000, 10000])
+def test_example_families(cuda, family, kernel, N):
+    """The two point families of examples/Kernel.jl:58-110 with the default
+    BLOCKRANK(Float64) = 20 and BLOCKSIZE = 80."""
+    if family == "chebyshev":
+        x = hmx.chebyshevpoints(N)
+        y = hmx.chebyshevpoints(N, kind=2)
+        iv = (1.0, -1.0, 1.0, -1.0)
+    else:
+        i = np.arange(N, 0, -1, dtype=np.float64)
+        x = i * (i + 1)
+        y = (i + 0.5) * (i + 1.5)
+        iv = (x.max(), x.min(), y.max(), y.min())
+    b = hmx.workloads.normal(N + KID[kernel], N)
+    t = O.Tree(KID[kernel], x, y, *iv, 80, 20)
+    ref = t.matvec(b)
+    for free in (False, True):
+        H = hmx.KernelMatrix(kernel, x, y, *iv, matrix_free=free)
+        assert H.rank == 20 and H.blocksize_ == 80
+        u = H.matvec(b)
+        assert rel(u, ref) <= TOL, (free, rel(u, ref))
