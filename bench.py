@@ -68,6 +68,24 @@ def pass_times(H, fn, steps):
     return ms / max(cnt.value, 1)
 
 
+def stream_peaks(stream):
+    """Measured HBM bandwidth of torch's copy (read + write) and sum (read) on a
+    4 GiB float64 buffer, for context beside the nominal 8 TB/s."""
+    import torch
+    n = 1 << 29
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+        src.sum()
+    ms_copy = time_steps(lambda: dst.copy_(src), 10, stream)
+    ms_sum = time_steps(lambda: src.sum(), 10, stream)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"copy_gbs": round(2 * 8 * n / (ms_copy * 1e-3) / 1e9, 1),
+            "read_gbs": round(8 * n / (ms_sum * 1e-3) / 1e9, 1)}
+
+
 def cpu_baseline(threads, n_log2):
     """Oracle traversal mul! (ref layout, bit-identical to the single-thread
     restatement) on the n = 2^n_log2 instance of the same family; matvecs/s
@@ -147,19 +165,25 @@ def main():
     dense = O.dense_matvec(O.CAUCHY, w["x"], w["y"], w["b"], rows=rows)
     err_dense = float(np.linalg.norm(u.cpu().numpy()[rows] - dense) / np.linalg.norm(dense))
 
-    # matrix-free variant
-    Hf = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=BS,
-                          rank=RANK, matrix_free=True)
-    uf = torch.empty_like(b)
+    # hybrid (stored A, F; Chebyshev-form basis) and matrix-free variants
+    variants = {}
+    for name, kw in (("hybrid", dict(hybrid=True)), ("matrix_free", dict(matrix_free=True))):
+        Hv = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=BS,
+                              rank=RANK, **kw)
+        uv = torch.empty_like(b)
 
-    def step_free():
-        Hf._mv(uf, b, 1, 1, 1, 1, 0, False, False, stream)
+        def step_v():
+            Hv._mv(uv, b, 1, 1, 1, 1, 0, False, False, stream)
 
-    for _ in range(args.warmup):
-        step_free()
-    ms_free = time_steps(step_free, args.steps, stream)
-    passes_free = pass_times(Hf, step_free, args.steps)
-    err_free = float(torch.linalg.norm(uf - u) / torch.linalg.norm(u))
+        for _ in range(args.warmup):
+            step_v()
+        ms_v = time_steps(step_v, args.steps, stream)
+        pv = pass_times(Hv, step_v, args.steps)
+        variants[name] = dict(ms=ms_v, passes=pv, bytes=Hv.info["bytes_stored"],
+                              err=float(torch.linalg.norm(uv - u) / torch.linalg.norm(u)))
+        Hv.close()
+        del uv
+        torch.cuda.empty_cache()
 
     names = ["up", "reduce", "translate", "down"]
     r = RANK
@@ -173,6 +197,7 @@ def main():
                 up_pass=dict(achieved=round(bytes_up / (passes[0] * 1e-3) / 1e9, 1),
                              avg_ms=round(float(passes[0]), 4)))
 
+    peaks = stream_peaks(stream)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -200,11 +225,13 @@ def main():
         "pass_ms": {k: round(float(v), 4) for k, v in zip(names, passes)},
         "rel_err_vs_dense_16rows": err_dense,
         "build_s": round(build_s, 3),
-        "matrix_free": {"value": round(1000.0 / ms_free, 3), "unit": "matvecs/s",
-                        "ms_per_step": round(ms_free, 4),
-                        "pass_ms": {k: round(float(v), 4) for k, v in zip(names, passes_free)},
-                        "rel_diff_vs_stored": err_free},
+        "variants": {k: {"value": round(1000.0 / v["ms"], 3), "unit": "matvecs/s",
+                         "ms_per_step": round(v["ms"], 4),
+                         "hbm_bytes_per_matvec": v["bytes"],
+                         "pass_ms": {q: round(float(t), 4) for q, t in zip(names, v["passes"])},
+                         "rel_diff_vs_stored": v["err"]} for k, v in variants.items()},
         "roofline": roof,
+        "measured_hbm_peaks": peaks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -16,6 +16,9 @@ struct DevPlan {
     int32_t nrc, ncc;
     const double *rnodes;    // nrc*r mapped nodes (x side)
     const double *cnodes;    // ncc*r mapped nodes (y side)
+    const double2 *rmap;     // per row cluster: (center, 1/halfwidth) of its interval
+    const double2 *cmap;     // per col cluster
+    const double *chebA;     // r x r, A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r (row-major)
     const DFarBlk *far;
     int32_t nfar;
     const int32_t *far_by_rc;  // far ids grouped by row cluster (CSR in DCluster.far_*)
@@ -44,9 +47,11 @@ struct MvArgs {
 
 // compression: dense blocks, U, V, F (bit-exact operation order)
 hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
-// one product y = beta*y + op(H) x for a single right-hand side; when ev is
-// not null, ev[0..4] are recorded before up, reduce, translate, down and after down
-hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, bool matrix_free,
+// one product y = beta*y + op(H) x for a single right-hand side; mode 0
+// stored, 1 hybrid (stored A, F; Chebyshev-form basis), 2 matrix-free. When
+// ev is not null, ev[0..4] are recorded before up, reduce, translate, down and
+// after down.
+hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode,
                              hipStream_t s, const hipEvent_t *ev = nullptr);
 // entries: kind[t] = 0 none, 1 dense (id), 2 far (id); rows/cols 0-based
 hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,

@@ -26,6 +26,21 @@
 #define HM_CHUNK 64          // rows per chunk = wavefront width
 #define HM_TASK_POINTS 8192  // points per upward-pass task
 
+// Stored matrices (dense blocks m x n, bases len x r) are kept in 64-row
+// tiles, column-major inside a tile: element (i, k) of a matrix with `cols`
+// columns sits at tile_index(i, k, cols). A wavefront that owns 64 rows then
+// reads one contiguous 64*cols run.
+#define HM_TILE 64
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline int64_t tile_index(int64_t i, int64_t k, int64_t cols) {
+    return (i >> 6) * (HM_TILE * cols) + k * HM_TILE + (i & (HM_TILE - 1));
+}
+inline int64_t tiled_size(int64_t rows, int64_t cols) {
+    return ((rows + HM_TILE - 1) / HM_TILE) * HM_TILE * cols;
+}
+
 struct DChunk {
     int64_t p0;        // first row (or column)
     int32_t np;        // number of rows in the chunk (<= HM_CHUNK)

@@ -12,9 +12,19 @@
 //              the chunk, written once as beta*y + acc
 // The transpose runs the same passes with rows and columns exchanged.
 //
-// Stored mode streams A, U, V, F from HBM. Matrix-free mode regenerates them
-// from the points: kernel values f(x_i, y_j) and the barycentric basis
-// lam_k/(x - t_k) / sum_m lam_m/(x - t_m) (src/BarycentricMatrix.jl:248-287).
+// Three modes:
+//   stored   streams A, U, V, F from HBM (U, V as update! forms them,
+//            src/BarycentricMatrix.jl:248-287)
+//   hybrid   streams A and F; the far-field basis is applied in Chebyshev form
+//   free     streams nothing but points; A and F are regenerated as well
+// Chebyshev form: on first-kind nodes x_k = cos((2k-1)pi/(2r)) the barycentric
+// Lagrange basis equals l_k(x) = sum_m A[k][m] T_m(x) with
+// A[k][m] = (m ? 2 : 1) T_m(x_k) / r (discrete orthogonality). So the upward
+// pass accumulates moments mu_m = sum_j T_m(xh_j) x_j by the three-term
+// recurrence, the reduce maps them to node values w = A mu, the translate
+// maps z to coefficients c = A^T z, and the downward pass evaluates
+// sum_m c_m T_m(xh_i) by Clenshaw: two FMAs per term, no division. Values
+// equal the barycentric ones up to rounding.
 //
 // Every per-wavefront quantity (chunk descriptors, block and cluster records,
 // the input vector entries and the z coefficients a lane multiplies with) is
@@ -120,7 +130,7 @@ __global__ __launch_bounds__(WG) void k_gen_dense(DevPlan p) {
     const int32_t tot = D.m * D.n;
     for (int32_t e = threadIdx.x; e < tot; e += WG) {
         int32_t j = e / D.m, i = e - j * D.m;
-        p.A[D.aoff + e] = kern_exact(p.kid, p.x[D.row0 + i], p.y[D.col0 + j]);
+        p.A[D.aoff + tile_index(i, j, D.n)] = kern_exact(p.kid, p.x[D.row0 + i], p.y[D.col0 + j]);
     }
 }
 
@@ -150,10 +160,10 @@ __global__ __launch_bounds__(WG) void k_gen_basis(const DChunk *ch, int nch, con
 #pragma unroll
         for (int k = 0; k < RMAX; ++k)
             if (k < r) s += row[k];
-        double *out = B + C.boff + f.off + lane;
+        double *out = B + C.boff + tile_index(f.off + lane, 0, r);
 #pragma unroll
         for (int k = 0; k < RMAX; ++k)
-            if (k < r) out[(int64_t)k * C.len] = row[k] / s;
+            if (k < r) out[k * HM_TILE] = row[k] / s;
     }
 }
 
@@ -186,13 +196,13 @@ __global__ __launch_bounds__(WG) void k_count_nonfinite(const double *v, int64_t
 
 // ------------------------------------------------------------------ product
 
-// Upward pass: partial[task.slot] = B[task points, :]^T x[task points], B = V
-// (forward) or U (transpose), stored or regenerated.
-template <int RMAX, bool FREE, int KID>
+// Upward pass: partial[task.slot] = B[task points, :]^T x[task points] with B =
+// V (forward) or U (transpose) read from HBM, or, in Chebyshev form, the
+// moments sum_j T_m(xh_j) x_j with xh = (p - center) * inv_halfwidth.
+template <int RMAX, bool CHEB>
 __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *cl, const double *B,
-                                          const double *nodes, const double *lam,
-                                          const double *pts, const double *x, int64_t incx,
-                                          double *partial, int r) {
+                                          const double2 *cmapv, const double *pts, const double *x,
+                                          int64_t incx, double *partial, int r) {
     __shared__ double sh[WPB][RMAX];
     const DTask T = tasks[blockIdx.x];
     const DCluster C = cl[T.cl];
@@ -201,29 +211,34 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
 #pragma unroll
     for (int k = 0; k < RMAX; ++k) acc[k] = 0.0;
     const int64_t first = C.first + T.off;
-    if constexpr (!FREE) {
-        const double *bp = B + C.boff + T.off;
+    if constexpr (!CHEB) {
         for (int i = tid; i < T.len; i += WG) {
             const double xv = x[(first + i) * incx];
+            const double *bp = B + C.boff + tile_index(T.off + i, 0, r);
+            // columns past r re-read column r-1 (in bounds) and are dropped,
+            // so all RMAX loads issue back to back without a branch
+            double bv[RMAX];
 #pragma unroll
-            for (int k = 0; k < RMAX; ++k)
-                if (k < r) acc[k] = __builtin_fma(bp[i + (int64_t)k * C.len], xv, acc[k]);
+            for (int k = 0; k < RMAX; ++k) bv[k] = bp[min(k, r - 1) * HM_TILE];
+#pragma unroll
+            for (int k = 0; k < RMAX; ++k) acc[k] = __builtin_fma(k < r ? bv[k] : 0.0, xv, acc[k]);
         }
     } else {
-        const double *t = nodes + (int64_t)T.cl * r;
+        const double2 cm = cmapv[T.cl];
         for (int i = tid; i < T.len; i += WG) {
             const double xv = x[(first + i) * incx];
-            const double pt = pts[first + i];
-            double wk[RMAX];
-            double S = 0.0;
+            const double xh = (pts[first + i] - cm.x) * cm.y;
+            const double x2 = xh + xh;
+            double t0 = xv, t1 = xh * xv;  // T_m(xh) * x, by the recurrence
+            acc[0] += t0;
+            if (RMAX > 1) acc[1] += t1;
 #pragma unroll
-            for (int k = 0; k < RMAX; ++k) {
-                wk[k] = k < r ? lam[k] * rcp64(pt - t[k]) : 0.0;
-                S += wk[k];
+            for (int k = 2; k < RMAX; ++k) {
+                const double t2 = __builtin_fma(x2, t1, -t0);
+                acc[k] += t2;
+                t0 = t1;
+                t1 = t2;
             }
-            const double sc = xv * rcp64(S);
-#pragma unroll
-            for (int k = 0; k < RMAX; ++k) acc[k] = __builtin_fma(wk[k], sc, acc[k]);
         }
     }
     int base = 0;
@@ -243,9 +258,11 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
     }
 }
 
-// w[c] = sum over the task slots of cluster c
+// w[c] = sum over the task slots of cluster c; in Chebyshev form the slots hold
+// moments and w = A mu.
+template <bool CHEB>
 __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const double *partial,
-                                              double *w, int r) {
+                                              const double *chebA, double *w, int r) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= ncl) return;
@@ -258,12 +275,20 @@ __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, cons
     }
     double tot = acc;
     for (int g = 1; g < G; ++g) tot += __shfl(acc, (lane + g * r) & (WAVE - 1));
+    if constexpr (CHEB) {
+        double v = 0.0;
+        for (int m = 0; m < r; ++m) {
+            const double mu = __shfl(tot, m);
+            if (lane < r) v = __builtin_fma(chebA[lane * r + m], mu, v);
+        }
+        tot = v;
+    }
     if (lane < r) w[(int64_t)wid * r + lane] = tot;
 }
 
 // z[d] = sum over far blocks b of cluster d of F_b w[other side of b]
 // (forward: F w[cc]; transpose: F^T w[rc]).
-template <bool FREE, int KID, bool TRANS>
+template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB>
 __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl, int ncl,
                                                  const int32_t *far_list, const double *w,
                                                  double *z) {
@@ -280,10 +305,16 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
         const double *ws = w + (int64_t)src * r;
         if constexpr (!FREE) {
             const double *F = p.F + b.foff;
-            for (int l = 0; l < r; ++l) {
-                double f = act ? (TRANS ? F[l + (int64_t)r * lane] : F[lane + (int64_t)r * l]) : 0.0;
-                acc = __builtin_fma(f, ws[l], acc);
+            double fv[RMAX];
+            const int la = act ? lane : 0;
+#pragma unroll
+            for (int l = 0; l < RMAX; ++l) {
+                const int lc = min(l, r - 1);
+                fv[l] = TRANS ? F[lc + r * la] : F[la + r * lc];
             }
+#pragma unroll
+            for (int l = 0; l < RMAX; ++l)
+                acc = __builtin_fma((act && l < r) ? fv[l] : 0.0, ws[min(l, r - 1)], acc);
         } else {
             // forward: lane = m, F[m, l] = f(tx_m, ty_l); transpose: lane = l, sum over m
             const double *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
@@ -296,16 +327,54 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
             }
         }
     }
+    if constexpr (CHEB) {  // coefficients c = A^T z
+        double c = 0.0;
+        for (int k = 0; k < r; ++k) {
+            const double zk = __shfl(acc, k);
+            if (act) c = __builtin_fma(p.chebA[k * r + lane], zk, c);
+        }
+        acc = c;
+    }
     if (act) z[(int64_t)wid * r + lane] = acc;
+}
+
+// Matrix-free near field of one lane: sum_j f(p, q_j) v_j (or f(q_j, p) v_j
+// when TR). For the power kernels 1/d^m two terms share one reciprocal:
+// v1/d1^m + v2/d2^m = (v1 d2^m + v2 d1^m) / (d1 d2)^m.
+template <int KID, bool TR>
+__device__ __forceinline__ double near_free(double p, const double *qs, const double *v,
+                                            int64_t inc, int n, bool act) {
+    double acc = 0.0;
+    if constexpr (KID <= 2) {
+        int j = 0;
+        for (; j + 2 <= n; j += 2) {
+            double d1 = TR ? qs[j] - p : p - qs[j];
+            double d2 = TR ? qs[j + 1] - p : p - qs[j + 1];
+            if constexpr (KID == 1) { d1 *= d1; d2 *= d2; }
+            if constexpr (KID == 2) { d1 *= d1 * d1; d2 *= d2 * d2; }
+            const double num = __builtin_fma(v[(int64_t)j * inc], d2, v[(int64_t)(j + 1) * inc] * d1);
+            acc = __builtin_fma(num, rcp64(d1 * d2), acc);
+        }
+        if (j < n) {
+            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
+            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
+        }
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
+            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
+        }
+    }
+    return act ? acc : 0.0;
 }
 
 // Downward pass. Forward: output = rows; near blocks read by rows; far basis U.
 // Transpose: output = columns; near blocks read by columns; far basis V.
-template <bool FREE, int KID, bool TRANS>
+template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB>
 __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nch,
                                             const DNear *nl, const DFar *fl,
                                             const DCluster *cl, const double *B,
-                                            const double *nodes, const double *pts_out,
+                                            const double2 *cmapv, const double *pts_out,
                                             const double *pts_in, const double *z, MvArgs a) {
     constexpr int UN = 16;
     const int wid = wave_id();
@@ -315,7 +384,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
     const DChunk c = ch[wid];
     const bool act = lane < c.np;
     double po = 0.0;
-    if constexpr (FREE) po = act ? pts_out[c.p0 + lane] : 0.0;
+    if constexpr (FREE || CHEB) po = act ? pts_out[c.p0 + lane] : 0.0;
     double acc = 0.0;
 
     // ---- near field
@@ -325,23 +394,19 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
         if constexpr (!TRANS) {
             const double *xin = a.x + D.col0 * a.incx;
             if constexpr (!FREE) {
-                const double *Aj = p.A + D.aoff + s.off + lane;
+                const double *Aj = p.A + D.aoff + tile_index(s.off + lane, 0, D.n);
                 int j = 0;
                 for (; j + UN <= D.n; j += UN) {
                     double av[UN];
 #pragma unroll
-                    for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(int64_t)(j + u) * D.m] : 0.0;
+                    for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : 0.0;
 #pragma unroll
                     for (int u = 0; u < UN; ++u) acc = __builtin_fma(av[u], xin[(int64_t)(j + u) * a.incx], acc);
                 }
                 for (; j < D.n; ++j)
-                    acc = __builtin_fma(act ? Aj[(int64_t)j * D.m] : 0.0, xin[(int64_t)j * a.incx], acc);
+                    acc = __builtin_fma(act ? Aj[j * HM_TILE] : 0.0, xin[(int64_t)j * a.incx], acc);
             } else {
-                const double *yin = pts_in + D.col0;
-                for (int j = 0; j < D.n; ++j) {
-                    const double f = kern_fast<KID>(po, yin[j]);
-                    acc = __builtin_fma(act ? f : 0.0, xin[(int64_t)j * a.incx], acc);
-                }
+                acc += near_free<KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
             }
         } else {
             const double *xin = a.x + D.row0 * a.incx;
@@ -351,23 +416,19 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                 double accT[WAVE];
 #pragma unroll
                 for (int q = 0; q < WAVE; ++q) accT[q] = 0.0;
-                const double *Ac = p.A + D.aoff + (int64_t)s.off * D.m;
                 for (int ib = 0; ib < D.m; ib += WAVE) {
                     const bool ra = ib + lane < D.m;
                     const double xv = ra ? xin[(int64_t)(ib + lane) * a.incx] : 0.0;
+                    const double *Ac = p.A + D.aoff + tile_index(ib + lane, s.off, D.n);
 #pragma unroll
                     for (int q = 0; q < WAVE; ++q)
-                        if (q < c.np) accT[q] += (ra ? Ac[ib + lane + (int64_t)q * D.m] : 0.0) * xv;
+                        if (q < c.np) accT[q] += (ra ? Ac[q * HM_TILE] : 0.0) * xv;
                 }
                 int base = 0;
                 Halve<WAVE, WAVE / 2>::run(accT, lane, base);
                 acc += accT[0];
             } else {
-                const double *pin = pts_in + D.row0;
-                for (int i = 0; i < D.m; ++i) {
-                    const double f = kern_fast<KID>(pin[i], po);
-                    acc = __builtin_fma(act ? f : 0.0, xin[(int64_t)i * a.incx], acc);
-                }
+                acc += near_free<KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act);
             }
         }
     }
@@ -376,27 +437,33 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
     for (int e = c.far_beg; e < c.far_end; ++e) {
         const DFar f = fl[e];
         const double *zc = z + (int64_t)f.cl * r;
-        if constexpr (!FREE) {
+        if constexpr (!CHEB) {
             const DCluster C = cl[f.cl];
-            const double *bp = B + C.boff + f.off + lane;
+            const double *bp = B + C.boff + tile_index(f.off + lane, 0, r);
             int k = 0;
             for (; k + 8 <= r; k += 8) {
                 double bv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) bv[u] = act ? bp[(int64_t)(k + u) * C.len] : 0.0;
+                for (int u = 0; u < 8; ++u) bv[u] = act ? bp[(k + u) * HM_TILE] : 0.0;
 #pragma unroll
                 for (int u = 0; u < 8; ++u) acc = __builtin_fma(bv[u], zc[k + u], acc);
             }
-            for (; k < r; ++k) acc = __builtin_fma(act ? bp[(int64_t)k * C.len] : 0.0, zc[k], acc);
+            for (; k < r; ++k) acc = __builtin_fma(act ? bp[k * HM_TILE] : 0.0, zc[k], acc);
         } else {
-            const double *tc = nodes + (int64_t)f.cl * r;
-            double num = 0.0, den = 0.0;
-            for (int k = 0; k < r; ++k) {
-                const double wk = p.lam[k] * rcp64(po - tc[k]);
-                den += wk;
-                num = __builtin_fma(wk, zc[k], num);
+            // Clenshaw: b_k = c_k + 2 xh b_{k+1} - b_{k+2}; sum = c_0 + xh b_1 - b_2
+            const double2 cm = cmapv[f.cl];
+            const double xh = (po - cm.x) * cm.y;
+            const double x2 = xh + xh;
+            double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+            for (int k = RMAX - 1; k >= 1; --k) {
+                const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
+                const double b0 = __builtin_fma(x2, b1, ck - b2);
+                b2 = b1;
+                b1 = b0;
             }
-            if (act) acc += num * rcp64(den);
+            const double sum = __builtin_fma(xh, b1, zc[0] - b2);
+            if (act) acc += sum;
         }
     }
     if (act) {
@@ -417,15 +484,15 @@ __global__ __launch_bounds__(WG) void k_entries(DevPlan p, int64_t n, const int6
     double v = 0.0;
     if (kind[t] == 1) {
         const DDense D = p.dense[id[t]];
-        v = p.A ? p.A[D.aoff + (i - D.row0) + (int64_t)D.m * (j - D.col0)] : kern_exact(p.kid, p.x[i], p.y[j]);
+        v = p.A ? p.A[D.aoff + tile_index(i - D.row0, j - D.col0, D.n)] : kern_exact(p.kid, p.x[i], p.y[j]);
     } else if (kind[t] == 2) {
         const DFarBlk b = p.far[id[t]];
         const DCluster R = p.rcl[b.rc], Cc = p.ccl[b.cc];
         double ur[64], vr[64];
         if (p.U) {
             for (int k = 0; k < r; ++k) {
-                ur[k] = p.U[R.boff + (i - R.first) + (int64_t)k * R.len];
-                vr[k] = p.V[Cc.boff + (j - Cc.first) + (int64_t)k * Cc.len];
+                ur[k] = p.U[R.boff + tile_index(i - R.first, k, r)];
+                vr[k] = p.V[Cc.boff + tile_index(j - Cc.first, k, r)];
             }
         } else {
             double su = 0.0, sv = 0.0;
@@ -455,17 +522,20 @@ inline int nblk(int64_t waves) { return (int)((waves + WPB - 1) / WPB); }
 
 template <int RMAX>
 hipError_t gen_basis_r(const DevPlan &p, hipStream_t s) {
-    if (p.ncch > 0)
+    if (p.ncch > 0 && p.V)
         k_gen_basis<RMAX><<<nblk(p.ncch), WG, 0, s>>>(p.cch, p.ncch, p.cfar, p.ccl, p.cnodes, p.lam,
                                                      p.y, p.V, p.r);
-    if (p.nrch > 0)
+    if (p.nrch > 0 && p.U)
         k_gen_basis<RMAX><<<nblk(p.nrch), WG, 0, s>>>(p.rch, p.nrch, p.rfar, p.rcl, p.rnodes, p.lam,
                                                      p.x, p.U, p.r);
     return hipGetLastError();
 }
 
-template <int RMAX, bool FREE, int KID>
+// MODE: 0 stored, 1 hybrid, 2 free
+template <int RMAX, int MODE, int KID>
 hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
+    constexpr bool CHEB = MODE != 0;
+    constexpr bool FREE = MODE == 2;
     auto mark = [&](int q) {
         if (ev) (void)hipEventRecord(ev[q], s);
     };
@@ -475,7 +545,7 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     const DCluster *in_cl = tr ? p.rcl : p.ccl;
     const int in_ncl = tr ? p.nrc : p.ncc;
     const double *in_B = tr ? p.U : p.V;
-    const double *in_nodes = tr ? p.rnodes : p.cnodes;
+    const double2 *in_map = tr ? p.rmap : p.cmap;
     const double *in_pts = tr ? p.x : p.y;
     const DChunk *out_ch = tr ? p.cch : p.rch;
     const int out_n = tr ? p.ncch : p.nrch;
@@ -484,23 +554,23 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     const DCluster *out_cl = tr ? p.ccl : p.rcl;
     const int out_ncl = tr ? p.ncc : p.nrc;
     const double *out_B = tr ? p.V : p.U;
-    const double *out_nodes = tr ? p.cnodes : p.rnodes;
+    const double2 *out_map = tr ? p.cmap : p.rmap;
     const double *out_pts = tr ? p.y : p.x;
     const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
 
     mark(0);
     if (p.nfar > 0 && in_ntask > 0) {
-        k_up<RMAX, FREE, KID><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_nodes, p.lam,
-                                                       in_pts, a.x, a.incx, a.partial, p.r);
+        k_up<RMAX, CHEB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x, a.incx,
+                                                  a.partial, p.r);
         mark(1);
-        k_reduce<<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, a.w, p.r);
+        k_reduce<CHEB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
         mark(2);
         if (tr)
-            k_translate<FREE, KID, true><<<nblk(out_ncl), WG, 0, s>>>(p, out_cl, out_ncl,
-                                                                     out_far_list, a.w, a.z);
+            k_translate<RMAX, FREE, KID, true, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, out_far_list, a.w, a.z);
         else
-            k_translate<FREE, KID, false><<<nblk(out_ncl), WG, 0, s>>>(p, out_cl, out_ncl,
-                                                                      out_far_list, a.w, a.z);
+            k_translate<RMAX, FREE, KID, false, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, out_far_list, a.w, a.z);
     } else {
         mark(1);
         mark(2);
@@ -508,36 +578,27 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     mark(3);
     if (out_n > 0) {
         if (tr)
-            k_down<FREE, KID, true><<<nblk(out_n), WG, 0, s>>>(p, out_ch, out_n, out_near, out_far,
-                                                              out_cl, out_B, out_nodes, out_pts,
-                                                              in_pts, a.z, a);
+            k_down<RMAX, FREE, KID, true, CHEB><<<nblk(out_n), WG, 0, s>>>(
+                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
         else
-            k_down<FREE, KID, false><<<nblk(out_n), WG, 0, s>>>(p, out_ch, out_n, out_near, out_far,
-                                                               out_cl, out_B, out_nodes, out_pts,
-                                                               in_pts, a.z, a);
+            k_down<RMAX, FREE, KID, false, CHEB><<<nblk(out_n), WG, 0, s>>>(
+                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
     }
     mark(4);
     return hipGetLastError();
 }
 
-template <int RMAX, bool FREE>
-hipError_t matvec_k(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
-    if constexpr (!FREE) {
-        return matvec_t<RMAX, false, 0>(p, a, tr, s, ev);
-    } else {
-        switch (p.kid) {
-        case 0: return matvec_t<RMAX, true, 0>(p, a, tr, s, ev);
-        case 1: return matvec_t<RMAX, true, 1>(p, a, tr, s, ev);
-        case 2: return matvec_t<RMAX, true, 2>(p, a, tr, s, ev);
-        default: return matvec_t<RMAX, true, 3>(p, a, tr, s, ev);
-        }
-    }
-}
-
 template <int RMAX>
-hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, bool free_, hipStream_t s,
+hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, hipStream_t s,
                     const hipEvent_t *ev) {
-    return free_ ? matvec_k<RMAX, true>(p, a, tr, s, ev) : matvec_k<RMAX, false>(p, a, tr, s, ev);
+    if (mode == 0) return matvec_t<RMAX, 0, 0>(p, a, tr, s, ev);
+    if (mode == 1) return matvec_t<RMAX, 1, 0>(p, a, tr, s, ev);
+    switch (p.kid) {
+    case 0: return matvec_t<RMAX, 2, 0>(p, a, tr, s, ev);
+    case 1: return matvec_t<RMAX, 2, 1>(p, a, tr, s, ev);
+    case 2: return matvec_t<RMAX, 2, 2>(p, a, tr, s, ev);
+    default: return matvec_t<RMAX, 2, 3>(p, a, tr, s, ev);
+    }
 }
 
 }  // namespace
@@ -546,7 +607,7 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s) {
     if (p.ndense > 0 && p.A) k_gen_dense<<<p.ndense, WG, 0, s>>>(p);
     if (p.nfar > 0 && p.F) k_gen_F<<<p.nfar, WG, 0, s>>>(p);
     hipError_t err = hipGetLastError();
-    if (err != hipSuccess || !p.U) return err;
+    if (err != hipSuccess || (!p.U && !p.V)) return err;
     if (p.r <= 8) return gen_basis_r<8>(p, s);
     if (p.r <= 16) return gen_basis_r<16>(p, s);
     if (p.r <= 24) return gen_basis_r<24>(p, s);
@@ -555,14 +616,14 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s) {
     return gen_basis_r<64>(p, s);
 }
 
-hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool tr, bool free_,
+hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool tr, int mode,
                              hipStream_t s, const hipEvent_t *ev) {
-    if (p.r <= 8) return matvec_r<8>(p, a, tr, free_, s, ev);
-    if (p.r <= 16) return matvec_r<16>(p, a, tr, free_, s, ev);
-    if (p.r <= 24) return matvec_r<24>(p, a, tr, free_, s, ev);
-    if (p.r <= 32) return matvec_r<32>(p, a, tr, free_, s, ev);
-    if (p.r <= 48) return matvec_r<48>(p, a, tr, free_, s, ev);
-    return matvec_r<64>(p, a, tr, free_, s, ev);
+    if (p.r <= 8) return matvec_r<8>(p, a, tr, mode, s, ev);
+    if (p.r <= 16) return matvec_r<16>(p, a, tr, mode, s, ev);
+    if (p.r <= 24) return matvec_r<24>(p, a, tr, mode, s, ev);
+    if (p.r <= 32) return matvec_r<32>(p, a, tr, mode, s, ev);
+    if (p.r <= 48) return matvec_r<48>(p, a, tr, mode, s, ev);
+    return matvec_r<64>(p, a, tr, mode, s, ev);
 }
 
 hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,

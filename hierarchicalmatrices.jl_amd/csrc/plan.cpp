@@ -355,7 +355,8 @@ struct hm_plan {
     // device
     DevPlan dp{};
     double *d_x = nullptr, *d_y = nullptr, *d_lam = nullptr, *d_rnodes = nullptr,
-           *d_cnodes = nullptr;
+           *d_cnodes = nullptr, *d_chebA = nullptr;
+    double2 *d_rmap = nullptr, *d_cmap = nullptr;
     DCluster *d_rcl = nullptr, *d_ccl = nullptr;
     DFarBlk *d_far = nullptr;
     int32_t *d_far_by_rc = nullptr, *d_far_by_cc = nullptr;
@@ -376,7 +377,7 @@ struct hm_plan {
 
     ~hm_plan() {
         drop_events();
-        void *bufs[] = {d_x, d_y, d_lam, d_rnodes, d_cnodes, d_rcl, d_ccl, d_far, d_far_by_rc,
+        void *bufs[] = {d_x, d_y, d_lam, d_rnodes, d_cnodes, d_chebA, d_rmap, d_cmap, d_rcl, d_ccl, d_far, d_far_by_rc,
                         d_far_by_cc, d_dense, d_rch, d_cch, d_rnear, d_cnear, d_rfar, d_cfar,
                         d_rtask, d_ctask,
                         d_A, d_U, d_V, d_F, d_partial, d_w, d_z};
@@ -446,7 +447,9 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     if (nx >= (1LL << 40) || ny >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many points");
     if (blocksize < 1) return fail(HM_ERR_INVALID, "blocksize must be >= 1");
     if (rank < 1 || rank > 64) return fail(HM_ERR_UNSUPPORTED, "rank %d outside 1..64", rank);
-    if (flags & ~7u) return fail(HM_ERR_INVALID, "unknown plan flags 0x%x", flags);
+    if (flags & ~15u) return fail(HM_ERR_INVALID, "unknown plan flags 0x%x", flags);
+    if ((flags & HM_PLAN_MATRIX_FREE) && (flags & HM_PLAN_HYBRID))
+        return fail(HM_ERR_INVALID, "HM_PLAN_MATRIX_FREE and HM_PLAN_HYBRID exclude each other");
     if (flags & HM_PLAN_FAR_F32) return fail(HM_ERR_UNSUPPORTED, "FP32 far-field plans are not built by this entry point");
     int64_t bad;
     if (!non_increasing(x, nx, &bad))
@@ -471,7 +474,7 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
 
     // storage offsets
     int64_t off = 0;
-    for (DDense &D : P->dense) { D.aoff = off; off += (int64_t)D.m * D.n; }
+    for (DDense &D : P->dense) { D.aoff = off; off += tiled_size(D.m, D.n); }
     const int64_t nA = off;
     auto mk = [&](const std::vector<HCluster> &hc, std::vector<DCluster> &dc) {
         dc.resize(hc.size());
@@ -482,7 +485,7 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
             dc[k].len = (int32_t)hc[k].len;
             dc[k].boff = o;
             dc[k].depth = hc[k].depth;
-            o += hc[k].len * r;
+            o += tiled_size(hc[k].len, r);
         }
         return o;
     };
@@ -524,6 +527,26 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
         return t;
     };
     const std::vector<double> rn = nodes(B.rcl), cn = nodes(B.ccl);
+    // (center, 1/halfwidth) per cluster: xh = (p - center)/halfwidth maps the
+    // interval onto [-1, 1] with the node order of the reference
+    auto maps = [&](const std::vector<HCluster> &hc) {
+        std::vector<double2> m(hc.size());
+        for (size_t k = 0; k < hc.size(); ++k) {
+            const double ab2 = 0.5 * (hc[k].lo + hc[k].hi), ba2 = 0.5 * (hc[k].hi - hc[k].lo);
+            m[k] = make_double2(ab2, 1.0 / ba2);
+        }
+        return m;
+    };
+    const std::vector<double2> rmap = maps(B.rcl), cmap = maps(B.ccl);
+    // A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r, T_m(cos t) = cos(m t), t_k = (2k-1) pi / (2r)
+    std::vector<double> chebA((size_t)r * r);
+    {
+        const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
+        for (int k = 1; k <= r; ++k)
+            for (int m = 0; m < r; ++m)
+                chebA[(size_t)(k - 1) * r + m] =
+                    (double)((m ? 2.0L : 1.0L) * cosl((long double)m * (2 * k - 1) * PI_L / (2.0L * r)) / r);
+    }
 
     if (flags & HM_PLAN_TREE_ONLY) {
         P->far_by_rc_h = std::move(by_rc);
@@ -538,6 +561,9 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     HM_TRY(upload(&P->d_lam, lam));
     HM_TRY(upload(&P->d_rnodes, rn));
     HM_TRY(upload(&P->d_cnodes, cn));
+    HM_TRY(upload(&P->d_rmap, rmap));
+    HM_TRY(upload(&P->d_cmap, cmap));
+    HM_TRY(upload(&P->d_chebA, chebA));
     HM_TRY(upload(&P->d_rcl, P->rcl));
     HM_TRY(upload(&P->d_ccl, P->ccl));
     HM_TRY(upload(&P->d_far, P->far));
@@ -559,11 +585,14 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     HM_TRY(dalloc(&P->d_w, ncl * r));
     HM_TRY(dalloc(&P->d_z, ncl * r));
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
+    const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
     if (stored) {
         HM_TRY(dalloc(&P->d_A, nA));
-        HM_TRY(dalloc(&P->d_U, nU));
-        HM_TRY(dalloc(&P->d_V, nV));
         HM_TRY(dalloc(&P->d_F, nF));
+        if (!hybrid) {
+            HM_TRY(dalloc(&P->d_U, nU));
+            HM_TRY(dalloc(&P->d_V, nV));
+        }
     }
 
     DevPlan &dp = P->dp;
@@ -572,6 +601,7 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     dp.rcl = P->d_rcl; dp.ccl = P->d_ccl;
     dp.nrc = (int32_t)P->rcl.size(); dp.ncc = (int32_t)P->ccl.size();
     dp.rnodes = P->d_rnodes; dp.cnodes = P->d_cnodes;
+    dp.rmap = P->d_rmap; dp.cmap = P->d_cmap; dp.chebA = P->d_chebA;
     dp.far = P->d_far; dp.nfar = (int32_t)P->far.size();
     dp.far_by_rc = P->d_far_by_rc; dp.far_by_cc = P->d_far_by_cc;
     dp.dense = P->d_dense; dp.ndense = (int32_t)P->dense.size();
@@ -586,6 +616,20 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
 
     hipStream_t s = (hipStream_t)stream;
     if (stored) {
+        // tile padding (rows beyond a block's or cluster's length) is zeroed
+        bool padded = false;
+        for (const DDense &D : P->dense) padded |= (D.m % HM_TILE) != 0;
+        for (const DCluster &C : P->rcl) padded |= (C.len % HM_TILE) != 0;
+        for (const DCluster &C : P->ccl) padded |= (C.len % HM_TILE) != 0;
+        if (padded) {
+            const double *arrs[] = {P->d_A, P->d_U, P->d_V};
+            const int64_t lens[] = {nA, nU, nV};
+            for (int q = 0; q < 3; ++q)
+                if (lens[q] > 0 && arrs[q]) {
+                    hipError_t e0 = hipMemsetAsync((void *)arrs[q], 0, sizeof(double) * lens[q], s);
+                    if (e0 != hipSuccess) { delete P; return fail(HM_ERR_HIP, "memset: %s", hipGetErrorString(e0)); }
+                }
+        }
         hipError_t e = hmx_launch_compress(dp, s);
         if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression launch: %s", hipGetErrorString(e)); }
         // count non-finite factors (coincident points give Inf, as in the reference)
@@ -597,7 +641,7 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
             const double *arrs[] = {P->d_A, P->d_U, P->d_V, P->d_F};
             const int64_t lens[] = {nA, nU, nV, nF};
             for (int q = 0; q < 4 && e == hipSuccess; ++q) {
-                if (lens[q] <= 0) continue;
+                if (lens[q] <= 0 || !arrs[q]) continue;
                 e = hipMemsetAsync(d_cnt, 0, 1024 * sizeof(unsigned long long), s);
                 if (e == hipSuccess) e = hmx_launch_count_nonfinite(arrs[q], lens[q], d_cnt, s);
                 if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_cnt, 1024 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
@@ -628,8 +672,12 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     int64_t v[HM_INFO_COUNT] = {0};
     const int64_t r = P->r;
     const int64_t bu = P->basis_rows(P->rcl), bv = P->basis_rows(P->ccl);
-    int64_t dense_live = 0;
-    for (const DDense &D : P->dense) dense_live += (int64_t)D.m * D.n;
+    int64_t dense_live = 0, basis_live = 0;
+    for (const DDense &D : P->dense) dense_live += tiled_size(D.m, D.n);
+    for (const DCluster &C : P->rcl) basis_live += tiled_size(C.len, r);
+    for (const DCluster &C : P->ccl) basis_live += tiled_size(C.len, r);
+    if (P->flags & HM_PLAN_HYBRID) basis_live = 0;
+    if (P->flags & HM_PLAN_MATRIX_FREE) dense_live = basis_live = 0;
     const int64_t vec = 8 * (P->nx + P->ny);
     v[HM_INFO_NROWS] = P->nx;
     v[HM_INFO_NCOLS] = P->ny;
@@ -649,7 +697,8 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     v[HM_INFO_DENSE_ENTRIES] = P->dense_entries;
     v[HM_INFO_BASIS_ROWS_U] = bu;
     v[HM_INFO_BASIS_ROWS_V] = bv;
-    v[HM_INFO_BYTES_STORED] = 8 * (dense_live + r * (bu + bv) + (int64_t)P->far.size() * r * r) + vec;
+    const int64_t fbytes = (P->flags & HM_PLAN_MATRIX_FREE) ? 0 : (int64_t)P->far.size() * r * r;
+    v[HM_INFO_BYTES_STORED] = 8 * (dense_live + basis_live + fbytes) + vec;
     v[HM_INFO_BYTES_REF_LAYOUT] = 8 * (P->dense_entries + r * (P->far_rows + P->far_cols) + nfar * r * r) + vec;
     v[HM_INFO_FLAGS] = P->flags;
     v[HM_INFO_N_ROWCHUNKS] = (int64_t)P->rows.ch.size();
@@ -684,11 +733,13 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec: tree-only plan has no device data");
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
-    if (flags & ~1u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (flags & ~3u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
     if (k == 0) return HM_OK;
     if (!y || !x) return fail(HM_ERR_INVALID, "matvec: NULL vector");
     if (k > 1 && (ldx < 1 || ldy < 1)) return fail(HM_ERR_INVALID, "matvec: bad ld");
-    const bool free_ = (P->flags & HM_PLAN_MATRIX_FREE) || (flags & HM_MV_MATRIX_FREE);
+    int mode = 0;
+    if ((P->flags & HM_PLAN_MATRIX_FREE) || (flags & HM_MV_MATRIX_FREE)) mode = 2;
+    else if ((P->flags & HM_PLAN_HYBRID) || (flags & HM_MV_CHEB_BASIS)) mode = 1;
     hipStream_t s = (hipStream_t)stream;
     for (int64_t q = 0; q < k; ++q) {
         MvArgs a{};
@@ -710,7 +761,7 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
             }
             ev = P->prof_events.data() + at;
         }
-        hipError_t e = hmx_launch_matvec(P->dp, a, tr, free_, s, ev);
+        hipError_t e = hmx_launch_matvec(P->dp, a, tr, mode, s, ev);
         if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
     }
     return HM_OK;

@@ -21,7 +21,9 @@ HM_PLAN_STORED = 0
 HM_PLAN_MATRIX_FREE = 1
 HM_PLAN_FAR_F32 = 2
 HM_PLAN_TREE_ONLY = 4
+HM_PLAN_HYBRID = 8
 HM_MV_MATRIX_FREE = 1
+HM_MV_CHEB_BASIS = 2
 
 INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
@@ -44,6 +46,15 @@ class HMError(RuntimeError):
 
 
 def _load():
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, requested by torch as "libamdhip64.so"). Loading torch
+    # first lets libhmx's NEEDED libamdhip64.so.7 resolve to that same copy;
+    # loading libhmx first would pull in /opt/rocm's copy and torch would then
+    # load a second runtime beside it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libhmx.so not found at {LIB_PATH}; build it with `python -c 'import "

@@ -155,11 +155,12 @@ class KernelMatrix:
     src/BarycentricMatrix.jl:299-307); (a, b), (c, d) are their intervals as
     (upper, lower). blocksize and rank default to BLOCKSIZE(Float64) = 80 and
     BLOCKRANK(Float64) = 20. matrix_free=True keeps only the points and
-    regenerates every factor in each product.
+    regenerates every factor in each product; hybrid=True keeps the dense blocks
+    and F and applies the far-field basis in Chebyshev form (no U, V).
     """
 
     def __init__(self, f, x, y, a, b, c, d, *, blocksize=None, rank=None, matrix_free=False,
-                 tree_only=False, stream=None):
+                 hybrid=False, tree_only=False, stream=None):
         if isinstance(f, str):
             f = KERNELS[f]
         if not isinstance(f, NamedKernel):
@@ -174,6 +175,8 @@ class KernelMatrix:
         flags = 0
         if matrix_free:
             flags |= _lib.HM_PLAN_MATRIX_FREE
+        if hybrid:
+            flags |= _lib.HM_PLAN_HYBRID
         if tree_only:
             flags |= _lib.HM_PLAN_TREE_ONLY
         self.flags = flags
@@ -228,7 +231,8 @@ class KernelMatrix:
         return ints[:nb], dbls[:nb]
 
     # -- products
-    def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream):
+    def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream,
+            cheb_basis=False):
         U = _Vec(u, True)
         V = _Vec(v, False)
         if U.k != V.k:
@@ -244,7 +248,8 @@ class KernelMatrix:
         if ioff + (m - 1) * incy >= U.n * U.inc or joff + (n - 1) * incx >= V.n * V.inc:
             raise ValueError("vector too short for H at the given offset and stride")
         fn = lib.hm_matvec_t if transpose else lib.hm_matvec
-        flags = _lib.HM_MV_MATRIX_FREE if matrix_free else 0
+        flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
+            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0)
         check(fn(self._h, U.ptr(), V.ptr(), ioff * U.inc, joff * V.inc, incy, incx, U.k,
                  U.ld, V.ld, beta, flags, _stream_ptr(stream)))
         U.writeback()
@@ -258,14 +263,14 @@ class KernelMatrix:
         """u += H^T v (accumulating)."""
         return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, True, stream)
 
-    def matvec(self, v, *, matrix_free=False, transpose=False, stream=None):
+    def matvec(self, v, *, matrix_free=False, transpose=False, cheb_basis=False, stream=None):
         """H v (or H^T v) into a new vector on the device of v."""
         torch = _torch()
         ishost = not isinstance(v, torch.Tensor)
         vt = torch.from_numpy(np.asarray(v, dtype=np.float64)).cuda() if ishost else v
         m = self.shape[1] if transpose else self.shape[0]
         out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=torch.float64, device=vt.device)
-        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream)
+        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis)
         if ishost:
             torch.cuda.synchronize()
             return out.cpu().numpy()

@@ -46,14 +46,18 @@ enum hm_plan_flags {
     HM_PLAN_STORED = 0,          /* generate and keep dense blocks, U, V, F in HBM */
     HM_PLAN_MATRIX_FREE = 1,     /* keep only points and nodes; regenerate in every product */
     HM_PLAN_FAR_F32 = 2,         /* stored far-field factors in FP32 (mixed precision) */
-    HM_PLAN_TREE_ONLY = 4        /* build the block tree and its tables on the host only:
+    HM_PLAN_TREE_ONLY = 4,       /* build the block tree and its tables on the host only:
                                     no device memory, no products (for inspection) */
+    HM_PLAN_HYBRID = 8           /* keep dense blocks and F in HBM; apply the far-field
+                                    basis in Chebyshev form at product time (no U, V) */
 };
 
 /* product flags (hm_matvec) */
 enum hm_mv_flags {
     HM_MV_DEFAULT = 0,
-    HM_MV_MATRIX_FREE = 1        /* regenerate factors even if the plan stores them */
+    HM_MV_MATRIX_FREE = 1,       /* regenerate factors even if the plan stores them */
+    HM_MV_CHEB_BASIS = 2         /* apply the far-field basis in Chebyshev form even if
+                                    the plan stores U and V */
 };
 
 /* Message of the last failing call on this thread ("" if none). */

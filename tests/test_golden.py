@@ -40,8 +40,9 @@ def test_gpu_matches_fixture(cuda, f):
     import torch
     import hmx
     g = load(f)
-    for free in (False, True):
+    for mode in ("stored", "hybrid", "free"):
         H = hmx.KernelMatrix(KNAMES[int(g["kernel"])], g["x"], g["y"], *g["intervals"],
-                             blocksize=int(g["blocksize"]), rank=int(g["rank"]), matrix_free=free)
+                             blocksize=int(g["blocksize"]), rank=int(g["rank"]),
+                             matrix_free=mode == "free", hybrid=mode == "hybrid")
         u = (H @ torch.from_numpy(g["b"]).to(cuda)).cpu().numpy()
         assert np.linalg.norm(u - g["u"]) <= 1e-12 * np.linalg.norm(g["u"])

@@ -17,6 +17,14 @@ pytestmark = pytest.mark.gpu
 KID = {"cauchykernel": O.CAUCHY, "coulombkernel": O.COULOMB,
        "coulombprimekernel": O.COULOMB3, "logkernel": O.LOG}
 TOL = 1e-12
+MODES = ["stored", "hybrid", "free"]
+
+
+def plan(kernel, w, bs, r, mode, x=None, y=None):
+    x = w["x"] if x is None else x
+    y = w["y"] if y is None else y
+    return hmx.KernelMatrix(kernel, x, y, *w["intervals"], blocksize=bs, rank=r,
+                            matrix_free=mode == "free", hybrid=mode == "hybrid")
 
 
 def rel(a, b):
@@ -27,17 +35,19 @@ def plain(n=4096, bs=64, r=16):
     return hmx.workloads.config_plain(n=n, blocksize=bs, rank=r)
 
 
-@pytest.mark.parametrize("free", [False, True], ids=["stored", "free"])
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("kernel", list(KID))
-def test_plain_vs_oracle(cuda, kernel, free):
+def test_plain_vs_oracle(cuda, kernel, mode):
     w = plain()
-    print(f"case: plain n=4096 kernel={kernel} free={free}", flush=True)
+    print(f"case: plain n=4096 kernel={kernel} mode={mode}", flush=True)
     ref = O.Tree(KID[kernel], w["x"], w["y"], *w["intervals"], 64, 16).matvec(w["b"])
-    H = hmx.KernelMatrix(kernel, w["x"], w["y"], *w["intervals"], blocksize=64, rank=16,
-                         matrix_free=free)
+    H = plan(kernel, w, 64, 16, mode)
     assert H.info["nonfinite"] == 0
-    u = (H @ torch.from_numpy(w["b"]).to(cuda)).cpu().numpy()
+    b = torch.from_numpy(w["b"]).to(cuda)
+    u = (H @ b).cpu().numpy()
     assert rel(u, ref) <= TOL
+    if mode == "stored":  # Chebyshev-form basis on the stored plan
+        assert rel(H.matvec(b, cheb_basis=True).cpu().numpy(), ref) <= TOL
 
 
 @pytest.mark.parametrize("kernel", ["cauchykernel", "coulombkernel", "coulombprimekernel"])
@@ -66,14 +76,13 @@ def test_log_entries_close(cuda):
     assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)) <= 4e-15
 
 
-@pytest.mark.parametrize("free", [False, True], ids=["stored", "free"])
-def test_headline_size_sampled_rows(cuda, free):
+@pytest.mark.parametrize("mode", MODES)
+def test_headline_size_sampled_rows(cuda, mode):
     """n = 2^20, leaf 256, rank 24: sampled rows against the oracle's
     traversal mul! (same per-row summation order as the full product)."""
     w = hmx.workloads.config_headline()
-    print(f"case: headline n=2^20 free={free}", flush=True)
-    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=256,
-                         rank=24, matrix_free=free)
+    print(f"case: headline n=2^20 mode={mode}", flush=True)
+    H = plan("cauchykernel", w, 256, 24, mode)
     u = (H @ torch.from_numpy(w["b"]).to(cuda)).cpu().numpy()
     H.close()
     t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], 256, 24, store=False)
@@ -94,6 +103,8 @@ def test_stored_equals_free_full_vector(cuda):
     u1 = H @ b
     u2 = H.matvec(b, matrix_free=True)
     assert rel(u2.cpu().numpy(), u1.cpu().numpy()) <= 1e-14
+    u4 = H.matvec(b, cheb_basis=True)
+    assert rel(u4.cpu().numpy(), u1.cpu().numpy()) <= 1e-14
     # linearity
     b2 = torch.from_numpy(hmx.workloads.normal(5, 1 << 16)).to(cuda)
     u3 = H @ (2.5 * b + b2)
@@ -141,8 +152,8 @@ def test_transpose(cuda):
     H = hmx.KernelMatrix("cauchykernel", x, y, *w["intervals"], blocksize=64, rank=16)
     r = torch.from_numpy(hmx.workloads.normal(21, 4096)).to(cuda)
     b = torch.from_numpy(w["b"]).to(cuda)
-    for free in (False, True):
-        g = H.matvec(r, transpose=True, matrix_free=free)
+    for free, cheb in ((False, False), (False, True), (True, False)):
+        g = H.matvec(r, transpose=True, matrix_free=free, cheb_basis=cheb)
         # adjoint identity <H^T r, b> = <r, H b>
         lhs = float(torch.dot(g, b))
         rhs = float(torch.dot(r, H @ b))
@@ -174,8 +185,9 @@ def test_example_families(cuda, family, kernel, N):
     b = hmx.workloads.normal(N + KID[kernel], N)
     t = O.Tree(KID[kernel], x, y, *iv, 80, 20)
     ref = t.matvec(b)
-    for free in (False, True):
-        H = hmx.KernelMatrix(kernel, x, y, *iv, matrix_free=free)
+    for mode in MODES:
+        H = hmx.KernelMatrix(kernel, x, y, *iv, matrix_free=mode == "free",
+                             hybrid=mode == "hybrid")
         assert H.rank == 20 and H.blocksize_ == 80
         u = H.matvec(b)
-        assert rel(u, ref) <= TOL, (free, rel(u, ref))
+        assert rel(u, ref) <= TOL, (mode, rel(u, ref))
