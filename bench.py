@@ -35,6 +35,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-n-log2", type=int, default=17)
+    ap.add_argument("--variants", default="hybrid,matrix_free",
+                    help="extra variants timed beside the stored product")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the torch bandwidth probes")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
     return ap.parse_args()
 
 
@@ -66,6 +71,20 @@ def pass_times(H, fn, steps):
                                    ctypes.byref(cnt)))
     check(lib.hm_plan_profile(H._h, 0))
     return ms / max(cnt.value, 1)
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the stored downward pass, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by a copy-kernel
+    calibration); None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_down_cauchy_n2p20_pmc_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch k_down_stored:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def stream_peaks(stream):
@@ -135,6 +154,17 @@ def main():
     import torch
     import hmx
 
+    if args.copy_calibration:
+        n = 1 << 27
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"doubles": n, "bytes_read": 8 * n,
+                                               "bytes_written": 8 * n, "launches": 5}}))
+        return
+
     dev = torch.device("cuda:0")
     w = hmx.workloads.config_headline()
     stream = torch.cuda.current_stream()
@@ -167,7 +197,10 @@ def main():
 
     # hybrid (stored A, F; Chebyshev-form basis) and matrix-free variants
     variants = {}
+    wanted = [v for v in args.variants.split(",") if v]
     for name, kw in (("hybrid", dict(hybrid=True)), ("matrix_free", dict(matrix_free=True))):
+        if name not in wanted:
+            continue
         Hv = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=BS,
                               rank=RANK, **kw)
         uv = torch.empty_like(b)
@@ -192,12 +225,13 @@ def main():
     ach_down = bytes_down / (passes[3] * 1e-3) / 1e9
     roof = dict(bound="hbm", kernel="k_down (near field + U z)", achieved=round(ach_down, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach_down / HBM_PEAK_GBS, 4),
-                traffic=None, algorithmic_bytes=bytes_down,
+                traffic=pmc_traffic(), algorithmic_bytes=bytes_down,
+                traffic_source="profiles/k_down_cauchy_n2p20_pmc_traffic.md (rocprofv3 --pmc)",
                 avg_ms=round(float(passes[3]), 4),
                 up_pass=dict(achieved=round(bytes_up / (passes[0] * 1e-3) / 1e9, 1),
                              avg_ms=round(float(passes[0]), 4)))
 
-    peaks = stream_peaks(stream)
+    peaks = None if args.no_peaks else stream_peaks(stream)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
