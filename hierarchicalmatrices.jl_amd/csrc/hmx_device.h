@@ -53,6 +53,11 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
 // after down.
 hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode,
                              hipStream_t s, const hipEvent_t *ev = nullptr);
+// gradients (matrix-free near field, Chebyshev-form far field):
+//   forward (tr = false): y = beta*y + s .* (d/dx_i of (H x)_i)
+//   transpose (tr = true): y = beta*y + H^T x, y2 = beta*y2 + s .* (d/dy_j of (H^T x)_j)
+hipError_t hmx_launch_grad(const DevPlan &p, const MvArgs &a, bool transpose, double *y2,
+                           int64_t incy2, const double *s, int64_t incs, hipStream_t stream);
 // entries: kind[t] = 0 none, 1 dense (id), 2 far (id); rows/cols 0-based
 hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,
                               const int32_t *kind, const int32_t *id, double *out, hipStream_t s);

@@ -82,6 +82,18 @@ __device__ __forceinline__ double kern_fast(double x, double y) {
     else return log(fabs(d));
 }
 
+// Value f(x, y) and derivative df/dx of the kernels, from one reciprocal of
+// d = x - y: Cauchy 1/d, -1/d^2; Coulomb 1/d^2, -2/d^3; 1/d^3, -3/d^4;
+// log|d|, 1/d. All are functions of x - y, so df/dy = -df/dx.
+template <int KID>
+__device__ __forceinline__ void kern_val_dx(double d, double &f, double &fx) {
+    const double r = rcp64(d);
+    if constexpr (KID == 0) { f = r; fx = -r * r; }
+    else if constexpr (KID == 1) { const double r2 = r * r; f = r2; fx = -2.0 * r2 * r; }
+    else if constexpr (KID == 2) { const double r2 = r * r; f = r2 * r; fx = -3.0 * r2 * r2; }
+    else { f = log(fabs(d)); fx = r; }
+}
+
 // ---------------------------------------------------------------- reductions
 // Recursive-halving reduction of P values held by every lane: at each xor
 // offset O the lane pair exchanges half of its values, so after log2 steps
@@ -472,6 +484,77 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
     }
 }
 
+// Downward pass of the gradients. With p the lane's output point (x_i forward,
+// y_j transposed), the chunk computes
+//   val_p = sum_q f(.,.) in_q + sum_clusters S_c(xh(p))        (the product)
+//   der_p = sum_q d f/d p in_q + sum_clusters S_c'(xh(p)) / halfwidth_c
+// where S_c is the Chebyshev series of the cluster's coefficients, so der_p is
+// the exact derivative of val_p with respect to the point p. The near field is
+// regenerated from the points. DM = 1 writes y = beta*y + s .* der; DM = 2
+// writes y = beta*y + val and y2 = s .* der.
+template <int RMAX, int KID, bool TRANS, int DM>
+__global__ __launch_bounds__(WG) void k_down_grad(DevPlan p, const DChunk *ch, int nch,
+                                                 const DNear *nl, const DFar *fl,
+                                                 const double2 *cmapv, const double *pts_out,
+                                                 const double *pts_in, const double *z, MvArgs a,
+                                                 double *y2, int64_t incy2, const double *sv,
+                                                 int64_t incs) {
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= nch) return;
+    const int r = p.r;
+    const DChunk c = ch[wid];
+    const bool act = lane < c.np;
+    const double po = act ? pts_out[c.p0 + lane] : 0.0;
+    double accv = 0.0, accd = 0.0;
+    for (int e = c.near_beg; e < c.near_end; ++e) {
+        const DNear s = nl[e];
+        const DDense D = p.dense[s.blk];
+        const int64_t q0 = TRANS ? D.row0 : D.col0;
+        const int nq = TRANS ? D.m : D.n;
+        const double *xin = a.x + q0 * a.incx;
+        const double *qin = pts_in + q0;
+        for (int j = 0; j < nq; ++j) {
+            const double d = TRANS ? qin[j] - po : po - qin[j];
+            double f, fx;
+            kern_val_dx<KID>(d, f, fx);
+            const double v = xin[(int64_t)j * a.incx];
+            if constexpr (DM == 2) accv = __builtin_fma(f, v, accv);
+            accd = __builtin_fma(TRANS ? -fx : fx, v, accd);
+        }
+    }
+    for (int e = c.far_beg; e < c.far_end; ++e) {
+        const DFar f = fl[e];
+        const double *zc = z + (int64_t)f.cl * r;
+        const double2 cm = cmapv[f.cl];
+        const double xh = (po - cm.x) * cm.y;
+        const double x2 = xh + xh;
+        double b1 = 0.0, b2 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int k = RMAX - 1; k >= 1; --k) {
+            const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
+            const double b0 = __builtin_fma(x2, b1, ck - b2);
+            const double d0 = __builtin_fma(x2, d1, 2.0 * b1 - d2);
+            b2 = b1; b1 = b0;
+            d2 = d1; d1 = d0;
+        }
+        if constexpr (DM == 2) accv += __builtin_fma(xh, b1, zc[0] - b2);
+        accd = __builtin_fma(__builtin_fma(xh, d1, b1 - d2), cm.y, accd);
+    }
+    if (act) {
+        const int64_t i = c.p0 + lane;
+        const double sc = sv[i * incs];
+        double *yo = a.y + i * a.incy;
+        if constexpr (DM == 1) {
+            *yo = (a.beta ? *yo : 0.0) + sc * accd;
+        } else {
+            *yo = (a.beta ? *yo : 0.0) + accv;
+            double *yd = y2 + i * incy2;
+            *yd = (a.beta ? *yd : 0.0) + sc * accd;
+        }
+    }
+}
+
 // getindex: one thread per requested entry
 __global__ __launch_bounds__(WG) void k_entries(DevPlan p, int64_t n, const int64_t *ii,
                                                const int64_t *jj, const int32_t *kind,
@@ -588,6 +671,57 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     return hipGetLastError();
 }
 
+template <int RMAX, int KID>
+hipError_t grad_t(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
+                  const double *sv, int64_t incs, hipStream_t s) {
+    const DTask *in_task = tr ? p.rtask : p.ctask;
+    const int in_ntask = tr ? p.nrtask : p.nctask;
+    const DCluster *in_cl = tr ? p.rcl : p.ccl;
+    const int in_ncl = tr ? p.nrc : p.ncc;
+    const double2 *in_map = tr ? p.rmap : p.cmap;
+    const double *in_pts = tr ? p.x : p.y;
+    const DChunk *out_ch = tr ? p.cch : p.rch;
+    const int out_n = tr ? p.ncch : p.nrch;
+    const DNear *out_near = tr ? p.cnear : p.rnear;
+    const DFar *out_far = tr ? p.cfar : p.rfar;
+    const DCluster *out_cl = tr ? p.ccl : p.rcl;
+    const int out_ncl = tr ? p.ncc : p.nrc;
+    const double2 *out_map = tr ? p.cmap : p.rmap;
+    const double *out_pts = tr ? p.y : p.x;
+    const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
+    if (p.nfar > 0 && in_ntask > 0) {
+        k_up<RMAX, true><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x, a.incx,
+                                                  a.partial, p.r);
+        k_reduce<true><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
+        if (tr)
+            k_translate<RMAX, true, KID, true, true><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, out_far_list, a.w, a.z);
+        else
+            k_translate<RMAX, true, KID, false, true><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, out_far_list, a.w, a.z);
+    }
+    if (out_n > 0) {
+        if (tr)
+            k_down_grad<RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
+                p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
+        else
+            k_down_grad<RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(
+                p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
+    }
+    return hipGetLastError();
+}
+
+template <int RMAX>
+hipError_t grad_r(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
+                  const double *sv, int64_t incs, hipStream_t s) {
+    switch (p.kid) {
+    case 0: return grad_t<RMAX, 0>(p, a, tr, y2, incy2, sv, incs, s);
+    case 1: return grad_t<RMAX, 1>(p, a, tr, y2, incy2, sv, incs, s);
+    case 2: return grad_t<RMAX, 2>(p, a, tr, y2, incy2, sv, incs, s);
+    default: return grad_t<RMAX, 3>(p, a, tr, y2, incy2, sv, incs, s);
+    }
+}
+
 template <int RMAX>
 hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, hipStream_t s,
                     const hipEvent_t *ev) {
@@ -624,6 +758,16 @@ hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool tr, int mod
     if (p.r <= 32) return matvec_r<32>(p, a, tr, mode, s, ev);
     if (p.r <= 48) return matvec_r<48>(p, a, tr, mode, s, ev);
     return matvec_r<64>(p, a, tr, mode, s, ev);
+}
+
+hipError_t hmx_launch_grad(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
+                           const double *sv, int64_t incs, hipStream_t s) {
+    if (p.r <= 8) return grad_r<8>(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 16) return grad_r<16>(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 24) return grad_r<24>(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 32) return grad_r<32>(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 48) return grad_r<48>(p, a, tr, y2, incy2, sv, incs, s);
+    return grad_r<64>(p, a, tr, y2, incy2, sv, incs, s);
 }
 
 hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,

@@ -779,6 +779,49 @@ int hm_matvec_t(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t j
     return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, true);
 }
 
+static MvArgs ws_args(hm_plan_t P) {
+    MvArgs a{};
+    a.partial = P->d_partial;
+    a.w = P->d_w;
+    a.z = P->d_z;
+    return a;
+}
+
+int hm_matvec_grad_x(hm_plan_t P, double *g, const double *b, const double *s, int64_t ioff,
+                     int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
+    if (!P || !g || !b || !s) return fail(HM_ERR_INVALID, "grad_x: NULL argument");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "grad_x: tree-only plan");
+    if (incg < 1 || incb < 1 || incs < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
+        return fail(HM_ERR_INVALID, "grad_x: bad offsets, strides or beta");
+    MvArgs a = ws_args(P);
+    a.y = g + ioff;
+    a.x = b + joff;
+    a.incy = incg;
+    a.incx = incb;
+    a.beta = beta;
+    hipError_t e = hmx_launch_grad(P->dp, a, false, nullptr, 0, s, incs, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "grad_x launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
+int hm_matvec_t_grad(hm_plan_t P, double *gb, double *gy, const double *r, const double *s,
+                     int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
+                     void *stream) {
+    if (!P || !gb || !gy || !r || !s) return fail(HM_ERR_INVALID, "t_grad: NULL argument");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "t_grad: tree-only plan");
+    if (incgb < 1 || incgy < 1 || incr < 1 || incs < 1 || (beta != 0 && beta != 1))
+        return fail(HM_ERR_INVALID, "t_grad: bad strides or beta");
+    MvArgs a = ws_args(P);
+    a.y = gb;
+    a.x = r;
+    a.incy = incgb;
+    a.incx = incr;
+    a.beta = beta;
+    hipError_t e = hmx_launch_grad(P->dp, a, true, gy, incgy, s, incs, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "t_grad launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
 int hm_plan_profile(hm_plan_t P, int enable) {
     if (!P) return fail(HM_ERR_INVALID, "profile: NULL plan");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "profile: tree-only plan");

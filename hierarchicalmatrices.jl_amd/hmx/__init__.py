@@ -10,6 +10,7 @@ from .kernelmatrix import (  # noqa: F401
     coulombprimekernel, indsplit, logkernel, mul_, size,
 )
 from . import workloads  # noqa: F401
+from . import autodiff  # noqa: F401
 
 __all__ = [
     "BLOCKRANK", "BLOCKSIZE", "Block", "KernelMatrix", "HMError", "blocksize", "size",

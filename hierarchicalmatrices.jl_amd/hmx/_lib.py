@@ -87,6 +87,12 @@ def _load():
         fn.restype = ctypes.c_int
         fn.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, ctypes.c_int,
                        ctypes.c_uint, _vp]
+    L.hm_matvec_grad_x.restype = ctypes.c_int
+    L.hm_matvec_grad_x.argtypes = [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                   ctypes.c_int, _vp]
+    L.hm_matvec_t_grad.restype = ctypes.c_int
+    L.hm_matvec_t_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int,
+                                   _vp]
     L.hm_plan_profile.restype = ctypes.c_int
     L.hm_plan_profile.argtypes = [_vp, ctypes.c_int]
     L.hm_plan_profile_read.restype = ctypes.c_int

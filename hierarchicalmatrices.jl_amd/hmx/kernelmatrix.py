@@ -276,6 +276,26 @@ class KernelMatrix:
             return out.cpu().numpy()
         return out
 
+    def grad_x(self, b, s, stream=None):
+        """g_i = s_i * d/dx_i (H(x, y) b)_i (device tensors, 1-D)."""
+        torch = _torch()
+        g = torch.empty(self.shape[0], dtype=torch.float64, device=b.device)
+        check(lib.hm_matvec_grad_x(self._h, ctypes.c_void_p(g.data_ptr()),
+                                   ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(s.data_ptr()),
+                                   0, 0, 1, b.stride(0), s.stride(0), 0, _stream_ptr(stream)))
+        return g
+
+    def tgrad(self, r, s, stream=None):
+        """(H^T r, s .* d/dy_j (H^T r)_j) for device tensors r (rows) and s (cols)."""
+        torch = _torch()
+        gb = torch.empty(self.shape[1], dtype=torch.float64, device=r.device)
+        gy = torch.empty_like(gb)
+        check(lib.hm_matvec_t_grad(self._h, ctypes.c_void_p(gb.data_ptr()),
+                                   ctypes.c_void_p(gy.data_ptr()), ctypes.c_void_p(r.data_ptr()),
+                                   ctypes.c_void_p(s.data_ptr()), 1, 1, r.stride(0), s.stride(0),
+                                   0, _stream_ptr(stream)))
+        return gb, gy
+
     def __matmul__(self, v):
         return self.matvec(v)
 

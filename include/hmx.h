@@ -137,6 +137,23 @@ int hm_matvec_t(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_
                 int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
                 unsigned flags, void *stream);
 
+/* Gradient of the product with respect to the row points x (no counterpart in
+ * the reference; it serves the rrule of L(x, y, b)):
+ *   g[ioff + i*incg] = beta*g[...] + s[i*incs] * d/dx_i sum_j H(x, y)[i, j] b[joff + j*incb]
+ * i.e. the exact derivative of the computed product (near field regenerated
+ * with df/dx, far field through the derivative of its Chebyshev-form basis).
+ * With s = the output cotangent this is the x-gradient. One right-hand side. */
+int hm_matvec_grad_x(hm_plan_t plan, double *g, const double *b, const double *s, int64_t ioff,
+                     int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream);
+
+/* Adjoint pass for the b- and y-gradients (no counterpart in the reference):
+ *   gb = beta*gb + H^T r,   gy[j] = beta*gy[j] + s[j] * d/dy_j (H^T r)_j
+ * with r the output cotangent; s = b gives the y-gradient. gb and gy are
+ * indexed by columns (gb[j*incgb], gy[j*incgy], s[j*incs], r[i*incr]). */
+int hm_matvec_t_grad(hm_plan_t plan, double *gb, double *gy, const double *r, const double *s,
+                     int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
+                     void *stream);
+
 /* Per-pass timing of the products of this plan (bench support, no counterpart
  * in the reference). enable=1 starts recording HIP events around the four
  * passes of every later product (up, reduce, translate, down) on its stream;
