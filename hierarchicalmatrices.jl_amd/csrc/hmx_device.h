@@ -39,10 +39,11 @@ struct MvArgs {
     double *y;
     const double *x;
     int64_t incy, incx;
+    int64_t ldy, ldx;  // stride between right-hand sides
     int beta;
-    double *partial;  // >= max(nslots) * r
-    double *w;        // >= max(nrc, ncc) * r
-    double *z;        // >= max(nrc, ncc) * r
+    double *partial;  // >= max(nslots) * r * kb
+    double *w;        // >= max(nrc, ncc) * r * kb
+    double *z;        // >= max(nrc, ncc) * r * kb
 };
 
 // compression: dense blocks, U, V, F (bit-exact operation order)
@@ -51,7 +52,8 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
 // stored, 1 hybrid (stored A, F; Chebyshev-form basis), 2 matrix-free. When
 // ev is not null, ev[0..4] are recorded before up, reduce, translate, down and
 // after down.
-hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode,
+// kb in {1, 2, 4, 8}: right-hand sides handled by this launch (forward only).
+hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode, int kb,
                              hipStream_t s, const hipEvent_t *ev = nullptr);
 // gradients (matrix-free near field, Chebyshev-form far field):
 //   forward (tr = false): y = beta*y + s .* (d/dx_i of (H x)_i)

@@ -25,6 +25,7 @@
 
 #define HM_CHUNK 64          // rows per chunk = wavefront width
 #define HM_TASK_POINTS 8192  // points per upward-pass task
+#define HM_MAX_KB 8          // right-hand sides per forward launch
 
 // Stored matrices (dense blocks m x n, bases len x r) are kept in 64-row
 // tiles, column-major inside a tile: element (i, k) of a matrix with `cols`

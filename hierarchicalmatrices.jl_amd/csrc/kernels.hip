@@ -1,780 +1,45 @@
-// kernels.hip -- gfx950 kernels of the KernelMatrix compression and product.
-//
-// One product y = beta*y + H x runs four passes over the flattened tree:
-//   up         per task (a column cluster and <= 8192 of its columns), one
-//              workgroup: w_task = B[cols, :]^T x[cols] reduced in registers,
-//              across the wavefront by recursive halving and across the four
-//              wavefronts in LDS; each task owns one partial slot (no atomics)
-//   reduce     per column cluster: w = sum of its task slots
-//   translate  per row cluster: z = sum over its far blocks of F w
-//   down       per row chunk (one wavefront, one lane per row): near field of
-//              the chunk's dense blocks plus U z of every row cluster covering
-//              the chunk, written once as beta*y + acc
-// The transpose runs the same passes with rows and columns exchanged.
-//
-// Three modes:
-//   stored   streams A, U, V, F from HBM (U, V as update! forms them,
-//            src/BarycentricMatrix.jl:248-287)
-//   hybrid   streams A and F; the far-field basis is applied in Chebyshev form
-//   free     streams nothing but points; A and F are regenerated as well
-// Chebyshev form: on first-kind nodes x_k = cos((2k-1)pi/(2r)) the barycentric
-// Lagrange basis equals l_k(x) = sum_m A[k][m] T_m(x) with
-// A[k][m] = (m ? 2 : 1) T_m(x_k) / r (discrete orthogonality). So the upward
-// pass accumulates moments mu_m = sum_j T_m(xh_j) x_j by the three-term
-// recurrence, the reduce maps them to node values w = A mu, the translate
-// maps z to coefficients c = A^T z, and the downward pass evaluates
-// sum_m c_m T_m(xh_i) by Clenshaw: two FMAs per term, no division. Values
-// equal the barycentric ones up to rounding.
-//
-// Every per-wavefront quantity (chunk descriptors, block and cluster records,
-// the input vector entries and the z coefficients a lane multiplies with) is
-// addressed through wave-uniform indices, so the compiler issues scalar loads
-// for it and the vector memory path only carries the streamed factors.
-//
-// Compression kernels follow the reference's operation order exactly and are
-// compiled without FMA contraction, so the stored factors are bit-identical to
-// the ones the reference computes from the same points and nodes.
-#include "hmx_device.h"
-
-#include <math.h>
-
-namespace {
-
-constexpr int WAVE = 64;
-constexpr int WG = 256;
-constexpr int WPB = WG / WAVE;  // wavefronts per workgroup
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ int wave_id() {
-    return uniform((int)(blockIdx.x * WPB + threadIdx.x / WAVE));
-}
-
-// Reciprocal: hardware estimate refined by two Newton-Raphson steps
-// r <- r + r(1 - d r) (Markstein's FMA formulation); error <= 1 ulp.
-__device__ __forceinline__ double rcp64(double d) {
-    double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-// f(x, y) with the reference's rounding sequence (examples/Kernel.jl:34-37).
-__device__ __forceinline__ double kern_exact(int kid, double x, double y) {
-#pragma clang fp contract(off)
-    double d = x - y;
-    switch (kid) {
-    case 0: return 1.0 / d;
-    case 1: return 1.0 / (d * d);
-    case 2: return 1.0 / ((d * d) * d);
-    default: return log(fabs(d));
-    }
-}
-
-// f(x, y) for the matrix-free passes.
-template <int KID>
-__device__ __forceinline__ double kern_fast(double x, double y) {
-    double d = x - y;
-    if constexpr (KID == 0) return rcp64(d);
-    else if constexpr (KID == 1) return rcp64(d * d);
-    else if constexpr (KID == 2) return rcp64(d * d * d);
-    else return log(fabs(d));
-}
-
-// Value f(x, y) and derivative df/dx of the kernels, from one reciprocal of
-// d = x - y: Cauchy 1/d, -1/d^2; Coulomb 1/d^2, -2/d^3; 1/d^3, -3/d^4;
-// log|d|, 1/d. All are functions of x - y, so df/dy = -df/dx.
-template <int KID>
-__device__ __forceinline__ void kern_val_dx(double d, double &f, double &fx) {
-    const double r = rcp64(d);
-    if constexpr (KID == 0) { f = r; fx = -r * r; }
-    else if constexpr (KID == 1) { const double r2 = r * r; f = r2; fx = -2.0 * r2 * r; }
-    else if constexpr (KID == 2) { const double r2 = r * r; f = r2 * r; fx = -3.0 * r2 * r2; }
-    else { f = log(fabs(d)); fx = r; }
-}
-
-// ---------------------------------------------------------------- reductions
-// Recursive-halving reduction of P values held by every lane: at each xor
-// offset O the lane pair exchanges half of its values, so after log2 steps
-// every lane holds P/2^s partial sums of distinct indices. Odd counts finish
-// with plain xor-reductions. `base` receives the first index a lane holds.
-template <int P, int O>
-struct Halve {
-    static __device__ __forceinline__ void run(double *acc, int lane, int &base) {
-        if constexpr (O > 0) {
-            if constexpr (P % 2 == 0) {
-                constexpr int H = P / 2;
-                const bool hi = (lane & O) != 0;
-#pragma unroll
-                for (int i = 0; i < H; ++i) {
-                    double send = hi ? acc[i] : acc[i + H];
-                    double keep = hi ? acc[i + H] : acc[i];
-                    acc[i] = keep + __shfl_xor(send, O);
-                }
-                if (hi) base += H;
-                Halve<H, O / 2>::run(acc, lane, base);
-            } else {
-#pragma unroll
-                for (int i = 0; i < P; ++i) acc[i] += __shfl_xor(acc[i], O);
-                Halve<P, O / 2>::run(acc, lane, base);
-            }
-        }
-    }
-};
-template <int P, int O>
-struct HalveShape {  // values left per lane, and the xor-mask of the plain steps
-    static constexpr int left = (O == 0) ? P : (P % 2 == 0 ? HalveShape<P / 2, O / 2>::left
-                                                           : HalveShape<P, O / 2>::left);
-    static constexpr int mask = (O == 0) ? 0 : (P % 2 == 0 ? HalveShape<P / 2, O / 2>::mask
-                                                           : (O | HalveShape<P, O / 2>::mask));
-};
-template <int P>
-struct HalveShape<P, 0> {
-    static constexpr int left = P;
-    static constexpr int mask = 0;
-};
-
-// ------------------------------------------------------------- compression
-
-__global__ __launch_bounds__(WG) void k_gen_dense(DevPlan p) {
-    const DDense D = p.dense[blockIdx.x];
-    const int32_t tot = D.m * D.n;
-    for (int32_t e = threadIdx.x; e < tot; e += WG) {
-        int32_t j = e / D.m, i = e - j * D.m;
-        p.A[D.aoff + tile_index(i, j, D.n)] = kern_exact(p.kid, p.x[D.row0 + i], p.y[D.col0 + j]);
-    }
-}
-
-// update! (src/BarycentricMatrix.jl:256-287): one lane per point of a chunk,
-// for every cluster covering the chunk.
-template <int RMAX>
-__global__ __launch_bounds__(WG) void k_gen_basis(const DChunk *ch, int nch, const DFar *fl,
-                                                 const DCluster *cl, const double *nodes,
-                                                 const double *lam, const double *pts,
-                                                 double *B, int r) {
-#pragma clang fp contract(off)
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= nch) return;
-    const DChunk c = ch[wid];
-    if (lane >= c.np) return;
-    const double xi = pts[c.p0 + lane];
-    for (int e = c.far_beg; e < c.far_end; ++e) {
-        const DFar f = fl[e];
-        const DCluster C = cl[f.cl];
-        const double *t = nodes + (int64_t)f.cl * r;
-        double row[RMAX];
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k)
-            if (k < r) row[k] = lam[k] * (1.0 / (xi - t[k]));
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k)
-            if (k < r) s += row[k];
-        double *out = B + C.boff + tile_index(f.off + lane, 0, r);
-#pragma unroll
-        for (int k = 0; k < RMAX; ++k)
-            if (k < r) out[k * HM_TILE] = row[k] / s;
-    }
-}
-
-// F[m, n] = f(t_m, s_n), src/BarycentricMatrix.jl:168-175
-__global__ __launch_bounds__(WG) void k_gen_F(DevPlan p) {
-    const DFarBlk b = p.far[blockIdx.x];
-    const int r = p.r;
-    const double *tx = p.rnodes + (int64_t)b.rc * r;
-    const double *ty = p.cnodes + (int64_t)b.cc * r;
-    for (int e = threadIdx.x; e < r * r; e += WG) {
-        int n = e / r, m = e - n * r;
-        p.F[b.foff + e] = kern_exact(p.kid, tx[m], ty[n]);
-    }
-}
-
-__global__ __launch_bounds__(WG) void k_count_nonfinite(const double *v, int64_t n,
-                                                        unsigned long long *blockcount) {
-    __shared__ unsigned int sh[WG];
-    unsigned int cnt = 0;
-    for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
-        cnt += isfinite(v[i]) ? 0u : 1u;
-    sh[threadIdx.x] = cnt;
-    __syncthreads();
-    for (int o = WG / 2; o > 0; o /= 2) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) blockcount[blockIdx.x] = sh[0];
-}
-
-// ------------------------------------------------------------------ product
-
-// Upward pass: partial[task.slot] = B[task points, :]^T x[task points] with B =
-// V (forward) or U (transpose) read from HBM, or, in Chebyshev form, the
-// moments sum_j T_m(xh_j) x_j with xh = (p - center) * inv_halfwidth.
-template <int RMAX, bool CHEB>
-__global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *cl, const double *B,
-                                          const double2 *cmapv, const double *pts, const double *x,
-                                          int64_t incx, double *partial, int r) {
-    __shared__ double sh[WPB][RMAX];
-    const DTask T = tasks[blockIdx.x];
-    const DCluster C = cl[T.cl];
-    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
-    double acc[RMAX];
-#pragma unroll
-    for (int k = 0; k < RMAX; ++k) acc[k] = 0.0;
-    const int64_t first = C.first + T.off;
-    if constexpr (!CHEB) {
-        for (int i = tid; i < T.len; i += WG) {
-            const double xv = x[(first + i) * incx];
-            const double *bp = B + C.boff + tile_index(T.off + i, 0, r);
-            // columns past r re-read column r-1 (in bounds) and are dropped,
-            // so all RMAX loads issue back to back without a branch
-            double bv[RMAX];
-#pragma unroll
-            for (int k = 0; k < RMAX; ++k) bv[k] = bp[min(k, r - 1) * HM_TILE];
-#pragma unroll
-            for (int k = 0; k < RMAX; ++k) acc[k] = __builtin_fma(k < r ? bv[k] : 0.0, xv, acc[k]);
-        }
-    } else {
-        const double2 cm = cmapv[T.cl];
-        for (int i = tid; i < T.len; i += WG) {
-            const double xv = x[(first + i) * incx];
-            const double xh = (pts[first + i] - cm.x) * cm.y;
-            const double x2 = xh + xh;
-            double t0 = xv, t1 = xh * xv;  // T_m(xh) * x, by the recurrence
-            acc[0] += t0;
-            if (RMAX > 1) acc[1] += t1;
-#pragma unroll
-            for (int k = 2; k < RMAX; ++k) {
-                const double t2 = __builtin_fma(x2, t1, -t0);
-                acc[k] += t2;
-                t0 = t1;
-                t1 = t2;
-            }
-        }
-    }
-    int base = 0;
-    Halve<RMAX, WAVE / 2>::run(acc, lane, base);
-    constexpr int LEFT = HalveShape<RMAX, WAVE / 2>::left;
-    constexpr int MASK = HalveShape<RMAX, WAVE / 2>::mask;
-    if ((lane & MASK) == 0) {
-#pragma unroll
-        for (int i = 0; i < LEFT; ++i) sh[wv][base + i] = acc[i];
-    }
-    __syncthreads();
-    if (tid < r) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < WPB; ++q) s += sh[q][tid];
-        partial[T.slot * r + tid] = s;
-    }
-}
-
-// w[c] = sum over the task slots of cluster c; in Chebyshev form the slots hold
-// moments and w = A mu.
-template <bool CHEB>
-__global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const double *partial,
-                                              const double *chebA, double *w, int r) {
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= ncl) return;
-    const DCluster C = cl[wid];
-    const int G = WAVE / r;  // lane groups; lane = g*r + k
-    double acc = 0.0;
-    if (lane < G * r) {
-        const int k = lane % r, g = lane / r;
-        for (int t = g; t < C.nslots; t += G) acc += partial[(C.pbase + t) * r + k];
-    }
-    double tot = acc;
-    for (int g = 1; g < G; ++g) tot += __shfl(acc, (lane + g * r) & (WAVE - 1));
-    if constexpr (CHEB) {
-        double v = 0.0;
-        for (int m = 0; m < r; ++m) {
-            const double mu = __shfl(tot, m);
-            if (lane < r) v = __builtin_fma(chebA[lane * r + m], mu, v);
-        }
-        tot = v;
-    }
-    if (lane < r) w[(int64_t)wid * r + lane] = tot;
-}
-
-// z[d] = sum over far blocks b of cluster d of F_b w[other side of b]
-// (forward: F w[cc]; transpose: F^T w[rc]).
-template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB>
-__global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl, int ncl,
-                                                 const int32_t *far_list, const double *w,
-                                                 double *z) {
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= ncl) return;
-    const int r = p.r;
-    const DCluster C = cl[wid];
-    const bool act = lane < r;
-    double acc = 0.0;
-    for (int e = C.far_beg; e < C.far_end; ++e) {
-        const DFarBlk b = p.far[far_list[e]];
-        const int src = TRANS ? b.rc : b.cc;
-        const double *ws = w + (int64_t)src * r;
-        if constexpr (!FREE) {
-            const double *F = p.F + b.foff;
-            double fv[RMAX];
-            const int la = act ? lane : 0;
-#pragma unroll
-            for (int l = 0; l < RMAX; ++l) {
-                const int lc = min(l, r - 1);
-                fv[l] = TRANS ? F[lc + r * la] : F[la + r * lc];
-            }
-#pragma unroll
-            for (int l = 0; l < RMAX; ++l)
-                acc = __builtin_fma((act && l < r) ? fv[l] : 0.0, ws[min(l, r - 1)], acc);
-        } else {
-            // forward: lane = m, F[m, l] = f(tx_m, ty_l); transpose: lane = l, sum over m
-            const double *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
-            const double *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
-            const double to = act ? own[lane] : 0.0;
-            for (int l = 0; l < r; ++l) {
-                double f = 0.0;
-                if (act) f = TRANS ? kern_fast<KID>(oth[l], to) : kern_fast<KID>(to, oth[l]);
-                acc = __builtin_fma(f, ws[l], acc);
-            }
-        }
-    }
-    if constexpr (CHEB) {  // coefficients c = A^T z
-        double c = 0.0;
-        for (int k = 0; k < r; ++k) {
-            const double zk = __shfl(acc, k);
-            if (act) c = __builtin_fma(p.chebA[k * r + lane], zk, c);
-        }
-        acc = c;
-    }
-    if (act) z[(int64_t)wid * r + lane] = acc;
-}
-
-// Matrix-free near field of one lane: sum_j f(p, q_j) v_j (or f(q_j, p) v_j
-// when TR). For the power kernels 1/d^m two terms share one reciprocal:
-// v1/d1^m + v2/d2^m = (v1 d2^m + v2 d1^m) / (d1 d2)^m.
-template <int KID, bool TR>
-__device__ __forceinline__ double near_free(double p, const double *qs, const double *v,
-                                            int64_t inc, int n, bool act) {
-    double acc = 0.0;
-    if constexpr (KID <= 2) {
-        int j = 0;
-        for (; j + 2 <= n; j += 2) {
-            double d1 = TR ? qs[j] - p : p - qs[j];
-            double d2 = TR ? qs[j + 1] - p : p - qs[j + 1];
-            if constexpr (KID == 1) { d1 *= d1; d2 *= d2; }
-            if constexpr (KID == 2) { d1 *= d1 * d1; d2 *= d2 * d2; }
-            const double num = __builtin_fma(v[(int64_t)j * inc], d2, v[(int64_t)(j + 1) * inc] * d1);
-            acc = __builtin_fma(num, rcp64(d1 * d2), acc);
-        }
-        if (j < n) {
-            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
-            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
-        }
-    } else {
-        for (int j = 0; j < n; ++j) {
-            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
-            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
-        }
-    }
-    return act ? acc : 0.0;
-}
-
-// Downward pass. Forward: output = rows; near blocks read by rows; far basis U.
-// Transpose: output = columns; near blocks read by columns; far basis V.
-template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB>
-__global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nch,
-                                            const DNear *nl, const DFar *fl,
-                                            const DCluster *cl, const double *B,
-                                            const double2 *cmapv, const double *pts_out,
-                                            const double *pts_in, const double *z, MvArgs a) {
-    constexpr int UN = 16;
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= nch) return;
-    const int r = p.r;
-    const DChunk c = ch[wid];
-    const bool act = lane < c.np;
-    double po = 0.0;
-    if constexpr (FREE || CHEB) po = act ? pts_out[c.p0 + lane] : 0.0;
-    double acc = 0.0;
-
-    // ---- near field
-    for (int e = c.near_beg; e < c.near_end; ++e) {
-        const DNear s = nl[e];
-        const DDense D = p.dense[s.blk];
-        if constexpr (!TRANS) {
-            const double *xin = a.x + D.col0 * a.incx;
-            if constexpr (!FREE) {
-                const double *Aj = p.A + D.aoff + tile_index(s.off + lane, 0, D.n);
-                int j = 0;
-                for (; j + UN <= D.n; j += UN) {
-                    double av[UN];
-#pragma unroll
-                    for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : 0.0;
-#pragma unroll
-                    for (int u = 0; u < UN; ++u) acc = __builtin_fma(av[u], xin[(int64_t)(j + u) * a.incx], acc);
-                }
-                for (; j < D.n; ++j)
-                    acc = __builtin_fma(act ? Aj[j * HM_TILE] : 0.0, xin[(int64_t)j * a.incx], acc);
-            } else {
-                acc += near_free<KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
-            }
-        } else {
-            const double *xin = a.x + D.row0 * a.incx;
-            if constexpr (!FREE) {
-                // lanes run down the rows of the block, 64 accumulators for the
-                // chunk's columns, then a recursive-halving transpose-reduce
-                double accT[WAVE];
-#pragma unroll
-                for (int q = 0; q < WAVE; ++q) accT[q] = 0.0;
-                for (int ib = 0; ib < D.m; ib += WAVE) {
-                    const bool ra = ib + lane < D.m;
-                    const double xv = ra ? xin[(int64_t)(ib + lane) * a.incx] : 0.0;
-                    const double *Ac = p.A + D.aoff + tile_index(ib + lane, s.off, D.n);
-#pragma unroll
-                    for (int q = 0; q < WAVE; ++q)
-                        if (q < c.np) accT[q] += (ra ? Ac[q * HM_TILE] : 0.0) * xv;
-                }
-                int base = 0;
-                Halve<WAVE, WAVE / 2>::run(accT, lane, base);
-                acc += accT[0];
-            } else {
-                acc += near_free<KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act);
-            }
-        }
-    }
-
-    // ---- far field: acc += B[chunk rows, :] z[cluster]
-    for (int e = c.far_beg; e < c.far_end; ++e) {
-        const DFar f = fl[e];
-        const double *zc = z + (int64_t)f.cl * r;
-        if constexpr (!CHEB) {
-            const DCluster C = cl[f.cl];
-            const double *bp = B + C.boff + tile_index(f.off + lane, 0, r);
-            int k = 0;
-            for (; k + 8 <= r; k += 8) {
-                double bv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) bv[u] = act ? bp[(k + u) * HM_TILE] : 0.0;
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc = __builtin_fma(bv[u], zc[k + u], acc);
-            }
-            for (; k < r; ++k) acc = __builtin_fma(act ? bp[k * HM_TILE] : 0.0, zc[k], acc);
-        } else {
-            // Clenshaw: b_k = c_k + 2 xh b_{k+1} - b_{k+2}; sum = c_0 + xh b_1 - b_2
-            const double2 cm = cmapv[f.cl];
-            const double xh = (po - cm.x) * cm.y;
-            const double x2 = xh + xh;
-            double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-            for (int k = RMAX - 1; k >= 1; --k) {
-                const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
-                const double b0 = __builtin_fma(x2, b1, ck - b2);
-                b2 = b1;
-                b1 = b0;
-            }
-            const double sum = __builtin_fma(xh, b1, zc[0] - b2);
-            if (act) acc += sum;
-        }
-    }
-    if (act) {
-        double *yo = a.y + (c.p0 + lane) * a.incy;
-        *yo = (a.beta ? *yo : 0.0) + acc;
-    }
-}
-
-// Downward pass of the gradients. With p the lane's output point (x_i forward,
-// y_j transposed), the chunk computes
-//   val_p = sum_q f(.,.) in_q + sum_clusters S_c(xh(p))        (the product)
-//   der_p = sum_q d f/d p in_q + sum_clusters S_c'(xh(p)) / halfwidth_c
-// where S_c is the Chebyshev series of the cluster's coefficients, so der_p is
-// the exact derivative of val_p with respect to the point p. The near field is
-// regenerated from the points. DM = 1 writes y = beta*y + s .* der; DM = 2
-// writes y = beta*y + val and y2 = s .* der.
-template <int RMAX, int KID, bool TRANS, int DM>
-__global__ __launch_bounds__(WG) void k_down_grad(DevPlan p, const DChunk *ch, int nch,
-                                                 const DNear *nl, const DFar *fl,
-                                                 const double2 *cmapv, const double *pts_out,
-                                                 const double *pts_in, const double *z, MvArgs a,
-                                                 double *y2, int64_t incy2, const double *sv,
-                                                 int64_t incs) {
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= nch) return;
-    const int r = p.r;
-    const DChunk c = ch[wid];
-    const bool act = lane < c.np;
-    const double po = act ? pts_out[c.p0 + lane] : 0.0;
-    double accv = 0.0, accd = 0.0;
-    for (int e = c.near_beg; e < c.near_end; ++e) {
-        const DNear s = nl[e];
-        const DDense D = p.dense[s.blk];
-        const int64_t q0 = TRANS ? D.row0 : D.col0;
-        const int nq = TRANS ? D.m : D.n;
-        const double *xin = a.x + q0 * a.incx;
-        const double *qin = pts_in + q0;
-        for (int j = 0; j < nq; ++j) {
-            const double d = TRANS ? qin[j] - po : po - qin[j];
-            double f, fx;
-            kern_val_dx<KID>(d, f, fx);
-            const double v = xin[(int64_t)j * a.incx];
-            if constexpr (DM == 2) accv = __builtin_fma(f, v, accv);
-            accd = __builtin_fma(TRANS ? -fx : fx, v, accd);
-        }
-    }
-    for (int e = c.far_beg; e < c.far_end; ++e) {
-        const DFar f = fl[e];
-        const double *zc = z + (int64_t)f.cl * r;
-        const double2 cm = cmapv[f.cl];
-        const double xh = (po - cm.x) * cm.y;
-        const double x2 = xh + xh;
-        double b1 = 0.0, b2 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-        for (int k = RMAX - 1; k >= 1; --k) {
-            const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
-            const double b0 = __builtin_fma(x2, b1, ck - b2);
-            const double d0 = __builtin_fma(x2, d1, 2.0 * b1 - d2);
-            b2 = b1; b1 = b0;
-            d2 = d1; d1 = d0;
-        }
-        if constexpr (DM == 2) accv += __builtin_fma(xh, b1, zc[0] - b2);
-        accd = __builtin_fma(__builtin_fma(xh, d1, b1 - d2), cm.y, accd);
-    }
-    if (act) {
-        const int64_t i = c.p0 + lane;
-        const double sc = sv[i * incs];
-        double *yo = a.y + i * a.incy;
-        if constexpr (DM == 1) {
-            *yo = (a.beta ? *yo : 0.0) + sc * accd;
-        } else {
-            *yo = (a.beta ? *yo : 0.0) + accv;
-            double *yd = y2 + i * incy2;
-            *yd = (a.beta ? *yd : 0.0) + sc * accd;
-        }
-    }
-}
-
-// getindex: one thread per requested entry
-__global__ __launch_bounds__(WG) void k_entries(DevPlan p, int64_t n, const int64_t *ii,
-                                               const int64_t *jj, const int32_t *kind,
-                                               const int32_t *id, double *out) {
-#pragma clang fp contract(off)
-    const int64_t t = (int64_t)blockIdx.x * WG + threadIdx.x;
-    if (t >= n) return;
-    const int64_t i = ii[t], j = jj[t];
-    const int r = p.r;
-    double v = 0.0;
-    if (kind[t] == 1) {
-        const DDense D = p.dense[id[t]];
-        v = p.A ? p.A[D.aoff + tile_index(i - D.row0, j - D.col0, D.n)] : kern_exact(p.kid, p.x[i], p.y[j]);
-    } else if (kind[t] == 2) {
-        const DFarBlk b = p.far[id[t]];
-        const DCluster R = p.rcl[b.rc], Cc = p.ccl[b.cc];
-        double ur[64], vr[64];
-        if (p.U) {
-            for (int k = 0; k < r; ++k) {
-                ur[k] = p.U[R.boff + tile_index(i - R.first, k, r)];
-                vr[k] = p.V[Cc.boff + tile_index(j - Cc.first, k, r)];
-            }
-        } else {
-            double su = 0.0, sv = 0.0;
-            for (int k = 0; k < r; ++k) {
-                ur[k] = p.lam[k] * (1.0 / (p.x[i] - p.rnodes[(int64_t)b.rc * r + k]));
-                vr[k] = p.lam[k] * (1.0 / (p.y[j] - p.cnodes[(int64_t)b.cc * r + k]));
-                su += ur[k];
-                sv += vr[k];
-            }
-            for (int k = 0; k < r; ++k) { ur[k] /= su; vr[k] /= sv; }
-        }
-        for (int k = 0; k < r; ++k) {
-            double tmp = 0.0;
-            for (int l = 0; l < r; ++l) {
-                double F = p.F ? p.F[b.foff + k + (int64_t)r * l]
-                               : kern_exact(p.kid, p.rnodes[(int64_t)b.rc * r + k],
-                                            p.cnodes[(int64_t)b.cc * r + l]);
-                tmp += F * vr[l];
-            }
-            v += ur[k] * tmp;
-        }
-    }
-    out[t] = v;
-}
-
-inline int nblk(int64_t waves) { return (int)((waves + WPB - 1) / WPB); }
-
-template <int RMAX>
-hipError_t gen_basis_r(const DevPlan &p, hipStream_t s) {
-    if (p.ncch > 0 && p.V)
-        k_gen_basis<RMAX><<<nblk(p.ncch), WG, 0, s>>>(p.cch, p.ncch, p.cfar, p.ccl, p.cnodes, p.lam,
-                                                     p.y, p.V, p.r);
-    if (p.nrch > 0 && p.U)
-        k_gen_basis<RMAX><<<nblk(p.nrch), WG, 0, s>>>(p.rch, p.nrch, p.rfar, p.rcl, p.rnodes, p.lam,
-                                                     p.x, p.U, p.r);
-    return hipGetLastError();
-}
-
-// MODE: 0 stored, 1 hybrid, 2 free
-template <int RMAX, int MODE, int KID>
-hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
-    constexpr bool CHEB = MODE != 0;
-    constexpr bool FREE = MODE == 2;
-    auto mark = [&](int q) {
-        if (ev) (void)hipEventRecord(ev[q], s);
-    };
-    // "in" = the side summed over (columns forward), "out" = the side written
-    const DTask *in_task = tr ? p.rtask : p.ctask;
-    const int in_ntask = tr ? p.nrtask : p.nctask;
-    const DCluster *in_cl = tr ? p.rcl : p.ccl;
-    const int in_ncl = tr ? p.nrc : p.ncc;
-    const double *in_B = tr ? p.U : p.V;
-    const double2 *in_map = tr ? p.rmap : p.cmap;
-    const double *in_pts = tr ? p.x : p.y;
-    const DChunk *out_ch = tr ? p.cch : p.rch;
-    const int out_n = tr ? p.ncch : p.nrch;
-    const DNear *out_near = tr ? p.cnear : p.rnear;
-    const DFar *out_far = tr ? p.cfar : p.rfar;
-    const DCluster *out_cl = tr ? p.ccl : p.rcl;
-    const int out_ncl = tr ? p.ncc : p.nrc;
-    const double *out_B = tr ? p.V : p.U;
-    const double2 *out_map = tr ? p.cmap : p.rmap;
-    const double *out_pts = tr ? p.y : p.x;
-    const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
-
-    mark(0);
-    if (p.nfar > 0 && in_ntask > 0) {
-        k_up<RMAX, CHEB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x, a.incx,
-                                                  a.partial, p.r);
-        mark(1);
-        k_reduce<CHEB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
-        mark(2);
-        if (tr)
-            k_translate<RMAX, FREE, KID, true, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
-        else
-            k_translate<RMAX, FREE, KID, false, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
-    } else {
-        mark(1);
-        mark(2);
-    }
-    mark(3);
-    if (out_n > 0) {
-        if (tr)
-            k_down<RMAX, FREE, KID, true, CHEB><<<nblk(out_n), WG, 0, s>>>(
-                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
-        else
-            k_down<RMAX, FREE, KID, false, CHEB><<<nblk(out_n), WG, 0, s>>>(
-                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
-    }
-    mark(4);
-    return hipGetLastError();
-}
-
-template <int RMAX, int KID>
-hipError_t grad_t(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
-                  const double *sv, int64_t incs, hipStream_t s) {
-    const DTask *in_task = tr ? p.rtask : p.ctask;
-    const int in_ntask = tr ? p.nrtask : p.nctask;
-    const DCluster *in_cl = tr ? p.rcl : p.ccl;
-    const int in_ncl = tr ? p.nrc : p.ncc;
-    const double2 *in_map = tr ? p.rmap : p.cmap;
-    const double *in_pts = tr ? p.x : p.y;
-    const DChunk *out_ch = tr ? p.cch : p.rch;
-    const int out_n = tr ? p.ncch : p.nrch;
-    const DNear *out_near = tr ? p.cnear : p.rnear;
-    const DFar *out_far = tr ? p.cfar : p.rfar;
-    const DCluster *out_cl = tr ? p.ccl : p.rcl;
-    const int out_ncl = tr ? p.ncc : p.nrc;
-    const double2 *out_map = tr ? p.cmap : p.rmap;
-    const double *out_pts = tr ? p.y : p.x;
-    const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
-    if (p.nfar > 0 && in_ntask > 0) {
-        k_up<RMAX, true><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x, a.incx,
-                                                  a.partial, p.r);
-        k_reduce<true><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
-        if (tr)
-            k_translate<RMAX, true, KID, true, true><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
-        else
-            k_translate<RMAX, true, KID, false, true><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
-    }
-    if (out_n > 0) {
-        if (tr)
-            k_down_grad<RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
-                p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
-        else
-            k_down_grad<RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(
-                p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
-    }
-    return hipGetLastError();
-}
-
-template <int RMAX>
-hipError_t grad_r(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
-                  const double *sv, int64_t incs, hipStream_t s) {
-    switch (p.kid) {
-    case 0: return grad_t<RMAX, 0>(p, a, tr, y2, incy2, sv, incs, s);
-    case 1: return grad_t<RMAX, 1>(p, a, tr, y2, incy2, sv, incs, s);
-    case 2: return grad_t<RMAX, 2>(p, a, tr, y2, incy2, sv, incs, s);
-    default: return grad_t<RMAX, 3>(p, a, tr, y2, incy2, sv, incs, s);
-    }
-}
-
-template <int RMAX>
-hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, hipStream_t s,
-                    const hipEvent_t *ev) {
-    if (mode == 0) return matvec_t<RMAX, 0, 0>(p, a, tr, s, ev);
-    if (mode == 1) return matvec_t<RMAX, 1, 0>(p, a, tr, s, ev);
-    switch (p.kid) {
-    case 0: return matvec_t<RMAX, 2, 0>(p, a, tr, s, ev);
-    case 1: return matvec_t<RMAX, 2, 1>(p, a, tr, s, ev);
-    case 2: return matvec_t<RMAX, 2, 2>(p, a, tr, s, ev);
-    default: return matvec_t<RMAX, 2, 3>(p, a, tr, s, ev);
-    }
-}
-
-}  // namespace
+// kernels.hip -- public launchers of the gfx950 kernels: dispatch on the rank
+// bucket (8, 16, 24, 32, 48, 64) to the instantiations in inst_r*.hip.
+#include "kernels_impl.h"
 
 hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s) {
-    if (p.ndense > 0 && p.A) k_gen_dense<<<p.ndense, WG, 0, s>>>(p);
-    if (p.nfar > 0 && p.F) k_gen_F<<<p.nfar, WG, 0, s>>>(p);
+    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<<<p.ndense, hmxk::WG, 0, s>>>(p);
+    if (p.nfar > 0 && p.F) hmxk::k_gen_F<<<p.nfar, hmxk::WG, 0, s>>>(p);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || (!p.U && !p.V)) return err;
-    if (p.r <= 8) return gen_basis_r<8>(p, s);
-    if (p.r <= 16) return gen_basis_r<16>(p, s);
-    if (p.r <= 24) return gen_basis_r<24>(p, s);
-    if (p.r <= 32) return gen_basis_r<32>(p, s);
-    if (p.r <= 48) return gen_basis_r<48>(p, s);
-    return gen_basis_r<64>(p, s);
+    if (p.r <= 8) return hmx_gen_basis_r8(p, s);
+    if (p.r <= 16) return hmx_gen_basis_r16(p, s);
+    if (p.r <= 24) return hmx_gen_basis_r24(p, s);
+    if (p.r <= 32) return hmx_gen_basis_r32(p, s);
+    if (p.r <= 48) return hmx_gen_basis_r48(p, s);
+    return hmx_gen_basis_r64(p, s);
 }
 
-hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool tr, int mode,
+hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb,
                              hipStream_t s, const hipEvent_t *ev) {
-    if (p.r <= 8) return matvec_r<8>(p, a, tr, mode, s, ev);
-    if (p.r <= 16) return matvec_r<16>(p, a, tr, mode, s, ev);
-    if (p.r <= 24) return matvec_r<24>(p, a, tr, mode, s, ev);
-    if (p.r <= 32) return matvec_r<32>(p, a, tr, mode, s, ev);
-    if (p.r <= 48) return matvec_r<48>(p, a, tr, mode, s, ev);
-    return matvec_r<64>(p, a, tr, mode, s, ev);
+    if (p.r <= 8) return hmx_matvec_r8(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 16) return hmx_matvec_r16(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 24) return hmx_matvec_r24(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 32) return hmx_matvec_r32(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 48) return hmx_matvec_r48(p, a, tr, mode, kb, s, ev);
+    return hmx_matvec_r64(p, a, tr, mode, kb, s, ev);
 }
 
 hipError_t hmx_launch_grad(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
                            const double *sv, int64_t incs, hipStream_t s) {
-    if (p.r <= 8) return grad_r<8>(p, a, tr, y2, incy2, sv, incs, s);
-    if (p.r <= 16) return grad_r<16>(p, a, tr, y2, incy2, sv, incs, s);
-    if (p.r <= 24) return grad_r<24>(p, a, tr, y2, incy2, sv, incs, s);
-    if (p.r <= 32) return grad_r<32>(p, a, tr, y2, incy2, sv, incs, s);
-    if (p.r <= 48) return grad_r<48>(p, a, tr, y2, incy2, sv, incs, s);
-    return grad_r<64>(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 8) return hmx_grad_r8(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 16) return hmx_grad_r16(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 24) return hmx_grad_r24(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 32) return hmx_grad_r32(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 48) return hmx_grad_r48(p, a, tr, y2, incy2, sv, incs, s);
+    return hmx_grad_r64(p, a, tr, y2, incy2, sv, incs, s);
 }
 
 hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, const int64_t *j,
                               const int32_t *kind, const int32_t *id, double *out,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    k_entries<<<(int)((n + WG - 1) / WG), WG, 0, s>>>(p, n, i, j, kind, id, out);
+    hmxk::k_entries<<<(int)((n + hmxk::WG - 1) / hmxk::WG), hmxk::WG, 0, s>>>(p, n, i, j, kind, id, out);
     return hipGetLastError();
 }
 
@@ -782,8 +47,8 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
                                       hipStream_t s) {
     // caller provides 1024 slots
     if (n <= 0) return hipSuccess;
-    int64_t nb = (n + WG - 1) / WG;
+    int64_t nb = (n + hmxk::WG - 1) / hmxk::WG;
     if (nb > 1024) nb = 1024;
-    k_count_nonfinite<<<(int)nb, WG, 0, s>>>(v, n, blockcount);
+    hmxk::k_count_nonfinite<<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
     return hipGetLastError();
 }

@@ -348,6 +348,7 @@ struct hm_plan {
     std::vector<DCluster> rcl, ccl;
     std::vector<DFarBlk> far;
     std::vector<int32_t> far_by_rc_h;
+    std::vector<int64_t> row_off, col_off;  // instance offsets (count + 1 each)
     Side rows, cols;
     int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
     int maxdepth = 0;
@@ -368,6 +369,8 @@ struct hm_plan {
     double *d_A = nullptr, *d_U = nullptr, *d_V = nullptr, *d_F = nullptr;
     double *d_partial = nullptr, *d_w = nullptr, *d_z = nullptr;
     bool profile = false;
+    int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
+    int tuned_kb = HM_MAX_KB, tuned_mode = -1;  // hm_plan_tune's choice
     std::vector<hipEvent_t> prof_events;  // 5 per product
 
     void drop_events() {
@@ -436,31 +439,82 @@ int hm_indsplit(const double *x, int64_t nx, int64_t first, int64_t len, double 
     return HM_OK;
 }
 
-int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double *y, int64_t ny,
-                        double a, double b, double c, double d, int blocksize, int rank,
-                        unsigned flags, void *stream, hm_plan_t *out) {
+struct Inst {
+    const double *x, *y;
+    int64_t nx, ny;
+    double a, b, c, d;
+};
+
+// Builds every instance's tree, then merges them into one forest over the
+// concatenated index space (instance t owns rows [rows_t, rows_t + nx_t) and
+// columns [cols_t, cols_t + ny_t)); blocks never cross instances, so every
+// pass of a product covers all instances in one launch.
+static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksize, int rank,
+                     unsigned flags, void *stream, hm_plan_t *out) {
     if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
     *out = nullptr;
     if (kernel_id < HM_CAUCHY || kernel_id > HM_LOG) return fail(HM_ERR_INVALID, "unknown kernel id %d", kernel_id);
-    if (!x || !y || nx < 1 || ny < 1)
-        return fail(HM_ERR_DEGENERATE, "KernelMatrix needs non-empty x and y (the reference indexes x[1])");
-    if (nx >= (1LL << 40) || ny >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many points");
+    if (count < 1) return fail(HM_ERR_INVALID, "no instances");
     if (blocksize < 1) return fail(HM_ERR_INVALID, "blocksize must be >= 1");
     if (rank < 1 || rank > 64) return fail(HM_ERR_UNSUPPORTED, "rank %d outside 1..64", rank);
     if (flags & ~15u) return fail(HM_ERR_INVALID, "unknown plan flags 0x%x", flags);
     if ((flags & HM_PLAN_MATRIX_FREE) && (flags & HM_PLAN_HYBRID))
         return fail(HM_ERR_INVALID, "HM_PLAN_MATRIX_FREE and HM_PLAN_HYBRID exclude each other");
     if (flags & HM_PLAN_FAR_F32) return fail(HM_ERR_UNSUPPORTED, "FP32 far-field plans are not built by this entry point");
-    int64_t bad;
-    if (!non_increasing(x, nx, &bad))
-        return fail(HM_ERR_NOT_DESCENDING, "x must be finite and non-increasing (index %lld)", (long long)bad);
-    if (!non_increasing(y, ny, &bad))
-        return fail(HM_ERR_NOT_DESCENDING, "y must be finite and non-increasing (index %lld)", (long long)bad);
-
+    int64_t NX = 0, NY = 0;
+    for (int64_t t = 0; t < count; ++t) {
+        const Inst &I = inst[t];
+        if (!I.x || !I.y || I.nx < 1 || I.ny < 1)
+            return fail(HM_ERR_DEGENERATE, "instance %lld: KernelMatrix needs non-empty x and y (the reference indexes x[1])", (long long)t);
+        int64_t bad;
+        if (!non_increasing(I.x, I.nx, &bad))
+            return fail(HM_ERR_NOT_DESCENDING, "instance %lld: x must be finite and non-increasing (index %lld)", (long long)t, (long long)bad);
+        if (!non_increasing(I.y, I.ny, &bad))
+            return fail(HM_ERR_NOT_DESCENDING, "instance %lld: y must be finite and non-increasing (index %lld)", (long long)t, (long long)bad);
+        NX += I.nx;
+        NY += I.ny;
+    }
+    if (NX >= (1LL << 40) || NY >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many points");
+    const double *x = nullptr, *y = nullptr;
+    std::vector<double> xcat, ycat;
+    if (count == 1) {
+        x = inst[0].x;
+        y = inst[0].y;
+    } else {
+        xcat.reserve(NX);
+        ycat.reserve(NY);
+        for (int64_t t = 0; t < count; ++t) {
+            xcat.insert(xcat.end(), inst[t].x, inst[t].x + inst[t].nx);
+            ycat.insert(ycat.end(), inst[t].y, inst[t].y + inst[t].ny);
+        }
+        x = xcat.data();
+        y = ycat.data();
+    }
+    const int64_t nx = NX, ny = NY;
     Builder B{};
-    B.x = x; B.y = y; B.nx = nx; B.ny = ny; B.bs = blocksize; B.r = rank;
-    B.node(0, 0, 0, nx, 0, ny, a, b, c, d);
-    if (B.err) return fail(B.err, "%s", B.msg.c_str());
+    std::vector<int64_t> row_off(count + 1, 0), col_off(count + 1, 0);
+    for (int64_t t = 0; t < count; ++t) {
+        const Inst &I = inst[t];
+        Builder Bt{};
+        Bt.x = I.x; Bt.y = I.y; Bt.nx = I.nx; Bt.ny = I.ny; Bt.bs = blocksize; Bt.r = rank;
+        Bt.node(0, 0, 0, I.nx, 0, I.ny, I.a, I.b, I.c, I.d);
+        if (Bt.err) {
+            return count == 1 ? fail(Bt.err, "%s", Bt.msg.c_str())
+                              : fail(Bt.err, "instance %lld: %s", (long long)t, Bt.msg.c_str());
+        }
+        const int64_t ro = row_off[t], co = col_off[t];
+        row_off[t + 1] = ro + I.nx;
+        col_off[t + 1] = co + I.ny;
+        const int32_t rb = (int32_t)B.rcl.size(), cb = (int32_t)B.ccl.size();
+        for (Leaf L : Bt.leaves) { L.row0 += ro; L.col0 += co; B.leaves.push_back(L); }
+        for (DDense D : Bt.dense) { D.row0 += ro; D.col0 += co; B.dense.push_back(D); }
+        for (HCluster h : Bt.rcl) { h.first += ro; B.rcl.push_back(h); }
+        for (HCluster h : Bt.ccl) { h.first += co; B.ccl.push_back(h); }
+        for (DFarBlk f : Bt.far) { f.rc += rb; f.cc += cb; B.far.push_back(f); }
+        B.nd += Bt.nd; B.nk1 += Bt.nk1; B.nk2 += Bt.nk2;
+        B.far_rows += Bt.far_rows; B.far_cols += Bt.far_cols; B.dense_entries += Bt.dense_entries;
+        B.maxdepth = std::max(B.maxdepth, Bt.maxdepth);
+    }
 
     hm_plan *P = new hm_plan();
     P->kid = kernel_id; P->r = rank; P->bs = blocksize; P->flags = flags;
@@ -468,6 +522,8 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     P->nd = B.nd; P->nk1 = B.nk1; P->nk2 = B.nk2; P->maxdepth = B.maxdepth;
     P->far_rows = B.far_rows; P->far_cols = B.far_cols; P->dense_entries = B.dense_entries;
     P->leaves = std::move(B.leaves);
+    P->row_off = std::move(row_off);
+    P->col_off = std::move(col_off);
     P->dense = std::move(B.dense);
     P->far = std::move(B.far);
     const int r = rank;
@@ -581,9 +637,9 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     HM_TRY(upload(&P->d_ctask, P->cols.tasks));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&P->d_partial, nslots * r));
-    HM_TRY(dalloc(&P->d_w, ncl * r));
-    HM_TRY(dalloc(&P->d_z, ncl * r));
+    HM_TRY(dalloc(&P->d_partial, nslots * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_w, ncl * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_z, ncl * r * HM_MAX_KB));
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
     const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
     if (stored) {
@@ -660,6 +716,35 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     return HM_OK;
 }
 
+int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double *y, int64_t ny,
+                        double a, double b, double c, double d, int blocksize, int rank,
+                        unsigned flags, void *stream, hm_plan_t *out) {
+    const Inst I{x, y, nx, ny, a, b, c, d};
+    return make_plan(kernel_id, &I, 1, blocksize, rank, flags, stream, out);
+}
+
+int hm_plan_batched(int kernel_id, int64_t count, const double *const *x, const int64_t *nx,
+                    const double *const *y, const int64_t *ny, const double *intervals,
+                    int blocksize, int rank, unsigned flags, void *stream, hm_plan_t *out) {
+    if (out) *out = nullptr;
+    if (count < 1 || !x || !nx || !y || !ny || !intervals)
+        return fail(HM_ERR_INVALID, "plan_batched: bad arguments");
+    std::vector<Inst> inst(count);
+    for (int64_t t = 0; t < count; ++t)
+        inst[t] = Inst{x[t], y[t], nx[t], ny[t], intervals[4 * t], intervals[4 * t + 1],
+                       intervals[4 * t + 2], intervals[4 * t + 3]};
+    return make_plan(kernel_id, inst.data(), count, blocksize, rank, flags, stream, out);
+}
+
+int hm_plan_batch_offsets(hm_plan_t P, int64_t *row_offsets, int64_t *col_offsets) {
+    if (!P) return fail(HM_ERR_INVALID, "batch_offsets: NULL plan");
+    for (size_t t = 0; t < P->row_off.size(); ++t) {
+        if (row_offsets) row_offsets[t] = P->row_off[t];
+        if (col_offsets) col_offsets[t] = P->col_off[t];
+    }
+    return HM_OK;
+}
+
 int hm_plan_destroy(hm_plan_t plan) {
     if (!plan) return HM_OK;
     if (!(plan->flags & HM_PLAN_TREE_ONLY)) (void)hipDeviceSynchronize();
@@ -704,6 +789,7 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     v[HM_INFO_N_ROWCHUNKS] = (int64_t)P->rows.ch.size();
     v[HM_INFO_N_COLCHUNKS] = (int64_t)P->cols.ch.size();
     v[HM_INFO_NONFINITE] = P->nonfinite;
+    v[HM_INFO_N_INSTANCES] = (int64_t)P->row_off.size() - 1;
     for (int k = 0; k < n && k < HM_INFO_COUNT; ++k) info[k] = v[k];
     for (int k = HM_INFO_COUNT; k < n; ++k) info[k] = 0;
     return HM_OK;
@@ -733,20 +819,31 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec: tree-only plan has no device data");
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
-    if (flags & ~3u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (flags & ~7u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
     if (k == 0) return HM_OK;
     if (!y || !x) return fail(HM_ERR_INVALID, "matvec: NULL vector");
     if (k > 1 && (ldx < 1 || ldy < 1)) return fail(HM_ERR_INVALID, "matvec: bad ld");
     int mode = 0;
     if ((P->flags & HM_PLAN_MATRIX_FREE) || (flags & HM_MV_MATRIX_FREE)) mode = 2;
     else if ((P->flags & HM_PLAN_HYBRID) || (flags & HM_MV_CHEB_BASIS)) mode = 1;
+    int max_kb = P->max_kb;
+    if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0) {
+        mode = std::max(mode, P->tuned_mode);
+        max_kb = P->tuned_kb;
+    }
     hipStream_t s = (hipStream_t)stream;
-    for (int64_t q = 0; q < k; ++q) {
+    for (int64_t q = 0; q < k;) {
+        // forward products take up to HM_MAX_KB right-hand sides per launch
+        int kb = 1;
+        if (!tr && max_kb > 1)
+            while (kb * 2 <= max_kb && q + kb * 2 <= k) kb *= 2;
         MvArgs a{};
         a.y = y + ioff + q * ldy;
         a.x = x + joff + q * ldx;
         a.incy = incy;
         a.incx = incx;
+        a.ldy = ldy;
+        a.ldx = ldx;
         a.beta = beta;
         a.partial = P->d_partial;
         a.w = P->d_w;
@@ -754,15 +851,16 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
         const hipEvent_t *ev = nullptr;
         if (P->profile) {
             const size_t at = P->prof_events.size();
-            for (int q = 0; q < 5; ++q) {
+            for (int e5 = 0; e5 < 5; ++e5) {
                 hipEvent_t h;
                 HM_HIP(hipEventCreate(&h));
                 P->prof_events.push_back(h);
             }
             ev = P->prof_events.data() + at;
         }
-        hipError_t e = hmx_launch_matvec(P->dp, a, tr, mode, s, ev);
+        hipError_t e = hmx_launch_matvec(P->dp, a, tr, mode, kb, s, ev);
         if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
+        q += kb;
     }
     return HM_OK;
 }
@@ -819,6 +917,103 @@ int hm_matvec_t_grad(hm_plan_t P, double *gb, double *gy, const double *r, const
     a.beta = beta;
     hipError_t e = hmx_launch_grad(P->dp, a, true, gy, incgy, s, incs, (hipStream_t)stream);
     if (e != hipSuccess) return fail(HM_ERR_HIP, "t_grad launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
+struct hm_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int hm_graph_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
+                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                    unsigned flags, hm_graph_t *out) {
+    if (!out) return fail(HM_ERR_INVALID, "graph: NULL output");
+    *out = nullptr;
+    if (!P || (P->flags & HM_PLAN_TREE_ONLY)) return fail(HM_ERR_INVALID, "graph: bad plan");
+    if (P->profile) return fail(HM_ERR_INVALID, "graph: profiling is on (it records events)");
+    hipStream_t s;
+    HM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return fail(HM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    const int rc = matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, s, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    (void)hipStreamDestroy(s);
+    if (rc != HM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    hm_graph *G = new hm_graph();
+    G->graph = g;
+    e = hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete G;
+        return fail(HM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    *out = G;
+    return HM_OK;
+}
+
+int hm_graph_launch(hm_graph_t G, void *stream) {
+    if (!G) return fail(HM_ERR_INVALID, "graph_launch: NULL graph");
+    HM_HIP(hipGraphLaunch(G->exec, (hipStream_t)stream));
+    return HM_OK;
+}
+
+int hm_graph_destroy(hm_graph_t G) {
+    if (!G) return HM_OK;
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+    delete G;
+    return HM_OK;
+}
+
+int hm_plan_tune(hm_plan_t P, double *y, const double *x, int64_t k, void *stream, int64_t out[3]) {
+    if (!P || !y || !x || k < 1 || (P->flags & HM_PLAN_TREE_ONLY))
+        return fail(HM_ERR_INVALID, "tune: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HM_HIP(hipEventCreate(&e0));
+    HM_HIP(hipEventCreate(&e1));
+    const int base_mode = (P->flags & HM_PLAN_MATRIX_FREE) ? 2 : (P->flags & HM_PLAN_HYBRID) ? 1 : 0;
+    float best = 1e30f;
+    int best_kb = 1, best_mode = base_mode;
+    const int saved_kb = P->max_kb;
+    for (int mode = base_mode; mode <= 2; ++mode) {
+        const unsigned fl = mode == 2 ? HM_MV_MATRIX_FREE : mode == 1 ? HM_MV_CHEB_BASIS : 0;
+        for (int kb = 1; kb <= HM_MAX_KB; kb *= 2) {
+            if (kb > 1 && kb > k) break;
+            P->max_kb = kb;
+            int rc = matvec_common(P, y, x, 0, 0, 1, 1, k, P->nx, P->ny, 0, fl, s, false);
+            if (rc != HM_OK) { P->max_kb = saved_kb; return rc; }
+            float ms = 0.f;
+            HM_HIP(hipEventRecord(e0, s));
+            for (int rep = 0; rep < 3; ++rep) {
+                rc = matvec_common(P, y, x, 0, 0, 1, 1, k, P->nx, P->ny, 0, fl, s, false);
+                if (rc != HM_OK) { P->max_kb = saved_kb; return rc; }
+            }
+            HM_HIP(hipEventRecord(e1, s));
+            HM_HIP(hipEventSynchronize(e1));
+            HM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_kb = kb; best_mode = mode; }
+        }
+    }
+    P->max_kb = saved_kb;
+    P->tuned_kb = best_kb;
+    P->tuned_mode = best_mode;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (out) {
+        out[0] = best_kb;
+        out[1] = best_mode;
+        out[2] = (int64_t)(1000.0f * best / 3.0f);
+    }
     return HM_OK;
 }
 

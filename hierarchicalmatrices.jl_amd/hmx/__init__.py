@@ -5,7 +5,7 @@ of libhmx.so (include/hmx.h). Importing fails if libhmx.so is not built.
 """
 from ._lib import HMError, lib  # noqa: F401  (raises ImportError if libhmx.so is missing)
 from .kernelmatrix import (  # noqa: F401
-    BLOCKRANK, BLOCKSIZE, Block, KernelMatrix, KERNELS, NamedKernel, blocksize,
+    BLOCKRANK, BLOCKSIZE, Block, KernelMatrix, KernelMatrixBatch, KERNELS, NamedKernel, blocksize,
     cauchykernel, chebyshevbarycentricweights, chebyshevpoints, coulombkernel,
     coulombprimekernel, indsplit, logkernel, mul_, size,
 )

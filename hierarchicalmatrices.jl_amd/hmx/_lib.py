@@ -24,11 +24,13 @@ HM_PLAN_TREE_ONLY = 4
 HM_PLAN_HYBRID = 8
 HM_MV_MATRIX_FREE = 1
 HM_MV_CHEB_BASIS = 2
+HM_MV_TUNED = 4
 
 INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
     "n_rowclusters", "n_colclusters", "depth", "dense_entries", "basis_rows_u", "basis_rows_v",
     "bytes_stored", "bytes_ref_layout", "flags", "n_rowchunks", "n_colchunks", "nonfinite",
+    "n_instances",
 ]
 
 _vp = ctypes.c_void_p
@@ -76,6 +78,12 @@ def _load():
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint, _vp,
                                       ctypes.POINTER(_vp)]
+    L.hm_plan_batched.restype = ctypes.c_int
+    L.hm_plan_batched.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(_dp), _ip,
+                                  ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_uint, _vp, ctypes.POINTER(_vp)]
+    L.hm_plan_batch_offsets.restype = ctypes.c_int
+    L.hm_plan_batch_offsets.argtypes = [_vp, _ip, _ip]
     L.hm_plan_destroy.restype = ctypes.c_int
     L.hm_plan_destroy.argtypes = [_vp]
     L.hm_plan_info.restype = ctypes.c_int
@@ -93,6 +101,15 @@ def _load():
     L.hm_matvec_t_grad.restype = ctypes.c_int
     L.hm_matvec_t_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int,
                                    _vp]
+    L.hm_graph_matvec.restype = ctypes.c_int
+    L.hm_graph_matvec.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                  ctypes.c_int, ctypes.c_uint, ctypes.POINTER(_vp)]
+    L.hm_graph_launch.restype = ctypes.c_int
+    L.hm_graph_launch.argtypes = [_vp, _vp]
+    L.hm_graph_destroy.restype = ctypes.c_int
+    L.hm_graph_destroy.argtypes = [_vp]
+    L.hm_plan_tune.restype = ctypes.c_int
+    L.hm_plan_tune.argtypes = [_vp, _vp, _vp, _i64, _vp, _ip]
     L.hm_plan_profile.restype = ctypes.c_int
     L.hm_plan_profile.argtypes = [_vp, ctypes.c_int]
     L.hm_plan_profile_read.restype = ctypes.c_int

@@ -232,7 +232,7 @@ class KernelMatrix:
 
     # -- products
     def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream,
-            cheb_basis=False):
+            cheb_basis=False, tuned=False):
         U = _Vec(u, True)
         V = _Vec(v, False)
         if U.k != V.k:
@@ -249,7 +249,7 @@ class KernelMatrix:
             raise ValueError("vector too short for H at the given offset and stride")
         fn = lib.hm_matvec_t if transpose else lib.hm_matvec
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
-            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0)
+            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0)
         check(fn(self._h, U.ptr(), V.ptr(), ioff * U.inc, joff * V.inc, incy, incx, U.k,
                  U.ld, V.ld, beta, flags, _stream_ptr(stream)))
         U.writeback()
@@ -263,14 +263,15 @@ class KernelMatrix:
         """u += H^T v (accumulating)."""
         return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, True, stream)
 
-    def matvec(self, v, *, matrix_free=False, transpose=False, cheb_basis=False, stream=None):
+    def matvec(self, v, *, matrix_free=False, transpose=False, cheb_basis=False, tuned=False,
+               stream=None):
         """H v (or H^T v) into a new vector on the device of v."""
         torch = _torch()
         ishost = not isinstance(v, torch.Tensor)
         vt = torch.from_numpy(np.asarray(v, dtype=np.float64)).cuda() if ishost else v
         m = self.shape[1] if transpose else self.shape[0]
         out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=torch.float64, device=vt.device)
-        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis)
+        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis, tuned)
         if ishost:
             torch.cuda.synchronize()
             return out.cpu().numpy()
@@ -296,6 +297,32 @@ class KernelMatrix:
                                    0, _stream_ptr(stream)))
         return gb, gy
 
+    def graph(self, out, v, *, beta=0, matrix_free=False, cheb_basis=False, tuned=False):
+        """Capture `out = beta*out + H v` (device tensors, 1-D or (n, k)) into an
+        executable HIP graph; returns a ProductGraph whose launch() replays it."""
+        U, V = _Vec(out, True), _Vec(v, False)
+        if U.host is not None or V.host is not None:
+            raise TypeError("graphs need device tensors")
+        flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
+            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0)
+        g = ctypes.c_void_p()
+        check(lib.hm_graph_matvec(self._h, U.ptr(), V.ptr(), 0, 0, U.inc, V.inc, U.k, U.ld, V.ld,
+                                  beta, flags, ctypes.byref(g)))
+        return ProductGraph(g, (out, v, self))
+
+    def tune(self, k=1, stream=None):
+        """Autotune launch shape and far-field form for k right-hand sides;
+        returns (rhs per launch, mode, microseconds). Use tuned=True later."""
+        torch = _torch()
+        m, n = self.shape
+        y = torch.empty(m * k, dtype=torch.float64, device="cuda")
+        x = torch.ones(n * k, dtype=torch.float64, device="cuda")
+        out = np.zeros(3, dtype=np.int64)
+        check(lib.hm_plan_tune(self._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                               k, _stream_ptr(stream),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return tuple(int(v) for v in out)
+
     def __matmul__(self, v):
         return self.matvec(v)
 
@@ -319,6 +346,69 @@ class KernelMatrix:
                              _stream_ptr(None)))
         res = out.cpu().numpy()
         return float(res.reshape(-1)[0]) if np.ndim(i) == 0 and np.ndim(j) == 0 else res
+
+
+class ProductGraph:
+    """An executable HIP graph of one product (KernelMatrix.graph)."""
+
+    def __init__(self, handle, keep):
+        self._g = handle
+        self._keep = keep  # the captured buffers and plan must outlive the graph
+
+    def launch(self, stream=None):
+        check(lib.hm_graph_launch(self._g, _stream_ptr(stream)))
+
+    def close(self):
+        if self._g and self._g.value:
+            lib.hm_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KernelMatrixBatch(KernelMatrix):
+    """Many independent KernelMatrix instances (same kernel, blocksize, rank)
+    in one grouped plan (hm_plan_batched): every pass of a product is one
+    launch over all instances. Vectors are the instances' vectors
+    concatenated; `row_offsets` / `col_offsets` give the instance boundaries.
+    `instances` is a sequence of (x, y, (a, b, c, d))."""
+
+    def __init__(self, f, instances, *, blocksize=None, rank=None, matrix_free=False,
+                 hybrid=False, tree_only=False, stream=None):
+        if isinstance(f, str):
+            f = KERNELS[f]
+        self.f = f
+        self.rank = BLOCKRANK(np.float64) if rank is None else int(rank)
+        self.blocksize_ = 4 * self.rank if blocksize is None else int(blocksize)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x, _, _ in instances]
+        ys = [np.ascontiguousarray(y, dtype=np.float64) for _, y, _ in instances]
+        iv = np.ascontiguousarray([list(map(float, v)) for _, _, v in instances], dtype=np.float64)
+        cnt = len(xs)
+        DP = ctypes.POINTER(ctypes.c_double)
+        xp = (DP * cnt)(*[_dptr(a) for a in xs])
+        yp = (DP * cnt)(*[_dptr(a) for a in ys])
+        nx = np.array([len(a) for a in xs], dtype=np.int64)
+        ny = np.array([len(a) for a in ys], dtype=np.int64)
+        flags = (_lib.HM_PLAN_MATRIX_FREE if matrix_free else 0) | \
+            (_lib.HM_PLAN_HYBRID if hybrid else 0) | (_lib.HM_PLAN_TREE_ONLY if tree_only else 0)
+        self.flags = flags
+        h = ctypes.c_void_p()
+        sp = ctypes.c_void_p(0) if tree_only else _stream_ptr(stream)
+        IP = ctypes.POINTER(ctypes.c_int64)
+        check(lib.hm_plan_batched(f.kid, cnt, xp, nx.ctypes.data_as(IP), yp, ny.ctypes.data_as(IP),
+                                  _dptr(iv), self.blocksize_, self.rank, flags, sp,
+                                  ctypes.byref(h)))
+        self._h = h
+        self.x = np.concatenate(xs)
+        self.y = np.concatenate(ys)
+        ro = np.zeros(cnt + 1, dtype=np.int64)
+        co = np.zeros(cnt + 1, dtype=np.int64)
+        check(lib.hm_plan_batch_offsets(h, ro.ctypes.data_as(IP), co.ctypes.data_as(IP)))
+        self.row_offsets, self.col_offsets = ro, co
 
 
 class _Transposed:

@@ -56,8 +56,9 @@ enum hm_plan_flags {
 enum hm_mv_flags {
     HM_MV_DEFAULT = 0,
     HM_MV_MATRIX_FREE = 1,       /* regenerate factors even if the plan stores them */
-    HM_MV_CHEB_BASIS = 2         /* apply the far-field basis in Chebyshev form even if
+    HM_MV_CHEB_BASIS = 2,        /* apply the far-field basis in Chebyshev form even if
                                     the plan stores U and V */
+    HM_MV_TUNED = 4              /* use the launch shape and mode chosen by hm_plan_tune */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -98,6 +99,22 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
                         double a, double b, double c, double d, int blocksize, int rank,
                         unsigned flags, void *stream, hm_plan_t *plan);
 
+/* Many independent KernelMatrix instances in one plan (no counterpart in the
+ * reference, which builds one tree per call of src/KernelMatrix.jl:47).
+ * Instance t has points x[t] (nx[t]), y[t] (ny[t]) and intervals
+ * intervals[4t..4t+3] = (a, b, c, d); kernel, blocksize and rank are shared.
+ * The trees are merged into one forest over the concatenated index space, so
+ * every pass of hm_matvec is one grouped launch over all instances: the
+ * product's vectors are the instances' vectors concatenated (offsets from
+ * hm_plan_batch_offsets). */
+int hm_plan_batched(int kernel_id, int64_t count, const double *const *x, const int64_t *nx,
+                    const double *const *y, const int64_t *ny, const double *intervals,
+                    int blocksize, int rank, unsigned flags, void *stream, hm_plan_t *plan);
+
+/* row_offsets[0..count] and col_offsets[0..count] of a batched plan
+ * (instance t owns rows [row_offsets[t], row_offsets[t+1])). */
+int hm_plan_batch_offsets(hm_plan_t plan, int64_t *row_offsets, int64_t *col_offsets);
+
 /* Frees every device buffer of the plan (synchronises the device first). */
 int hm_plan_destroy(hm_plan_t plan);
 
@@ -111,6 +128,7 @@ enum hm_info {
     HM_INFO_BYTES_STORED,      /* bytes of every stored array read by one product */
     HM_INFO_BYTES_REF_LAYOUT,  /* same arrays in the reference's per-block layout */
     HM_INFO_FLAGS, HM_INFO_N_ROWCHUNKS, HM_INFO_N_COLCHUNKS, HM_INFO_NONFINITE,
+    HM_INFO_N_INSTANCES,
     HM_INFO_COUNT
 };
 int hm_plan_info(hm_plan_t plan, int64_t *info, int n);
@@ -153,6 +171,30 @@ int hm_matvec_grad_x(hm_plan_t plan, double *g, const double *b, const double *s
 int hm_matvec_t_grad(hm_plan_t plan, double *gb, double *gy, const double *r, const double *s,
                      int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
                      void *stream);
+
+/* Executable HIP graph of one hm_matvec call (no counterpart in the
+ * reference): the launches of hm_matvec(plan, y, x, ...) with these exact
+ * arguments are captured once and replayed by hm_graph_launch on any stream.
+ * The graph holds only kernel nodes (no memset or copy nodes); the plan must
+ * outlive it, and replays on one plan must not overlap (they share the
+ * plan's workspace). */
+typedef struct hm_graph *hm_graph_t;
+int hm_graph_matvec(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_t joff,
+                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                    unsigned flags, hm_graph_t *graph);
+int hm_graph_launch(hm_graph_t graph, void *stream);
+int hm_graph_destroy(hm_graph_t graph);
+
+/* Autotuner (no counterpart in the reference): times hm_matvec on this plan
+ * for k right-hand sides (device buffers y, x as in hm_matvec with unit
+ * stride and ld = n) over the candidate launch shapes (right-hand sides per
+ * launch 1, 2, 4, 8) and far-field forms (the plan's own, Chebyshev basis,
+ * matrix-free), keeps the fastest for later products that pass HM_MV_TUNED,
+ * and writes its choice to out[0] = right-hand sides per launch, out[1] =
+ * mode (0 stored, 1 Chebyshev basis, 2 matrix-free), out[2] = its time in
+ * microseconds. Every candidate computes the same product up to rounding. */
+int hm_plan_tune(hm_plan_t plan, double *y, const double *x, int64_t k, void *stream,
+                 int64_t out[3]);
 
 /* Per-pass timing of the products of this plan (bench support, no counterpart
  * in the reference). enable=1 starts recording HIP events around the four

@@ -141,9 +141,12 @@ class Tree:
         self.store = store
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_tree_free(self.h)
-            self.h = None
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.or_tree_free(self.h)
+        except Exception:
+            pass
+        self.h = None
 
     def stats(self):
         out = np.zeros(9, dtype=np.int64)

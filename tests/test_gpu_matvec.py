@@ -142,8 +142,8 @@ def test_multiple_rhs_columns(cuda):
     H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
     B = torch.from_numpy(hmx.workloads.normal(9, 2048 * 5).reshape(2048, 5)).to(cuda)
     Y = H @ B
-    for q in range(5):
-        assert torch.equal(Y[:, q], H @ B[:, q].contiguous())
+    for q in range(5):  # several right-hand sides per launch: same values up to rounding
+        assert rel(Y[:, q].cpu().numpy(), (H @ B[:, q].contiguous()).cpu().numpy()) <= 1e-14
 
 
 def test_transpose(cuda):
