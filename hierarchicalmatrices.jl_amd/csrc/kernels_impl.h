@@ -286,38 +286,40 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
 }
 
 // w[(c, q)] = sum over the task slots of cluster c; in Chebyshev form the
-// slots hold moments and w = A mu. One wavefront per (cluster, rhs).
+// slots hold moments and w = A mu. One wavefront per cluster: lane = g*r + m
+// handles row m of right-hand sides q = g, g + G, ... (G = 64 / r groups).
 template <bool CHEB, int KB>
 __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const double *partial,
                                               const double *chebA, double *w, int r) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    if (wid >= ncl * KB) return;
-    const int c = wid / KB, q = wid % KB;
-    const DCluster C = cl[c];
-    const int G = WAVE / r;  // lane groups; lane = g*r + k
-    double acc = 0.0;
-    if (lane < G * r) {
-        const int k = lane % r, g = lane / r;
-        for (int t = g; t < C.nslots; t += G) acc += partial[((C.pbase + t) * KB + q) * r + k];
-    }
-    double tot = acc;
-    for (int g = 1; g < G; ++g) tot += __shfl(acc, (lane + g * r) & (WAVE - 1));
-    if constexpr (CHEB) {
-        double v = 0.0;
-        for (int m = 0; m < r; ++m) {
-            const double mu = __shfl(tot, m);
-            if (lane < r) v = __builtin_fma(chebA[lane * r + m], mu, v);
+    if (wid >= ncl) return;
+    const DCluster C = cl[wid];
+    const int G = WAVE / r;
+    const int m = lane % r, g = lane / r;
+    const bool act = g < G;
+    for (int q0 = 0; q0 < KB; q0 += G) {
+        const int q = q0 + g;
+        const bool aq = act && q < KB;
+        double acc = 0.0;
+        if (aq)
+            for (int t = 0; t < C.nslots; ++t) acc += partial[((C.pbase + t) * KB + q) * r + m];
+        if constexpr (CHEB) {
+            double v = 0.0;
+            for (int k = 0; k < r; ++k) {
+                const double mu = __shfl(acc, (g * r + k) & (WAVE - 1));
+                v = __builtin_fma(chebA[m * r + k], mu, v);
+            }
+            acc = v;
         }
-        tot = v;
+        if (aq) w[((int64_t)wid * KB + q) * r + m] = acc;
     }
-    if (lane < r) w[((int64_t)c * KB + q) * r + lane] = tot;
 }
 
 // z[(d, q)] = sum over far blocks b of cluster d of F_b w[(other side, q)]
 // (forward: F w[cc]; transpose: F^T w[rc]); in Chebyshev form z is then
-// mapped to coefficients c = A^T z. Lane = row of F; F is read once per block
-// and applied to all KB right-hand sides.
+// mapped to coefficients c = A^T z. One wavefront per cluster: lane = g*r + m
+// owns row m of F for right-hand sides q = g, g + G, ...
 template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
 __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl, int ncl,
                                                  const int32_t *far_list, const double *w,
@@ -327,53 +329,50 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
     if (wid >= ncl) return;
     const int r = p.r;
     const DCluster C = cl[wid];
-    const bool act = lane < r;
-    const int la = act ? lane : 0;
-    double acc[KB];
+    const int G = WAVE / r;
+    const int m = lane % r, g = lane / r;
+    const bool act = g < G;
+    for (int q0 = 0; q0 < KB; q0 += G) {
+        const int q = q0 + g;
+        const bool aq = act && q < KB;
+        const int qa = aq ? q : 0;
+        double acc = 0.0;
+        for (int e = C.far_beg; e < C.far_end; ++e) {
+            const DFarBlk b = p.far[far_list[e]];
+            const int src = TRANS ? b.rc : b.cc;
+            const double *ws = w + ((int64_t)src * KB + qa) * r;
+            double fv[RMAX];
+            if constexpr (!FREE) {
+                const double *F = p.F + b.foff;
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = 0.0;
-    for (int e = C.far_beg; e < C.far_end; ++e) {
-        const DFarBlk b = p.far[far_list[e]];
-        const int src = TRANS ? b.rc : b.cc;
-        const double *ws = w + (int64_t)src * KB * r;
-        double fv[RMAX];
-        if constexpr (!FREE) {
-            const double *F = p.F + b.foff;
+                for (int l = 0; l < RMAX; ++l) {
+                    const int lc = min(l, r - 1);
+                    fv[l] = TRANS ? F[lc + r * m] : F[m + r * lc];
+                }
+            } else {
+                // forward: F[m, l] = f(tx_m, ty_l); transpose: lane row m is F's column
+                const double *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
+                const double *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
+                const double to = own[m];
 #pragma unroll
-            for (int l = 0; l < RMAX; ++l) {
-                const int lc = min(l, r - 1);
-                fv[l] = TRANS ? F[lc + r * la] : F[la + r * lc];
+                for (int l = 0; l < RMAX; ++l) {
+                    const double t = oth[min(l, r - 1)];
+                    fv[l] = TRANS ? kern_fast<KID>(t, to) : kern_fast<KID>(to, t);
+                }
             }
-        } else {
-            // forward: lane = m, F[m, l] = f(tx_m, ty_l); transpose: lane = l, sum over m
-            const double *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
-            const double *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
-            const double to = own[la];
 #pragma unroll
-            for (int l = 0; l < RMAX; ++l) {
-                const double t = oth[min(l, r - 1)];
-                fv[l] = TRANS ? kern_fast<KID>(t, to) : kern_fast<KID>(to, t);
-            }
+            for (int l = 0; l < RMAX; ++l)
+                acc = __builtin_fma(l < r ? fv[l] : 0.0, ws[min(l, r - 1)], acc);
         }
-#pragma unroll
-        for (int l = 0; l < RMAX; ++l) {
-            const double f = (act && l < r) ? fv[l] : 0.0;
-#pragma unroll
-            for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(f, ws[q * r + min(l, r - 1)], acc[q]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < KB; ++q) {
-        double v = acc[q];
         if constexpr (CHEB) {  // coefficients c = A^T z
             double c = 0.0;
             for (int k = 0; k < r; ++k) {
-                const double zk = __shfl(v, k);
-                if (act) c = __builtin_fma(p.chebA[k * r + lane], zk, c);
+                const double zk = __shfl(acc, (g * r + k) & (WAVE - 1));
+                c = __builtin_fma(p.chebA[k * r + m], zk, c);
             }
-            v = c;
+            acc = c;
         }
-        if (act) z[((int64_t)wid * KB + q) * r + lane] = v;
+        if (aq) z[((int64_t)wid * KB + q) * r + m] = acc;
     }
 }
 
@@ -418,8 +417,13 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                                             const double *pts_in, const double *z, MvArgs a) {
     static_assert(!TRANS || KB == 1, "the transpose handles one right-hand side per pass");
     constexpr int UN = KB == 1 ? 16 : 8;
+    // per-wavefront staging of right-hand-side tiles and far coefficients
+    constexpr int XS = KB == 1 ? 1 : (WAVE * KB > KB * RMAX ? WAVE * KB : KB * RMAX);
+    __shared__ double xs_all[WPB][XS];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
+    double *xs = xs_all[(threadIdx.x / WAVE) % WPB];
+    (void)xs;
     if (wid >= nch) return;
     const int r = p.r;
     const DChunk c = ch[wid];
@@ -438,35 +442,73 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
             const double *xin = a.x + D.col0 * a.incx;
             if constexpr (!FREE) {
                 const double *Aj = p.A + D.aoff + tile_index(s.off + lane, 0, D.n);
-                int j = 0;
-                for (; j + UN <= D.n; j += UN) {
-                    double av[UN];
+                if constexpr (KB == 1) {
+                    int j = 0;
+                    for (; j + UN <= D.n; j += UN) {
+                        double av[UN];
 #pragma unroll
-                    for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : 0.0;
+                        for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : 0.0;
 #pragma unroll
-                    for (int u = 0; u < UN; ++u)
+                        for (int u = 0; u < UN; ++u)
+                            acc[0] = __builtin_fma(av[u], xin[(int64_t)(j + u) * a.incx], acc[0]);
+                    }
+                    for (; j < D.n; ++j)
+                        acc[0] = __builtin_fma(act ? Aj[j * HM_TILE] : 0.0, xin[(int64_t)j * a.incx], acc[0]);
+                } else {
+                    for (int jt = 0; jt < D.n; jt += WAVE) {
+                        const int nj = min(WAVE, D.n - jt);
+                        if (lane < nj) {
 #pragma unroll
-                        for (int q = 0; q < KB; ++q)
-                            acc[q] = __builtin_fma(av[u], xin[(int64_t)(j + u) * a.incx + q * a.ldx], acc[q]);
-                }
-                for (; j < D.n; ++j) {
-                    const double av = act ? Aj[j * HM_TILE] : 0.0;
+                            for (int q = 0; q < KB; ++q)
+                                xs[lane * KB + q] = xin[(int64_t)(jt + lane) * a.incx + q * a.ldx];
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                        int j = 0;
+                        for (; j + UN <= nj; j += UN) {
+                            double av[UN];
 #pragma unroll
-                    for (int q = 0; q < KB; ++q)
-                        acc[q] = __builtin_fma(av, xin[(int64_t)j * a.incx + q * a.ldx], acc[q]);
+                            for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(jt + j + u) * HM_TILE] : 0.0;
+#pragma unroll
+                            for (int u = 0; u < UN; ++u)
+#pragma unroll
+                                for (int q = 0; q < KB; ++q)
+                                    acc[q] = __builtin_fma(av[u], xs[(j + u) * KB + q], acc[q]);
+                        }
+                        for (; j < nj; ++j) {
+                            const double av = act ? Aj[(jt + j) * HM_TILE] : 0.0;
+#pragma unroll
+                            for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(av, xs[j * KB + q], acc[q]);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    }
                 }
             } else if constexpr (KB == 1) {
                 acc[0] += near_free<KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
             } else {
+                // right-hand sides of a 64-column tile staged in LDS, read as
+                // broadcasts; one kernel value per (row, column)
                 const double *yin = pts_in + D.col0;
-                for (int j = 0; j < D.n; ++j) {
-                    const double f = act ? kern_fast<KID>(po, yin[j]) : 0.0;
+                for (int jt = 0; jt < D.n; jt += WAVE) {
+                    const int nj = min(WAVE, D.n - jt);
+                    if (lane < nj) {
 #pragma unroll
-                    for (int q = 0; q < KB; ++q)
-                        acc[q] = __builtin_fma(f, xin[(int64_t)j * a.incx + q * a.ldx], acc[q]);
+                        for (int q = 0; q < KB; ++q)
+                            xs[lane * KB + q] = xin[(int64_t)(jt + lane) * a.incx + q * a.ldx];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    for (int j = 0; j < nj; ++j) {
+                        const double f = act ? kern_fast<KID>(po, yin[jt + j]) : 0.0;
+#pragma unroll
+                        for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(f, xs[j * KB + q], acc[q]);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 }
             }
-        } else {
+                } else {
             const double *xin = a.x + D.row0 * a.incx;
             if constexpr (!FREE) {
                 // lanes run down the rows of the block, 64 accumulators for the
@@ -498,6 +540,13 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
         if constexpr (!CHEB) {
             const DCluster C = cl[f.cl];
             const double *bp = B + C.boff + tile_index(f.off + lane, 0, r);
+            const double *zs = zc;
+            if constexpr (KB > 1) {
+                for (int idx = lane; idx < KB * r; idx += WAVE) xs[idx] = zc[idx];
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                zs = xs;
+            }
             int k = 0;
             for (; k + 8 <= r; k += 8) {
                 double bv[8];
@@ -506,12 +555,16 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
 #pragma unroll
-                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv[u], zc[q * r + k + u], acc[q]);
+                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv[u], zs[q * r + k + u], acc[q]);
             }
             for (; k < r; ++k) {
                 const double bv = act ? bp[k * HM_TILE] : 0.0;
 #pragma unroll
-                for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv, zc[q * r + k], acc[q]);
+                for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv, zs[q * r + k], acc[q]);
+            }
+            if constexpr (KB > 1) {
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             }
         } else {
             const double2 cm = cmapv[f.cl];
@@ -530,7 +583,11 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                 const double sum = __builtin_fma(xh, b1, zc[0] - b2);
                 if (act) acc[0] += sum;
             } else {
-                // T_k(xh) once, then one dot product per right-hand side
+                // coefficients of the cluster (KB x r) staged in LDS; T_k(xh) once,
+                // then one dot product per right-hand side
+                for (int idx = lane; idx < KB * r; idx += WAVE) xs[idx] = zc[idx];
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 double t0 = 1.0, t1 = xh;
 #pragma unroll
                 for (int k = 0; k < RMAX; ++k) {
@@ -543,8 +600,10 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                     }
                     const double tu = (act && k < r) ? tv : 0.0;
 #pragma unroll
-                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(tu, zc[q * r + min(k, r - 1)], acc[q]);
+                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(tu, xs[q * r + min(k, r - 1)], acc[q]);
                 }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             }
         }
     }
@@ -719,7 +778,7 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
         k_up<RMAX, CHEB, KB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x,
                                                       a.incx, a.ldx, a.partial, p.r);
         mark(1);
-        k_reduce<CHEB, KB><<<nblk((int64_t)in_ncl * KB), WG, 0, s>>>(in_cl, in_ncl, a.partial,
+        k_reduce<CHEB, KB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial,
                                                                       p.chebA, a.w, p.r);
         mark(2);
         if (tr) {

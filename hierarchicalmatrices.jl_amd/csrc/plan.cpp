@@ -369,6 +369,7 @@ struct hm_plan {
     double *d_A = nullptr, *d_U = nullptr, *d_V = nullptr, *d_F = nullptr;
     double *d_partial = nullptr, *d_w = nullptr, *d_z = nullptr;
     bool profile = false;
+    bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
     int tuned_kb = HM_MAX_KB, tuned_mode = -1;  // hm_plan_tune's choice
     std::vector<hipEvent_t> prof_events;  // 5 per product
@@ -723,6 +724,148 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
     return make_plan(kernel_id, &I, 1, blocksize, rank, flags, stream, out);
 }
 
+int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                        const int64_t *row0, const int64_t *col0, const int64_t *m,
+                        const int64_t *n, const int64_t *rank, const double *const *A,
+                        const double *const *U, const double *const *S,
+                        const double *const *V, void *stream, hm_plan_t *out) {
+    if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
+    *out = nullptr;
+    if (nrows < 1 || ncols < 1 || nleaves < 0 || (nleaves > 0 && (!kind || !row0 || !col0 || !m || !n)))
+        return fail(HM_ERR_INVALID, "plan_from_blocks: bad arguments");
+    if (nrows >= (1LL << 40) || ncols >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many rows");
+    int R = 1;
+    for (int64_t t = 0; t < nleaves; ++t) {
+        if (kind[t] != 2 && kind[t] != 3) return fail(HM_ERR_INVALID, "leaf %lld: kind must be 2 or 3", (long long)t);
+        if (m[t] < 0 || n[t] < 0 || row0[t] < 0 || col0[t] < 0 || row0[t] + m[t] > nrows || col0[t] + n[t] > ncols)
+            return fail(HM_ERR_INVALID, "leaf %lld out of range", (long long)t);
+        if (m[t] >= (1LL << 31) || n[t] >= (1LL << 31)) return fail(HM_ERR_INVALID, "leaf %lld too large", (long long)t);
+        if (kind[t] == 3 && m[t] * n[t] > 0 && (!A || !A[t])) return fail(HM_ERR_INVALID, "leaf %lld: no data", (long long)t);
+        if (kind[t] == 2) {
+            if (!rank || rank[t] < 0) return fail(HM_ERR_INVALID, "leaf %lld: bad rank", (long long)t);
+            if (rank[t] > 64) return fail(HM_ERR_UNSUPPORTED, "leaf %lld: rank %lld > 64", (long long)t, (long long)rank[t]);
+            if (rank[t] > 0 && (!U || !S || !V || !U[t] || !S[t] || !V[t]))
+                return fail(HM_ERR_INVALID, "leaf %lld: no factors", (long long)t);
+            R = std::max<int>(R, (int)rank[t]);
+        }
+    }
+    hm_plan *P = new hm_plan();
+    P->from_blocks = true;
+    P->kid = 0; P->r = R; P->bs = 0; P->flags = 0;
+    P->nx = nrows; P->ny = ncols;
+    P->row_off = {0, nrows};
+    P->col_off = {0, ncols};
+    std::vector<HCluster> hr, hc;
+    std::vector<int64_t> lr_leaf;  // leaf index of each far block
+    for (int64_t t = 0; t < nleaves; ++t) {
+        P->leaves.push_back({kind[t], 1, row0[t], m[t], col0[t], n[t], 0, 0, 0, 0});
+        if (m[t] == 0 || n[t] == 0) continue;
+        if (kind[t] == 3) {
+            P->dense.push_back({row0[t], col0[t], (int32_t)m[t], (int32_t)n[t], 0});
+            P->dense_entries += m[t] * n[t];
+        } else if (rank[t] > 0) {
+            const int32_t rc = (int32_t)hr.size(), cc = (int32_t)hc.size();
+            hr.push_back({1, row0[t], m[t], 0, 0});
+            hc.push_back({1, col0[t], n[t], 0, 0});
+            P->far.push_back({rc, cc, 0});
+            lr_leaf.push_back(t);
+            P->far_rows += m[t];
+            P->far_cols += n[t];
+        }
+    }
+    const int r = R;
+    int64_t off = 0;
+    for (DDense &D : P->dense) { D.aoff = off; off += tiled_size(D.m, D.n); }
+    const int64_t nA = off;
+    auto mk = [&](const std::vector<HCluster> &h, std::vector<DCluster> &dc) {
+        dc.resize(h.size());
+        int64_t o = 0;
+        for (size_t k = 0; k < h.size(); ++k) {
+            dc[k] = DCluster{};
+            dc[k].first = h[k].first;
+            dc[k].len = (int32_t)h[k].len;
+            dc[k].boff = o;
+            dc[k].depth = 1;
+            dc[k].far_beg = (int32_t)k;
+            dc[k].far_end = (int32_t)k + 1;
+            o += tiled_size(h[k].len, r);
+        }
+        return o;
+    };
+    const int64_t nU = mk(hr, P->rcl), nV = mk(hc, P->ccl);
+    for (size_t k = 0; k < P->far.size(); ++k) P->far[k].foff = (int64_t)k * r * r;
+    const int64_t nF = (int64_t)P->far.size() * r * r;
+    std::vector<int32_t> by_rc(P->far.size()), by_cc(P->far.size());
+    for (size_t k = 0; k < P->far.size(); ++k) by_rc[k] = by_cc[k] = (int32_t)k;
+    build_side(nrows, P->dense, true, P->rcl, P->rows);
+    build_side(ncols, P->dense, false, P->ccl, P->cols);
+    // host-side tiling of the data
+    std::vector<double> hA(nA, 0.0), hU(nU, 0.0), hV(nV, 0.0), hF(nF, 0.0);
+    {
+        size_t d = 0;
+        for (int64_t t = 0; t < nleaves; ++t) {
+            if (kind[t] != 3 || m[t] == 0 || n[t] == 0) continue;
+            const DDense &D = P->dense[d++];
+            for (int64_t j = 0; j < D.n; ++j)
+                for (int64_t i = 0; i < D.m; ++i) hA[D.aoff + tile_index(i, j, D.n)] = A[t][i + D.m * j];
+        }
+        for (size_t f = 0; f < lr_leaf.size(); ++f) {
+            const int64_t t = lr_leaf[f], k = rank[t];
+            const DCluster &Cr = P->rcl[f], &Cc = P->ccl[f];
+            for (int64_t c = 0; c < k; ++c) {
+                for (int64_t i = 0; i < m[t]; ++i) hU[Cr.boff + tile_index(i, c, r)] = U[t][i + m[t] * c];
+                for (int64_t j = 0; j < n[t]; ++j) hV[Cc.boff + tile_index(j, c, r)] = V[t][j + n[t] * c];
+                hF[(int64_t)f * r * r + c + (int64_t)r * c] = S[t][c];
+            }
+        }
+    }
+    int rc_;
+#define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
+    HM_TRY(upload(&P->d_rcl, P->rcl));
+    HM_TRY(upload(&P->d_ccl, P->ccl));
+    HM_TRY(upload(&P->d_far, P->far));
+    HM_TRY(upload(&P->d_far_by_rc, by_rc));
+    HM_TRY(upload(&P->d_far_by_cc, by_cc));
+    P->far_by_rc_h = by_rc;
+    HM_TRY(upload(&P->d_dense, P->dense));
+    HM_TRY(upload(&P->d_rch, P->rows.ch));
+    HM_TRY(upload(&P->d_cch, P->cols.ch));
+    HM_TRY(upload(&P->d_rnear, P->rows.near));
+    HM_TRY(upload(&P->d_cnear, P->cols.near));
+    HM_TRY(upload(&P->d_rfar, P->rows.farl));
+    HM_TRY(upload(&P->d_cfar, P->cols.farl));
+    HM_TRY(upload(&P->d_rtask, P->rows.tasks));
+    HM_TRY(upload(&P->d_ctask, P->cols.tasks));
+    HM_TRY(upload(&P->d_A, hA));
+    HM_TRY(upload(&P->d_U, hU));
+    HM_TRY(upload(&P->d_V, hV));
+    HM_TRY(upload(&P->d_F, hF));
+    const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
+    const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
+    HM_TRY(dalloc(&P->d_partial, std::max<int64_t>(nslots, 1) * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_w, std::max<int64_t>(ncl, 1) * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_z, std::max<int64_t>(ncl, 1) * r * HM_MAX_KB));
+#undef HM_TRY
+    DevPlan &dp = P->dp;
+    dp.kid = 0; dp.r = r; dp.nx = nrows; dp.ny = ncols;
+    dp.rcl = P->d_rcl; dp.ccl = P->d_ccl;
+    dp.nrc = (int32_t)P->rcl.size(); dp.ncc = (int32_t)P->ccl.size();
+    dp.far = P->d_far; dp.nfar = (int32_t)P->far.size();
+    dp.far_by_rc = P->d_far_by_rc; dp.far_by_cc = P->d_far_by_cc;
+    dp.dense = P->d_dense; dp.ndense = (int32_t)P->dense.size();
+    dp.rch = P->d_rch; dp.cch = P->d_cch;
+    dp.nrch = (int32_t)P->rows.ch.size(); dp.ncch = (int32_t)P->cols.ch.size();
+    dp.rnear = P->d_rnear; dp.cnear = P->d_cnear;
+    dp.rfar = P->d_rfar; dp.cfar = P->d_cfar;
+    dp.rtask = P->d_rtask; dp.ctask = P->d_ctask;
+    dp.nrtask = (int32_t)P->rows.tasks.size(); dp.nctask = (int32_t)P->cols.tasks.size();
+    dp.nslots_r = P->rows.nslots; dp.nslots_c = P->cols.nslots;
+    dp.A = P->d_A; dp.U = P->d_U; dp.V = P->d_V; dp.F = P->d_F;
+    (void)stream;
+    *out = P;
+    return HM_OK;
+}
+
 int hm_plan_batched(int kernel_id, int64_t count, const double *const *x, const int64_t *nx,
                     const double *const *y, const int64_t *ny, const double *intervals,
                     int blocksize, int rank, unsigned flags, void *stream, hm_plan_t *out) {
@@ -790,6 +933,7 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     v[HM_INFO_N_COLCHUNKS] = (int64_t)P->cols.ch.size();
     v[HM_INFO_NONFINITE] = P->nonfinite;
     v[HM_INFO_N_INSTANCES] = (int64_t)P->row_off.size() - 1;
+    v[HM_INFO_FROM_BLOCKS] = P->from_blocks ? 1 : 0;
     for (int k = 0; k < n && k < HM_INFO_COUNT; ++k) info[k] = v[k];
     for (int k = HM_INFO_COUNT; k < n; ++k) info[k] = 0;
     return HM_OK;
@@ -826,8 +970,10 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
     int mode = 0;
     if ((P->flags & HM_PLAN_MATRIX_FREE) || (flags & HM_MV_MATRIX_FREE)) mode = 2;
     else if ((P->flags & HM_PLAN_HYBRID) || (flags & HM_MV_CHEB_BASIS)) mode = 1;
+    if (P->from_blocks && (mode != 0 || (flags & (HM_MV_MATRIX_FREE | HM_MV_CHEB_BASIS))))
+        return fail(HM_ERR_UNSUPPORTED, "a plan built from blocks has no points: stored form only");
     int max_kb = P->max_kb;
-    if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0) {
+    if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0 && !P->from_blocks) {
         mode = std::max(mode, P->tuned_mode);
         max_kb = P->tuned_kb;
     }

@@ -30,7 +30,7 @@ INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
     "n_rowclusters", "n_colclusters", "depth", "dense_entries", "basis_rows_u", "basis_rows_v",
     "bytes_stored", "bytes_ref_layout", "flags", "n_rowchunks", "n_colchunks", "nonfinite",
-    "n_instances",
+    "n_instances", "from_blocks",
 ]
 
 _vp = ctypes.c_void_p
@@ -77,6 +77,11 @@ def _load():
     L.hm_plan_from_points.argtypes = [ctypes.c_int, _dp, _i64, _dp, _i64, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint, _vp,
+                                      ctypes.POINTER(_vp)]
+    L.hm_plan_from_blocks.restype = ctypes.c_int
+    L.hm_plan_from_blocks.argtypes = [_i64, _i64, _i64, ctypes.POINTER(ctypes.c_int32), _ip, _ip,
+                                      _ip, _ip, _ip, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                      ctypes.POINTER(_dp), ctypes.POINTER(_dp), _vp,
                                       ctypes.POINTER(_vp)]
     L.hm_plan_batched.restype = ctypes.c_int
     L.hm_plan_batched.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(_dp), _ip,

@@ -1,0 +1,220 @@
+"""The generic block tree of HierarchicalMatrices.jl: `HierarchicalMatrix` with
+`LowRankMatrix` and dense leaves, mirrored on the host and run on the device.
+
+  LowRankMatrix(U, S, V)          src/LowRankMatrix.jl:28-41 (U diag(S) V')
+  HierarchicalMatrix(M, N)        src/hierarchical.jl:54-69 (@hierarchical
+                                  HierarchicalMatrix LowRankMatrix Matrix,
+                                  src/HierarchicalMatrix.jl:1)
+  H[Block(m), Block(n)] = A       src/hierarchical.jl:149-172 (assigned codes
+                                  1 = HierarchicalMatrix, 2 = LowRankMatrix,
+                                  3 = Matrix; other types stay unassigned)
+  blocksize(H, m, n, k), size(H)  src/hierarchical.jl:31-47,76-97
+  mul_(y, H, x, istart, jstart, INCX, INCY)
+                                  src/HierarchicalMatrix.jl:24-52 with the leaf
+                                  loops of src/algebra.jl:37-48,110-131
+
+The tree is flattened on first use into the leaves' absolute offsets (the
+traversal's istart + p, jstart + q) and handed to hm_plan_from_blocks; the
+product then runs the same gfx950 kernels as the KernelMatrix product.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .kernelmatrix import Block, KernelMatrix, _stream_ptr
+
+
+class LowRankMatrix:
+    """U * Diagonal(S) * V' (src/LowRankMatrix.jl:28-38)."""
+
+    def __init__(self, U, S, V):
+        self.U = np.asfortranarray(U, dtype=np.float64)
+        self.S = np.ascontiguousarray(np.diag(S) if np.ndim(S) == 2 else S, dtype=np.float64)
+        self.V = np.asfortranarray(V, dtype=np.float64)
+        if self.U.shape[1] != len(self.S) or self.V.shape[1] != len(self.S):
+            raise ValueError("U, S, V ranks differ")
+
+    @property
+    def shape(self):
+        return (self.U.shape[0], self.V.shape[0])
+
+    def size(self, k=None):
+        return self.shape if k is None else self.shape[k - 1]
+
+    def rank(self):
+        return len(self.S)
+
+    def __getitem__(self, ij):
+        """Entry (i, j), 0-based; sum over k from r down to 1 as
+        src/LowRankMatrix.jl:50-58."""
+        i, j = ij
+        ret = 0.0
+        for k in range(self.rank() - 1, -1, -1):
+            ret += self.U[i, k] * self.S[k] * self.V[j, k]
+        return ret
+
+
+def _child_size(c, k):
+    if c is None:
+        return 0
+    if isinstance(c, HierarchicalMatrix):
+        return c.size(k)
+    return c.shape[k - 1]
+
+
+class HierarchicalMatrix:
+    """An M x N block tree whose children are HierarchicalMatrix,
+    LowRankMatrix or dense (2-D float64 numpy) blocks."""
+
+    def __init__(self, M, N):
+        self.blocks = [[None] * N for _ in range(M)]
+        self.assigned = np.zeros((M, N), dtype=np.int64)
+        self._plan = None
+
+    def blocksize(self, m=None, n=None, k=None):
+        """blocksize(H) = size(H.assigned); blocksize(H, m, n, k) = size of
+        child (m, n) (1-based) along k, 0 when unassigned."""
+        if m is None:
+            return self.assigned.shape
+        return _child_size(self.blocks[m - 1][n - 1], k)
+
+    def size(self, k=None):
+        M, N = self.assigned.shape
+        rows = sum(self.blocksize(m, N, 1) for m in range(1, M + 1))
+        cols = sum(self.blocksize(1, n, 2) for n in range(1, N + 1))
+        return (rows, cols) if k is None else (rows, cols)[k - 1]
+
+    @property
+    def shape(self):
+        return self.size()
+
+    def __setitem__(self, key, A):
+        b1, b2 = key
+        m, n = (b1.K if isinstance(b1, Block) else int(b1)), (b2.K if isinstance(b2, Block) else int(b2))
+        if isinstance(A, HierarchicalMatrix):
+            code = 1
+        elif isinstance(A, LowRankMatrix):
+            code = 2
+        elif isinstance(A, np.ndarray) and A.ndim == 2 and A.dtype == np.float64:
+            code = 3
+            A = np.asfortranarray(A)
+        else:  # no matching field: the reference leaves the block unassigned
+            return
+        self.blocks[m - 1][n - 1] = A
+        self.assigned[m - 1, n - 1] = code
+        self._plan = None
+
+    def __getitem__(self, key):
+        b1, b2 = key
+        if isinstance(b1, Block):
+            return self.blocks[b1.K - 1][b2.K - 1]
+        return self.getindex(int(b1), int(b2))
+
+    def getindex(self, i, j):
+        """Entry (i, j), 0-based, by the block walk of src/hierarchical.jl:120-147."""
+        M, N = self.assigned.shape
+        m = 0
+        while m < M and i >= self.blocksize(m + 1, N, 1):
+            i -= self.blocksize(m + 1, N, 1)
+            m += 1
+        n = 0
+        while n < N and j >= self.blocksize(1, n + 1, 2):
+            j -= self.blocksize(1, n + 1, 2)
+            n += 1
+        if m >= M or n >= N:
+            raise IndexError("index out of range")
+        c = self.blocks[m][n]
+        if c is None:
+            return 0.0
+        if isinstance(c, (HierarchicalMatrix, LowRankMatrix)):
+            return c[i, j] if isinstance(c, LowRankMatrix) else c.getindex(i, j)
+        return float(c[i, j])
+
+    def leaves(self, istart=0, jstart=0):
+        """Leaves in the traversal order of src/HierarchicalMatrix.jl:24-52,
+        as (block, row0, col0) with 0-based absolute offsets."""
+        out = []
+        M, N = self.assigned.shape
+        p = 0
+        for m in range(M):
+            q = 0
+            for n in range(N):
+                c = self.blocks[m][n]
+                if isinstance(c, HierarchicalMatrix):
+                    out.extend(c.leaves(istart + p, jstart + q))
+                elif c is not None:
+                    out.append((c, istart + p, jstart + q))
+                q += self.blocksize(1, n + 1, 2)
+            p += self.blocksize(m + 1, N, 1)
+        return out
+
+    def plan(self, stream=None):
+        """The device plan (built on first use)."""
+        if self._plan is None:
+            self._plan = _BlocksPlan(self, stream)
+        return self._plan
+
+    # products, as KernelMatrix
+    def mul(self, u, v, istart=1, jstart=1, INCX=1, INCY=1, **kw):
+        return self.plan().mul(u, v, istart, jstart, INCX, INCY, **kw)
+
+    def matvec(self, v, **kw):
+        return self.plan().matvec(v, **kw)
+
+    def __matmul__(self, v):
+        return self.matvec(v)
+
+    __mul__ = __matmul__
+
+
+class _BlocksPlan(KernelMatrix):
+    """hm_plan_from_blocks over the flattened leaves of a HierarchicalMatrix."""
+
+    def __init__(self, H, stream=None):
+        rows, cols = H.size()
+        leaves = H.leaves()
+        nl = len(leaves)
+        kind = np.zeros(max(nl, 1), dtype=np.int32)
+        r0 = np.zeros(max(nl, 1), dtype=np.int64)
+        c0 = np.zeros_like(r0)
+        mm = np.zeros_like(r0)
+        nn = np.zeros_like(r0)
+        rk = np.zeros_like(r0)
+        DP = ctypes.POINTER(ctypes.c_double)
+        A = (DP * max(nl, 1))()
+        U = (DP * max(nl, 1))()
+        S = (DP * max(nl, 1))()
+        V = (DP * max(nl, 1))()
+        keep = []
+        for t, (c, i0, j0) in enumerate(leaves):
+            r0[t], c0[t] = i0, j0
+            if isinstance(c, LowRankMatrix):
+                kind[t] = 2
+                mm[t], nn[t] = c.shape
+                rk[t] = c.rank()
+                for arr, dst in ((c.U, U), (c.S, S), (c.V, V)):
+                    a = np.ascontiguousarray(arr.ravel(order="F"))
+                    keep.append(a)
+                    dst[t] = a.ctypes.data_as(DP)
+            else:
+                kind[t] = 3
+                mm[t], nn[t] = c.shape
+                a = np.ascontiguousarray(c.ravel(order="F"))
+                keep.append(a)
+                A[t] = a.ctypes.data_as(DP)
+        IP = ctypes.POINTER(ctypes.c_int64)
+        h = ctypes.c_void_p()
+        check(lib.hm_plan_from_blocks(rows, cols, nl, kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      r0.ctypes.data_as(IP), c0.ctypes.data_as(IP),
+                                      mm.ctypes.data_as(IP), nn.ctypes.data_as(IP),
+                                      rk.ctypes.data_as(IP), A, U, S, V, _stream_ptr(stream),
+                                      ctypes.byref(h)))
+        self._h = h
+        self.f = None
+        self.rank = int(rk.max()) if nl else 0
+        self.blocksize_ = 0
+        self.flags = 0
+        self.x = np.zeros(rows)
+        self.y = np.zeros(cols)

@@ -99,6 +99,22 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
                         double a, double b, double c, double d, int blocksize, int rank,
                         unsigned flags, void *stream, hm_plan_t *plan);
 
+/* A HierarchicalMatrix given by its leaves (src/HierarchicalMatrix.jl:1-52 with
+ * LowRankMatrix and Matrix leaves, src/LowRankMatrix.jl:28-38): leaf t sits at
+ * (row0[t], col0[t]) with size m[t] x n[t] in an nrows x ncols matrix, placed
+ * where the reference's traversal offsets put it (src/HierarchicalMatrix.jl:45,47).
+ * kind[t] = 3: dense, A[t] is m x n column-major. kind[t] = 2: low rank
+ * U Diagonal(S) V' with U[t] m x rank[t], S[t] rank[t], V[t] n x rank[t]
+ * (column-major). Leaves may be missing (unassigned blocks are zero). Host
+ * pointers; the data is copied into the plan. Products: hm_matvec /
+ * hm_matvec_t in stored form (the leaf loops of src/algebra.jl:37-48,110-131
+ * and their transposes :52-65,137-159). Ranks up to 64. */
+int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                        const int64_t *row0, const int64_t *col0, const int64_t *m,
+                        const int64_t *n, const int64_t *rank, const double *const *A,
+                        const double *const *U, const double *const *S,
+                        const double *const *V, void *stream, hm_plan_t *plan);
+
 /* Many independent KernelMatrix instances in one plan (no counterpart in the
  * reference, which builds one tree per call of src/KernelMatrix.jl:47).
  * Instance t has points x[t] (nx[t]), y[t] (ny[t]) and intervals
@@ -128,7 +144,7 @@ enum hm_info {
     HM_INFO_BYTES_STORED,      /* bytes of every stored array read by one product */
     HM_INFO_BYTES_REF_LAYOUT,  /* same arrays in the reference's per-block layout */
     HM_INFO_FLAGS, HM_INFO_N_ROWCHUNKS, HM_INFO_N_COLCHUNKS, HM_INFO_NONFINITE,
-    HM_INFO_N_INSTANCES,
+    HM_INFO_N_INSTANCES, HM_INFO_FROM_BLOCKS,
     HM_INFO_COUNT
 };
 int hm_plan_info(hm_plan_t plan, int64_t *info, int n);
