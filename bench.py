@@ -202,6 +202,14 @@ def run_batched(args):
         out["variants"][name] = {"ms_per_step": round(ms, 4), "value": round(1000 / ms, 3),
                                  "hbm_bytes_stored": info["bytes_stored"],
                                  "rel_diff_vs_stored": err}
+        for prec in ("f32", "bf16x3"):
+            Ym = torch.empty_like(Y)
+            ms_m = timed(lambda: H._mv(Ym, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
+                                       prec), args.steps, args.warmup, stream)
+            out["variants"][name][f"far_{prec}"] = {
+                "ms_per_step": round(ms_m, 4), "value": round(1000 / ms_m, 3),
+                "rel_err_vs_fp64": float(torch.linalg.norm(Ym - ref) / torch.linalg.norm(ref))}
+            del Ym
         if name == "matrix_free" or name == "hybrid":
             kb, mode, us = H.tune(k=32)
             ms_t = timed(lambda: H.matvec(B, tuned=True), args.steps, args.warmup, stream)
@@ -211,6 +219,12 @@ def run_batched(args):
                                                 "ms_per_step": round(ms_t, 4)},
                                          graph_ms_per_step=round(ms_g, 4))
             g.close()
+            gm = H.graph(Y, B, far_precision="f32")
+            ms_gm = timed(lambda: gm.launch(), args.steps, args.warmup, stream)
+            out["variants"][name]["far_f32"]["graph_ms_per_step"] = round(ms_gm, 4)
+            out["variants"][name]["far_f32"]["graph_rel_err_vs_fp64"] = float(
+                torch.linalg.norm(Y - ref) / torch.linalg.norm(ref))
+            gm.close()
         H.close()
         del Y
         torch.cuda.empty_cache()
@@ -295,6 +309,13 @@ def run_scale(args):
         out["variants"][name] = {"ms_per_step": round(ms, 4), "value": round(1000 / ms, 3),
                                  "build_s": round(build, 3), "hbm_bytes_stored": H.info["bytes_stored"],
                                  "rel_err_vs_dense_4096rows": err}
+        for prec in ("f32", "bf16x3"):
+            ms_m = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
+                                       prec), args.steps, args.warmup, stream)
+            out["variants"][name][f"far_{prec}"] = {
+                "ms_per_step": round(ms_m, 4), "value": round(1000 / ms_m, 3),
+                "rel_err_vs_dense_4096rows":
+                    float(torch.linalg.norm(Y[rows] - dense) / torch.linalg.norm(dense))}
         H.close()
         del Y
         torch.cuda.empty_cache()

@@ -25,7 +25,8 @@
 
 #define HM_CHUNK 64          // rows per chunk = wavefront width
 #define HM_TASK_POINTS 8192  // points per upward-pass task
-#define HM_MAX_KB 8          // right-hand sides per forward launch
+#define HM_MAX_KB 8          // right-hand sides per forward launch (FP64 passes)
+#define HM_WS_RHS 16         // right-hand sides the workspace holds (mixed precision: 16)
 
 // Stored matrices (dense blocks m x n, bases len x r) are kept in 64-row
 // tiles, column-major inside a tile: element (i, k) of a matrix with `cols`

@@ -12,3 +12,8 @@ hipError_t hmx_grad_r24(const DevPlan &p, const MvArgs &a, bool tr, double *y2, 
 }
 
 hipError_t hmx_gen_basis_r24(const DevPlan &p, hipStream_t s) { return hmxk::gen_basis_r<24>(p, s); }
+
+hipError_t hmx_matvec_mixed_r24(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                                hipStream_t s) {
+    return hmxk::matvec_mixed_r<32>(p, a, mode, prec, nq, s);
+}

@@ -52,3 +52,13 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
     hmxk::k_count_nonfinite<<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
     return hipGetLastError();
 }
+
+hipError_t hmx_launch_matvec_mixed(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                                   hipStream_t s) {
+    if (p.r <= 8) return hmx_matvec_mixed_r8(p, a, mode, prec, nq, s);
+    if (p.r <= 16) return hmx_matvec_mixed_r16(p, a, mode, prec, nq, s);
+    if (p.r <= 24) return hmx_matvec_mixed_r24(p, a, mode, prec, nq, s);
+    if (p.r <= 32) return hmx_matvec_mixed_r32(p, a, mode, prec, nq, s);
+    if (p.r <= 48) return hmx_matvec_mixed_r48(p, a, mode, prec, nq, s);
+    return hmx_matvec_mixed_r64(p, a, mode, prec, nq, s);
+}

@@ -859,6 +859,282 @@ hipError_t grad_r(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_
     }
 }
 
+
+// ------------------------------------------------- mixed-precision far field
+// Far field of the Chebyshev-form passes on MFMA with FP32 accumulation, for
+// 16 right-hand sides per launch. T_m(xh) comes from FP64 point differences
+// and an FP64 recurrence; only the products T * X (upward) and T * C
+// (downward) run in reduced precision:
+//   PREC 0: v_mfma_f32_16x16x4f32 (FP32 operands)
+//   PREC 1: split BF16, x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+//           hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16
+// Operand layouts (16x16 output tile, lane l): A[i][k] from lane i = l % 16,
+// k = K/4 * (l / 16) + v; B[k][j] from lane j = l % 16, same k; D[i][j] in
+// lane j = l % 16, i = 4 * (l / 16) + v.
+typedef float mf4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 mbf8_t __attribute__((ext_vector_type(8)));
+constexpr int MQ = 16;  // right-hand sides per mixed-precision launch
+
+__device__ __forceinline__ __bf16 bf16_hi(float x) { return (__bf16)x; }
+__device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)(__bf16)x); }
+
+// Upward pass: moments mu[m][q] = sum_j T_m(xh_j) x[j, q] of one task. Each
+// wavefront forms T for 64 points (one per lane) into LDS, then multiplies.
+template <int RP, int PREC>
+__global__ __launch_bounds__(WG) void k_up_mfma(const DTask *tasks, const DCluster *cl,
+                                               const double2 *cmapv, const double *pts,
+                                               const double *x, int64_t incx, int64_t ldx,
+                                               int nq, double *partial, int r) {
+    constexpr int MT = RP / 16;
+    __shared__ float Tl[WPB][RP][WAVE + 1];
+    float (*red)[RP][MQ] = reinterpret_cast<float (*)[RP][MQ]>(&Tl[0][0][0]);  // reused after the loop
+    const DTask T = tasks[blockIdx.x];
+    const DCluster C = cl[T.cl];
+    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+    const double2 cm = cmapv[T.cl];
+    const int64_t first = C.first + T.off;
+    mf4_t D[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) D[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    const int j16 = lane % 16, g = lane / 16;
+    for (int base = wv * WAVE; base < T.len; base += WG) {
+        // T_m at this lane's point, FP64 recurrence, stored as FP32
+        {
+            const int i = base + lane;
+            const bool ok = i < T.len;
+            const double xh = ok ? (pts[first + i] - cm.x) * cm.y : 0.0;
+            double t0 = 1.0, t1 = xh;
+#pragma unroll
+            for (int m = 0; m < RP; ++m) {
+                double tv = m == 0 ? t0 : t1;
+                if (m >= 2) {
+                    tv = __builtin_fma(xh + xh, t1, -t0);
+                    t0 = t1;
+                    t1 = tv;
+                }
+                Tl[wv][m][lane] = (ok && m < r) ? (float)tv : 0.f;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if constexpr (PREC == 0) {
+#pragma unroll 4
+            for (int k0 = 0; k0 < WAVE; k0 += 4) {
+                const int i = base + k0 + g;
+                const float b = (i < T.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[wv][t * 16 + j16][k0 + g], b, D[t], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < WAVE; k0 += 32) {
+                mbf8_t bh, bl;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int i = base + k0 + 8 * g + v;
+                    const float b = (i < T.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
+                    bh[v] = bf16_hi(b);
+                    bl[v] = bf16_lo(b);
+                }
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+                    mbf8_t ah, al;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        const float tv = Tl[wv][t * 16 + j16][k0 + 8 * g + v];
+                        ah[v] = bf16_hi(tv);
+                        al[v] = bf16_lo(tv);
+                    }
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D[t], 0, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    __syncthreads();  // every wavefront is done with Tl before it is reused
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[wv][t * 16 + 4 * g + v][j16] = D[t][v];
+    __syncthreads();
+    for (int idx = tid; idx < r * MQ; idx += WG) {
+        const int q = idx / r, m = idx % r;
+        if (q < nq) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int w = 0; w < WPB; ++w) sacc += (double)red[w][m][q];
+            partial[(T.slot * MQ + q) * r + m] = sacc;
+        }
+    }
+}
+
+// Downward pass: near field in FP64 (stored A or regenerated, right-hand
+// sides staged in LDS), far field sum over clusters of T(xh_rows) * C on
+// MFMA in FP32; the two are added in FP64 when the chunk is written.
+template <int RP, int PREC, bool FREE, int KID>
+__global__ __launch_bounds__(WG) void k_down_mfma(DevPlan p, const DChunk *ch, int nch,
+                                                 const DNear *nl, const DFar *fl,
+                                                 const double2 *cmapv, const double *pts_out,
+                                                 const double *pts_in, const double *z, MvArgs a,
+                                                 int nq) {
+    constexpr int MT = RP / 16;
+    __shared__ float Tl[WPB][WAVE][RP + 1];
+    __shared__ double xs_all[WPB][WAVE * MQ];
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    const int wv = (threadIdx.x / WAVE) % WPB;
+    double *xs = xs_all[wv];
+    if (wid >= nch) return;
+    const int r = p.r;
+    const DChunk c = ch[wid];
+    const bool act = lane < c.np;
+    const double po = act ? pts_out[c.p0 + lane] : 0.0;
+    const int j16 = lane % 16, g = lane / 16;
+    double acc[MQ];
+#pragma unroll
+    for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
+    // ---- near field, FP64
+    for (int e = c.near_beg; e < c.near_end; ++e) {
+        const DNear s = nl[e];
+        const DDense Dd = p.dense[s.blk];
+        const double *xin = a.x + Dd.col0 * a.incx;
+        const double *Aj = FREE ? nullptr : p.A + Dd.aoff + tile_index(s.off + lane, 0, Dd.n);
+        for (int jt = 0; jt < Dd.n; jt += WAVE) {
+            const int nj = min(WAVE, Dd.n - jt);
+            if (lane < nj) {
+#pragma unroll
+                for (int q = 0; q < MQ; ++q)
+                    xs[lane * MQ + q] = q < nq ? xin[(int64_t)(jt + lane) * a.incx + q * a.ldx] : 0.0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            for (int j = 0; j < nj; ++j) {
+                double f;
+                if constexpr (FREE) f = act ? kern_fast<KID>(po, pts_in[Dd.col0 + jt + j]) : 0.0;
+                else f = act ? Aj[(jt + j) * HM_TILE] : 0.0;
+#pragma unroll
+                for (int q = 0; q < MQ; ++q) acc[q] = __builtin_fma(f, xs[j * MQ + q], acc[q]);
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+    }
+    // ---- far field on MFMA: 4 row tiles of 16
+    mf4_t D[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) D[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    for (int e = c.far_beg; e < c.far_end; ++e) {
+        const DFar f = fl[e];
+        const double2 cm = cmapv[f.cl];
+        const double xh = (po - cm.x) * cm.y;
+        {
+            double t0 = 1.0, t1 = xh;
+#pragma unroll
+            for (int m = 0; m < RP; ++m) {
+                double tv = m == 0 ? t0 : t1;
+                if (m >= 2) {
+                    tv = __builtin_fma(xh + xh, t1, -t0);
+                    t0 = t1;
+                    t1 = tv;
+                }
+                Tl[wv][lane][m] = (act && m < r) ? (float)tv : 0.f;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const double *zc = z + (int64_t)f.cl * MQ * r;
+        if constexpr (PREC == 0) {
+#pragma unroll
+            for (int k0 = 0; k0 < RP; k0 += 4) {
+                const int m = k0 + g;
+                const float b = (m < r && j16 < nq) ? (float)zc[j16 * r + m] : 0.f;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[wv][t * 16 + j16][k0 + g], b, D[t], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < RP; k0 += 32) {
+                mbf8_t bh, bl;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int m = k0 + 8 * g + v;
+                    const float b = (m < r && j16 < nq) ? (float)zc[j16 * r + min(m, r - 1)] : 0.f;
+                    bh[v] = bf16_hi(b);
+                    bl[v] = bf16_lo(b);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    mbf8_t ah, al;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        const int m = k0 + 8 * g + v;
+                        const float tv = m < RP ? Tl[wv][t * 16 + j16][min(m, RP - 1)] : 0.f;
+                        ah[v] = bf16_hi(tv);
+                        al[v] = bf16_lo(tv);
+                    }
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D[t], 0, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    // D (rows x rhs, FP32) through LDS back to the row-per-lane layout
+    float *Dl = reinterpret_cast<float *>(xs);  // 64 x 16 floats fit in the staging buffer
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Dl[(t * 16 + 4 * g + v) * MQ + j16] = D[t][v];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < MQ; ++q) {
+            if (q < nq) {
+                double *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
+                *yo = (a.beta ? *yo : 0.0) + acc[q] + (double)Dl[lane * MQ + q];
+            }
+        }
+    }
+}
+
+template <int RP, int PREC, int MODE, int KID>
+hipError_t matvec_mixed_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) {
+    constexpr bool FREE = MODE == 2;
+    if (p.nfar > 0 && p.nctask > 0) {
+        k_up_mfma<RP, PREC><<<p.nctask, WG, 0, s>>>(p.ctask, p.ccl, p.cmap, p.y, a.x, a.incx,
+                                                     a.ldx, nq, a.partial, p.r);
+        k_reduce<true, MQ><<<nblk(p.ncc), WG, 0, s>>>(p.ccl, p.ncc, a.partial, p.chebA, a.w, p.r);
+        k_translate<RP, FREE, KID, false, true, MQ><<<nblk(p.nrc), WG, 0, s>>>(
+            p, p.rcl, p.nrc, p.far_by_rc, a.w, a.z);
+    }
+    if (p.nrch > 0)
+        k_down_mfma<RP, PREC, FREE, KID><<<nblk(p.nrch), WG, 0, s>>>(
+            p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, a.z, a, nq);
+    return hipGetLastError();
+}
+
+template <int RP>
+hipError_t matvec_mixed_r(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                          hipStream_t s) {
+    if (mode != 2) {
+        return prec == 0 ? matvec_mixed_t<RP, 0, 1, 0>(p, a, nq, s) : matvec_mixed_t<RP, 1, 1, 0>(p, a, nq, s);
+    }
+    switch (p.kid) {
+    case 0: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 0>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 0>(p, a, nq, s);
+    case 1: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 1>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 1>(p, a, nq, s);
+    case 2: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 2>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 2>(p, a, nq, s);
+    default: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 3>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 3>(p, a, nq, s);
+    }
+}
+
 template <int RMAX, int KB>
 hipError_t matvec_rk(const DevPlan &p, const MvArgs &a, bool tr, int mode, hipStream_t s,
                      const hipEvent_t *ev) {
@@ -890,7 +1166,9 @@ hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb
                                hipStream_t s, const hipEvent_t *ev);                            \
     hipError_t hmx_grad_r##R(const DevPlan &p, const MvArgs &a, bool tr, double *y2,            \
                              int64_t incy2, const double *sv, int64_t incs, hipStream_t s);     \
-    hipError_t hmx_gen_basis_r##R(const DevPlan &p, hipStream_t s);
+    hipError_t hmx_gen_basis_r##R(const DevPlan &p, hipStream_t s);                              \
+    hipError_t hmx_matvec_mixed_r##R(const DevPlan &p, const MvArgs &a, int mode, int prec,      \
+                                     int nq, hipStream_t s);
 HMX_DECLARE_BUCKET(8)
 HMX_DECLARE_BUCKET(16)
 HMX_DECLARE_BUCKET(24)

@@ -638,9 +638,9 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     HM_TRY(upload(&P->d_ctask, P->cols.tasks));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&P->d_partial, nslots * r * HM_MAX_KB));
-    HM_TRY(dalloc(&P->d_w, ncl * r * HM_MAX_KB));
-    HM_TRY(dalloc(&P->d_z, ncl * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_partial, nslots * r * HM_WS_RHS));
+    HM_TRY(dalloc(&P->d_w, ncl * r * HM_WS_RHS));
+    HM_TRY(dalloc(&P->d_z, ncl * r * HM_WS_RHS));
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
     const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
     if (stored) {
@@ -842,9 +842,9 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     HM_TRY(upload(&P->d_F, hF));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&P->d_partial, std::max<int64_t>(nslots, 1) * r * HM_MAX_KB));
-    HM_TRY(dalloc(&P->d_w, std::max<int64_t>(ncl, 1) * r * HM_MAX_KB));
-    HM_TRY(dalloc(&P->d_z, std::max<int64_t>(ncl, 1) * r * HM_MAX_KB));
+    HM_TRY(dalloc(&P->d_partial, std::max<int64_t>(nslots, 1) * r * HM_WS_RHS));
+    HM_TRY(dalloc(&P->d_w, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
+    HM_TRY(dalloc(&P->d_z, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
 #undef HM_TRY
     DevPlan &dp = P->dp;
     dp.kid = 0; dp.r = r; dp.nx = nrows; dp.ny = ncols;
@@ -963,7 +963,10 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec: tree-only plan has no device data");
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
-    if (flags & ~7u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (flags & ~31u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    const int prec = (flags & HM_MV_FAR_BF16X3) ? 1 : (flags & HM_MV_FAR_F32) ? 0 : -1;
+    if ((flags & HM_MV_FAR_BF16X3) && (flags & HM_MV_FAR_F32))
+        return fail(HM_ERR_INVALID, "matvec: choose one far-field precision");
     if (k == 0) return HM_OK;
     if (!y || !x) return fail(HM_ERR_INVALID, "matvec: NULL vector");
     if (k > 1 && (ldx < 1 || ldy < 1)) return fail(HM_ERR_INVALID, "matvec: bad ld");
@@ -978,6 +981,29 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
         max_kb = P->tuned_kb;
     }
     hipStream_t s = (hipStream_t)stream;
+    if (prec >= 0) {
+        // far field on MFMA in reduced precision: Chebyshev form, 16 rhs per launch
+        if (tr || P->from_blocks)
+            return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: forward KernelMatrix products only");
+        const int mmode = std::max(mode, 1);
+        for (int64_t q = 0; q < k; q += 16) {
+            MvArgs a{};
+            a.y = y + ioff + q * ldy;
+            a.x = x + joff + q * ldx;
+            a.incy = incy;
+            a.incx = incx;
+            a.ldy = ldy;
+            a.ldx = ldx;
+            a.beta = beta;
+            a.partial = P->d_partial;
+            a.w = P->d_w;
+            a.z = P->d_z;
+            const int nq = (int)std::min<int64_t>(16, k - q);
+            hipError_t e = hmx_launch_matvec_mixed(P->dp, a, mmode, prec, nq, s);
+            if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
+        }
+        return HM_OK;
+    }
     for (int64_t q = 0; q < k;) {
         // forward products take up to HM_MAX_KB right-hand sides per launch
         int kb = 1;

@@ -25,6 +25,8 @@ HM_PLAN_HYBRID = 8
 HM_MV_MATRIX_FREE = 1
 HM_MV_CHEB_BASIS = 2
 HM_MV_TUNED = 4
+HM_MV_FAR_F32 = 8
+HM_MV_FAR_BF16X3 = 16
 
 INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",

@@ -148,6 +148,9 @@ class _Vec:
             self.host[...] = self.t.cpu().numpy()
 
 
+_FAR = {"f64": 0, "f32": _lib.HM_MV_FAR_F32, "bf16x3": _lib.HM_MV_FAR_BF16X3}
+
+
 class KernelMatrix:
     """KernelMatrix(f, x, y, a, b, c, d), src/KernelMatrix.jl:47-116.
 
@@ -232,7 +235,7 @@ class KernelMatrix:
 
     # -- products
     def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream,
-            cheb_basis=False, tuned=False):
+            cheb_basis=False, tuned=False, far_precision="f64"):
         U = _Vec(u, True)
         V = _Vec(v, False)
         if U.k != V.k:
@@ -249,7 +252,8 @@ class KernelMatrix:
             raise ValueError("vector too short for H at the given offset and stride")
         fn = lib.hm_matvec_t if transpose else lib.hm_matvec
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
-            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0)
+            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
+            _FAR[far_precision]
         check(fn(self._h, U.ptr(), V.ptr(), ioff * U.inc, joff * V.inc, incy, incx, U.k,
                  U.ld, V.ld, beta, flags, _stream_ptr(stream)))
         U.writeback()
@@ -264,14 +268,15 @@ class KernelMatrix:
         return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, True, stream)
 
     def matvec(self, v, *, matrix_free=False, transpose=False, cheb_basis=False, tuned=False,
-               stream=None):
+               far_precision="f64", stream=None):
         """H v (or H^T v) into a new vector on the device of v."""
         torch = _torch()
         ishost = not isinstance(v, torch.Tensor)
         vt = torch.from_numpy(np.asarray(v, dtype=np.float64)).cuda() if ishost else v
         m = self.shape[1] if transpose else self.shape[0]
         out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=torch.float64, device=vt.device)
-        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis, tuned)
+        self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis, tuned,
+                 far_precision)
         if ishost:
             torch.cuda.synchronize()
             return out.cpu().numpy()
@@ -297,14 +302,16 @@ class KernelMatrix:
                                    0, _stream_ptr(stream)))
         return gb, gy
 
-    def graph(self, out, v, *, beta=0, matrix_free=False, cheb_basis=False, tuned=False):
+    def graph(self, out, v, *, beta=0, matrix_free=False, cheb_basis=False, tuned=False,
+              far_precision="f64"):
         """Capture `out = beta*out + H v` (device tensors, 1-D or (n, k)) into an
         executable HIP graph; returns a ProductGraph whose launch() replays it."""
         U, V = _Vec(out, True), _Vec(v, False)
         if U.host is not None or V.host is not None:
             raise TypeError("graphs need device tensors")
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
-            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0)
+            (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
+            _FAR[far_precision]
         g = ctypes.c_void_p()
         check(lib.hm_graph_matvec(self._h, U.ptr(), V.ptr(), 0, 0, U.inc, V.inc, U.k, U.ld, V.ld,
                                   beta, flags, ctypes.byref(g)))

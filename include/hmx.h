@@ -58,7 +58,11 @@ enum hm_mv_flags {
     HM_MV_MATRIX_FREE = 1,       /* regenerate factors even if the plan stores them */
     HM_MV_CHEB_BASIS = 2,        /* apply the far-field basis in Chebyshev form even if
                                     the plan stores U and V */
-    HM_MV_TUNED = 4              /* use the launch shape and mode chosen by hm_plan_tune */
+    HM_MV_TUNED = 4,             /* use the launch shape and mode chosen by hm_plan_tune */
+    HM_MV_FAR_F32 = 8,           /* far field (Chebyshev form) on FP32 MFMA, FP32 accumulate;
+                                    points, differences, near field and F stay FP64 */
+    HM_MV_FAR_BF16X3 = 16        /* far field on BF16 MFMA with operands split hi + lo
+                                    (hi*hi + hi*lo + lo*hi), FP32 accumulate */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
