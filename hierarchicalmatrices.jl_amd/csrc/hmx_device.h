@@ -4,21 +4,28 @@
 #include <hip/hip_runtime.h>
 #include "hmx_internal.h"
 
-// Device-side view of a plan (all pointers are device pointers).
-struct DevPlan {
+template <class T> struct Real2;
+template <> struct Real2<double> { using type = double2; };
+template <> struct Real2<float> { using type = float2; };
+
+// Device-side view of a plan (all pointers are device pointers). T is the
+// element type of the plan: double (Float64) or float (Float32).
+template <class T>
+struct DevPlanT {
+    using V2 = typename Real2<T>::type;
     int kid;       // enum hm_kernel
     int r;         // Chebyshev rank
     int64_t nx, ny;
-    const double *x, *y;     // points (non-increasing)
-    const double *lam;       // r barycentric weights
+    const T *x, *y;          // points (non-increasing)
+    const T *lam;            // r barycentric weights
     const DCluster *rcl;     // row clusters
     const DCluster *ccl;     // col clusters
     int32_t nrc, ncc;
-    const double *rnodes;    // nrc*r mapped nodes (x side)
-    const double *cnodes;    // ncc*r mapped nodes (y side)
-    const double2 *rmap;     // per row cluster: (center, 1/halfwidth) of its interval
-    const double2 *cmap;     // per col cluster
-    const double *chebA;     // r x r, A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r (row-major)
+    const T *rnodes;         // nrc*r mapped nodes (x side)
+    const T *cnodes;         // ncc*r mapped nodes (y side)
+    const V2 *rmap;          // per row cluster: (center, 1/halfwidth) of its interval
+    const V2 *cmap;          // per col cluster
+    const T *chebA;          // r x r, A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r (row-major)
     const DFarBlk *far;
     int32_t nfar;
     const int32_t *far_by_rc;  // far ids grouped by row cluster (CSR in DCluster.far_*)
@@ -32,19 +39,25 @@ struct DevPlan {
     const DTask *rtask, *ctask;  // upward-pass tasks over row / col clusters
     int32_t nrtask, nctask;
     int64_t nslots_r, nslots_c;
-    double *A, *U, *V, *F;     // stored factors (null when matrix-free)
+    T *A, *U, *V, *F;          // stored factors (null when matrix-free)
 };
 
-struct MvArgs {
-    double *y;
-    const double *x;
+template <class T>
+struct MvArgsT {
+    T *y;
+    const T *x;
     int64_t incy, incx;
     int64_t ldy, ldx;  // stride between right-hand sides
     int beta;
-    double *partial;  // >= max(nslots) * r * kb
-    double *w;        // >= max(nrc, ncc) * r * kb
-    double *z;        // >= max(nrc, ncc) * r * kb
+    T *partial;  // >= max(nslots) * r * kb
+    T *w;        // >= max(nrc, ncc) * r * kb
+    T *z;        // >= max(nrc, ncc) * r * kb
 };
+
+using DevPlan = DevPlanT<double>;
+using DevPlanF = DevPlanT<float>;
+using MvArgs = MvArgsT<double>;
+using MvArgsF = MvArgsT<float>;
 
 // compression: dense blocks, U, V, F (bit-exact operation order)
 hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
@@ -69,4 +82,15 @@ hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, con
                               const int32_t *kind, const int32_t *id, double *out, hipStream_t s);
 // number of non-finite values among generated factors (after compression)
 hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long long *count,
+                                      hipStream_t s);
+
+// The same passes for FP32 plans (ranks up to 32; no reduced-precision form).
+hipError_t hmx_launch_compress(const DevPlanF &p, hipStream_t s);
+hipError_t hmx_launch_matvec(const DevPlanF &p, const MvArgsF &a, bool transpose, int mode, int kb,
+                             hipStream_t s, const hipEvent_t *ev = nullptr);
+hipError_t hmx_launch_grad(const DevPlanF &p, const MvArgsF &a, bool transpose, float *y2,
+                           int64_t incy2, const float *s, int64_t incs, hipStream_t stream);
+hipError_t hmx_launch_entries(const DevPlanF &p, int64_t n, const int64_t *i, const int64_t *j,
+                              const int32_t *kind, const int32_t *id, float *out, hipStream_t s);
+hipError_t hmx_launch_count_nonfinite(const float *v, int64_t n, unsigned long long *count,
                                       hipStream_t s);

@@ -3,15 +3,15 @@
 
 hipError_t hmx_matvec_r64(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb,
                           hipStream_t s, const hipEvent_t *ev) {
-    return hmxk::matvec_r<64>(p, a, tr, mode, kb, s, ev);
+    return hmxk::matvec_r<double, 64>(p, a, tr, mode, kb, s, ev);
 }
 
 hipError_t hmx_grad_r64(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
                         const double *sv, int64_t incs, hipStream_t s) {
-    return hmxk::grad_r<64>(p, a, tr, y2, incy2, sv, incs, s);
+    return hmxk::grad_r<double, 64>(p, a, tr, y2, incy2, sv, incs, s);
 }
 
-hipError_t hmx_gen_basis_r64(const DevPlan &p, hipStream_t s) { return hmxk::gen_basis_r<64>(p, s); }
+hipError_t hmx_gen_basis_r64(const DevPlan &p, hipStream_t s) { return hmxk::gen_basis_r<double, 64>(p, s); }
 
 hipError_t hmx_matvec_mixed_r64(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
                                 hipStream_t s) {

@@ -1,10 +1,11 @@
 // kernels.hip -- public launchers of the gfx950 kernels: dispatch on the rank
-// bucket (8, 16, 24, 32, 48, 64) to the instantiations in inst_r*.hip.
+// bucket to the instantiations in inst_r*.hip (FP64 plans: 8, 16, 24, 32, 48,
+// 64) and inst_f32_r*.hip (FP32 plans: 8, 16, 32).
 #include "kernels_impl.h"
 
 hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s) {
-    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<<<p.ndense, hmxk::WG, 0, s>>>(p);
-    if (p.nfar > 0 && p.F) hmxk::k_gen_F<<<p.nfar, hmxk::WG, 0, s>>>(p);
+    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<double><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    if (p.nfar > 0 && p.F) hmxk::k_gen_F<double><<<p.nfar, hmxk::WG, 0, s>>>(p);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || (!p.U && !p.V)) return err;
     if (p.r <= 8) return hmx_gen_basis_r8(p, s);
@@ -39,7 +40,7 @@ hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, con
                               const int32_t *kind, const int32_t *id, double *out,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hmxk::k_entries<<<(int)((n + hmxk::WG - 1) / hmxk::WG), hmxk::WG, 0, s>>>(p, n, i, j, kind, id, out);
+    hmxk::k_entries<double><<<(int)((n + hmxk::WG - 1) / hmxk::WG), hmxk::WG, 0, s>>>(p, n, i, j, kind, id, out);
     return hipGetLastError();
 }
 
@@ -49,7 +50,7 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
     if (n <= 0) return hipSuccess;
     int64_t nb = (n + hmxk::WG - 1) / hmxk::WG;
     if (nb > 1024) nb = 1024;
-    hmxk::k_count_nonfinite<<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
+    hmxk::k_count_nonfinite<double><<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
     return hipGetLastError();
 }
 
@@ -61,4 +62,49 @@ hipError_t hmx_launch_matvec_mixed(const DevPlan &p, const MvArgs &a, int mode, 
     if (p.r <= 32) return hmx_matvec_mixed_r32(p, a, mode, prec, nq, s);
     if (p.r <= 48) return hmx_matvec_mixed_r48(p, a, mode, prec, nq, s);
     return hmx_matvec_mixed_r64(p, a, mode, prec, nq, s);
+}
+
+// ------------------------------------------------------------- FP32 plans
+
+hipError_t hmx_launch_compress(const DevPlanF &p, hipStream_t s) {
+    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<float><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    if (p.nfar > 0 && p.F) hmxk::k_gen_F<float><<<p.nfar, hmxk::WG, 0, s>>>(p);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || (!p.U && !p.V)) return err;
+    if (p.r <= 8) return hmx_gen_basis_f32_r8(p, s);
+    if (p.r <= 16) return hmx_gen_basis_f32_r16(p, s);
+    if (p.r <= 32) return hmx_gen_basis_f32_r32(p, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t hmx_launch_matvec(const DevPlanF &p, const MvArgsF &a, bool tr, int mode, int kb,
+                             hipStream_t s, const hipEvent_t *ev) {
+    if (p.r <= 8) return hmx_matvec_f32_r8(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 16) return hmx_matvec_f32_r16(p, a, tr, mode, kb, s, ev);
+    if (p.r <= 32) return hmx_matvec_f32_r32(p, a, tr, mode, kb, s, ev);
+    return hipErrorInvalidValue;
+}
+
+hipError_t hmx_launch_grad(const DevPlanF &p, const MvArgsF &a, bool tr, float *y2, int64_t incy2,
+                           const float *sv, int64_t incs, hipStream_t s) {
+    if (p.r <= 8) return hmx_grad_f32_r8(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 16) return hmx_grad_f32_r16(p, a, tr, y2, incy2, sv, incs, s);
+    if (p.r <= 32) return hmx_grad_f32_r32(p, a, tr, y2, incy2, sv, incs, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t hmx_launch_entries(const DevPlanF &p, int64_t n, const int64_t *i, const int64_t *j,
+                              const int32_t *kind, const int32_t *id, float *out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hmxk::k_entries<float><<<(int)((n + hmxk::WG - 1) / hmxk::WG), hmxk::WG, 0, s>>>(p, n, i, j, kind, id, out);
+    return hipGetLastError();
+}
+
+hipError_t hmx_launch_count_nonfinite(const float *v, int64_t n, unsigned long long *blockcount,
+                                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t nb = (n + hmxk::WG - 1) / hmxk::WG;
+    if (nb > 1024) nb = 1024;
+    hmxk::k_count_nonfinite<float><<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
+    return hipGetLastError();
 }

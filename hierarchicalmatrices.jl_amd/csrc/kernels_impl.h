@@ -53,48 +53,68 @@ __device__ __forceinline__ int wave_id() {
     return uniform((int)(blockIdx.x * WPB + threadIdx.x / WAVE));
 }
 
-// Reciprocal: hardware estimate refined by two Newton-Raphson steps
-// r <- r + r(1 - d r) (Markstein's FMA formulation); error <= 1 ulp.
-__device__ __forceinline__ double rcp64(double d) {
+// Scalar helpers for the two element types (FP64 plans and FP32 plans).
+template <class T> struct V2Of;
+template <> struct V2Of<double> { using type = double2; };
+template <> struct V2Of<float> { using type = float2; };
+template <class T> using V2 = typename V2Of<T>::type;
+
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double logT(double v) { return log(v); }
+__device__ __forceinline__ float logT(float v) { return logf(v); }
+__device__ __forceinline__ double fabsT(double v) { return fabs(v); }
+__device__ __forceinline__ float fabsT(float v) { return fabsf(v); }
+
+// Reciprocal: hardware estimate refined by Newton-Raphson steps
+// r <- r + r(1 - d r) (Markstein's FMA formulation), two in FP64, one in
+// FP32; error <= 1 ulp.
+__device__ __forceinline__ double rcpT(double d) {
     double r = __builtin_amdgcn_rcp(d);
     double e = __builtin_fma(-d, r, 1.0);
     r = __builtin_fma(r, e, r);
     e = __builtin_fma(-d, r, 1.0);
     return __builtin_fma(r, e, r);
 }
+__device__ __forceinline__ float rcpT(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
 
 // f(x, y) with the reference's rounding sequence (examples/Kernel.jl:34-37).
-__device__ __forceinline__ double kern_exact(int kid, double x, double y) {
+template <class T>
+__device__ __forceinline__ T kern_exact(int kid, T x, T y) {
 #pragma clang fp contract(off)
-    double d = x - y;
+    T d = x - y;
     switch (kid) {
-    case 0: return 1.0 / d;
-    case 1: return 1.0 / (d * d);
-    case 2: return 1.0 / ((d * d) * d);
-    default: return log(fabs(d));
+    case 0: return T(1) / d;
+    case 1: return T(1) / (d * d);
+    case 2: return T(1) / ((d * d) * d);
+    default: return logT(fabsT(d));
     }
 }
 
 // f(x, y) for the matrix-free passes.
-template <int KID>
-__device__ __forceinline__ double kern_fast(double x, double y) {
-    double d = x - y;
-    if constexpr (KID == 0) return rcp64(d);
-    else if constexpr (KID == 1) return rcp64(d * d);
-    else if constexpr (KID == 2) return rcp64(d * d * d);
-    else return log(fabs(d));
+template <class T, int KID>
+__device__ __forceinline__ T kern_fast(T x, T y) {
+    T d = x - y;
+    if constexpr (KID == 0) return rcpT(d);
+    else if constexpr (KID == 1) return rcpT(d * d);
+    else if constexpr (KID == 2) return rcpT(d * d * d);
+    else return logT(fabsT(d));
 }
 
 // Value f(x, y) and derivative df/dx of the kernels, from one reciprocal of
 // d = x - y: Cauchy 1/d, -1/d^2; Coulomb 1/d^2, -2/d^3; 1/d^3, -3/d^4;
 // log|d|, 1/d. All are functions of x - y, so df/dy = -df/dx.
-template <int KID>
-__device__ __forceinline__ void kern_val_dx(double d, double &f, double &fx) {
-    const double r = rcp64(d);
+template <class T, int KID>
+__device__ __forceinline__ void kern_val_dx(T d, T &f, T &fx) {
+    const T r = rcpT(d);
     if constexpr (KID == 0) { f = r; fx = -r * r; }
-    else if constexpr (KID == 1) { const double r2 = r * r; f = r2; fx = -2.0 * r2 * r; }
-    else if constexpr (KID == 2) { const double r2 = r * r; f = r2 * r; fx = -3.0 * r2 * r2; }
-    else { f = log(fabs(d)); fx = r; }
+    else if constexpr (KID == 1) { const T r2 = r * r; f = r2; fx = -T(2) * r2 * r; }
+    else if constexpr (KID == 2) { const T r2 = r * r; f = r2 * r; fx = -T(3) * r2 * r2; }
+    else { f = logT(fabsT(d)); fx = r; }
 }
 
 // ---------------------------------------------------------------- reductions
@@ -106,15 +126,16 @@ __device__ __forceinline__ void kern_val_dx(double d, double &f, double &fx) {
 // right-hand side index of the multi-RHS passes) keep separate sums.
 template <int P, int O, int OMIN = 1>
 struct Halve {
-    static __device__ __forceinline__ void run(double *acc, int lane, int &base) {
+    template <class T>
+    static __device__ __forceinline__ void run(T *acc, int lane, int &base) {
         if constexpr (O >= OMIN && O > 0) {
             if constexpr (P % 2 == 0) {
                 constexpr int H = P / 2;
                 const bool hi = (lane & O) != 0;
 #pragma unroll
                 for (int i = 0; i < H; ++i) {
-                    double send = hi ? acc[i] : acc[i + H];
-                    double keep = hi ? acc[i + H] : acc[i];
+                    T send = hi ? acc[i] : acc[i + H];
+                    T keep = hi ? acc[i + H] : acc[i];
                     acc[i] = keep + __shfl_xor(send, O);
                 }
                 if (hi) base += H;
@@ -143,42 +164,43 @@ struct HalveShape<P, 0, OMIN> {
 
 // ------------------------------------------------------------- compression
 
-__global__ __launch_bounds__(WG) void k_gen_dense(DevPlan p) {
+template <class T>
+__global__ __launch_bounds__(WG) void k_gen_dense(DevPlanT<T> p) {
     const DDense D = p.dense[blockIdx.x];
     const int32_t tot = D.m * D.n;
     for (int32_t e = threadIdx.x; e < tot; e += WG) {
         int32_t j = e / D.m, i = e - j * D.m;
-        p.A[D.aoff + tile_index(i, j, D.n)] = kern_exact(p.kid, p.x[D.row0 + i], p.y[D.col0 + j]);
+        p.A[D.aoff + tile_index(i, j, D.n)] = kern_exact<T>(p.kid, p.x[D.row0 + i], p.y[D.col0 + j]);
     }
 }
 
 // update! (src/BarycentricMatrix.jl:256-287): one lane per point of a chunk,
 // for every cluster covering the chunk.
-template <int RMAX>
+template <class T, int RMAX>
 __global__ __launch_bounds__(WG) void k_gen_basis(const DChunk *ch, int nch, const DFar *fl,
-                                                 const DCluster *cl, const double *nodes,
-                                                 const double *lam, const double *pts,
-                                                 double *B, int r) {
+                                                 const DCluster *cl, const T *nodes,
+                                                 const T *lam, const T *pts,
+                                                 T *B, int r) {
 #pragma clang fp contract(off)
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= nch) return;
     const DChunk c = ch[wid];
     if (lane >= c.np) return;
-    const double xi = pts[c.p0 + lane];
+    const T xi = pts[c.p0 + lane];
     for (int e = c.far_beg; e < c.far_end; ++e) {
         const DFar f = fl[e];
         const DCluster C = cl[f.cl];
-        const double *t = nodes + (int64_t)f.cl * r;
-        double row[RMAX];
-        double s = 0.0;
+        const T *t = nodes + (int64_t)f.cl * r;
+        T row[RMAX];
+        T s = T(0);
 #pragma unroll
         for (int k = 0; k < RMAX; ++k)
-            if (k < r) row[k] = lam[k] * (1.0 / (xi - t[k]));
+            if (k < r) row[k] = lam[k] * (T(1) / (xi - t[k]));
 #pragma unroll
         for (int k = 0; k < RMAX; ++k)
             if (k < r) s += row[k];
-        double *out = B + C.boff + tile_index(f.off + lane, 0, r);
+        T *out = B + C.boff + tile_index(f.off + lane, 0, r);
 #pragma unroll
         for (int k = 0; k < RMAX; ++k)
             if (k < r) out[k * HM_TILE] = row[k] / s;
@@ -186,18 +208,20 @@ __global__ __launch_bounds__(WG) void k_gen_basis(const DChunk *ch, int nch, con
 }
 
 // F[m, n] = f(t_m, s_n), src/BarycentricMatrix.jl:168-175
-__global__ __launch_bounds__(WG) void k_gen_F(DevPlan p) {
+template <class T>
+__global__ __launch_bounds__(WG) void k_gen_F(DevPlanT<T> p) {
     const DFarBlk b = p.far[blockIdx.x];
     const int r = p.r;
-    const double *tx = p.rnodes + (int64_t)b.rc * r;
-    const double *ty = p.cnodes + (int64_t)b.cc * r;
+    const T *tx = p.rnodes + (int64_t)b.rc * r;
+    const T *ty = p.cnodes + (int64_t)b.cc * r;
     for (int e = threadIdx.x; e < r * r; e += WG) {
         int n = e / r, m = e - n * r;
-        p.F[b.foff + e] = kern_exact(p.kid, tx[m], ty[n]);
+        p.F[b.foff + e] = kern_exact<T>(p.kid, tx[m], ty[n]);
     }
 }
 
-__global__ __launch_bounds__(WG) void k_count_nonfinite(const double *v, int64_t n,
+template <class T>
+__global__ __launch_bounds__(WG) void k_count_nonfinite(const T *v, int64_t n,
                                                         unsigned long long *blockcount) {
     __shared__ unsigned int sh[WG];
     unsigned int cnt = 0;
@@ -222,45 +246,45 @@ __global__ __launch_bounds__(WG) void k_count_nonfinite(const double *v, int64_t
 // V (forward) or U (transpose) read from HBM, or, in Chebyshev form, the
 // moments sum_j T_m(xh_j) x_j with xh = (p - center) * inv_halfwidth. A lane
 // works on one (point, right-hand side) pair: lane = point*KB + q.
-template <int RMAX, bool CHEB, int KB>
-__global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *cl, const double *B,
-                                          const double2 *cmapv, const double *pts, const double *x,
-                                          int64_t incx, int64_t ldx, double *partial, int r) {
+template <class T, int RMAX, bool CHEB, int KB>
+__global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *cl, const T *B,
+                                          const V2<T> *cmapv, const T *pts, const T *x,
+                                          int64_t incx, int64_t ldx, T *partial, int r) {
     constexpr int PP = WAVE / KB;  // points per wavefront step
-    __shared__ double sh[WPB][KB][RMAX];
-    const DTask T = tasks[blockIdx.x];
-    const DCluster C = cl[T.cl];
+    __shared__ T sh[WPB][KB][RMAX];
+    const DTask Tk = tasks[blockIdx.x];
+    const DCluster C = cl[Tk.cl];
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const int q = lane % KB, pp = lane / KB;
-    double acc[RMAX];
+    T acc[RMAX];
 #pragma unroll
-    for (int k = 0; k < RMAX; ++k) acc[k] = 0.0;
-    const int64_t first = C.first + T.off;
-    const double *xq = x + (int64_t)q * ldx;
+    for (int k = 0; k < RMAX; ++k) acc[k] = T(0);
+    const int64_t first = C.first + Tk.off;
+    const T *xq = x + (int64_t)q * ldx;
     if constexpr (!CHEB) {
-        for (int i = wv * PP + pp; i < T.len; i += WPB * PP) {
-            const double xv = xq[(first + i) * incx];
-            const double *bp = B + C.boff + tile_index(T.off + i, 0, r);
+        for (int i = wv * PP + pp; i < Tk.len; i += WPB * PP) {
+            const T xv = xq[(first + i) * incx];
+            const T *bp = B + C.boff + tile_index(Tk.off + i, 0, r);
             // columns past r re-read column r-1 (in bounds) and are dropped,
             // so all RMAX loads issue back to back without a branch
-            double bv[RMAX];
+            T bv[RMAX];
 #pragma unroll
             for (int k = 0; k < RMAX; ++k) bv[k] = bp[min(k, r - 1) * HM_TILE];
 #pragma unroll
-            for (int k = 0; k < RMAX; ++k) acc[k] = __builtin_fma(k < r ? bv[k] : 0.0, xv, acc[k]);
+            for (int k = 0; k < RMAX; ++k) acc[k] = fmaT(k < r ? bv[k] : T(0), xv, acc[k]);
         }
     } else {
-        const double2 cm = cmapv[T.cl];
-        for (int i = wv * PP + pp; i < T.len; i += WPB * PP) {
-            const double xv = xq[(first + i) * incx];
-            const double xh = (pts[first + i] - cm.x) * cm.y;
-            const double x2 = xh + xh;
-            double t0 = xv, t1 = xh * xv;  // T_m(xh) * x, by the recurrence
+        const V2<T> cm = cmapv[Tk.cl];
+        for (int i = wv * PP + pp; i < Tk.len; i += WPB * PP) {
+            const T xv = xq[(first + i) * incx];
+            const T xh = (pts[first + i] - cm.x) * cm.y;
+            const T x2 = xh + xh;
+            T t0 = xv, t1 = xh * xv;  // T_m(xh) * x, by the recurrence
             acc[0] += t0;
             if (RMAX > 1) acc[1] += t1;
 #pragma unroll
             for (int k = 2; k < RMAX; ++k) {
-                const double t2 = __builtin_fma(x2, t1, -t0);
+                const T t2 = fmaT(x2, t1, -t0);
                 acc[k] += t2;
                 t0 = t1;
                 t1 = t2;
@@ -278,19 +302,19 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
     __syncthreads();
     if (tid < KB * r) {
         const int qq = tid / r, m = tid % r;
-        double s = 0.0;
+        T s = T(0);
 #pragma unroll
         for (int w = 0; w < WPB; ++w) s += sh[w][qq][m];
-        partial[(T.slot * KB + qq) * r + m] = s;
+        partial[(Tk.slot * KB + qq) * r + m] = s;
     }
 }
 
 // w[(c, q)] = sum over the task slots of cluster c; in Chebyshev form the
 // slots hold moments and w = A mu. One wavefront per cluster: lane = g*r + m
 // handles row m of right-hand sides q = g, g + G, ... (G = 64 / r groups).
-template <bool CHEB, int KB>
-__global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const double *partial,
-                                              const double *chebA, double *w, int r) {
+template <class T, bool CHEB, int KB>
+__global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const T *partial,
+                                              const T *chebA, T *w, int r) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= ncl) return;
@@ -301,14 +325,14 @@ __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, cons
     for (int q0 = 0; q0 < KB; q0 += G) {
         const int q = q0 + g;
         const bool aq = act && q < KB;
-        double acc = 0.0;
+        T acc = T(0);
         if (aq)
             for (int t = 0; t < C.nslots; ++t) acc += partial[((C.pbase + t) * KB + q) * r + m];
         if constexpr (CHEB) {
-            double v = 0.0;
+            T v = T(0);
             for (int k = 0; k < r; ++k) {
-                const double mu = __shfl(acc, (g * r + k) & (WAVE - 1));
-                v = __builtin_fma(chebA[m * r + k], mu, v);
+                const T mu = __shfl(acc, (g * r + k) & (WAVE - 1));
+                v = fmaT(chebA[m * r + k], mu, v);
             }
             acc = v;
         }
@@ -320,10 +344,10 @@ __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, cons
 // (forward: F w[cc]; transpose: F^T w[rc]); in Chebyshev form z is then
 // mapped to coefficients c = A^T z. One wavefront per cluster: lane = g*r + m
 // owns row m of F for right-hand sides q = g, g + G, ...
-template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
-__global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl, int ncl,
-                                                 const int32_t *far_list, const double *w,
-                                                 double *z) {
+template <class T, int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
+__global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster *cl, int ncl,
+                                                 const int32_t *far_list, const T *w,
+                                                 T *z) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= ncl) return;
@@ -336,14 +360,14 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
         const int q = q0 + g;
         const bool aq = act && q < KB;
         const int qa = aq ? q : 0;
-        double acc = 0.0;
+        T acc = T(0);
         for (int e = C.far_beg; e < C.far_end; ++e) {
             const DFarBlk b = p.far[far_list[e]];
             const int src = TRANS ? b.rc : b.cc;
-            const double *ws = w + ((int64_t)src * KB + qa) * r;
-            double fv[RMAX];
+            const T *ws = w + ((int64_t)src * KB + qa) * r;
+            T fv[RMAX];
             if constexpr (!FREE) {
-                const double *F = p.F + b.foff;
+                const T *F = p.F + b.foff;
 #pragma unroll
                 for (int l = 0; l < RMAX; ++l) {
                     const int lc = min(l, r - 1);
@@ -351,24 +375,24 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
                 }
             } else {
                 // forward: F[m, l] = f(tx_m, ty_l); transpose: lane row m is F's column
-                const double *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
-                const double *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
-                const double to = own[m];
+                const T *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
+                const T *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
+                const T to = own[m];
 #pragma unroll
                 for (int l = 0; l < RMAX; ++l) {
-                    const double t = oth[min(l, r - 1)];
-                    fv[l] = TRANS ? kern_fast<KID>(t, to) : kern_fast<KID>(to, t);
+                    const T t = oth[min(l, r - 1)];
+                    fv[l] = TRANS ? kern_fast<T, KID>(t, to) : kern_fast<T, KID>(to, t);
                 }
             }
 #pragma unroll
             for (int l = 0; l < RMAX; ++l)
-                acc = __builtin_fma(l < r ? fv[l] : 0.0, ws[min(l, r - 1)], acc);
+                acc = fmaT(l < r ? fv[l] : T(0), ws[min(l, r - 1)], acc);
         }
         if constexpr (CHEB) {  // coefficients c = A^T z
-            double c = 0.0;
+            T c = T(0);
             for (int k = 0; k < r; ++k) {
-                const double zk = __shfl(acc, (g * r + k) & (WAVE - 1));
-                c = __builtin_fma(p.chebA[k * r + m], zk, c);
+                const T zk = __shfl(acc, (g * r + k) & (WAVE - 1));
+                c = fmaT(p.chebA[k * r + m], zk, c);
             }
             acc = c;
         }
@@ -379,81 +403,81 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlan p, const DCluster *cl,
 // Matrix-free near field of one lane and one right-hand side: sum_j f(p, q_j)
 // v_j (or f(q_j, p) v_j when TR). For the power kernels 1/d^m two terms share
 // one reciprocal: v1/d1^m + v2/d2^m = (v1 d2^m + v2 d1^m) / (d1 d2)^m.
-template <int KID, bool TR>
-__device__ __forceinline__ double near_free(double p, const double *qs, const double *v,
+template <class T, int KID, bool TR>
+__device__ __forceinline__ T near_free(T p, const T *qs, const T *v,
                                             int64_t inc, int n, bool act) {
-    double acc = 0.0;
+    T acc = T(0);
     if constexpr (KID <= 2) {
         int j = 0;
         for (; j + 2 <= n; j += 2) {
-            double d1 = TR ? qs[j] - p : p - qs[j];
-            double d2 = TR ? qs[j + 1] - p : p - qs[j + 1];
+            T d1 = TR ? qs[j] - p : p - qs[j];
+            T d2 = TR ? qs[j + 1] - p : p - qs[j + 1];
             if constexpr (KID == 1) { d1 *= d1; d2 *= d2; }
             if constexpr (KID == 2) { d1 *= d1 * d1; d2 *= d2 * d2; }
-            const double num = __builtin_fma(v[(int64_t)j * inc], d2, v[(int64_t)(j + 1) * inc] * d1);
-            acc = __builtin_fma(num, rcp64(d1 * d2), acc);
+            const T num = fmaT(v[(int64_t)j * inc], d2, v[(int64_t)(j + 1) * inc] * d1);
+            acc = fmaT(num, rcpT(d1 * d2), acc);
         }
         if (j < n) {
-            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
-            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
+            const T f = TR ? kern_fast<T, KID>(qs[j], p) : kern_fast<T, KID>(p, qs[j]);
+            acc = fmaT(f, v[(int64_t)j * inc], acc);
         }
     } else {
         for (int j = 0; j < n; ++j) {
-            const double f = TR ? kern_fast<KID>(qs[j], p) : kern_fast<KID>(p, qs[j]);
-            acc = __builtin_fma(f, v[(int64_t)j * inc], acc);
+            const T f = TR ? kern_fast<T, KID>(qs[j], p) : kern_fast<T, KID>(p, qs[j]);
+            acc = fmaT(f, v[(int64_t)j * inc], acc);
         }
     }
-    return act ? acc : 0.0;
+    return act ? acc : T(0);
 }
 
 // Downward pass. Forward: output = rows; near blocks read by rows; far basis U
 // (or its Chebyshev form). Transpose (KB = 1 only): output = columns; near
 // blocks read by columns; far basis V.
-template <int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
-__global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nch,
+template <class T, int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
+__global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, int nch,
                                             const DNear *nl, const DFar *fl,
-                                            const DCluster *cl, const double *B,
-                                            const double2 *cmapv, const double *pts_out,
-                                            const double *pts_in, const double *z, MvArgs a) {
+                                            const DCluster *cl, const T *B,
+                                            const V2<T> *cmapv, const T *pts_out,
+                                            const T *pts_in, const T *z, MvArgsT<T> a) {
     static_assert(!TRANS || KB == 1, "the transpose handles one right-hand side per pass");
     constexpr int UN = KB == 1 ? 16 : 8;
     // per-wavefront staging of right-hand-side tiles and far coefficients
     constexpr int XS = KB == 1 ? 1 : (WAVE * KB > KB * RMAX ? WAVE * KB : KB * RMAX);
-    __shared__ double xs_all[WPB][XS];
+    __shared__ T xs_all[WPB][XS];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    double *xs = xs_all[(threadIdx.x / WAVE) % WPB];
+    T *xs = xs_all[(threadIdx.x / WAVE) % WPB];
     (void)xs;
     if (wid >= nch) return;
     const int r = p.r;
     const DChunk c = ch[wid];
     const bool act = lane < c.np;
-    double po = 0.0;
-    if constexpr (FREE || CHEB) po = act ? pts_out[c.p0 + lane] : 0.0;
-    double acc[KB];
+    T po = T(0);
+    if constexpr (FREE || CHEB) po = act ? pts_out[c.p0 + lane] : T(0);
+    T acc[KB];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = 0.0;
+    for (int q = 0; q < KB; ++q) acc[q] = T(0);
 
     // ---- near field
     for (int e = c.near_beg; e < c.near_end; ++e) {
         const DNear s = nl[e];
         const DDense D = p.dense[s.blk];
         if constexpr (!TRANS) {
-            const double *xin = a.x + D.col0 * a.incx;
+            const T *xin = a.x + D.col0 * a.incx;
             if constexpr (!FREE) {
-                const double *Aj = p.A + D.aoff + tile_index(s.off + lane, 0, D.n);
+                const T *Aj = p.A + D.aoff + tile_index(s.off + lane, 0, D.n);
                 if constexpr (KB == 1) {
                     int j = 0;
                     for (; j + UN <= D.n; j += UN) {
-                        double av[UN];
+                        T av[UN];
 #pragma unroll
-                        for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : 0.0;
+                        for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(j + u) * HM_TILE] : T(0);
 #pragma unroll
                         for (int u = 0; u < UN; ++u)
-                            acc[0] = __builtin_fma(av[u], xin[(int64_t)(j + u) * a.incx], acc[0]);
+                            acc[0] = fmaT(av[u], xin[(int64_t)(j + u) * a.incx], acc[0]);
                     }
                     for (; j < D.n; ++j)
-                        acc[0] = __builtin_fma(act ? Aj[j * HM_TILE] : 0.0, xin[(int64_t)j * a.incx], acc[0]);
+                        acc[0] = fmaT(act ? Aj[j * HM_TILE] : T(0), xin[(int64_t)j * a.incx], acc[0]);
                 } else {
                     for (int jt = 0; jt < D.n; jt += WAVE) {
                         const int nj = min(WAVE, D.n - jt);
@@ -466,30 +490,30 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                         int j = 0;
                         for (; j + UN <= nj; j += UN) {
-                            double av[UN];
+                            T av[UN];
 #pragma unroll
-                            for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(jt + j + u) * HM_TILE] : 0.0;
+                            for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(jt + j + u) * HM_TILE] : T(0);
 #pragma unroll
                             for (int u = 0; u < UN; ++u)
 #pragma unroll
                                 for (int q = 0; q < KB; ++q)
-                                    acc[q] = __builtin_fma(av[u], xs[(j + u) * KB + q], acc[q]);
+                                    acc[q] = fmaT(av[u], xs[(j + u) * KB + q], acc[q]);
                         }
                         for (; j < nj; ++j) {
-                            const double av = act ? Aj[(jt + j) * HM_TILE] : 0.0;
+                            const T av = act ? Aj[(jt + j) * HM_TILE] : T(0);
 #pragma unroll
-                            for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(av, xs[j * KB + q], acc[q]);
+                            for (int q = 0; q < KB; ++q) acc[q] = fmaT(av, xs[j * KB + q], acc[q]);
                         }
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                     }
                 }
             } else if constexpr (KB == 1) {
-                acc[0] += near_free<KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
+                acc[0] += near_free<T, KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
             } else {
                 // right-hand sides of a 64-column tile staged in LDS, read as
                 // broadcasts; one kernel value per (row, column)
-                const double *yin = pts_in + D.col0;
+                const T *yin = pts_in + D.col0;
                 for (int jt = 0; jt < D.n; jt += WAVE) {
                     const int nj = min(WAVE, D.n - jt);
                     if (lane < nj) {
@@ -500,35 +524,35 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                     for (int j = 0; j < nj; ++j) {
-                        const double f = act ? kern_fast<KID>(po, yin[jt + j]) : 0.0;
+                        const T f = act ? kern_fast<T, KID>(po, yin[jt + j]) : T(0);
 #pragma unroll
-                        for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(f, xs[j * KB + q], acc[q]);
+                        for (int q = 0; q < KB; ++q) acc[q] = fmaT(f, xs[j * KB + q], acc[q]);
                     }
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 }
             }
                 } else {
-            const double *xin = a.x + D.row0 * a.incx;
+            const T *xin = a.x + D.row0 * a.incx;
             if constexpr (!FREE) {
                 // lanes run down the rows of the block, 64 accumulators for the
                 // chunk's columns, then a recursive-halving transpose-reduce
-                double accT[WAVE];
+                T accT[WAVE];
 #pragma unroll
-                for (int q = 0; q < WAVE; ++q) accT[q] = 0.0;
+                for (int q = 0; q < WAVE; ++q) accT[q] = T(0);
                 for (int ib = 0; ib < D.m; ib += WAVE) {
                     const bool ra = ib + lane < D.m;
-                    const double xv = ra ? xin[(int64_t)(ib + lane) * a.incx] : 0.0;
-                    const double *Ac = p.A + D.aoff + tile_index(ib + lane, s.off, D.n);
+                    const T xv = ra ? xin[(int64_t)(ib + lane) * a.incx] : T(0);
+                    const T *Ac = p.A + D.aoff + tile_index(ib + lane, s.off, D.n);
 #pragma unroll
                     for (int q = 0; q < WAVE; ++q)
-                        if (q < c.np) accT[q] += (ra ? Ac[q * HM_TILE] : 0.0) * xv;
+                        if (q < c.np) accT[q] += (ra ? Ac[q * HM_TILE] : T(0)) * xv;
                 }
                 int base = 0;
                 Halve<WAVE, WAVE / 2>::run(accT, lane, base);
                 acc[0] += accT[0];
             } else {
-                acc[0] += near_free<KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act);
+                acc[0] += near_free<T, KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act);
             }
         }
     }
@@ -536,11 +560,11 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
     // ---- far field: acc += B[chunk rows, :] z[cluster]
     for (int e = c.far_beg; e < c.far_end; ++e) {
         const DFar f = fl[e];
-        const double *zc = z + (int64_t)f.cl * KB * r;
+        const T *zc = z + (int64_t)f.cl * KB * r;
         if constexpr (!CHEB) {
             const DCluster C = cl[f.cl];
-            const double *bp = B + C.boff + tile_index(f.off + lane, 0, r);
-            const double *zs = zc;
+            const T *bp = B + C.boff + tile_index(f.off + lane, 0, r);
+            const T *zs = zc;
             if constexpr (KB > 1) {
                 for (int idx = lane; idx < KB * r; idx += WAVE) xs[idx] = zc[idx];
                 __builtin_amdgcn_wave_barrier();
@@ -549,38 +573,38 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
             }
             int k = 0;
             for (; k + 8 <= r; k += 8) {
-                double bv[8];
+                T bv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) bv[u] = act ? bp[(k + u) * HM_TILE] : 0.0;
+                for (int u = 0; u < 8; ++u) bv[u] = act ? bp[(k + u) * HM_TILE] : T(0);
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
 #pragma unroll
-                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv[u], zs[q * r + k + u], acc[q]);
+                    for (int q = 0; q < KB; ++q) acc[q] = fmaT(bv[u], zs[q * r + k + u], acc[q]);
             }
             for (; k < r; ++k) {
-                const double bv = act ? bp[k * HM_TILE] : 0.0;
+                const T bv = act ? bp[k * HM_TILE] : T(0);
 #pragma unroll
-                for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(bv, zs[q * r + k], acc[q]);
+                for (int q = 0; q < KB; ++q) acc[q] = fmaT(bv, zs[q * r + k], acc[q]);
             }
             if constexpr (KB > 1) {
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             }
         } else {
-            const double2 cm = cmapv[f.cl];
-            const double xh = (po - cm.x) * cm.y;
-            const double x2 = xh + xh;
+            const V2<T> cm = cmapv[f.cl];
+            const T xh = (po - cm.x) * cm.y;
+            const T x2 = xh + xh;
             if constexpr (KB == 1) {
                 // Clenshaw: b_k = c_k + 2 xh b_{k+1} - b_{k+2}; sum = c_0 + xh b_1 - b_2
-                double b1 = 0.0, b2 = 0.0;
+                T b1 = T(0), b2 = T(0);
 #pragma unroll
                 for (int k = RMAX - 1; k >= 1; --k) {
-                    const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
-                    const double b0 = __builtin_fma(x2, b1, ck - b2);
+                    const T ck = k < r ? zc[min(k, r - 1)] : T(0);
+                    const T b0 = fmaT(x2, b1, ck - b2);
                     b2 = b1;
                     b1 = b0;
                 }
-                const double sum = __builtin_fma(xh, b1, zc[0] - b2);
+                const T sum = fmaT(xh, b1, zc[0] - b2);
                 if (act) acc[0] += sum;
             } else {
                 // coefficients of the cluster (KB x r) staged in LDS; T_k(xh) once,
@@ -588,19 +612,19 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
                 for (int idx = lane; idx < KB * r; idx += WAVE) xs[idx] = zc[idx];
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                double t0 = 1.0, t1 = xh;
+                T t0 = T(1), t1 = xh;
 #pragma unroll
                 for (int k = 0; k < RMAX; ++k) {
-                    const double tk = k == 0 ? t0 : (k == 1 ? t1 : 0.0);
-                    double tv = tk;
+                    const T tk = k == 0 ? t0 : (k == 1 ? t1 : T(0));
+                    T tv = tk;
                     if (k >= 2) {
-                        tv = __builtin_fma(x2, t1, -t0);
+                        tv = fmaT(x2, t1, -t0);
                         t0 = t1;
                         t1 = tv;
                     }
-                    const double tu = (act && k < r) ? tv : 0.0;
+                    const T tu = (act && k < r) ? tv : T(0);
 #pragma unroll
-                    for (int q = 0; q < KB; ++q) acc[q] = __builtin_fma(tu, xs[q * r + min(k, r - 1)], acc[q]);
+                    for (int q = 0; q < KB; ++q) acc[q] = fmaT(tu, xs[q * r + min(k, r - 1)], acc[q]);
                 }
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -610,8 +634,8 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
     if (act) {
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
-            double *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
-            *yo = (a.beta ? *yo : 0.0) + acc[q];
+            T *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
+            *yo = (a.beta ? *yo : T(0)) + acc[q];
         }
     }
 }
@@ -624,12 +648,12 @@ __global__ __launch_bounds__(WG) void k_down(DevPlan p, const DChunk *ch, int nc
 // the exact derivative of val_p with respect to the point p. The near field is
 // regenerated from the points. DM = 1 writes y = beta*y + s .* der; DM = 2
 // writes y = beta*y + val and y2 = s .* der.
-template <int RMAX, int KID, bool TRANS, int DM>
-__global__ __launch_bounds__(WG) void k_down_grad(DevPlan p, const DChunk *ch, int nch,
+template <class T, int RMAX, int KID, bool TRANS, int DM>
+__global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *ch, int nch,
                                                  const DNear *nl, const DFar *fl,
-                                                 const double2 *cmapv, const double *pts_out,
-                                                 const double *pts_in, const double *z, MvArgs a,
-                                                 double *y2, int64_t incy2, const double *sv,
+                                                 const V2<T> *cmapv, const T *pts_out,
+                                                 const T *pts_in, const T *z, MvArgsT<T> a,
+                                                 T *y2, int64_t incy2, const T *sv,
                                                  int64_t incs) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
@@ -637,93 +661,94 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlan p, const DChunk *ch, i
     const int r = p.r;
     const DChunk c = ch[wid];
     const bool act = lane < c.np;
-    const double po = act ? pts_out[c.p0 + lane] : 0.0;
-    double accv = 0.0, accd = 0.0;
+    const T po = act ? pts_out[c.p0 + lane] : T(0);
+    T accv = T(0), accd = T(0);
     for (int e = c.near_beg; e < c.near_end; ++e) {
         const DNear s = nl[e];
         const DDense D = p.dense[s.blk];
         const int64_t q0 = TRANS ? D.row0 : D.col0;
         const int nq = TRANS ? D.m : D.n;
-        const double *xin = a.x + q0 * a.incx;
-        const double *qin = pts_in + q0;
+        const T *xin = a.x + q0 * a.incx;
+        const T *qin = pts_in + q0;
         for (int j = 0; j < nq; ++j) {
-            const double d = TRANS ? qin[j] - po : po - qin[j];
-            double f, fx;
-            kern_val_dx<KID>(d, f, fx);
-            const double v = xin[(int64_t)j * a.incx];
-            if constexpr (DM == 2) accv = __builtin_fma(f, v, accv);
-            accd = __builtin_fma(TRANS ? -fx : fx, v, accd);
+            const T d = TRANS ? qin[j] - po : po - qin[j];
+            T f, fx;
+            kern_val_dx<T, KID>(d, f, fx);
+            const T v = xin[(int64_t)j * a.incx];
+            if constexpr (DM == 2) accv = fmaT(f, v, accv);
+            accd = fmaT(TRANS ? -fx : fx, v, accd);
         }
     }
     for (int e = c.far_beg; e < c.far_end; ++e) {
         const DFar f = fl[e];
-        const double *zc = z + (int64_t)f.cl * r;
-        const double2 cm = cmapv[f.cl];
-        const double xh = (po - cm.x) * cm.y;
-        const double x2 = xh + xh;
-        double b1 = 0.0, b2 = 0.0, d1 = 0.0, d2 = 0.0;
+        const T *zc = z + (int64_t)f.cl * r;
+        const V2<T> cm = cmapv[f.cl];
+        const T xh = (po - cm.x) * cm.y;
+        const T x2 = xh + xh;
+        T b1 = T(0), b2 = T(0), d1 = T(0), d2 = T(0);
 #pragma unroll
         for (int k = RMAX - 1; k >= 1; --k) {
-            const double ck = k < r ? zc[min(k, r - 1)] : 0.0;
-            const double b0 = __builtin_fma(x2, b1, ck - b2);
-            const double d0 = __builtin_fma(x2, d1, 2.0 * b1 - d2);
+            const T ck = k < r ? zc[min(k, r - 1)] : T(0);
+            const T b0 = fmaT(x2, b1, ck - b2);
+            const T d0 = fmaT(x2, d1, T(2) * b1 - d2);
             b2 = b1; b1 = b0;
             d2 = d1; d1 = d0;
         }
-        if constexpr (DM == 2) accv += __builtin_fma(xh, b1, zc[0] - b2);
-        accd = __builtin_fma(__builtin_fma(xh, d1, b1 - d2), cm.y, accd);
+        if constexpr (DM == 2) accv += fmaT(xh, b1, zc[0] - b2);
+        accd = fmaT(fmaT(xh, d1, b1 - d2), cm.y, accd);
     }
     if (act) {
         const int64_t i = c.p0 + lane;
-        const double sc = sv[i * incs];
-        double *yo = a.y + i * a.incy;
+        const T sc = sv[i * incs];
+        T *yo = a.y + i * a.incy;
         if constexpr (DM == 1) {
-            *yo = (a.beta ? *yo : 0.0) + sc * accd;
+            *yo = (a.beta ? *yo : T(0)) + sc * accd;
         } else {
-            *yo = (a.beta ? *yo : 0.0) + accv;
-            double *yd = y2 + i * incy2;
-            *yd = (a.beta ? *yd : 0.0) + sc * accd;
+            *yo = (a.beta ? *yo : T(0)) + accv;
+            T *yd = y2 + i * incy2;
+            *yd = (a.beta ? *yd : T(0)) + sc * accd;
         }
     }
 }
 
 // getindex: one thread per requested entry
-__global__ __launch_bounds__(WG) void k_entries(DevPlan p, int64_t n, const int64_t *ii,
+template <class T>
+__global__ __launch_bounds__(WG) void k_entries(DevPlanT<T> p, int64_t n, const int64_t *ii,
                                                const int64_t *jj, const int32_t *kind,
-                                               const int32_t *id, double *out) {
+                                               const int32_t *id, T *out) {
 #pragma clang fp contract(off)
     const int64_t t = (int64_t)blockIdx.x * WG + threadIdx.x;
     if (t >= n) return;
     const int64_t i = ii[t], j = jj[t];
     const int r = p.r;
-    double v = 0.0;
+    T v = T(0);
     if (kind[t] == 1) {
         const DDense D = p.dense[id[t]];
-        v = p.A ? p.A[D.aoff + tile_index(i - D.row0, j - D.col0, D.n)] : kern_exact(p.kid, p.x[i], p.y[j]);
+        v = p.A ? p.A[D.aoff + tile_index(i - D.row0, j - D.col0, D.n)] : kern_exact<T>(p.kid, p.x[i], p.y[j]);
     } else if (kind[t] == 2) {
         const DFarBlk b = p.far[id[t]];
         const DCluster R = p.rcl[b.rc], Cc = p.ccl[b.cc];
-        double ur[64], vr[64];
+        T ur[64], vr[64];
         if (p.U) {
             for (int k = 0; k < r; ++k) {
                 ur[k] = p.U[R.boff + tile_index(i - R.first, k, r)];
                 vr[k] = p.V[Cc.boff + tile_index(j - Cc.first, k, r)];
             }
         } else {
-            double su = 0.0, sv = 0.0;
+            T su = T(0), sv = T(0);
             for (int k = 0; k < r; ++k) {
-                ur[k] = p.lam[k] * (1.0 / (p.x[i] - p.rnodes[(int64_t)b.rc * r + k]));
-                vr[k] = p.lam[k] * (1.0 / (p.y[j] - p.cnodes[(int64_t)b.cc * r + k]));
+                ur[k] = p.lam[k] * (T(1) / (p.x[i] - p.rnodes[(int64_t)b.rc * r + k]));
+                vr[k] = p.lam[k] * (T(1) / (p.y[j] - p.cnodes[(int64_t)b.cc * r + k]));
                 su += ur[k];
                 sv += vr[k];
             }
             for (int k = 0; k < r; ++k) { ur[k] /= su; vr[k] /= sv; }
         }
         for (int k = 0; k < r; ++k) {
-            double tmp = 0.0;
+            T tmp = T(0);
             for (int l = 0; l < r; ++l) {
-                double F = p.F ? p.F[b.foff + k + (int64_t)r * l]
-                               : kern_exact(p.kid, p.rnodes[(int64_t)b.rc * r + k],
+                T F = p.F ? p.F[b.foff + k + (int64_t)r * l]
+                               : kern_exact<T>(p.kid, p.rnodes[(int64_t)b.rc * r + k],
                                             p.cnodes[(int64_t)b.cc * r + l]);
                 tmp += F * vr[l];
             }
@@ -735,20 +760,20 @@ __global__ __launch_bounds__(WG) void k_entries(DevPlan p, int64_t n, const int6
 
 inline int nblk(int64_t waves) { return (int)((waves + WPB - 1) / WPB); }
 
-template <int RMAX>
-hipError_t gen_basis_r(const DevPlan &p, hipStream_t s) {
+template <class T, int RMAX>
+hipError_t gen_basis_r(const DevPlanT<T> &p, hipStream_t s) {
     if (p.ncch > 0 && p.V)
-        k_gen_basis<RMAX><<<nblk(p.ncch), WG, 0, s>>>(p.cch, p.ncch, p.cfar, p.ccl, p.cnodes, p.lam,
+        k_gen_basis<T, RMAX><<<nblk(p.ncch), WG, 0, s>>>(p.cch, p.ncch, p.cfar, p.ccl, p.cnodes, p.lam,
                                                      p.y, p.V, p.r);
     if (p.nrch > 0 && p.U)
-        k_gen_basis<RMAX><<<nblk(p.nrch), WG, 0, s>>>(p.rch, p.nrch, p.rfar, p.rcl, p.rnodes, p.lam,
+        k_gen_basis<T, RMAX><<<nblk(p.nrch), WG, 0, s>>>(p.rch, p.nrch, p.rfar, p.rcl, p.rnodes, p.lam,
                                                      p.x, p.U, p.r);
     return hipGetLastError();
 }
 
 // MODE: 0 stored, 1 hybrid, 2 free
-template <int RMAX, int MODE, int KID, int KB>
-hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
+template <class T, int RMAX, int MODE, int KID, int KB>
+hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
     constexpr bool CHEB = MODE != 0;
     constexpr bool FREE = MODE == 2;
     auto mark = [&](int q) {
@@ -759,34 +784,34 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     const int in_ntask = tr ? p.nrtask : p.nctask;
     const DCluster *in_cl = tr ? p.rcl : p.ccl;
     const int in_ncl = tr ? p.nrc : p.ncc;
-    const double *in_B = tr ? p.U : p.V;
-    const double2 *in_map = tr ? p.rmap : p.cmap;
-    const double *in_pts = tr ? p.x : p.y;
+    const T *in_B = tr ? p.U : p.V;
+    const V2<T> *in_map = tr ? p.rmap : p.cmap;
+    const T *in_pts = tr ? p.x : p.y;
     const DChunk *out_ch = tr ? p.cch : p.rch;
     const int out_n = tr ? p.ncch : p.nrch;
     const DNear *out_near = tr ? p.cnear : p.rnear;
     const DFar *out_far = tr ? p.cfar : p.rfar;
     const DCluster *out_cl = tr ? p.ccl : p.rcl;
     const int out_ncl = tr ? p.ncc : p.nrc;
-    const double *out_B = tr ? p.V : p.U;
-    const double2 *out_map = tr ? p.cmap : p.rmap;
-    const double *out_pts = tr ? p.y : p.x;
+    const T *out_B = tr ? p.V : p.U;
+    const V2<T> *out_map = tr ? p.cmap : p.rmap;
+    const T *out_pts = tr ? p.y : p.x;
     const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
 
     mark(0);
     if (p.nfar > 0 && in_ntask > 0) {
-        k_up<RMAX, CHEB, KB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x,
+        k_up<T, RMAX, CHEB, KB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x,
                                                       a.incx, a.ldx, a.partial, p.r);
         mark(1);
-        k_reduce<CHEB, KB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial,
+        k_reduce<T, CHEB, KB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial,
                                                                       p.chebA, a.w, p.r);
         mark(2);
         if (tr) {
             if constexpr (KB == 1)
-                k_translate<RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_ncl), WG, 0, s>>>(
+                k_translate<T, RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_ncl), WG, 0, s>>>(
                     p, out_cl, out_ncl, out_far_list, a.w, a.z);
         } else {
-            k_translate<RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_ncl), WG, 0, s>>>(
+            k_translate<T, RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_ncl), WG, 0, s>>>(
                 p, out_cl, out_ncl, out_far_list, a.w, a.z);
         }
     } else {
@@ -797,10 +822,10 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     if (out_n > 0) {
         if (tr) {
             if constexpr (KB == 1)
-                k_down<RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_n), WG, 0, s>>>(
+                k_down<T, RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_n), WG, 0, s>>>(
                     p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
         } else {
-            k_down<RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_n), WG, 0, s>>>(
+            k_down<T, RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_n), WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
         }
     }
@@ -808,54 +833,54 @@ hipError_t matvec_t(const DevPlan &p, const MvArgs &a, bool tr, hipStream_t s, c
     return hipGetLastError();
 }
 
-template <int RMAX, int KID>
-hipError_t grad_t(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
-                  const double *sv, int64_t incs, hipStream_t s) {
+template <class T, int RMAX, int KID>
+hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int64_t incy2,
+                  const T *sv, int64_t incs, hipStream_t s) {
     const DTask *in_task = tr ? p.rtask : p.ctask;
     const int in_ntask = tr ? p.nrtask : p.nctask;
     const DCluster *in_cl = tr ? p.rcl : p.ccl;
     const int in_ncl = tr ? p.nrc : p.ncc;
-    const double2 *in_map = tr ? p.rmap : p.cmap;
-    const double *in_pts = tr ? p.x : p.y;
+    const V2<T> *in_map = tr ? p.rmap : p.cmap;
+    const T *in_pts = tr ? p.x : p.y;
     const DChunk *out_ch = tr ? p.cch : p.rch;
     const int out_n = tr ? p.ncch : p.nrch;
     const DNear *out_near = tr ? p.cnear : p.rnear;
     const DFar *out_far = tr ? p.cfar : p.rfar;
     const DCluster *out_cl = tr ? p.ccl : p.rcl;
     const int out_ncl = tr ? p.ncc : p.nrc;
-    const double2 *out_map = tr ? p.cmap : p.rmap;
-    const double *out_pts = tr ? p.y : p.x;
+    const V2<T> *out_map = tr ? p.cmap : p.rmap;
+    const T *out_pts = tr ? p.y : p.x;
     const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
     if (p.nfar > 0 && in_ntask > 0) {
-        k_up<RMAX, true, 1><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x,
+        k_up<T, RMAX, true, 1><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x,
                                                      a.incx, 0, a.partial, p.r);
-        k_reduce<true, 1><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
+        k_reduce<T, true, 1><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
         if (tr)
-            k_translate<RMAX, true, KID, true, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
+            k_translate<T, RMAX, true, KID, true, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
                 p, out_cl, out_ncl, out_far_list, a.w, a.z);
         else
-            k_translate<RMAX, true, KID, false, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
+            k_translate<T, RMAX, true, KID, false, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
                 p, out_cl, out_ncl, out_far_list, a.w, a.z);
     }
     if (out_n > 0) {
         if (tr)
-            k_down_grad<RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
+            k_down_grad<T, RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
         else
-            k_down_grad<RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(
+            k_down_grad<T, RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
     }
     return hipGetLastError();
 }
 
-template <int RMAX>
-hipError_t grad_r(const DevPlan &p, const MvArgs &a, bool tr, double *y2, int64_t incy2,
-                  const double *sv, int64_t incs, hipStream_t s) {
+template <class T, int RMAX>
+hipError_t grad_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int64_t incy2,
+                  const T *sv, int64_t incs, hipStream_t s) {
     switch (p.kid) {
-    case 0: return grad_t<RMAX, 0>(p, a, tr, y2, incy2, sv, incs, s);
-    case 1: return grad_t<RMAX, 1>(p, a, tr, y2, incy2, sv, incs, s);
-    case 2: return grad_t<RMAX, 2>(p, a, tr, y2, incy2, sv, incs, s);
-    default: return grad_t<RMAX, 3>(p, a, tr, y2, incy2, sv, incs, s);
+    case 0: return grad_t<T, RMAX, 0>(p, a, tr, y2, incy2, sv, incs, s);
+    case 1: return grad_t<T, RMAX, 1>(p, a, tr, y2, incy2, sv, incs, s);
+    case 2: return grad_t<T, RMAX, 2>(p, a, tr, y2, incy2, sv, incs, s);
+    default: return grad_t<T, RMAX, 3>(p, a, tr, y2, incy2, sv, incs, s);
     }
 }
 
@@ -1014,7 +1039,7 @@ __global__ __launch_bounds__(WG) void k_down_mfma(DevPlan p, const DChunk *ch, i
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             for (int j = 0; j < nj; ++j) {
                 double f;
-                if constexpr (FREE) f = act ? kern_fast<KID>(po, pts_in[Dd.col0 + jt + j]) : 0.0;
+                if constexpr (FREE) f = act ? kern_fast<double, KID>(po, pts_in[Dd.col0 + jt + j]) : 0.0;
                 else f = act ? Aj[(jt + j) * HM_TILE] : 0.0;
 #pragma unroll
                 for (int q = 0; q < MQ; ++q) acc[q] = __builtin_fma(f, xs[j * MQ + q], acc[q]);
@@ -1111,8 +1136,8 @@ hipError_t matvec_mixed_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t
     if (p.nfar > 0 && p.nctask > 0) {
         k_up_mfma<RP, PREC><<<p.nctask, WG, 0, s>>>(p.ctask, p.ccl, p.cmap, p.y, a.x, a.incx,
                                                      a.ldx, nq, a.partial, p.r);
-        k_reduce<true, MQ><<<nblk(p.ncc), WG, 0, s>>>(p.ccl, p.ncc, a.partial, p.chebA, a.w, p.r);
-        k_translate<RP, FREE, KID, false, true, MQ><<<nblk(p.nrc), WG, 0, s>>>(
+        k_reduce<double, true, MQ><<<nblk(p.ncc), WG, 0, s>>>(p.ccl, p.ncc, a.partial, p.chebA, a.w, p.r);
+        k_translate<double, RP, FREE, KID, false, true, MQ><<<nblk(p.nrc), WG, 0, s>>>(
             p, p.rcl, p.nrc, p.far_by_rc, a.w, a.z);
     }
     if (p.nrch > 0)
@@ -1135,32 +1160,33 @@ hipError_t matvec_mixed_r(const DevPlan &p, const MvArgs &a, int mode, int prec,
     }
 }
 
-template <int RMAX, int KB>
-hipError_t matvec_rk(const DevPlan &p, const MvArgs &a, bool tr, int mode, hipStream_t s,
+template <class T, int RMAX, int KB>
+hipError_t matvec_rk(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, int mode, hipStream_t s,
                      const hipEvent_t *ev) {
-    if (mode == 0) return matvec_t<RMAX, 0, 0, KB>(p, a, tr, s, ev);
-    if (mode == 1) return matvec_t<RMAX, 1, 0, KB>(p, a, tr, s, ev);
+    if (mode == 0) return matvec_t<T, RMAX, 0, 0, KB>(p, a, tr, s, ev);
+    if (mode == 1) return matvec_t<T, RMAX, 1, 0, KB>(p, a, tr, s, ev);
     switch (p.kid) {
-    case 0: return matvec_t<RMAX, 2, 0, KB>(p, a, tr, s, ev);
-    case 1: return matvec_t<RMAX, 2, 1, KB>(p, a, tr, s, ev);
-    case 2: return matvec_t<RMAX, 2, 2, KB>(p, a, tr, s, ev);
-    default: return matvec_t<RMAX, 2, 3, KB>(p, a, tr, s, ev);
+    case 0: return matvec_t<T, RMAX, 2, 0, KB>(p, a, tr, s, ev);
+    case 1: return matvec_t<T, RMAX, 2, 1, KB>(p, a, tr, s, ev);
+    case 2: return matvec_t<T, RMAX, 2, 2, KB>(p, a, tr, s, ev);
+    default: return matvec_t<T, RMAX, 2, 3, KB>(p, a, tr, s, ev);
     }
 }
 
-template <int RMAX>
-hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb, hipStream_t s,
+template <class T, int RMAX>
+hipError_t matvec_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, int mode, int kb, hipStream_t s,
                     const hipEvent_t *ev) {
-    if (tr || kb == 1) return matvec_rk<RMAX, 1>(p, a, tr, mode, s, ev);
-    if (kb == 2) return matvec_rk<RMAX, 2>(p, a, false, mode, s, ev);
-    if (kb == 4) return matvec_rk<RMAX, 4>(p, a, false, mode, s, ev);
-    return matvec_rk<RMAX, 8>(p, a, false, mode, s, ev);
+    if (tr || kb == 1) return matvec_rk<T, RMAX, 1>(p, a, tr, mode, s, ev);
+    if (kb == 2) return matvec_rk<T, RMAX, 2>(p, a, false, mode, s, ev);
+    if (kb == 4) return matvec_rk<T, RMAX, 4>(p, a, false, mode, s, ev);
+    return matvec_rk<T, RMAX, 8>(p, a, false, mode, s, ev);
 }
 
 }  // namespace
 }  // namespace hmxk
 
-// per-rank-bucket entry points (inst_r*.hip)
+// per-rank-bucket entry points (inst_r*.hip: FP64 plans, ranks up to 64;
+// inst_f32_r*.hip: FP32 plans, ranks up to 32)
 #define HMX_DECLARE_BUCKET(R)                                                                   \
     hipError_t hmx_matvec_r##R(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb,    \
                                hipStream_t s, const hipEvent_t *ev);                            \
@@ -1169,9 +1195,18 @@ hipError_t matvec_r(const DevPlan &p, const MvArgs &a, bool tr, int mode, int kb
     hipError_t hmx_gen_basis_r##R(const DevPlan &p, hipStream_t s);                              \
     hipError_t hmx_matvec_mixed_r##R(const DevPlan &p, const MvArgs &a, int mode, int prec,      \
                                      int nq, hipStream_t s);
+#define HMX_DECLARE_BUCKET_F32(R)                                                               \
+    hipError_t hmx_matvec_f32_r##R(const DevPlanF &p, const MvArgsF &a, bool tr, int mode,      \
+                                   int kb, hipStream_t s, const hipEvent_t *ev);                \
+    hipError_t hmx_grad_f32_r##R(const DevPlanF &p, const MvArgsF &a, bool tr, float *y2,       \
+                                 int64_t incy2, const float *sv, int64_t incs, hipStream_t s);  \
+    hipError_t hmx_gen_basis_f32_r##R(const DevPlanF &p, hipStream_t s);
 HMX_DECLARE_BUCKET(8)
 HMX_DECLARE_BUCKET(16)
 HMX_DECLARE_BUCKET(24)
 HMX_DECLARE_BUCKET(32)
 HMX_DECLARE_BUCKET(48)
 HMX_DECLARE_BUCKET(64)
+HMX_DECLARE_BUCKET_F32(8)
+HMX_DECLARE_BUCKET_F32(16)
+HMX_DECLARE_BUCKET_F32(32)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -94,6 +95,7 @@ struct Builder {
     const double *x, *y;
     int64_t nx, ny;
     int bs, r;
+    bool f32 = false;  // Float32 plan: points and intervals are floats, midpoints rounded to float
     std::vector<Leaf> leaves;
     std::vector<HCluster> rcl, ccl;
     std::unordered_map<uint64_t, int32_t> rmap, cmap;
@@ -108,8 +110,15 @@ struct Builder {
     // indsplit (src/BarycentricMatrix.jl:299-307) on a validated non-increasing
     // range: the linear scan stops at the first point below the midpoint, which
     // a binary search finds directly.
+    // half(T)*(a+b) in the element type (src/BarycentricMatrix.jl:300,
+    // src/KernelMatrix.jl:51); float values are exact in double
+    double mid(double a, double b) const {
+        if (f32) return (double)(0.5f * ((float)a + (float)b));
+        return 0.5 * (a + b);
+    }
+
     int64_t split(const double *p, int64_t np, int64_t first, int64_t len, double a, double b) {
-        const double ab2 = 0.5 * (a + b);
+        const double ab2 = mid(a, b);
         if (len == 0) {
             if (first >= np || p[first] >= ab2) {
                 err = HM_ERR_DEGENERATE;
@@ -178,7 +187,7 @@ struct Builder {
         const int64_t nj1 = split(y, ny, j0, nj, c, d), nj2 = nj - nj1;
         if (err) return;
         const int64_t i2 = i0 + ni1, j2 = j0 + nj1;
-        const double ab2 = 0.5 * (a + b), cd2 = 0.5 * (c + d);
+        const double ab2 = mid(a, b), cd2 = mid(c, d);
         const bool leaf = ni1 < bs && ni2 < bs && nj1 < bs && nj2 < bs;
         const int cd_ = depth + 1;
         if (type == 0) {  // KernelMatrix, src/KernelMatrix.jl:49-70
@@ -323,18 +332,38 @@ int upload(T **dst, const std::vector<T> &src) {
     return HM_OK;
 }
 
-int dalloc(double **dst, int64_t n) {
+template <class T>
+int dalloc(T **dst, int64_t n) {
     *dst = nullptr;
     if (n <= 0) return HM_OK;
-    hipError_t e = hipMalloc((void **)dst, sizeof(double) * (size_t)n);
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * (size_t)n);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *dst = nullptr;
-        return fail(HM_ERR_OOM, "hipMalloc of %lld doubles failed: %s", (long long)n,
-                    hipGetErrorString(e));
+        return fail(HM_ERR_OOM, "hipMalloc of %lld %d-byte values failed: %s", (long long)n,
+                    (int)sizeof(T), hipGetErrorString(e));
     }
     return HM_OK;
 }
+
+// Device arrays whose element type is the plan's (double or float), and the
+// device view built from them.
+template <class T>
+struct PlanBufs {
+    using V2 = typename Real2<T>::type;
+    T *x = nullptr, *y = nullptr, *lam = nullptr, *rnodes = nullptr, *cnodes = nullptr,
+      *chebA = nullptr;
+    V2 *rmap = nullptr, *cmap = nullptr;
+    T *A = nullptr, *U = nullptr, *V = nullptr, *F = nullptr;
+    T *partial = nullptr, *w = nullptr, *z = nullptr;
+    DevPlanT<T> dp{};
+
+    void release() {
+        void *b[] = {x, y, lam, rnodes, cnodes, chebA, rmap, cmap, A, U, V, F, partial, w, z};
+        for (void *q : b)
+            if (q) (void)hipFree(q);
+    }
+};
 
 }  // namespace
 
@@ -353,11 +382,10 @@ struct hm_plan {
     int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
     int maxdepth = 0;
     int64_t nonfinite = 0;
-    // device
-    DevPlan dp{};
-    double *d_x = nullptr, *d_y = nullptr, *d_lam = nullptr, *d_rnodes = nullptr,
-           *d_cnodes = nullptr, *d_chebA = nullptr;
-    double2 *d_rmap = nullptr, *d_cmap = nullptr;
+    // device: element-typed arrays (b64 for Float64 plans, b32 for Float32)
+    bool f32 = false;
+    PlanBufs<double> b64;
+    PlanBufs<float> b32;
     DCluster *d_rcl = nullptr, *d_ccl = nullptr;
     DFarBlk *d_far = nullptr;
     int32_t *d_far_by_rc = nullptr, *d_far_by_cc = nullptr;
@@ -366,8 +394,6 @@ struct hm_plan {
     DNear *d_rnear = nullptr, *d_cnear = nullptr;
     DFar *d_rfar = nullptr, *d_cfar = nullptr;
     DTask *d_rtask = nullptr, *d_ctask = nullptr;
-    double *d_A = nullptr, *d_U = nullptr, *d_V = nullptr, *d_F = nullptr;
-    double *d_partial = nullptr, *d_w = nullptr, *d_z = nullptr;
     bool profile = false;
     bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
@@ -381,13 +407,39 @@ struct hm_plan {
 
     ~hm_plan() {
         drop_events();
-        void *bufs[] = {d_x, d_y, d_lam, d_rnodes, d_cnodes, d_chebA, d_rmap, d_cmap, d_rcl, d_ccl, d_far, d_far_by_rc,
-                        d_far_by_cc, d_dense, d_rch, d_cch, d_rnear, d_cnear, d_rfar, d_cfar,
-                        d_rtask, d_ctask,
-                        d_A, d_U, d_V, d_F, d_partial, d_w, d_z};
+        void *bufs[] = {d_rcl, d_ccl, d_far, d_far_by_rc, d_far_by_cc, d_dense, d_rch, d_cch,
+                        d_rnear, d_cnear, d_rfar, d_cfar, d_rtask, d_ctask};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
+        b64.release();
+        b32.release();
     }
+
+    template <class T>
+    PlanBufs<T> &bufs() {
+        if constexpr (std::is_same_v<T, double>) return b64;
+        else return b32;
+    }
+
+    // the element-type-independent part of the device view
+    template <class T>
+    void fill_structure(DevPlanT<T> &dp) const {
+        dp.kid = kid; dp.r = r; dp.nx = nx; dp.ny = ny;
+        dp.rcl = d_rcl; dp.ccl = d_ccl;
+        dp.nrc = (int32_t)rcl.size(); dp.ncc = (int32_t)ccl.size();
+        dp.far = d_far; dp.nfar = (int32_t)far.size();
+        dp.far_by_rc = d_far_by_rc; dp.far_by_cc = d_far_by_cc;
+        dp.dense = d_dense; dp.ndense = (int32_t)dense.size();
+        dp.rch = d_rch; dp.cch = d_cch;
+        dp.nrch = (int32_t)rows.ch.size(); dp.ncch = (int32_t)cols.ch.size();
+        dp.rnear = d_rnear; dp.cnear = d_cnear;
+        dp.rfar = d_rfar; dp.cfar = d_cfar;
+        dp.rtask = d_rtask; dp.ctask = d_ctask;
+        dp.nrtask = (int32_t)rows.tasks.size(); dp.nctask = (int32_t)cols.tasks.size();
+        dp.nslots_r = rows.nslots; dp.nslots_c = cols.nslots;
+    }
+
+    int elem_bytes() const { return f32 ? 4 : 8; }
 
     int64_t basis_rows(const std::vector<DCluster> &cl) const {
         int64_t s = 0;
@@ -446,18 +498,171 @@ struct Inst {
     double a, b, c, d;
 };
 
+// chebyshevpoints(T, n) and chebyshevbarycentricweights(T, n) of the first
+// kind (src/BarycentricMatrix.jl:92-136) in the element type T: the argument
+// of sinpi is formed in T, sin(pi q) is evaluated in long double and rounded
+// once to T.
+extern "C++" {
+template <class T>
+void cheb_points_t(T *x, int n) {
+    const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
+    std::fill(x, x + n, T(0));
+    for (int k = 1; k <= n / 2; ++k) {
+        const T q = ((T)(n - 2 * k) + T(1)) / (T)(2 * n);
+        x[k - 1] = (T)sinl(PI_L * (long double)q);
+    }
+    for (int k = 1; k <= n / 2; ++k) x[n - k] = -x[k - 1];
+}
+}  // extern "C++"
+
+extern "C++" {
+template <class T>
+void cheb_weights_t(T *l, int n) {
+    const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
+    std::fill(l, l + n, T(0));
+    for (int k = 1; k <= n / 2 + 1 && k <= n; ++k) {
+        const T q = ((T)(2 * k) - T(1)) / (T)(2 * n);
+        l[k - 1] = (T)sinl(PI_L * (long double)q);
+    }
+    for (int k = 1; k <= n / 2; ++k) l[n - k] = l[k - 1];
+    for (int k = 2; k <= n; k += 2) l[k - 1] = -l[k - 1];
+}
+}  // extern "C++"
+
+// The element-typed part of a plan: nodes, weights, cluster maps and the
+// Chebyshev-form matrix in T, the device arrays, the device view, and the
+// compression (dense blocks, F, U, V) for stored and hybrid plans.
+extern "C++" {
+template <class T>
+int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<HCluster> &hr,
+                const std::vector<HCluster> &hc, unsigned flags, int64_t nA, int64_t nU,
+                int64_t nV, int64_t nF, hipStream_t s) {
+#pragma clang fp contract(off)
+    using V2 = typename Real2<T>::type;
+    const int r = P->r;
+    PlanBufs<T> &b = P->bufs<T>();
+    std::vector<T> xhat(r), lam(r);
+    cheb_points_t(xhat.data(), r);
+    cheb_weights_t(lam.data(), r);
+    // mapped nodes t = ab2 + ba2 * xhat with ab2 = (a+b)/2, ba2 = (b-a)/2 in T
+    // (src/BarycentricMatrix.jl:156-166), and per cluster (center,
+    // 1/halfwidth): xh = (p - center)/halfwidth maps the interval onto [-1, 1]
+    // with the node order of the reference
+    auto nodes = [&](const std::vector<HCluster> &h, std::vector<T> &t, std::vector<V2> &m) {
+        t.resize(h.size() * r);
+        m.resize(h.size());
+        for (size_t k = 0; k < h.size(); ++k) {
+            const T lo = (T)h[k].lo, hi = (T)h[k].hi;
+            const T ab2 = T(0.5) * (lo + hi), ba2 = T(0.5) * (hi - lo);
+            for (int q = 0; q < r; ++q) t[k * r + q] = ab2 + ba2 * xhat[q];
+            m[k].x = ab2;
+            m[k].y = T(1) / ba2;
+        }
+    };
+    std::vector<T> rn, cn;
+    std::vector<V2> rmap, cmap;
+    nodes(hr, rn, rmap);
+    nodes(hc, cn, cmap);
+    // A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r, T_m(cos t) = cos(m t), t_k = (2k-1) pi / (2r)
+    std::vector<T> chebA((size_t)r * r);
+    {
+        const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
+        for (int k = 1; k <= r; ++k)
+            for (int m = 0; m < r; ++m)
+                chebA[(size_t)(k - 1) * r + m] =
+                    (T)((m ? 2.0L : 1.0L) * cosl((long double)m * (2 * k - 1) * PI_L / (2.0L * r)) / r);
+    }
+    const std::vector<T> xv(x, x + P->nx), yv(y, y + P->ny);
+    int rc;
+#define HM_TRY_T(expr) do { rc = (expr); if (rc != HM_OK) return rc; } while (0)
+    HM_TRY_T(upload(&b.x, xv));
+    HM_TRY_T(upload(&b.y, yv));
+    HM_TRY_T(upload(&b.lam, lam));
+    HM_TRY_T(upload(&b.rnodes, rn));
+    HM_TRY_T(upload(&b.cnodes, cn));
+    HM_TRY_T(upload(&b.rmap, rmap));
+    HM_TRY_T(upload(&b.cmap, cmap));
+    HM_TRY_T(upload(&b.chebA, chebA));
+    const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
+    const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
+    HM_TRY_T(dalloc(&b.partial, nslots * r * HM_WS_RHS));
+    HM_TRY_T(dalloc(&b.w, ncl * r * HM_WS_RHS));
+    HM_TRY_T(dalloc(&b.z, ncl * r * HM_WS_RHS));
+    const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
+    const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
+    if (stored) {
+        HM_TRY_T(dalloc(&b.A, nA));
+        HM_TRY_T(dalloc(&b.F, nF));
+        if (!hybrid) {
+            HM_TRY_T(dalloc(&b.U, nU));
+            HM_TRY_T(dalloc(&b.V, nV));
+        }
+    }
+#undef HM_TRY_T
+    DevPlanT<T> &dp = b.dp;
+    P->fill_structure(dp);
+    dp.x = b.x; dp.y = b.y; dp.lam = b.lam;
+    dp.rnodes = b.rnodes; dp.cnodes = b.cnodes;
+    dp.rmap = b.rmap; dp.cmap = b.cmap; dp.chebA = b.chebA;
+    dp.A = b.A; dp.U = b.U; dp.V = b.V; dp.F = b.F;
+
+    if (stored) {
+        // tile padding (rows beyond a block's or cluster's length) is zeroed
+        bool padded = false;
+        for (const DDense &D : P->dense) padded |= (D.m % HM_TILE) != 0;
+        for (const DCluster &C : P->rcl) padded |= (C.len % HM_TILE) != 0;
+        for (const DCluster &C : P->ccl) padded |= (C.len % HM_TILE) != 0;
+        if (padded) {
+            T *arrs[] = {b.A, b.U, b.V};
+            const int64_t lens[] = {nA, nU, nV};
+            for (int q = 0; q < 3; ++q)
+                if (lens[q] > 0 && arrs[q]) {
+                    hipError_t e0 = hipMemsetAsync((void *)arrs[q], 0, sizeof(T) * lens[q], s);
+                    if (e0 != hipSuccess) return fail(HM_ERR_HIP, "memset: %s", hipGetErrorString(e0));
+                }
+        }
+        hipError_t e = hmx_launch_compress(dp, s);
+        if (e != hipSuccess) return fail(HM_ERR_HIP, "compression launch: %s", hipGetErrorString(e));
+        // count non-finite factors (coincident points give Inf, as in the reference)
+        unsigned long long *d_cnt = nullptr;
+        e = hipMalloc((void **)&d_cnt, 1024 * sizeof(unsigned long long));
+        if (e == hipSuccess) {
+            std::vector<unsigned long long> h(1024);
+            int64_t total = 0;
+            const T *arrs[] = {b.A, b.U, b.V, b.F};
+            const int64_t lens[] = {nA, nU, nV, nF};
+            for (int q = 0; q < 4 && e == hipSuccess; ++q) {
+                if (lens[q] <= 0 || !arrs[q]) continue;
+                e = hipMemsetAsync(d_cnt, 0, 1024 * sizeof(unsigned long long), s);
+                if (e == hipSuccess) e = hmx_launch_count_nonfinite(arrs[q], lens[q], d_cnt, s);
+                if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_cnt, 1024 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                for (auto v : h) total += (int64_t)v;
+            }
+            (void)hipFree(d_cnt);
+            P->nonfinite = total;
+        }
+        if (e != hipSuccess) return fail(HM_ERR_HIP, "compression: %s", hipGetErrorString(e));
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "compression sync: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+}  // extern "C++"
+
 // Builds every instance's tree, then merges them into one forest over the
 // concatenated index space (instance t owns rows [rows_t, rows_t + nx_t) and
 // columns [cols_t, cols_t + ny_t)); blocks never cross instances, so every
 // pass of a product covers all instances in one launch.
 static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksize, int rank,
-                     unsigned flags, void *stream, hm_plan_t *out) {
+                     unsigned flags, void *stream, hm_plan_t *out, bool f32 = false) {
     if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
     *out = nullptr;
     if (kernel_id < HM_CAUCHY || kernel_id > HM_LOG) return fail(HM_ERR_INVALID, "unknown kernel id %d", kernel_id);
     if (count < 1) return fail(HM_ERR_INVALID, "no instances");
     if (blocksize < 1) return fail(HM_ERR_INVALID, "blocksize must be >= 1");
     if (rank < 1 || rank > 64) return fail(HM_ERR_UNSUPPORTED, "rank %d outside 1..64", rank);
+    if (f32 && rank > 32) return fail(HM_ERR_UNSUPPORTED, "Float32 plans take rank <= 32 (got %d)", rank);
     if (flags & ~15u) return fail(HM_ERR_INVALID, "unknown plan flags 0x%x", flags);
     if ((flags & HM_PLAN_MATRIX_FREE) && (flags & HM_PLAN_HYBRID))
         return fail(HM_ERR_INVALID, "HM_PLAN_MATRIX_FREE and HM_PLAN_HYBRID exclude each other");
@@ -498,6 +703,7 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         const Inst &I = inst[t];
         Builder Bt{};
         Bt.x = I.x; Bt.y = I.y; Bt.nx = I.nx; Bt.ny = I.ny; Bt.bs = blocksize; Bt.r = rank;
+        Bt.f32 = f32;
         Bt.node(0, 0, 0, I.nx, 0, I.ny, I.a, I.b, I.c, I.d);
         if (Bt.err) {
             return count == 1 ? fail(Bt.err, "%s", Bt.msg.c_str())
@@ -520,6 +726,7 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     hm_plan *P = new hm_plan();
     P->kid = kernel_id; P->r = rank; P->bs = blocksize; P->flags = flags;
     P->nx = nx; P->ny = ny;
+    P->f32 = f32;
     P->nd = B.nd; P->nk1 = B.nk1; P->nk2 = B.nk2; P->maxdepth = B.maxdepth;
     P->far_rows = B.far_rows; P->far_cols = B.far_cols; P->dense_entries = B.dense_entries;
     P->leaves = std::move(B.leaves);
@@ -571,56 +778,13 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     build_side(nx, P->dense, true, P->rcl, P->rows);
     build_side(ny, P->dense, false, P->ccl, P->cols);
 
-    // mapped nodes t = (a+b)/2 + (b-a)/2 * xhat, src/BarycentricMatrix.jl:156-166
-    std::vector<double> xhat(r), lam(r);
-    cheb_points(xhat.data(), r, 1);
-    cheb_weights(lam.data(), r, 1);
-    auto nodes = [&](const std::vector<HCluster> &hc) {
-        std::vector<double> t(hc.size() * r);
-        for (size_t k = 0; k < hc.size(); ++k) {
-            const double ab2 = 0.5 * (hc[k].lo + hc[k].hi), ba2 = 0.5 * (hc[k].hi - hc[k].lo);
-            for (int p = 0; p < r; ++p) t[k * r + p] = ab2 + ba2 * xhat[p];
-        }
-        return t;
-    };
-    const std::vector<double> rn = nodes(B.rcl), cn = nodes(B.ccl);
-    // (center, 1/halfwidth) per cluster: xh = (p - center)/halfwidth maps the
-    // interval onto [-1, 1] with the node order of the reference
-    auto maps = [&](const std::vector<HCluster> &hc) {
-        std::vector<double2> m(hc.size());
-        for (size_t k = 0; k < hc.size(); ++k) {
-            const double ab2 = 0.5 * (hc[k].lo + hc[k].hi), ba2 = 0.5 * (hc[k].hi - hc[k].lo);
-            m[k] = make_double2(ab2, 1.0 / ba2);
-        }
-        return m;
-    };
-    const std::vector<double2> rmap = maps(B.rcl), cmap = maps(B.ccl);
-    // A[k][m] = (m ? 2 : 1) T_m(xhat_k) / r, T_m(cos t) = cos(m t), t_k = (2k-1) pi / (2r)
-    std::vector<double> chebA((size_t)r * r);
-    {
-        const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
-        for (int k = 1; k <= r; ++k)
-            for (int m = 0; m < r; ++m)
-                chebA[(size_t)(k - 1) * r + m] =
-                    (double)((m ? 2.0L : 1.0L) * cosl((long double)m * (2 * k - 1) * PI_L / (2.0L * r)) / r);
-    }
-
     if (flags & HM_PLAN_TREE_ONLY) {
         P->far_by_rc_h = std::move(by_rc);
         *out = P;
         return HM_OK;
     }
-    std::vector<double> xv(x, x + nx), yv(y, y + ny);
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
-    HM_TRY(upload(&P->d_x, xv));
-    HM_TRY(upload(&P->d_y, yv));
-    HM_TRY(upload(&P->d_lam, lam));
-    HM_TRY(upload(&P->d_rnodes, rn));
-    HM_TRY(upload(&P->d_cnodes, cn));
-    HM_TRY(upload(&P->d_rmap, rmap));
-    HM_TRY(upload(&P->d_cmap, cmap));
-    HM_TRY(upload(&P->d_chebA, chebA));
     HM_TRY(upload(&P->d_rcl, P->rcl));
     HM_TRY(upload(&P->d_ccl, P->ccl));
     HM_TRY(upload(&P->d_far, P->far));
@@ -636,82 +800,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     HM_TRY(upload(&P->d_cfar, P->cols.farl));
     HM_TRY(upload(&P->d_rtask, P->rows.tasks));
     HM_TRY(upload(&P->d_ctask, P->cols.tasks));
-    const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
-    const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&P->d_partial, nslots * r * HM_WS_RHS));
-    HM_TRY(dalloc(&P->d_w, ncl * r * HM_WS_RHS));
-    HM_TRY(dalloc(&P->d_z, ncl * r * HM_WS_RHS));
-    const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
-    const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
-    if (stored) {
-        HM_TRY(dalloc(&P->d_A, nA));
-        HM_TRY(dalloc(&P->d_F, nF));
-        if (!hybrid) {
-            HM_TRY(dalloc(&P->d_U, nU));
-            HM_TRY(dalloc(&P->d_V, nV));
-        }
-    }
-
-    DevPlan &dp = P->dp;
-    dp.kid = kernel_id; dp.r = r; dp.nx = nx; dp.ny = ny;
-    dp.x = P->d_x; dp.y = P->d_y; dp.lam = P->d_lam;
-    dp.rcl = P->d_rcl; dp.ccl = P->d_ccl;
-    dp.nrc = (int32_t)P->rcl.size(); dp.ncc = (int32_t)P->ccl.size();
-    dp.rnodes = P->d_rnodes; dp.cnodes = P->d_cnodes;
-    dp.rmap = P->d_rmap; dp.cmap = P->d_cmap; dp.chebA = P->d_chebA;
-    dp.far = P->d_far; dp.nfar = (int32_t)P->far.size();
-    dp.far_by_rc = P->d_far_by_rc; dp.far_by_cc = P->d_far_by_cc;
-    dp.dense = P->d_dense; dp.ndense = (int32_t)P->dense.size();
-    dp.rch = P->d_rch; dp.cch = P->d_cch;
-    dp.nrch = (int32_t)P->rows.ch.size(); dp.ncch = (int32_t)P->cols.ch.size();
-    dp.rnear = P->d_rnear; dp.cnear = P->d_cnear;
-    dp.rfar = P->d_rfar; dp.cfar = P->d_cfar;
-    dp.rtask = P->d_rtask; dp.ctask = P->d_ctask;
-    dp.nrtask = (int32_t)P->rows.tasks.size(); dp.nctask = (int32_t)P->cols.tasks.size();
-    dp.nslots_r = P->rows.nslots; dp.nslots_c = P->cols.nslots;
-    dp.A = P->d_A; dp.U = P->d_U; dp.V = P->d_V; dp.F = P->d_F;
-
-    hipStream_t s = (hipStream_t)stream;
-    if (stored) {
-        // tile padding (rows beyond a block's or cluster's length) is zeroed
-        bool padded = false;
-        for (const DDense &D : P->dense) padded |= (D.m % HM_TILE) != 0;
-        for (const DCluster &C : P->rcl) padded |= (C.len % HM_TILE) != 0;
-        for (const DCluster &C : P->ccl) padded |= (C.len % HM_TILE) != 0;
-        if (padded) {
-            const double *arrs[] = {P->d_A, P->d_U, P->d_V};
-            const int64_t lens[] = {nA, nU, nV};
-            for (int q = 0; q < 3; ++q)
-                if (lens[q] > 0 && arrs[q]) {
-                    hipError_t e0 = hipMemsetAsync((void *)arrs[q], 0, sizeof(double) * lens[q], s);
-                    if (e0 != hipSuccess) { delete P; return fail(HM_ERR_HIP, "memset: %s", hipGetErrorString(e0)); }
-                }
-        }
-        hipError_t e = hmx_launch_compress(dp, s);
-        if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression launch: %s", hipGetErrorString(e)); }
-        // count non-finite factors (coincident points give Inf, as in the reference)
-        unsigned long long *d_cnt = nullptr;
-        e = hipMalloc((void **)&d_cnt, 1024 * sizeof(unsigned long long));
-        if (e == hipSuccess) {
-            std::vector<unsigned long long> h(1024);
-            int64_t total = 0;
-            const double *arrs[] = {P->d_A, P->d_U, P->d_V, P->d_F};
-            const int64_t lens[] = {nA, nU, nV, nF};
-            for (int q = 0; q < 4 && e == hipSuccess; ++q) {
-                if (lens[q] <= 0 || !arrs[q]) continue;
-                e = hipMemsetAsync(d_cnt, 0, 1024 * sizeof(unsigned long long), s);
-                if (e == hipSuccess) e = hmx_launch_count_nonfinite(arrs[q], lens[q], d_cnt, s);
-                if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_cnt, 1024 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
-                if (e == hipSuccess) e = hipStreamSynchronize(s);
-                for (auto v : h) total += (int64_t)v;
-            }
-            (void)hipFree(d_cnt);
-            P->nonfinite = total;
-        }
-        if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression: %s", hipGetErrorString(e)); }
-    }
-    hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { delete P; return fail(HM_ERR_HIP, "compression sync: %s", hipGetErrorString(e)); }
+    if (f32) HM_TRY(setup_typed<float>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
+    else HM_TRY(setup_typed<double>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
 #undef HM_TRY
     *out = P;
     return HM_OK;
@@ -722,6 +812,19 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
                         unsigned flags, void *stream, hm_plan_t *out) {
     const Inst I{x, y, nx, ny, a, b, c, d};
     return make_plan(kernel_id, &I, 1, blocksize, rank, flags, stream, out);
+}
+
+int hm_plan_from_points_f32(int kernel_id, const float *x, int64_t nx, const float *y, int64_t ny,
+                            float a, float b, float c, float d, int blocksize, int rank,
+                            unsigned flags, void *stream, hm_plan_t *out) {
+    if (out) *out = nullptr;
+    if (!x || !y || nx < 1 || ny < 1)
+        return fail(HM_ERR_DEGENERATE, "KernelMatrix needs non-empty x and y (the reference indexes x[1])");
+    // every float is exact in double; the tree and the factors are formed in
+    // float arithmetic from these values
+    const std::vector<double> xd(x, x + nx), yd(y, y + ny);
+    const Inst I{xd.data(), yd.data(), nx, ny, a, b, c, d};
+    return make_plan(kernel_id, &I, 1, blocksize, rank, flags, stream, out, true);
 }
 
 int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
@@ -836,31 +939,20 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     HM_TRY(upload(&P->d_cfar, P->cols.farl));
     HM_TRY(upload(&P->d_rtask, P->rows.tasks));
     HM_TRY(upload(&P->d_ctask, P->cols.tasks));
-    HM_TRY(upload(&P->d_A, hA));
-    HM_TRY(upload(&P->d_U, hU));
-    HM_TRY(upload(&P->d_V, hV));
-    HM_TRY(upload(&P->d_F, hF));
+    PlanBufs<double> &b = P->b64;
+    HM_TRY(upload(&b.A, hA));
+    HM_TRY(upload(&b.U, hU));
+    HM_TRY(upload(&b.V, hV));
+    HM_TRY(upload(&b.F, hF));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&P->d_partial, std::max<int64_t>(nslots, 1) * r * HM_WS_RHS));
-    HM_TRY(dalloc(&P->d_w, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
-    HM_TRY(dalloc(&P->d_z, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
+    HM_TRY(dalloc(&b.partial, std::max<int64_t>(nslots, 1) * r * HM_WS_RHS));
+    HM_TRY(dalloc(&b.w, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
+    HM_TRY(dalloc(&b.z, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
 #undef HM_TRY
-    DevPlan &dp = P->dp;
-    dp.kid = 0; dp.r = r; dp.nx = nrows; dp.ny = ncols;
-    dp.rcl = P->d_rcl; dp.ccl = P->d_ccl;
-    dp.nrc = (int32_t)P->rcl.size(); dp.ncc = (int32_t)P->ccl.size();
-    dp.far = P->d_far; dp.nfar = (int32_t)P->far.size();
-    dp.far_by_rc = P->d_far_by_rc; dp.far_by_cc = P->d_far_by_cc;
-    dp.dense = P->d_dense; dp.ndense = (int32_t)P->dense.size();
-    dp.rch = P->d_rch; dp.cch = P->d_cch;
-    dp.nrch = (int32_t)P->rows.ch.size(); dp.ncch = (int32_t)P->cols.ch.size();
-    dp.rnear = P->d_rnear; dp.cnear = P->d_cnear;
-    dp.rfar = P->d_rfar; dp.cfar = P->d_cfar;
-    dp.rtask = P->d_rtask; dp.ctask = P->d_ctask;
-    dp.nrtask = (int32_t)P->rows.tasks.size(); dp.nctask = (int32_t)P->cols.tasks.size();
-    dp.nslots_r = P->rows.nslots; dp.nslots_c = P->cols.nslots;
-    dp.A = P->d_A; dp.U = P->d_U; dp.V = P->d_V; dp.F = P->d_F;
+    DevPlan &dp = b.dp;
+    P->fill_structure(dp);
+    dp.A = b.A; dp.U = b.U; dp.V = b.V; dp.F = b.F;
     (void)stream;
     *out = P;
     return HM_OK;
@@ -906,7 +998,8 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     for (const DCluster &C : P->ccl) basis_live += tiled_size(C.len, r);
     if (P->flags & HM_PLAN_HYBRID) basis_live = 0;
     if (P->flags & HM_PLAN_MATRIX_FREE) dense_live = basis_live = 0;
-    const int64_t vec = 8 * (P->nx + P->ny);
+    const int64_t eb = P->elem_bytes();
+    const int64_t vec = eb * (P->nx + P->ny);
     v[HM_INFO_NROWS] = P->nx;
     v[HM_INFO_NCOLS] = P->ny;
     v[HM_INFO_RANK] = P->r;
@@ -926,14 +1019,15 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     v[HM_INFO_BASIS_ROWS_U] = bu;
     v[HM_INFO_BASIS_ROWS_V] = bv;
     const int64_t fbytes = (P->flags & HM_PLAN_MATRIX_FREE) ? 0 : (int64_t)P->far.size() * r * r;
-    v[HM_INFO_BYTES_STORED] = 8 * (dense_live + basis_live + fbytes) + vec;
-    v[HM_INFO_BYTES_REF_LAYOUT] = 8 * (P->dense_entries + r * (P->far_rows + P->far_cols) + nfar * r * r) + vec;
+    v[HM_INFO_BYTES_STORED] = eb * (dense_live + basis_live + fbytes) + vec;
+    v[HM_INFO_BYTES_REF_LAYOUT] = eb * (P->dense_entries + r * (P->far_rows + P->far_cols) + nfar * r * r) + vec;
     v[HM_INFO_FLAGS] = P->flags;
     v[HM_INFO_N_ROWCHUNKS] = (int64_t)P->rows.ch.size();
     v[HM_INFO_N_COLCHUNKS] = (int64_t)P->cols.ch.size();
     v[HM_INFO_NONFINITE] = P->nonfinite;
     v[HM_INFO_N_INSTANCES] = (int64_t)P->row_off.size() - 1;
     v[HM_INFO_FROM_BLOCKS] = P->from_blocks ? 1 : 0;
+    v[HM_INFO_FLOAT32] = P->f32 ? 1 : 0;
     for (int k = 0; k < n && k < HM_INFO_COUNT; ++k) info[k] = v[k];
     for (int k = HM_INFO_COUNT; k < n; ++k) info[k] = 0;
     return HM_OK;
@@ -956,11 +1050,39 @@ int hm_plan_blocks(hm_plan_t P, int64_t *ints, double *dbls, int64_t cap, int64_
     return HM_OK;
 }
 
-static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
+// A product entry point for element type T may only be called on a plan of
+// that type.
+extern "C++" {
+template <class T>
+static int check_type(hm_plan_t P, const char *what) {
+    const bool want32 = std::is_same_v<T, float>;
+    if (P->f32 != want32)
+        return fail(HM_ERR_INVALID, "%s: the plan holds %s data; use the %s entry point", what,
+                    P->f32 ? "Float32" : "Float64", P->f32 ? "_f32" : "Float64");
+    return HM_OK;
+}
+}  // extern "C++"
+
+extern "C++" {
+template <class T>
+static MvArgsT<T> ws_args(hm_plan_t P) {
+    MvArgsT<T> a{};
+    PlanBufs<T> &b = P->bufs<T>();
+    a.partial = b.partial;
+    a.w = b.w;
+    a.z = b.z;
+    return a;
+}
+}  // extern "C++"
+
+extern "C++" {
+template <class T>
+static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t joff,
                          int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
                          unsigned flags, void *stream, bool tr) {
     if (!P) return fail(HM_ERR_INVALID, "matvec: NULL plan");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec: tree-only plan has no device data");
+    if (int rc = check_type<T>(P, "matvec")) return rc;
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
     if (flags & ~31u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
@@ -983,33 +1105,34 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
     hipStream_t s = (hipStream_t)stream;
     if (prec >= 0) {
         // far field on MFMA in reduced precision: Chebyshev form, 16 rhs per launch
-        if (tr || P->from_blocks)
-            return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: forward KernelMatrix products only");
-        const int mmode = std::max(mode, 1);
-        for (int64_t q = 0; q < k; q += 16) {
-            MvArgs a{};
-            a.y = y + ioff + q * ldy;
-            a.x = x + joff + q * ldx;
-            a.incy = incy;
-            a.incx = incx;
-            a.ldy = ldy;
-            a.ldx = ldx;
-            a.beta = beta;
-            a.partial = P->d_partial;
-            a.w = P->d_w;
-            a.z = P->d_z;
-            const int nq = (int)std::min<int64_t>(16, k - q);
-            hipError_t e = hmx_launch_matvec_mixed(P->dp, a, mmode, prec, nq, s);
-            if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
+        if constexpr (std::is_same_v<T, double>) {
+            if (tr || P->from_blocks)
+                return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: forward KernelMatrix products only");
+            const int mmode = std::max(mode, 1);
+            for (int64_t q = 0; q < k; q += 16) {
+                MvArgs a = ws_args<double>(P);
+                a.y = y + ioff + q * ldy;
+                a.x = x + joff + q * ldx;
+                a.incy = incy;
+                a.incx = incx;
+                a.ldy = ldy;
+                a.ldx = ldx;
+                a.beta = beta;
+                const int nq = (int)std::min<int64_t>(16, k - q);
+                hipError_t e = hmx_launch_matvec_mixed(P->b64.dp, a, mmode, prec, nq, s);
+                if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
+            }
+            return HM_OK;
+        } else {
+            return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: Float64 plans only");
         }
-        return HM_OK;
     }
     for (int64_t q = 0; q < k;) {
         // forward products take up to HM_MAX_KB right-hand sides per launch
         int kb = 1;
         if (!tr && max_kb > 1)
             while (kb * 2 <= max_kb && q + kb * 2 <= k) kb *= 2;
-        MvArgs a{};
+        MvArgsT<T> a = ws_args<T>(P);
         a.y = y + ioff + q * ldy;
         a.x = x + joff + q * ldx;
         a.incy = incy;
@@ -1017,9 +1140,6 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
         a.ldy = ldy;
         a.ldx = ldx;
         a.beta = beta;
-        a.partial = P->d_partial;
-        a.w = P->d_w;
-        a.z = P->d_z;
         const hipEvent_t *ev = nullptr;
         if (P->profile) {
             const size_t at = P->prof_events.size();
@@ -1030,12 +1150,13 @@ static int matvec_common(hm_plan_t P, double *y, const double *x, int64_t ioff, 
             }
             ev = P->prof_events.data() + at;
         }
-        hipError_t e = hmx_launch_matvec(P->dp, a, tr, mode, kb, s, ev);
+        hipError_t e = hmx_launch_matvec(P->bufs<T>().dp, a, tr, mode, kb, s, ev);
         if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
         q += kb;
     }
     return HM_OK;
 }
+}  // extern "C++"
 
 int hm_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff, int64_t incy,
               int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta, unsigned flags,
@@ -1049,47 +1170,83 @@ int hm_matvec_t(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t j
     return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, true);
 }
 
-static MvArgs ws_args(hm_plan_t P) {
-    MvArgs a{};
-    a.partial = P->d_partial;
-    a.w = P->d_w;
-    a.z = P->d_z;
-    return a;
+int hm_matvec_f32(hm_plan_t P, float *y, const float *x, int64_t ioff, int64_t joff,
+                  int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                  unsigned flags, void *stream) {
+    return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, false);
 }
 
-int hm_matvec_grad_x(hm_plan_t P, double *g, const double *b, const double *s, int64_t ioff,
-                     int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
+int hm_matvec_t_f32(hm_plan_t P, float *y, const float *x, int64_t ioff, int64_t joff,
+                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                    unsigned flags, void *stream) {
+    return matvec_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, stream, true);
+}
+
+extern "C++" {
+template <class T>
+static int grad_x_common(hm_plan_t P, T *g, const T *b, const T *s, int64_t ioff, int64_t joff,
+                         int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
     if (!P || !g || !b || !s) return fail(HM_ERR_INVALID, "grad_x: NULL argument");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "grad_x: tree-only plan");
+    if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "grad_x: a plan built from blocks has no points");
+    if (int rc = check_type<T>(P, "grad_x")) return rc;
     if (incg < 1 || incb < 1 || incs < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "grad_x: bad offsets, strides or beta");
-    MvArgs a = ws_args(P);
+    MvArgsT<T> a = ws_args<T>(P);
     a.y = g + ioff;
     a.x = b + joff;
     a.incy = incg;
     a.incx = incb;
     a.beta = beta;
-    hipError_t e = hmx_launch_grad(P->dp, a, false, nullptr, 0, s, incs, (hipStream_t)stream);
+    hipError_t e = hmx_launch_grad(P->bufs<T>().dp, a, false, (T *)nullptr, 0, s, incs, (hipStream_t)stream);
     if (e != hipSuccess) return fail(HM_ERR_HIP, "grad_x launch: %s", hipGetErrorString(e));
     return HM_OK;
 }
+}  // extern "C++"
 
-int hm_matvec_t_grad(hm_plan_t P, double *gb, double *gy, const double *r, const double *s,
-                     int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
-                     void *stream) {
+extern "C++" {
+template <class T>
+static int t_grad_common(hm_plan_t P, T *gb, T *gy, const T *r, const T *s, int64_t incgb,
+                         int64_t incgy, int64_t incr, int64_t incs, int beta, void *stream) {
     if (!P || !gb || !gy || !r || !s) return fail(HM_ERR_INVALID, "t_grad: NULL argument");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "t_grad: tree-only plan");
+    if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "t_grad: a plan built from blocks has no points");
+    if (int rc = check_type<T>(P, "t_grad")) return rc;
     if (incgb < 1 || incgy < 1 || incr < 1 || incs < 1 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "t_grad: bad strides or beta");
-    MvArgs a = ws_args(P);
+    MvArgsT<T> a = ws_args<T>(P);
     a.y = gb;
     a.x = r;
     a.incy = incgb;
     a.incx = incr;
     a.beta = beta;
-    hipError_t e = hmx_launch_grad(P->dp, a, true, gy, incgy, s, incs, (hipStream_t)stream);
+    hipError_t e = hmx_launch_grad(P->bufs<T>().dp, a, true, gy, incgy, s, incs, (hipStream_t)stream);
     if (e != hipSuccess) return fail(HM_ERR_HIP, "t_grad launch: %s", hipGetErrorString(e));
     return HM_OK;
+}
+}  // extern "C++"
+
+int hm_matvec_grad_x(hm_plan_t P, double *g, const double *b, const double *s, int64_t ioff,
+                     int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
+    return grad_x_common(P, g, b, s, ioff, joff, incg, incb, incs, beta, stream);
+}
+
+int hm_matvec_t_grad(hm_plan_t P, double *gb, double *gy, const double *r, const double *s,
+                     int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
+                     void *stream) {
+    return t_grad_common(P, gb, gy, r, s, incgb, incgy, incr, incs, beta, stream);
+}
+
+int hm_matvec_grad_x_f32(hm_plan_t P, float *g, const float *b, const float *s, int64_t ioff,
+                         int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta,
+                         void *stream) {
+    return grad_x_common(P, g, b, s, ioff, joff, incg, incb, incs, beta, stream);
+}
+
+int hm_matvec_t_grad_f32(hm_plan_t P, float *gb, float *gy, const float *r, const float *s,
+                         int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
+                         void *stream) {
+    return t_grad_common(P, gb, gy, r, s, incgb, incgy, incr, incs, beta, stream);
 }
 
 struct hm_graph {
@@ -1097,9 +1254,11 @@ struct hm_graph {
     hipGraphExec_t exec = nullptr;
 };
 
-int hm_graph_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
-                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
-                    unsigned flags, hm_graph_t *out) {
+extern "C++" {
+template <class T>
+static int graph_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t joff, int64_t incy,
+                        int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                        unsigned flags, hm_graph_t *out) {
     if (!out) return fail(HM_ERR_INVALID, "graph: NULL output");
     *out = nullptr;
     if (!P || (P->flags & HM_PLAN_TREE_ONLY)) return fail(HM_ERR_INVALID, "graph: bad plan");
@@ -1131,6 +1290,19 @@ int hm_graph_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64
     *out = G;
     return HM_OK;
 }
+}  // extern "C++"
+
+int hm_graph_matvec(hm_plan_t P, double *y, const double *x, int64_t ioff, int64_t joff,
+                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                    unsigned flags, hm_graph_t *out) {
+    return graph_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, out);
+}
+
+int hm_graph_matvec_f32(hm_plan_t P, float *y, const float *x, int64_t ioff, int64_t joff,
+                        int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                        unsigned flags, hm_graph_t *out) {
+    return graph_common(P, y, x, ioff, joff, incy, incx, k, ldy, ldx, beta, flags, out);
+}
 
 int hm_graph_launch(hm_graph_t G, void *stream) {
     if (!G) return fail(HM_ERR_INVALID, "graph_launch: NULL graph");
@@ -1146,9 +1318,9 @@ int hm_graph_destroy(hm_graph_t G) {
     return HM_OK;
 }
 
-int hm_plan_tune(hm_plan_t P, double *y, const double *x, int64_t k, void *stream, int64_t out[3]) {
-    if (!P || !y || !x || k < 1 || (P->flags & HM_PLAN_TREE_ONLY))
-        return fail(HM_ERR_INVALID, "tune: bad arguments");
+extern "C++" {
+template <class T>
+static int tune_common(hm_plan_t P, T *y, const T *x, int64_t k, void *stream, int64_t out[3]) {
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
     HM_HIP(hipEventCreate(&e0));
@@ -1188,6 +1360,14 @@ int hm_plan_tune(hm_plan_t P, double *y, const double *x, int64_t k, void *strea
     }
     return HM_OK;
 }
+}  // extern "C++"
+
+int hm_plan_tune(hm_plan_t P, void *y, const void *x, int64_t k, void *stream, int64_t out[3]) {
+    if (!P || !y || !x || k < 1 || (P->flags & HM_PLAN_TREE_ONLY))
+        return fail(HM_ERR_INVALID, "tune: bad arguments");
+    if (P->f32) return tune_common(P, (float *)y, (const float *)x, k, stream, out);
+    return tune_common(P, (double *)y, (const double *)x, k, stream, out);
+}
 
 int hm_plan_profile(hm_plan_t P, int enable) {
     if (!P) return fail(HM_ERR_INVALID, "profile: NULL plan");
@@ -1214,9 +1394,12 @@ int hm_plan_profile_read(hm_plan_t P, double ms[4], int64_t *count) {
     return HM_OK;
 }
 
-int hm_entries(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, double *out,
-               void *stream) {
+extern "C++" {
+template <class T>
+static int entries_common(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, T *out,
+                          void *stream) {
     if (!P || n < 0) return fail(HM_ERR_INVALID, "entries: bad arguments");
+    if (int rc = check_type<T>(P, "entries")) return rc;
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "entries: tree-only plan has no device data");
     if (n == 0) return HM_OK;
     if (!i || !j || !out) return fail(HM_ERR_INVALID, "entries: NULL pointer");
@@ -1252,12 +1435,23 @@ int hm_entries(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, doubl
     hipError_t e = hipMalloc((void **)&d_id, n * sizeof(int32_t));
     if (e == hipSuccess) e = hipMemcpyAsync(d_kind, kind.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_id, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hmx_launch_entries(P->dp, n, i, j, d_kind, d_id, out, s);
+    if (e == hipSuccess) e = hmx_launch_entries(P->bufs<T>().dp, n, i, j, d_kind, d_id, out, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d_kind);
     if (d_id) (void)hipFree(d_id);
     if (e != hipSuccess) return fail(HM_ERR_HIP, "entries: %s", hipGetErrorString(e));
     return HM_OK;
+}
+}  // extern "C++"
+
+int hm_entries(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, double *out,
+               void *stream) {
+    return entries_common(P, n, i, j, out, stream);
+}
+
+int hm_entries_f32(hm_plan_t P, int64_t n, const int64_t *i, const int64_t *j, float *out,
+                   void *stream) {
+    return entries_common(P, n, i, j, out, stream);
 }
 
 }  // extern "C"

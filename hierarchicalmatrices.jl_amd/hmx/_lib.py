@@ -32,12 +32,13 @@ INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
     "n_rowclusters", "n_colclusters", "depth", "dense_entries", "basis_rows_u", "basis_rows_v",
     "bytes_stored", "bytes_ref_layout", "flags", "n_rowchunks", "n_colchunks", "nonfinite",
-    "n_instances", "from_blocks",
+    "n_instances", "from_blocks", "float32",
 ]
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int64)
 
 
@@ -123,6 +124,25 @@ def _load():
     L.hm_plan_profile_read.argtypes = [_vp, _dp, _ip]
     L.hm_entries.restype = ctypes.c_int
     L.hm_entries.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp]
+    # Float32 plans
+    L.hm_plan_from_points_f32.restype = ctypes.c_int
+    L.hm_plan_from_points_f32.argtypes = [ctypes.c_int, _fp, _i64, _fp, _i64, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_uint, _vp,
+                                          ctypes.POINTER(_vp)]
+    for name in ("hm_matvec_f32", "hm_matvec_t_f32"):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, ctypes.c_int,
+                       ctypes.c_uint, _vp]
+    L.hm_matvec_grad_x_f32.restype = ctypes.c_int
+    L.hm_matvec_grad_x_f32.argtypes = L.hm_matvec_grad_x.argtypes
+    L.hm_matvec_t_grad_f32.restype = ctypes.c_int
+    L.hm_matvec_t_grad_f32.argtypes = L.hm_matvec_t_grad.argtypes
+    L.hm_graph_matvec_f32.restype = ctypes.c_int
+    L.hm_graph_matvec_f32.argtypes = L.hm_graph_matvec.argtypes
+    L.hm_entries_f32.restype = ctypes.c_int
+    L.hm_entries_f32.argtypes = L.hm_entries.argtypes
     return L
 
 

@@ -43,5 +43,6 @@ class _KernelMatVec(torch.autograd.Function):
 
 def kernel_matvec(kernel, x, y, b, intervals, blocksize, rank):
     """u = H(x, y) b with H = KernelMatrix(kernel, x, y, *intervals) (x, y
-    non-increasing float64 CUDA tensors); differentiable in x, y and b."""
+    non-increasing CUDA tensors, float64 or float32: the element type of x
+    selects a Float64 or a Float32 plan); differentiable in x, y and b."""
     return _KernelMatVec.apply(x, y, b, kernel, tuple(intervals), blocksize, rank)

@@ -59,6 +59,8 @@ def BLOCKSIZE(T=np.float64):
 
 
 def _dptr(a):
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
@@ -113,19 +115,22 @@ def _stream_ptr(stream):
 
 
 class _Vec:
-    """A float64 vector or (n, k) matrix on the device, with its strides."""
+    """A vector or (n, k) matrix of the plan's element type on the device,
+    with its strides."""
 
-    def __init__(self, a, writable):
+    def __init__(self, a, writable, dtype=np.float64):
         torch = _torch()
+        tdt = torch.float32 if dtype == np.float32 else torch.float64
+        name = np.dtype(dtype).name
         self.host = None
         if isinstance(a, torch.Tensor):
-            if a.device.type != "cuda" or a.dtype != torch.float64:
-                raise TypeError("device vectors must be float64 CUDA tensors")
+            if a.device.type != "cuda" or a.dtype != tdt:
+                raise TypeError(f"device vectors must be {name} CUDA tensors (the eltype of H)")
             self.t = a
         else:
             arr = np.asarray(a)
-            if arr.dtype != np.float64:
-                raise TypeError("vectors must be float64 (the eltype of H, src/KernelMatrix.jl:17)")
+            if arr.dtype != dtype:
+                raise TypeError(f"vectors must be {name} (the eltype of H, src/KernelMatrix.jl:17)")
             self.host = arr
             self.t = torch.from_numpy(np.array(arr, copy=True)).cuda()
         self.writable = writable
@@ -156,24 +161,32 @@ class KernelMatrix:
 
     x and y must be non-increasing (the reference's indsplit assumes it,
     src/BarycentricMatrix.jl:299-307); (a, b), (c, d) are their intervals as
-    (upper, lower). blocksize and rank default to BLOCKSIZE(Float64) = 80 and
-    BLOCKRANK(Float64) = 20. matrix_free=True keeps only the points and
-    regenerates every factor in each product; hybrid=True keeps the dense blocks
-    and F and applies the far-field basis in Chebyshev form (no U, V).
+    (upper, lower). The element type T is that of x (float32 or float64; pass
+    dtype to force one); blocksize and rank default to BLOCKSIZE(T) and
+    BLOCKRANK(T) (80 and 20 for Float64, 40 and 10 for Float32). A Float32
+    matrix forms its tree, factors and products in FP32, as the reference's
+    generic code does for T = Float32. matrix_free=True keeps only the points
+    and regenerates every factor in each product; hybrid=True keeps the dense
+    blocks and F and applies the far-field basis in Chebyshev form (no U, V).
     """
 
     def __init__(self, f, x, y, a, b, c, d, *, blocksize=None, rank=None, matrix_free=False,
-                 hybrid=False, tree_only=False, stream=None):
+                 hybrid=False, tree_only=False, stream=None, dtype=None):
         if isinstance(f, str):
             f = KERNELS[f]
         if not isinstance(f, NamedKernel):
             raise TypeError("f must be one of cauchykernel, coulombkernel, coulombprimekernel, "
                             "logkernel")
         self.f = f
-        self.rank = BLOCKRANK(np.float64) if rank is None else int(rank)
+        if dtype is None:
+            dtype = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
+        self.dtype = np.dtype(dtype).type
+        if self.dtype not in (np.float32, np.float64):
+            raise TypeError("element type must be float32 or float64")
+        self.rank = BLOCKRANK(self.dtype) if rank is None else int(rank)
         self.blocksize_ = 4 * self.rank if blocksize is None else int(blocksize)
-        self.x = np.ascontiguousarray(x, dtype=np.float64)
-        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.x = np.ascontiguousarray(x, dtype=self.dtype)
+        self.y = np.ascontiguousarray(y, dtype=self.dtype)
         self.intervals = (float(a), float(b), float(c), float(d))
         flags = 0
         if matrix_free:
@@ -185,10 +198,21 @@ class KernelMatrix:
         self.flags = flags
         h = ctypes.c_void_p()
         sp = ctypes.c_void_p(0) if tree_only else _stream_ptr(stream)
-        check(lib.hm_plan_from_points(f.kid, _dptr(self.x), len(self.x), _dptr(self.y),
-                                      len(self.y), float(a), float(b), float(c), float(d),
-                                      self.blocksize_, self.rank, flags, sp, ctypes.byref(h)))
+        ctor = lib.hm_plan_from_points_f32 if self.f32 else lib.hm_plan_from_points
+        if self.f32:  # the intervals are Float32 values too
+            a, b, c, d = (float(np.float32(v)) for v in (a, b, c, d))
+        check(ctor(f.kid, _dptr(self.x), len(self.x), _dptr(self.y), len(self.y), float(a),
+                   float(b), float(c), float(d), self.blocksize_, self.rank, flags, sp,
+                   ctypes.byref(h)))
         self._h = h
+
+    @property
+    def f32(self):
+        return getattr(self, "dtype", np.float64) == np.float32
+
+    def _tdtype(self):
+        torch = _torch()
+        return torch.float32 if self.f32 else torch.float64
 
     # -- lifetime
     def close(self):
@@ -236,8 +260,9 @@ class KernelMatrix:
     # -- products
     def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream,
             cheb_basis=False, tuned=False, far_precision="f64"):
-        U = _Vec(u, True)
-        V = _Vec(v, False)
+        dt = np.float32 if self.f32 else np.float64
+        U = _Vec(u, True, dt)
+        V = _Vec(v, False, dt)
         if U.k != V.k:
             raise ValueError("u and v hold different numbers of right-hand sides")
         m, n = self.shape
@@ -250,7 +275,10 @@ class KernelMatrix:
             raise ValueError("istart and jstart are 1-based")
         if ioff + (m - 1) * incy >= U.n * U.inc or joff + (n - 1) * incx >= V.n * V.inc:
             raise ValueError("vector too short for H at the given offset and stride")
-        fn = lib.hm_matvec_t if transpose else lib.hm_matvec
+        if self.f32:
+            fn = lib.hm_matvec_t_f32 if transpose else lib.hm_matvec_f32
+        else:
+            fn = lib.hm_matvec_t if transpose else lib.hm_matvec
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
             (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
             _FAR[far_precision]
@@ -272,9 +300,10 @@ class KernelMatrix:
         """H v (or H^T v) into a new vector on the device of v."""
         torch = _torch()
         ishost = not isinstance(v, torch.Tensor)
-        vt = torch.from_numpy(np.asarray(v, dtype=np.float64)).cuda() if ishost else v
+        dt = np.float32 if self.f32 else np.float64
+        vt = torch.from_numpy(np.asarray(v, dtype=dt)).cuda() if ishost else v
         m = self.shape[1] if transpose else self.shape[0]
-        out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=torch.float64, device=vt.device)
+        out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=self._tdtype(), device=vt.device)
         self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis, tuned,
                  far_precision)
         if ishost:
@@ -285,8 +314,9 @@ class KernelMatrix:
     def grad_x(self, b, s, stream=None):
         """g_i = s_i * d/dx_i (H(x, y) b)_i (device tensors, 1-D)."""
         torch = _torch()
-        g = torch.empty(self.shape[0], dtype=torch.float64, device=b.device)
-        check(lib.hm_matvec_grad_x(self._h, ctypes.c_void_p(g.data_ptr()),
+        g = torch.empty(self.shape[0], dtype=self._tdtype(), device=b.device)
+        fn = lib.hm_matvec_grad_x_f32 if self.f32 else lib.hm_matvec_grad_x
+        check(fn(self._h, ctypes.c_void_p(g.data_ptr()),
                                    ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(s.data_ptr()),
                                    0, 0, 1, b.stride(0), s.stride(0), 0, _stream_ptr(stream)))
         return g
@@ -294,9 +324,10 @@ class KernelMatrix:
     def tgrad(self, r, s, stream=None):
         """(H^T r, s .* d/dy_j (H^T r)_j) for device tensors r (rows) and s (cols)."""
         torch = _torch()
-        gb = torch.empty(self.shape[1], dtype=torch.float64, device=r.device)
+        gb = torch.empty(self.shape[1], dtype=self._tdtype(), device=r.device)
         gy = torch.empty_like(gb)
-        check(lib.hm_matvec_t_grad(self._h, ctypes.c_void_p(gb.data_ptr()),
+        fn = lib.hm_matvec_t_grad_f32 if self.f32 else lib.hm_matvec_t_grad
+        check(fn(self._h, ctypes.c_void_p(gb.data_ptr()),
                                    ctypes.c_void_p(gy.data_ptr()), ctypes.c_void_p(r.data_ptr()),
                                    ctypes.c_void_p(s.data_ptr()), 1, 1, r.stride(0), s.stride(0),
                                    0, _stream_ptr(stream)))
@@ -306,15 +337,17 @@ class KernelMatrix:
               far_precision="f64"):
         """Capture `out = beta*out + H v` (device tensors, 1-D or (n, k)) into an
         executable HIP graph; returns a ProductGraph whose launch() replays it."""
-        U, V = _Vec(out, True), _Vec(v, False)
+        dt = np.float32 if self.f32 else np.float64
+        U, V = _Vec(out, True, dt), _Vec(v, False, dt)
         if U.host is not None or V.host is not None:
             raise TypeError("graphs need device tensors")
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
             (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
             _FAR[far_precision]
         g = ctypes.c_void_p()
-        check(lib.hm_graph_matvec(self._h, U.ptr(), V.ptr(), 0, 0, U.inc, V.inc, U.k, U.ld, V.ld,
-                                  beta, flags, ctypes.byref(g)))
+        fn = lib.hm_graph_matvec_f32 if self.f32 else lib.hm_graph_matvec
+        check(fn(self._h, U.ptr(), V.ptr(), 0, 0, U.inc, V.inc, U.k, U.ld, V.ld, beta, flags,
+                 ctypes.byref(g)))
         return ProductGraph(g, (out, v, self))
 
     def tune(self, k=1, stream=None):
@@ -322,8 +355,8 @@ class KernelMatrix:
         returns (rhs per launch, mode, microseconds). Use tuned=True later."""
         torch = _torch()
         m, n = self.shape
-        y = torch.empty(m * k, dtype=torch.float64, device="cuda")
-        x = torch.ones(n * k, dtype=torch.float64, device="cuda")
+        y = torch.empty(m * k, dtype=self._tdtype(), device="cuda")
+        x = torch.ones(n * k, dtype=self._tdtype(), device="cuda")
         out = np.zeros(3, dtype=np.int64)
         check(lib.hm_plan_tune(self._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                                k, _stream_ptr(stream),
@@ -347,8 +380,9 @@ class KernelMatrix:
         jj = torch.as_tensor(np.atleast_1d(np.asarray(j, dtype=np.int64))).cuda()
         ii, jj = torch.broadcast_tensors(ii, jj)
         ii, jj = ii.contiguous(), jj.contiguous()
-        out = torch.empty(ii.shape, dtype=torch.float64, device=ii.device)
-        check(lib.hm_entries(self._h, ii.numel(), ctypes.c_void_p(ii.data_ptr()),
+        out = torch.empty(ii.shape, dtype=self._tdtype(), device=ii.device)
+        fn = lib.hm_entries_f32 if self.f32 else lib.hm_entries
+        check(fn(self._h, ii.numel(), ctypes.c_void_p(ii.data_ptr()),
                              ctypes.c_void_p(jj.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                              _stream_ptr(None)))
         res = out.cpu().numpy()

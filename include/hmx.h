@@ -149,6 +149,7 @@ enum hm_info {
     HM_INFO_BYTES_REF_LAYOUT,  /* same arrays in the reference's per-block layout */
     HM_INFO_FLAGS, HM_INFO_N_ROWCHUNKS, HM_INFO_N_COLCHUNKS, HM_INFO_NONFINITE,
     HM_INFO_N_INSTANCES, HM_INFO_FROM_BLOCKS,
+    HM_INFO_FLOAT32,           /* 1 for a plan built by hm_plan_from_points_f32 */
     HM_INFO_COUNT
 };
 int hm_plan_info(hm_plan_t plan, int64_t *info, int n);
@@ -213,8 +214,8 @@ int hm_graph_destroy(hm_graph_t graph);
  * and writes its choice to out[0] = right-hand sides per launch, out[1] =
  * mode (0 stored, 1 Chebyshev basis, 2 matrix-free), out[2] = its time in
  * microseconds. Every candidate computes the same product up to rounding. */
-int hm_plan_tune(hm_plan_t plan, double *y, const double *x, int64_t k, void *stream,
-                 int64_t out[3]);
+int hm_plan_tune(hm_plan_t plan, void *y, const void *x, int64_t k, void *stream,
+                 int64_t out[3]);  /* y, x hold the plan's element type */
 
 /* Per-pass timing of the products of this plan (bench support, no counterpart
  * in the reference). enable=1 starts recording HIP events around the four
@@ -229,6 +230,39 @@ int hm_plan_profile_read(hm_plan_t plan, double ms[4], int64_t *count);
  * src/BarycentricMatrix.jl:222-234 at far blocks. */
 int hm_entries(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j, double *out,
                void *stream);
+
+/* ---------------------------------------------------------------- Float32
+ * KernelMatrix{Float32}: the reference's constructors and mul! are generic in
+ * the element type T (src/KernelMatrix.jl:47-116, src/BarycentricMatrix.jl:
+ * 92-136,147-178,248-307, src/algebra.jl:34-48,243-277), so a Float32 tree
+ * splits at Float32 midpoints, its nodes, weights, F, U, V and dense blocks are
+ * Float32 values formed in Float32 arithmetic, and mul! accumulates in
+ * Float32. These entry points build and apply such a plan: points, intervals
+ * and vectors are float; the tree rule, layout and passes are those of the
+ * Float64 plan, with every stored array and every accumulation in FP32.
+ * Ranks up to 32 (BLOCKRANK(Float32) = 10). Reduced-precision far-field flags
+ * do not apply. The Float64 product entry points reject a Float32 plan and
+ * the _f32 ones a Float64 plan (HM_ERR_INVALID). */
+int hm_plan_from_points_f32(int kernel_id, const float *x, int64_t nx, const float *y, int64_t ny,
+                            float a, float b, float c, float d, int blocksize, int rank,
+                            unsigned flags, void *stream, hm_plan_t *plan);
+int hm_matvec_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, int64_t joff,
+                  int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                  unsigned flags, void *stream);
+int hm_matvec_t_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, int64_t joff,
+                    int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                    unsigned flags, void *stream);
+int hm_matvec_grad_x_f32(hm_plan_t plan, float *g, const float *b, const float *s, int64_t ioff,
+                         int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta,
+                         void *stream);
+int hm_matvec_t_grad_f32(hm_plan_t plan, float *gb, float *gy, const float *r, const float *s,
+                         int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
+                         void *stream);
+int hm_graph_matvec_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, int64_t joff,
+                        int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
+                        unsigned flags, hm_graph_t *graph);
+int hm_entries_f32(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j, float *out,
+                   void *stream);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,7 @@
  *                                    closed form F = f(t_m, s_n)
  *   leaf kernels                  -> restated known-answer tests of test/runtests.jl:15-52
  *                                    (exact on small integers) and numpy GEMV
- *   traversal mul!                -> dense long-double K*b within the interpolation bound
+ *   traversal mul!                -> dense long-or_real K*b within the interpolation bound
  */
 #include "hmx_oracle.h"
 
@@ -36,18 +36,18 @@
 
 /* examples/Kernel.jl:34-37. Julia lowers (x-y)^2 to d*d and (x-y)^3 to (d*d)*d
  * (Base.literal_pow), and inv(d) to 1/d. */
-double or_kernel(int kernel_id, double x, double y) {
-    double d = x - y;
+or_real or_kernel(int kernel_id, or_real x, or_real y) {
+    or_real d = x - y;
     switch (kernel_id) {
-    case OR_CAUCHY: return 1.0 / d;
-    case OR_COULOMB: return 1.0 / (d * d);
-    case OR_COULOMB3: return 1.0 / ((d * d) * d);
-    case OR_LOG: return log(fabs(d));
+    case OR_CAUCHY: return R(1) / d;
+    case OR_COULOMB: return R(1) / (d * d);
+    case OR_COULOMB3: return R(1) / ((d * d) * d);
+    case OR_LOG: return OR_LOGF(OR_FABS(d));
     default: return NAN;
     }
 }
 
-/* src/HierarchicalMatrices.jl:5 in double arithmetic; for mant_bits <= 52 the
+/* src/HierarchicalMatrices.jl:5 in or_real arithmetic; for mant_bits <= 52 the
  * result equals the one computed in the narrower type (values 4, 10, 20, 100
  * for Float16, Float32, Float64, BigFloat(256)). */
 int or_blockrank(int mant_bits) {
@@ -56,49 +56,49 @@ int or_blockrank(int mant_bits) {
     return 2 * (int)lround(v);
 }
 
-double or_sinpi(double q) {
+or_real or_sinpi(or_real q) {
     const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
-    return (double)sinl(PI_L * (long double)q);
+    return (or_real)sinl(PI_L * (long double)q);
 }
 
 /* src/BarycentricMatrix.jl:92-111 */
-void or_chebyshevpoints(double *x, int n, int kind) {
+void or_chebyshevpoints(or_real *x, int n, int kind) {
     for (int k = 0; k < n; ++k) x[k] = 0.0;
     int nd2 = n / 2;
     if (kind == 1) {
         for (int k = 1; k <= nd2; ++k)
-            x[k - 1] = or_sinpi(((double)(n - 2 * k) + 1.0) / (double)(2 * n));
+            x[k - 1] = or_sinpi(((or_real)(n - 2 * k) + R(1)) / (or_real)(2 * n));
         for (int k = 1; k <= nd2; ++k) x[n - k] = -x[k - 1];
     } else {
         for (int k = 1; k <= nd2; ++k)
-            x[k - 1] = or_sinpi(((double)(n - 2 * k) + 1.0) / (double)(2 * (n - 1)));
+            x[k - 1] = or_sinpi(((or_real)(n - 2 * k) + R(1)) / (or_real)(2 * (n - 1)));
         for (int k = 1; k <= nd2; ++k) x[n - k] = -x[k - 1];
     }
 }
 
 /* src/BarycentricMatrix.jl:114-136 */
-void or_chebyshevbarycentricweights(double *l, int n, int kind) {
+void or_chebyshevbarycentricweights(or_real *l, int n, int kind) {
     for (int k = 0; k < n; ++k) l[k] = 0.0;
     int nd2 = n / 2;
     if (kind == 1) {
         for (int k = 1; k <= nd2 + 1 && k <= n; ++k)
-            l[k - 1] = or_sinpi(((double)(2 * k) - 1.0) / (double)(2 * n));
+            l[k - 1] = or_sinpi(((or_real)(2 * k) - R(1)) / (or_real)(2 * n));
         for (int k = 1; k <= nd2; ++k) l[n - k] = l[k - 1];
-        for (int k = 2; k <= n; k += 2) l[k - 1] *= -1.0;
+        for (int k = 2; k <= n; k += 2) l[k - 1] *= R(-1);
     } else {
         for (int k = 0; k < n; ++k) l[k] = 1.0;
-        for (int k = 2; k <= n; k += 2) l[k - 1] *= -1.0;
-        l[0] *= 0.5;
-        l[n - 1] *= 0.5;
+        for (int k = 2; k <= n; k += 2) l[k - 1] *= R(-1);
+        l[0] *= R(0.5);
+        l[n - 1] *= R(0.5);
     }
 }
 
 /* src/BarycentricMatrix.jl:299-307 */
-int or_indsplit(const double *x, int64_t nx, int64_t first, int64_t len, double a,
-                double b, int64_t out[4]) {
+int or_indsplit(const or_real *x, int64_t nx, int64_t first, int64_t len, or_real a,
+                or_real b, int64_t out[4]) {
     int64_t last = first + len - 1;
     int64_t i = first;
-    double ab2 = 0.5 * (a + b);
+    or_real ab2 = R(0.5) * (a + b);
     for (;;) {
         if (i < 1 || i > nx) return -1; /* Julia BoundsError */
         if (!(x[i - 1] >= ab2)) break;
@@ -122,11 +122,11 @@ typedef struct {
     int depth;         /* depth of this block (root children have depth 1) */
     int64_t i0, m;     /* 1-based first row index into x, and row count */
     int64_t j0, n;     /* 1-based first column index into y, and column count */
-    double a, b, c, d; /* intervals the block was generated with */
+    or_real a, b, c, d; /* intervals the block was generated with */
     or_node *node;     /* code KM */
-    double *A;         /* dense m x n, column-major (store_factors only) */
-    double *tx, *ty;   /* far: mapped nodes (always) */
-    double *U, *V, *F; /* far factors (store_factors only) */
+    or_real *A;         /* dense m x n, column-major (store_factors only) */
+    or_real *tx, *ty;   /* far: mapped nodes (always) */
+    or_real *U, *V, *F; /* far factors (store_factors only) */
 } or_child;
 
 struct or_node {
@@ -141,10 +141,10 @@ typedef struct {
 
 struct or_tree {
     int kernel_id, r, bs, store, quirk, err;
-    const double *x, *y; /* borrowed during build only */
-    double *xs, *ys;     /* owned copies */
+    const or_real *x, *y; /* borrowed during build only */
+    or_real *xs, *ys;     /* owned copies */
     int64_t nx, ny;
-    double *xhat, *lam; /* chebyshevpoints / weights of rank r */
+    or_real *xhat, *lam; /* chebyshevpoints / weights of rank r */
     or_child root;
     int64_t nd, nk1, nk2, ndense, nfar, dense_entries, far_rows, far_cols;
     int maxdepth;
@@ -159,7 +159,7 @@ static void *xmalloc(size_t n) {
 
 /* src/KernelMatrix.jl:57-60 (and :82, :104): T[f(x[i], y[j]) for i in ir, j in jr] */
 static void gen_dense(or_tree *t, or_child *ch) {
-    ch->A = (double *)xmalloc(sizeof(double) * (size_t)(ch->m * ch->n));
+    ch->A = (or_real *)xmalloc(sizeof(or_real) * (size_t)(ch->m * ch->n));
     for (int64_t j = 0; j < ch->n; ++j)
         for (int64_t i = 0; i < ch->m; ++i)
             ch->A[i + ch->m * j] = or_kernel(t->kernel_id, t->xs[ch->i0 - 1 + i], t->ys[ch->j0 - 1 + j]);
@@ -168,19 +168,19 @@ static void gen_dense(or_tree *t, or_child *ch) {
 /* nodes of BarycentricPoly2D, src/BarycentricMatrix.jl:147-166 */
 static void gen_nodes(or_tree *t, or_child *ch) {
     int r = t->r;
-    ch->tx = (double *)xmalloc(sizeof(double) * r);
-    ch->ty = (double *)xmalloc(sizeof(double) * r);
-    double ab2 = 0.5 * (ch->a + ch->b), ba2 = 0.5 * (ch->b - ch->a);
+    ch->tx = (or_real *)xmalloc(sizeof(or_real) * r);
+    ch->ty = (or_real *)xmalloc(sizeof(or_real) * r);
+    or_real ab2 = R(0.5) * (ch->a + ch->b), ba2 = R(0.5) * (ch->b - ch->a);
     for (int p = 0; p < r; ++p) ch->tx[p] = ab2 + ba2 * t->xhat[p];
-    double cd2 = 0.5 * (ch->c + ch->d), dc2 = 0.5 * (ch->d - ch->c);
+    or_real cd2 = R(0.5) * (ch->c + ch->d), dc2 = R(0.5) * (ch->d - ch->c);
     for (int q = 0; q < r; ++q) ch->ty[q] = cd2 + dc2 * t->xhat[q];
 }
 
 /* one row of update!, src/BarycentricMatrix.jl:256-267 (U) / :272-283 (V):
  * row[m] = lam[m]*inv(p - t[m]), then row[m] /= sum(row) */
-static void basis_row(const double *tn, const double *lam, int r, double p, double *row) {
-    for (int m = 0; m < r; ++m) row[m] = lam[m] * (1.0 / (p - tn[m]));
-    double s = 0.0;
+static void basis_row(const or_real *tn, const or_real *lam, int r, or_real p, or_real *row) {
+    for (int m = 0; m < r; ++m) row[m] = lam[m] * (R(1) / (p - tn[m]));
+    or_real s = 0.0;
     for (int m = 0; m < r; ++m) s += row[m];
     for (int m = 0; m < r; ++m) row[m] /= s;
 }
@@ -189,12 +189,12 @@ static void basis_row(const double *tn, const double *lam, int r, double p, doub
  * (src/BarycentricMatrix.jl:248-287). */
 static void gen_far(or_tree *t, or_child *ch) {
     int r = t->r;
-    ch->F = (double *)xmalloc(sizeof(double) * r * r);
+    ch->F = (or_real *)xmalloc(sizeof(or_real) * r * r);
     for (int n = 0; n < r; ++n)
         for (int m = 0; m < r; ++m) ch->F[m + r * n] = or_kernel(t->kernel_id, ch->tx[m], ch->ty[n]);
-    ch->U = (double *)xmalloc(sizeof(double) * (size_t)(ch->m * r));
-    ch->V = (double *)xmalloc(sizeof(double) * (size_t)(ch->n * r));
-    double row[256];
+    ch->U = (or_real *)xmalloc(sizeof(or_real) * (size_t)(ch->m * r));
+    ch->V = (or_real *)xmalloc(sizeof(or_real) * (size_t)(ch->n * r));
+    or_real row[256];
     for (int64_t i = 0; i < ch->m; ++i) {
         basis_row(ch->tx, t->lam, r, t->xs[ch->i0 - 1 + i], row);
         for (int m = 0; m < r; ++m) ch->U[i + ch->m * m] = row[m];
@@ -217,7 +217,7 @@ static void set_dense(or_tree *t, or_child *ch, int depth, int64_t i0, int64_t m
 }
 
 /* BarycentricMatrix2D(T, f, a, b, c, d, x, y, ir, jr), src/BarycentricMatrix.jl:236-246 */
-static void set_far(or_tree *t, or_child *ch, int depth, double a, double b, double c, double d,
+static void set_far(or_tree *t, or_child *ch, int depth, or_real a, or_real b, or_real c, or_real d,
                     int64_t i0, int64_t m, int64_t j0, int64_t n) {
     memset(ch, 0, sizeof(*ch));
     ch->code = OR_CODE_BARY;
@@ -232,10 +232,10 @@ static void set_far(or_tree *t, or_child *ch, int depth, double a, double b, dou
 }
 
 static or_node *build_node(or_tree *t, int type, int depth, int64_t i0, int64_t ni, int64_t j0,
-                           int64_t nj, double a, double b, double c, double d);
+                           int64_t nj, or_real a, or_real b, or_real c, or_real d);
 
 static void set_node(or_tree *t, or_child *ch, int type, int depth, int64_t i0, int64_t ni,
-                     int64_t j0, int64_t nj, double a, double b, double c, double d) {
+                     int64_t j0, int64_t nj, or_real a, or_real b, or_real c, or_real d) {
     memset(ch, 0, sizeof(*ch));
     ch->code = OR_CODE_KM;
     ch->depth = depth;
@@ -246,7 +246,7 @@ static void set_node(or_tree *t, or_child *ch, int type, int depth, int64_t i0, 
 
 /* KernelMatrix / KernelMatrix1 / KernelMatrix2, src/KernelMatrix.jl:49-116 */
 static or_node *build_node(or_tree *t, int type, int depth, int64_t i0, int64_t ni, int64_t j0,
-                           int64_t nj, double a, double b, double c, double d) {
+                           int64_t nj, or_real a, or_real b, or_real c, or_real d) {
     if (t->err) return NULL;
     if (depth > t->maxdepth) t->maxdepth = depth;
     if (depth > 200) { t->err = 2; return NULL; } /* the reference would recurse without bound */
@@ -261,7 +261,7 @@ static or_node *build_node(or_tree *t, int type, int depth, int64_t i0, int64_t 
     }
     int64_t ir1f = is[0], ir1n = is[1], ir2f = is[2], ir2n = is[3];
     int64_t jr1f = js[0], jr1n = js[1], jr2f = js[2], jr2n = js[3];
-    double ab2 = 0.5 * (a + b), cd2 = 0.5 * (c + d);
+    or_real ab2 = R(0.5) * (a + b), cd2 = R(0.5) * (c + d);
     int bs = t->bs;
     int leaf = ir1n < bs && ir2n < bs && jr1n < bs && jr2n < bs;
     or_node *H = (or_node *)calloc(1, sizeof(or_node));
@@ -336,8 +336,8 @@ static void flatten(or_tree *t, or_node *H, int64_t istart, int64_t jstart) {
     }
 }
 
-or_tree *or_tree_build(int kernel_id, const double *x, const double *y, int64_t nx, int64_t ny,
-                       double a, double b, double c, double d, int blocksize, int rank,
+or_tree *or_tree_build(int kernel_id, const or_real *x, const or_real *y, int64_t nx, int64_t ny,
+                       or_real a, or_real b, or_real c, or_real d, int blocksize, int rank,
                        int store_factors, int *quirk) {
     if (rank < 1 || rank > 256 || blocksize < 1) return NULL;
     or_tree *t = (or_tree *)calloc(1, sizeof(or_tree));
@@ -347,12 +347,12 @@ or_tree *or_tree_build(int kernel_id, const double *x, const double *y, int64_t 
     t->store = store_factors;
     t->nx = nx;
     t->ny = ny;
-    t->xs = (double *)xmalloc(sizeof(double) * (size_t)nx);
-    t->ys = (double *)xmalloc(sizeof(double) * (size_t)ny);
-    memcpy(t->xs, x, sizeof(double) * (size_t)nx);
-    memcpy(t->ys, y, sizeof(double) * (size_t)ny);
-    t->xhat = (double *)xmalloc(sizeof(double) * rank);
-    t->lam = (double *)xmalloc(sizeof(double) * rank);
+    t->xs = (or_real *)xmalloc(sizeof(or_real) * (size_t)nx);
+    t->ys = (or_real *)xmalloc(sizeof(or_real) * (size_t)ny);
+    memcpy(t->xs, x, sizeof(or_real) * (size_t)nx);
+    memcpy(t->ys, y, sizeof(or_real) * (size_t)ny);
+    t->xhat = (or_real *)xmalloc(sizeof(or_real) * rank);
+    t->lam = (or_real *)xmalloc(sizeof(or_real) * rank);
     or_chebyshevpoints(t->xhat, rank, 1);
     or_chebyshevbarycentricweights(t->lam, rank, 1);
     /* KernelMatrix(f, x, y, a, b, c, d), src/KernelMatrix.jl:47 */
@@ -388,28 +388,28 @@ void or_tree_stats(const or_tree *t, int64_t out[9]) {
     out[6] = t->far_rows; out[7] = t->far_cols; out[8] = t->maxdepth;
 }
 
-int64_t or_tree_blocks(const or_tree *t, int64_t *ints, double *dbls, int64_t cap) {
+int64_t or_tree_blocks(const or_tree *t, int64_t *ints, or_real *dbls, int64_t cap) {
     for (int64_t k = 0; k < t->nflat && k < cap; ++k) {
         const or_child *c = t->flat[k].c;
         int64_t *o = ints + 6 * k;
         o[0] = c->code; o[1] = c->depth; o[2] = t->flat[k].row0; o[3] = t->flat[k].col0;
         o[4] = c->m; o[5] = c->n;
-        double *g = dbls + 4 * k;
+        or_real *g = dbls + 4 * k;
         g[0] = c->a; g[1] = c->b; g[2] = c->c; g[3] = c->d;
     }
     return t->nflat;
 }
 
-int or_tree_block_data(const or_tree *t, int64_t bidx, double *A, double *U, double *V, double *F) {
+int or_tree_block_data(const or_tree *t, int64_t bidx, or_real *A, or_real *U, or_real *V, or_real *F) {
     if (!t->store || bidx < 0 || bidx >= t->nflat) return -1;
     const or_child *c = t->flat[bidx].c;
     if (c->code == OR_CODE_DENSE) {
-        if (A) memcpy(A, c->A, sizeof(double) * (size_t)(c->m * c->n));
+        if (A) memcpy(A, c->A, sizeof(or_real) * (size_t)(c->m * c->n));
     } else {
         int r = t->r;
-        if (U) memcpy(U, c->U, sizeof(double) * (size_t)(c->m * r));
-        if (V) memcpy(V, c->V, sizeof(double) * (size_t)(c->n * r));
-        if (F) memcpy(F, c->F, sizeof(double) * (size_t)(r * r));
+        if (U) memcpy(U, c->U, sizeof(or_real) * (size_t)(c->m * r));
+        if (V) memcpy(V, c->V, sizeof(or_real) * (size_t)(c->n * r));
+        if (F) memcpy(F, c->F, sizeof(or_real) * (size_t)(r * r));
     }
     return 0;
 }
@@ -417,80 +417,80 @@ int or_tree_block_data(const or_tree *t, int64_t bidx, double *A, double *U, dou
 /* ------------------------------------------------------------ leaf kernels */
 
 /* src/algebra.jl:37-48 */
-void or_dense_mul(double *y, const double *A, int64_t m, int64_t n, const double *x, int64_t istart,
+void or_dense_mul(or_real *y, const or_real *A, int64_t m, int64_t n, const or_real *x, int64_t istart,
                   int64_t jstart, int64_t incx, int64_t incy) {
     int64_t ishift = istart - incy, jshift = jstart - incx;
     for (int64_t j = 1; j <= n; ++j) {
-        double xj = x[jshift + j * incx - 1];
+        or_real xj = x[jshift + j * incx - 1];
         for (int64_t i = 1; i <= m; ++i) y[ishift + i * incy - 1] += A[(i - 1) + m * (j - 1)] * xj;
     }
 }
 
 /* src/algebra.jl:52-65 (transpose(A) with A m x n) */
-void or_dense_tmul(double *y, const double *A, int64_t m, int64_t n, const double *x,
+void or_dense_tmul(or_real *y, const or_real *A, int64_t m, int64_t n, const or_real *x,
                    int64_t istart, int64_t jstart, int64_t incx, int64_t incy) {
     int64_t ishift = istart - incy, jshift = jstart - incx;
     for (int64_t i = 1; i <= n; ++i) {
-        double yi = 0.0;
+        or_real yi = 0.0;
         for (int64_t j = 1; j <= m; ++j) yi += A[(j - 1) + m * (i - 1)] * x[jshift + j * incx - 1];
         y[ishift + i * incy - 1] += yi;
     }
 }
 
 /* src/algebra.jl:110-131 (U m x r, V n x r, S = diag) */
-void or_lowrank_mul(double *y, const double *U, const double *S, const double *V, int64_t m,
-                    int64_t n, int64_t r, const double *x, int64_t istart, int64_t jstart,
+void or_lowrank_mul(or_real *y, const or_real *U, const or_real *S, const or_real *V, int64_t m,
+                    int64_t n, int64_t r, const or_real *x, int64_t istart, int64_t jstart,
                     int64_t incx, int64_t incy) {
     int64_t ishift = istart - incy, jshift = jstart - incx;
-    double *temp = (double *)xmalloc(sizeof(double) * (size_t)r);
+    or_real *temp = (or_real *)xmalloc(sizeof(or_real) * (size_t)r);
     for (int64_t k = 0; k < r; ++k) {
         temp[k] = 0.0;
         for (int64_t j = 1; j <= n; ++j) temp[k] += V[(j - 1) + n * k] * x[jshift + j * incx - 1];
         temp[k] *= S[k];
     }
     for (int64_t k = 0; k < r; ++k) {
-        double tk = temp[k];
+        or_real tk = temp[k];
         for (int64_t i = 1; i <= m; ++i) y[ishift + i * incy - 1] += U[(i - 1) + m * k] * tk;
     }
     free(temp);
 }
 
 /* src/algebra.jl:137-159: adjoint(L) = V S U', size n x m */
-void or_lowrank_adjmul(double *y, const double *U, const double *S, const double *V, int64_t m,
-                       int64_t n, int64_t r, const double *x, int64_t istart, int64_t jstart,
+void or_lowrank_adjmul(or_real *y, const or_real *U, const or_real *S, const or_real *V, int64_t m,
+                       int64_t n, int64_t r, const or_real *x, int64_t istart, int64_t jstart,
                        int64_t incx, int64_t incy) {
     int64_t ishift = istart - incy, jshift = jstart - incx;
-    double *temp = (double *)xmalloc(sizeof(double) * (size_t)r);
+    or_real *temp = (or_real *)xmalloc(sizeof(or_real) * (size_t)r);
     for (int64_t k = 0; k < r; ++k) {
         temp[k] = 0.0;
         for (int64_t j = 1; j <= m; ++j) temp[k] += U[(j - 1) + m * k] * x[jshift + j * incx - 1];
         temp[k] *= S[k];
     }
     for (int64_t k = 0; k < r; ++k) {
-        double tk = temp[k];
+        or_real tk = temp[k];
         for (int64_t i = 1; i <= n; ++i) y[ishift + i * incy - 1] += V[(i - 1) + n * k] * tk;
     }
     free(temp);
 }
 
 /* src/algebra.jl:243-277 */
-void or_bary_mul(double *u, const double *U, const double *F, const double *V, int64_t m,
-                 int64_t n, int64_t r, const double *v, int64_t istart, int64_t jstart) {
+void or_bary_mul(or_real *u, const or_real *U, const or_real *F, const or_real *V, int64_t m,
+                 int64_t n, int64_t r, const or_real *v, int64_t istart, int64_t jstart) {
     int64_t ishift = istart - 1, jshift = jstart - 1;
-    double *t1 = (double *)xmalloc(sizeof(double) * (size_t)r);
-    double *t2 = (double *)xmalloc(sizeof(double) * (size_t)r);
+    or_real *t1 = (or_real *)xmalloc(sizeof(or_real) * (size_t)r);
+    or_real *t2 = (or_real *)xmalloc(sizeof(or_real) * (size_t)r);
     for (int64_t k = 0; k < r; ++k) {
-        double s = 0.0;
+        or_real s = 0.0;
         for (int64_t j = 1; j <= n; ++j) s += V[(j - 1) + n * k] * v[jshift + j - 1];
         t1[k] = s;
         t2[k] = 0.0;
     }
     for (int64_t l = 0; l < r; ++l) {
-        double t1l = t1[l];
+        or_real t1l = t1[l];
         for (int64_t k = 0; k < r; ++k) t2[k] += F[k + r * l] * t1l;
     }
     for (int64_t k = 0; k < r; ++k) {
-        double t2k = t2[k];
+        or_real t2k = t2[k];
         for (int64_t i = 1; i <= m; ++i) u[ishift + i - 1] += U[(i - 1) + m * k] * t2k;
     }
     free(t1);
@@ -499,7 +499,7 @@ void or_bary_mul(double *u, const double *U, const double *F, const double *V, i
 
 /* --------------------------------------------------------------- matvecs */
 
-int or_tree_matvec(or_tree *t, double *u, const double *v) {
+int or_tree_matvec(or_tree *t, or_real *u, const or_real *v) {
     if (!t->store) return -1;
     for (int64_t k = 0; k < t->nflat; ++k) {
         or_child *c = t->flat[k].c;
@@ -512,13 +512,13 @@ int or_tree_matvec(or_tree *t, double *u, const double *v) {
 
 /* temp2 = F * (V' v_block) of src/algebra.jl:249-265, with V rows regenerated
  * (bit-identical to update!) unless stored. */
-static void far_temp2(const or_tree *t, const or_flat *fb, const double *v, double *t2) {
+static void far_temp2(const or_tree *t, const or_flat *fb, const or_real *v, or_real *t2) {
     const or_child *c = fb->c;
     int r = t->r;
-    double t1[256], row[256];
+    or_real t1[256], row[256];
     for (int k = 0; k < r; ++k) t1[k] = 0.0;
     for (int64_t j = 0; j < c->n; ++j) {
-        double vj = v[fb->col0 + j];
+        or_real vj = v[fb->col0 + j];
         if (c->V) {
             for (int k = 0; k < r; ++k) t1[k] += c->V[j + c->n * k] * vj;
         } else {
@@ -528,19 +528,19 @@ static void far_temp2(const or_tree *t, const or_flat *fb, const double *v, doub
     }
     for (int k = 0; k < r; ++k) t2[k] = 0.0;
     for (int l = 0; l < r; ++l) {
-        double t1l = t1[l];
+        or_real t1l = t1[l];
         for (int k = 0; k < r; ++k) {
-            double F = c->F ? c->F[k + r * l] : or_kernel(t->kernel_id, c->tx[k], c->ty[l]);
+            or_real F = c->F ? c->F[k + r * l] : or_kernel(t->kernel_id, c->tx[k], c->ty[l]);
             t2[k] += F * t1l;
         }
     }
 }
 
 /* phase-2 worker: u[row] accumulated over blocks in traversal order */
-static void row_accumulate(const or_tree *t, const double *v, const double *t2all,
-                           const int64_t *faridx, int64_t row, double *acc) {
+static void row_accumulate(const or_tree *t, const or_real *v, const or_real *t2all,
+                           const int64_t *faridx, int64_t row, or_real *acc) {
     int r = t->r;
-    double rowb[256];
+    or_real rowb[256];
     for (int64_t k = 0; k < t->nflat; ++k) {
         const or_flat *fb = &t->flat[k];
         const or_child *c = fb->c;
@@ -548,12 +548,12 @@ static void row_accumulate(const or_tree *t, const double *v, const double *t2al
         int64_t i = row - fb->row0;
         if (c->code == OR_CODE_DENSE) {
             for (int64_t j = 0; j < c->n; ++j) {
-                double A = c->A ? c->A[i + c->m * j]
+                or_real A = c->A ? c->A[i + c->m * j]
                                 : or_kernel(t->kernel_id, t->xs[c->i0 - 1 + i], t->ys[c->j0 - 1 + j]);
                 *acc += A * v[fb->col0 + j];
             }
         } else {
-            const double *t2 = t2all + (size_t)r * (size_t)faridx[k];
+            const or_real *t2 = t2all + (size_t)r * (size_t)faridx[k];
             if (c->U) {
                 for (int kk = 0; kk < r; ++kk) *acc += c->U[i + c->m * kk] * t2[kk];
             } else {
@@ -564,14 +564,14 @@ static void row_accumulate(const or_tree *t, const double *v, const double *t2al
     }
 }
 
-static double *all_temp2(const or_tree *t, const double *v, int64_t **faridx_out, int nthreads) {
+static or_real *all_temp2(const or_tree *t, const or_real *v, int64_t **faridx_out, int nthreads) {
     int64_t *faridx = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)t->nflat);
     int64_t nf = 0;
     for (int64_t k = 0; k < t->nflat; ++k) faridx[k] = t->flat[k].c->code == OR_CODE_BARY ? nf++ : -1;
     int64_t *list = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(nf + 1));
     for (int64_t k = 0; k < t->nflat; ++k)
         if (faridx[k] >= 0) list[faridx[k]] = k;
-    double *t2 = (double *)xmalloc(sizeof(double) * (size_t)(nf * t->r + 1));
+    or_real *t2 = (or_real *)xmalloc(sizeof(or_real) * (size_t)(nf * t->r + 1));
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
@@ -582,10 +582,10 @@ static double *all_temp2(const or_tree *t, const double *v, int64_t **faridx_out
     return t2;
 }
 
-int or_tree_matvec_omp(or_tree *t, double *u, const double *v, int nthreads) {
+int or_tree_matvec_omp(or_tree *t, or_real *u, const or_real *v, int nthreads) {
     if (!t->store) return -1;
     int64_t *faridx;
-    double *t2 = all_temp2(t, v, &faridx, nthreads);
+    or_real *t2 = all_temp2(t, v, &faridx, nthreads);
     /* bucket blocks by 256-row chunks, keeping traversal order */
     const int64_t CH = 256;
     int64_t nch = (t->nx + CH - 1) / CH;
@@ -618,13 +618,13 @@ int or_tree_matvec_omp(or_tree *t, double *u, const double *v, int nthreads) {
             int64_t r1 = fb->row0 + c->m < hi ? fb->row0 + c->m : hi;
             if (c->code == OR_CODE_DENSE) {
                 for (int64_t j = 0; j < c->n; ++j) {
-                    double xj = v[fb->col0 + j];
+                    or_real xj = v[fb->col0 + j];
                     for (int64_t i = r0; i < r1; ++i) u[i] += c->A[(i - fb->row0) + c->m * j] * xj;
                 }
             } else {
-                const double *tt = t2 + (size_t)r * (size_t)faridx[lst[e]];
+                const or_real *tt = t2 + (size_t)r * (size_t)faridx[lst[e]];
                 for (int kk = 0; kk < r; ++kk) {
-                    double tk = tt[kk];
+                    or_real tk = tt[kk];
                     for (int64_t i = r0; i < r1; ++i) u[i] += c->U[(i - fb->row0) + c->m * kk] * tk;
                 }
             }
@@ -634,16 +634,16 @@ int or_tree_matvec_omp(or_tree *t, double *u, const double *v, int nthreads) {
     return 0;
 }
 
-int or_tree_matvec_rows(or_tree *t, const double *v, const int64_t *rows, int64_t nrows,
-                        double *out, int nthreads) {
+int or_tree_matvec_rows(or_tree *t, const or_real *v, const int64_t *rows, int64_t nrows,
+                        or_real *out, int nthreads) {
     int64_t *faridx;
-    double *t2 = all_temp2(t, v, &faridx, nthreads);
+    or_real *t2 = all_temp2(t, v, &faridx, nthreads);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
 #endif
     for (int64_t k = 0; k < nrows; ++k) {
-        double acc = 0.0;
+        or_real acc = 0.0;
         row_accumulate(t, v, t2, faridx, rows[k], &acc);
         out[k] = acc;
     }
@@ -654,7 +654,7 @@ int or_tree_matvec_rows(or_tree *t, const double *v, const int64_t *rows, int64_
 
 /* getindex, src/hierarchical.jl:120-147 with the leaf getindex of
  * src/BarycentricMatrix.jl:222-234 and a dense lookup */
-double or_tree_getindex(or_tree *t, int64_t i, int64_t j) {
+or_real or_tree_getindex(or_tree *t, int64_t i, int64_t j) {
     or_node *H = t->root.node;
     int64_t ii = i + 1, jj = j + 1; /* 1-based, as in the reference */
     for (;;) {
@@ -676,14 +676,14 @@ double or_tree_getindex(or_tree *t, int64_t i, int64_t j) {
             return or_kernel(t->kernel_id, t->xs[c->i0 - 2 + ii], t->ys[c->j0 - 2 + jj]);
         }
         int r = t->r;
-        double ur[256], vr[256];
+        or_real ur[256], vr[256];
         basis_row(c->tx, t->lam, r, t->xs[c->i0 - 2 + ii], ur);
         basis_row(c->ty, t->lam, r, t->ys[c->j0 - 2 + jj], vr);
-        double ret = 0.0;
+        or_real ret = 0.0;
         for (int k = 0; k < r; ++k) {
-            double tmp = 0.0;
+            or_real tmp = 0.0;
             for (int l = 0; l < r; ++l) {
-                double F = c->F ? c->F[k + r * l] : or_kernel(t->kernel_id, c->tx[k], c->ty[l]);
+                or_real F = c->F ? c->F[k + r * l] : or_kernel(t->kernel_id, c->tx[k], c->ty[l]);
                 tmp += F * vr[l];
             }
             ret += ur[k] * tmp;
@@ -694,8 +694,8 @@ double or_tree_getindex(or_tree *t, int64_t i, int64_t j) {
 
 /* ------------------------------------------------------------ dense oracle */
 
-void or_dense_matvec_rows(int kernel_id, const double *x, const double *y, int64_t ny,
-                          const double *v, const int64_t *rows, int64_t nrows, double *out,
+void or_dense_matvec_rows(int kernel_id, const or_real *x, const or_real *y, int64_t ny,
+                          const or_real *v, const int64_t *rows, int64_t nrows, or_real *out,
                           int nthreads) {
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -704,8 +704,8 @@ void or_dense_matvec_rows(int kernel_id, const double *x, const double *y, int64
     for (int64_t k = 0; k < nrows; ++k) {
         int64_t i = rows ? rows[k] : k;
         long double acc = 0.0L;
-        double xi = x[i];
+        or_real xi = x[i];
         for (int64_t j = 0; j < ny; ++j) acc += (long double)or_kernel(kernel_id, xi, y[j]) * (long double)v[j];
-        out[k] = (double)acc;
+        out[k] = (or_real)acc;
     }
 }

@@ -13,6 +13,21 @@
 
 #include <stdint.h>
 
+/* The same source builds two libraries: liboracle.so with or_real = double
+ * (Float64) and liboracle_f32.so with or_real = float (Float32, every
+ * operation rounded to single precision as Julia does for T = Float32).
+ * liboracle_f32.so is built with -DOR_F32. */
+#ifdef OR_F32
+typedef float or_real;
+#define OR_LOGF logf
+#define OR_FABS fabsf
+#else
+typedef double or_real;
+#define OR_LOGF log
+#define OR_FABS fabs
+#endif
+#define R(v) ((or_real)(v))
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -22,24 +37,24 @@ enum { OR_CODE_NONE = 0, OR_CODE_KM = 1, OR_CODE_BARY = 2, OR_CODE_DENSE = 3 };
 enum { OR_NODE_D = 0, OR_NODE_K1 = 1, OR_NODE_K2 = 2 };
 
 /* scalar kernels of examples/Kernel.jl:34-37 */
-double or_kernel(int kernel_id, double x, double y);
+or_real or_kernel(int kernel_id, or_real x, or_real y);
 
 /* BLOCKRANK for an IEEE type with `mant_bits` explicit mantissa bits
  * (src/HierarchicalMatrices.jl:5): 2*round(0.5*log_{3+sqrt 8}(1/eps)). */
 int or_blockrank(int mant_bits);
 
 /* first/second-kind Chebyshev points, src/BarycentricMatrix.jl:92-111 */
-void or_chebyshevpoints(double *out, int n, int kind);
+void or_chebyshevpoints(or_real *out, int n, int kind);
 /* barycentric weights, src/BarycentricMatrix.jl:114-136 */
-void or_chebyshevbarycentricweights(double *out, int n, int kind);
+void or_chebyshevbarycentricweights(or_real *out, int n, int kind);
 /* sinpi used by the two functions above */
-double or_sinpi(double q);
+or_real or_sinpi(or_real q);
 
 /* indsplit, src/BarycentricMatrix.jl:299-307. x is 1-based in the sense that
  * index i reads x[i-1]. Writes (first1,len1,first2,len2). Returns 0, or -1 when
  * the reference would raise BoundsError. */
-int or_indsplit(const double *x, int64_t nx, int64_t first, int64_t len,
-                double a, double b, int64_t out[4]);
+int or_indsplit(const or_real *x, int64_t nx, int64_t first, int64_t len,
+                or_real a, or_real b, int64_t out[4]);
 
 /* KernelMatrix tree, src/KernelMatrix.jl:47-116 */
 typedef struct or_tree or_tree;
@@ -48,8 +63,8 @@ typedef struct or_tree or_tree;
  * (ref layout); 0 keeps only the skeleton (ranges, intervals) for the
  * row-sampled matvec. Returns NULL on failure; *quirk is set to 1 if an empty
  * range went through indsplit (the reference then returns a length-1 range). */
-or_tree *or_tree_build(int kernel_id, const double *x, const double *y, int64_t nx,
-                       int64_t ny, double a, double b, double c, double d,
+or_tree *or_tree_build(int kernel_id, const or_real *x, const or_real *y, int64_t nx,
+                       int64_t ny, or_real a, or_real b, or_real c, or_real d,
                        int blocksize, int rank, int store_factors, int *quirk);
 void or_tree_free(or_tree *t);
 
@@ -61,50 +76,50 @@ void or_tree_stats(const or_tree *t, int64_t out[9]);
  * code, depth, row0 (0-based), col0, m, n, then 4 doubles a,b,c,d (far only).
  * Returns the number of blocks; writes at most cap records. */
 int64_t or_tree_blocks(const or_tree *t, int64_t *ints /* 6 per block */,
-                       double *dbls /* 4 per block */, int64_t cap);
+                       or_real *dbls /* 4 per block */, int64_t cap);
 
 /* Faithful traversal mul! (src/KernelMatrix.jl:14-45): u += H*v with the
  * reference's loop order. Needs store_factors=1. */
-int or_tree_matvec(or_tree *t, double *u, const double *v);
+int or_tree_matvec(or_tree *t, or_real *u, const or_real *v);
 
 /* Same values, OpenMP over row chunks: phase 1 forms F*(V'v) per far block,
  * phase 2 accumulates each row in traversal order, so every u[i] is
  * bit-identical to or_tree_matvec. nthreads<=0 uses the OpenMP default. */
-int or_tree_matvec_omp(or_tree *t, double *u, const double *v, int nthreads);
+int or_tree_matvec_omp(or_tree *t, or_real *u, const or_real *v, int nthreads);
 
 /* Row-sampled faithful mul!: out[k] = (H*v)[rows[k]] (0-based rows), with the
  * reference's per-row summation order; far-field factors are regenerated on
  * the fly (bit-identical to the stored ones), so it works at any n. */
-int or_tree_matvec_rows(or_tree *t, const double *v, const int64_t *rows,
-                        int64_t nrows, double *out, int nthreads);
+int or_tree_matvec_rows(or_tree *t, const or_real *v, const int64_t *rows,
+                        int64_t nrows, or_real *out, int nthreads);
 
 /* getindex(H,i,j), src/hierarchical.jl:120-147 (0-based i,j here) */
-double or_tree_getindex(or_tree *t, int64_t i, int64_t j);
+or_real or_tree_getindex(or_tree *t, int64_t i, int64_t j);
 
 /* Copy generated factors of the b-th block (traversal order) out: dense A
  * (m*n), or far U (m*r), V (n*r), F (r*r). Needs store_factors=1. */
-int or_tree_block_data(const or_tree *t, int64_t bidx, double *A, double *U, double *V,
-                       double *F);
+int or_tree_block_data(const or_tree *t, int64_t bidx, or_real *A, or_real *U, or_real *V,
+                       or_real *F);
 
 /* Leaf kernels, src/algebra.jl. istart/jstart are 1-based as in Julia. */
-void or_dense_mul(double *y, const double *A, int64_t m, int64_t n, const double *x,
+void or_dense_mul(or_real *y, const or_real *A, int64_t m, int64_t n, const or_real *x,
                   int64_t istart, int64_t jstart, int64_t incx, int64_t incy);
-void or_dense_tmul(double *y, const double *A, int64_t m, int64_t n, const double *x,
+void or_dense_tmul(or_real *y, const or_real *A, int64_t m, int64_t n, const or_real *x,
                    int64_t istart, int64_t jstart, int64_t incx, int64_t incy);
-void or_lowrank_mul(double *y, const double *U, const double *S, const double *V,
-                    int64_t m, int64_t n, int64_t r, const double *x, int64_t istart,
+void or_lowrank_mul(or_real *y, const or_real *U, const or_real *S, const or_real *V,
+                    int64_t m, int64_t n, int64_t r, const or_real *x, int64_t istart,
                     int64_t jstart, int64_t incx, int64_t incy);
-void or_lowrank_adjmul(double *y, const double *U, const double *S, const double *V,
-                       int64_t m, int64_t n, int64_t r, const double *x, int64_t istart,
+void or_lowrank_adjmul(or_real *y, const or_real *U, const or_real *S, const or_real *V,
+                       int64_t m, int64_t n, int64_t r, const or_real *x, int64_t istart,
                        int64_t jstart, int64_t incx, int64_t incy);
-void or_bary_mul(double *u, const double *U, const double *F, const double *V, int64_t m,
-                 int64_t n, int64_t r, const double *v, int64_t istart, int64_t jstart);
+void or_bary_mul(or_real *u, const or_real *U, const or_real *F, const or_real *V, int64_t m,
+                 int64_t n, int64_t r, const or_real *v, int64_t istart, int64_t jstart);
 
 /* Dense oracle: out[k] = sum_j f(x[rows[k]], y[j]) * v[j] accumulated in long
- * double (rows==NULL means all rows 0..nrows-1). OpenMP over rows. */
-void or_dense_matvec_rows(int kernel_id, const double *x, const double *y, int64_t ny,
-                          const double *v, const int64_t *rows, int64_t nrows,
-                          double *out, int nthreads);
+ * or_real (rows==NULL means all rows 0..nrows-1). OpenMP over rows. */
+void or_dense_matvec_rows(int kernel_id, const or_real *x, const or_real *y, int64_t ny,
+                          const or_real *v, const int64_t *rows, int64_t nrows,
+                          or_real *out, int nthreads);
 
 #ifdef __cplusplus
 }

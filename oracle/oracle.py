@@ -15,13 +15,16 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH_F32 = os.path.join(HERE, "liboracle_f32.so")
 
 CAUCHY, COULOMB, COULOMB3, LOG = 0, 1, 2, 3
 CODE_KM, CODE_BARY, CODE_DENSE = 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int64)
 _lib = None
+_lib32 = None
 
 
 def build():
@@ -29,61 +32,68 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
+def _setup(L, _dp, real):
+    L.or_kernel.restype = real
+    L.or_kernel.argtypes = [ctypes.c_int, real, real]
+    L.or_blockrank.restype = ctypes.c_int
+    L.or_blockrank.argtypes = [ctypes.c_int]
+    L.or_sinpi.restype = real
+    L.or_sinpi.argtypes = [real]
+    L.or_chebyshevpoints.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+    L.or_chebyshevbarycentricweights.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+    L.or_indsplit.restype = ctypes.c_int
+    L.or_indsplit.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, real, real, _ip]
+    L.or_tree_build.restype = ctypes.c_void_p
+    L.or_tree_build.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                                real, real, real, real, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_int)]
+    L.or_tree_free.argtypes = [ctypes.c_void_p]
+    L.or_tree_stats.argtypes = [ctypes.c_void_p, _ip]
+    L.or_tree_blocks.restype = ctypes.c_int64
+    L.or_tree_blocks.argtypes = [ctypes.c_void_p, _ip, _dp, ctypes.c_int64]
+    L.or_tree_matvec.restype = ctypes.c_int
+    L.or_tree_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.or_tree_matvec_omp.restype = ctypes.c_int
+    L.or_tree_matvec_omp.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+    L.or_tree_matvec_rows.restype = ctypes.c_int
+    L.or_tree_matvec_rows.argtypes = [ctypes.c_void_p, _dp, _ip, ctypes.c_int64, _dp, ctypes.c_int]
+    L.or_tree_getindex.restype = real
+    L.or_tree_getindex.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
+    L.or_tree_block_data.restype = ctypes.c_int
+    L.or_tree_block_data.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]
+    for name in ("or_dense_mul", "or_dense_tmul"):
+        getattr(L, name).argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _dp,
+                                     ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int64]
+    for name in ("or_lowrank_mul", "or_lowrank_adjmul"):
+        getattr(L, name).argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int64, ctypes.c_int64]
+    L.or_bary_mul.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
+                              ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64]
+    L.or_dense_matvec_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _ip,
+                                       ctypes.c_int64, _dp, ctypes.c_int]
+    return L
+
+
+def lib(f32=False):
+    """The Float64 oracle, or its Float32 twin (same source, or_real = float)."""
+    global _lib, _lib32
+    if f32:
+        if _lib32 is None:
+            if not os.path.exists(LIB_PATH_F32):
+                build()
+            _lib32 = _setup(ctypes.CDLL(LIB_PATH_F32), _fp, ctypes.c_float)
+        return _lib32
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.or_kernel.restype = ctypes.c_double
-        L.or_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
-        L.or_blockrank.restype = ctypes.c_int
-        L.or_blockrank.argtypes = [ctypes.c_int]
-        L.or_sinpi.restype = ctypes.c_double
-        L.or_sinpi.argtypes = [ctypes.c_double]
-        L.or_chebyshevpoints.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
-        L.or_chebyshevbarycentricweights.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
-        L.or_indsplit.restype = ctypes.c_int
-        L.or_indsplit.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                  ctypes.c_double, ctypes.c_double, _ip]
-        L.or_tree_build.restype = ctypes.c_void_p
-        L.or_tree_build.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
-                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                    ctypes.POINTER(ctypes.c_int)]
-        L.or_tree_free.argtypes = [ctypes.c_void_p]
-        L.or_tree_stats.argtypes = [ctypes.c_void_p, _ip]
-        L.or_tree_blocks.restype = ctypes.c_int64
-        L.or_tree_blocks.argtypes = [ctypes.c_void_p, _ip, _dp, ctypes.c_int64]
-        L.or_tree_matvec.restype = ctypes.c_int
-        L.or_tree_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.or_tree_matvec_omp.restype = ctypes.c_int
-        L.or_tree_matvec_omp.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
-        L.or_tree_matvec_rows.restype = ctypes.c_int
-        L.or_tree_matvec_rows.argtypes = [ctypes.c_void_p, _dp, _ip, ctypes.c_int64, _dp,
-                                          ctypes.c_int]
-        L.or_tree_getindex.restype = ctypes.c_double
-        L.or_tree_getindex.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
-        L.or_tree_block_data.restype = ctypes.c_int
-        L.or_tree_block_data.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]
-        for name in ("or_dense_mul", "or_dense_tmul"):
-            getattr(L, name).argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, _dp,
-                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                         ctypes.c_int64]
-        for name in ("or_lowrank_mul", "or_lowrank_adjmul"):
-            getattr(L, name).argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
-                                         ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64,
-                                         ctypes.c_int64, ctypes.c_int64]
-        L.or_bary_mul.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
-                                  ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64]
-        L.or_dense_matvec_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _ip,
-                                           ctypes.c_int64, _dp, ctypes.c_int]
-        _lib = L
+        _lib = _setup(ctypes.CDLL(LIB_PATH), _dp, ctypes.c_double)
     return _lib
 
 
 def _d(a):
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_fp if a.dtype == np.float32 else _dp)
 
 
 def _i(a):
@@ -94,31 +104,35 @@ def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def kernel(kid, x, y):
-    return lib().or_kernel(kid, float(x), float(y))
+def _dt(f32):
+    return np.float32 if f32 else np.float64
+
+
+def kernel(kid, x, y, f32=False):
+    return lib(f32).or_kernel(kid, float(x), float(y))
 
 
 def blockrank(mant_bits):
     return lib().or_blockrank(mant_bits)
 
 
-def chebyshevpoints(n, kind=1):
-    out = np.zeros(n)
-    lib().or_chebyshevpoints(_d(out), n, kind)
+def chebyshevpoints(n, kind=1, f32=False):
+    out = np.zeros(n, dtype=_dt(f32))
+    lib(f32).or_chebyshevpoints(_d(out), n, kind)
     return out
 
 
-def chebyshevbarycentricweights(n, kind=1):
-    out = np.zeros(n)
-    lib().or_chebyshevbarycentricweights(_d(out), n, kind)
+def chebyshevbarycentricweights(n, kind=1, f32=False):
+    out = np.zeros(n, dtype=_dt(f32))
+    lib(f32).or_chebyshevbarycentricweights(_d(out), n, kind)
     return out
 
 
-def indsplit(x, first, length, a, b):
+def indsplit(x, first, length, a, b, f32=False):
     """Returns ((first1, len1), (first2, len2)) with 1-based firsts."""
-    x = f64(x)
+    x = np.ascontiguousarray(x, dtype=_dt(f32))
     out = np.zeros(4, dtype=np.int64)
-    rc = lib().or_indsplit(_d(x), len(x), first, length, a, b, _i(out))
+    rc = lib(f32).or_indsplit(_d(x), len(x), first, length, a, b, _i(out))
     if rc:
         raise IndexError("BoundsError in indsplit")
     return (int(out[0]), int(out[1])), (int(out[2]), int(out[3]))
@@ -127,12 +141,15 @@ def indsplit(x, first, length, a, b):
 class Tree:
     """KernelMatrix tree built by the oracle (src/KernelMatrix.jl:47-116)."""
 
-    def __init__(self, kid, x, y, a, b, c, d, blocksize, rank, store=True):
-        self.x = f64(x)
-        self.y = f64(y)
+    def __init__(self, kid, x, y, a, b, c, d, blocksize, rank, store=True, f32=False):
+        self.f32 = f32
+        self.dt = _dt(f32)
+        self.L = lib(f32)
+        self.x = np.ascontiguousarray(x, dtype=self.dt)
+        self.y = np.ascontiguousarray(y, dtype=self.dt)
         self.kid, self.rank, self.blocksize = kid, rank, blocksize
         quirk = ctypes.c_int(0)
-        h = lib().or_tree_build(kid, _d(self.x), _d(self.y), len(self.x), len(self.y), a, b, c, d,
+        h = self.L.or_tree_build(kid, _d(self.x), _d(self.y), len(self.x), len(self.y), a, b, c, d,
                                 blocksize, rank, 1 if store else 0, ctypes.byref(quirk))
         if not h:
             raise ValueError("oracle tree build failed (bounds error or unbounded recursion)")
@@ -142,25 +159,25 @@ class Tree:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None) and _lib is not None:
-                _lib.or_tree_free(self.h)
+            if getattr(self, "h", None) and getattr(self, "L", None) is not None:
+                self.L.or_tree_free(self.h)
         except Exception:
             pass
         self.h = None
 
     def stats(self):
         out = np.zeros(9, dtype=np.int64)
-        lib().or_tree_stats(self.h, _i(out))
+        self.L.or_tree_stats(self.h, _i(out))
         keys = ["d", "k1", "k2", "dense", "far", "dense_entries", "far_rows", "far_cols", "depth"]
         return dict(zip(keys, (int(v) for v in out)))
 
     def blocks(self):
         """Leaf blocks in traversal order: ints (nb,6) = code, depth, row0,
         col0, m, n; dbls (nb,4) = a, b, c, d."""
-        nb = lib().or_tree_blocks(self.h, None, None, 0)
+        nb = self.L.or_tree_blocks(self.h, None, None, 0)
         ints = np.zeros((max(nb, 1), 6), dtype=np.int64)
-        dbls = np.zeros((max(nb, 1), 4))
-        lib().or_tree_blocks(self.h, _i(ints), _d(dbls), nb)
+        dbls = np.zeros((max(nb, 1), 4), dtype=self.dt)
+        self.L.or_tree_blocks(self.h, _i(ints), _d(dbls), nb)
         return ints[:nb], dbls[:nb]
 
     def block_data(self, k):
@@ -168,38 +185,38 @@ class Tree:
         code, _, _, _, m, n = ints[k]
         r = self.rank
         if code == CODE_DENSE:
-            A = np.zeros(m * n)
-            lib().or_tree_block_data(self.h, k, _d(A), None, None, None)
+            A = np.zeros(m * n, dtype=self.dt)
+            self.L.or_tree_block_data(self.h, k, _d(A), None, None, None)
             return {"A": A.reshape((m, n), order="F")}
-        U, V, F = np.zeros(m * r), np.zeros(n * r), np.zeros(r * r)
-        lib().or_tree_block_data(self.h, k, None, _d(U), _d(V), _d(F))
+        U, V, F = (np.zeros(k, dtype=self.dt) for k in (m * r, n * r, r * r))
+        self.L.or_tree_block_data(self.h, k, None, _d(U), _d(V), _d(F))
         return {"U": U.reshape((m, r), order="F"), "V": V.reshape((n, r), order="F"),
                 "F": F.reshape((r, r), order="F")}
 
     def matvec(self, v, u=None):
         """Faithful mul!: returns u + H*v (u defaults to zeros)."""
-        v = f64(v)
-        u = np.zeros(len(self.x)) if u is None else f64(u).copy()
-        if lib().or_tree_matvec(self.h, _d(u), _d(v)):
+        v = np.ascontiguousarray(v, dtype=self.dt)
+        u = np.zeros(len(self.x), dtype=self.dt) if u is None else np.array(u, dtype=self.dt)
+        if self.L.or_tree_matvec(self.h, _d(u), _d(v)):
             raise ValueError("tree has no stored factors")
         return u
 
     def matvec_omp(self, v, nthreads=0, u=None):
-        v = f64(v)
-        u = np.zeros(len(self.x)) if u is None else f64(u).copy()
-        if lib().or_tree_matvec_omp(self.h, _d(u), _d(v), nthreads):
+        v = np.ascontiguousarray(v, dtype=self.dt)
+        u = np.zeros(len(self.x), dtype=self.dt) if u is None else np.array(u, dtype=self.dt)
+        if self.L.or_tree_matvec_omp(self.h, _d(u), _d(v), nthreads):
             raise ValueError("tree has no stored factors")
         return u
 
     def matvec_rows(self, v, rows, nthreads=0):
-        v = f64(v)
+        v = np.ascontiguousarray(v, dtype=self.dt)
         rows = np.ascontiguousarray(rows, dtype=np.int64)
-        out = np.zeros(len(rows))
-        lib().or_tree_matvec_rows(self.h, _d(v), _i(rows), len(rows), _d(out), nthreads)
+        out = np.zeros(len(rows), dtype=self.dt)
+        self.L.or_tree_matvec_rows(self.h, _d(v), _i(rows), len(rows), _d(out), nthreads)
         return out
 
     def getindex(self, i, j):
-        return lib().or_tree_getindex(self.h, i, j)
+        return self.L.or_tree_getindex(self.h, i, j)
 
 
 def dense_matvec(kid, x, y, v, rows=None, nthreads=0):

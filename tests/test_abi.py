@@ -160,3 +160,52 @@ def test_empty_cluster_is_reported():
     x2 = np.linspace(0.99, 0.01, n)
     compare_structure(tree("cauchykernel", x2, y, (1.0, 0.0, 1.0, 0.0), 16, 8), x2, y,
                       (1.0, 0.0, 1.0, 0.0), 16, 8)
+
+
+# ------------------------------------------------------------ Float32 plans
+
+def compare_structure_f32(x, y, iv, bs, r, kid=O.CAUCHY, kernel="cauchykernel"):
+    x, y = x.astype(np.float32), y.astype(np.float32)
+    H = hmx.KernelMatrix(kernel, x, y, *iv, blocksize=bs, rank=r, tree_only=True)
+    assert H.dtype == np.float32 and H.info["float32"] == 1
+    t = O.Tree(kid, x, y, *iv, bs, r, store=False, f32=True)
+    ib, db = H.blocks()
+    ob, odb = t.blocks()
+    assert np.array_equal(ib, ob)
+    assert np.array_equal(db, odb.astype(np.float64))
+    st = t.stats()
+    info = H.info
+    assert (info["n_d"], info["n_k1"], info["n_k2"], info["n_dense"], info["n_far"]) == \
+        (st["d"], st["k1"], st["k2"], st["dense"], st["far"])
+    return info
+
+
+def test_structure_f32_families():
+    """Float32 trees (midpoints rounded to Float32) equal the Float32
+    oracle's leaf for leaf: grid, random points, the example families."""
+    w = hmx.workloads.config_plain()
+    info = compare_structure_f32(w["x"], w["y"], w["intervals"], 64, 16)
+    assert info["bytes_ref_layout"] == 20447232 // 2  # 4-byte elements
+    for seed, n in [(0, 1034), (3, 20000)]:
+        x, y, iv = hmx.workloads.sorted_uniform_points(n, seed)
+        compare_structure_f32(x, y, iv, 40, 10)
+    x = hmx.chebyshevpoints(1000)
+    y = hmx.chebyshevpoints(1000, kind=2)
+    compare_structure_f32(x, y, (1.0, -1.0, 1.0, -1.0), 40, 10)
+    x, y, iv = hmx.workloads.sorted_uniform_points(1 << 16, 1)
+    compare_structure_f32(x, y, iv, 512, 32, O.LOG, "logkernel")
+
+
+def test_f32_defaults_and_errors():
+    x, y, iv = hmx.workloads.shifted_grid(256)
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    H = hmx.KernelMatrix("cauchykernel", x32, y32, *iv, tree_only=True)
+    assert (H.rank, H.blocksize_) == (hmx.BLOCKRANK(np.float32), hmx.BLOCKSIZE(np.float32)) == (10, 40)
+    # Float32 plans take rank <= 32
+    assert status_of(lambda: hmx.KernelMatrix("cauchykernel", x32, y32, *iv, blocksize=16,
+                                              rank=33, tree_only=True)) == _lib.HM_ERR_UNSUPPORTED
+    assert status_of(lambda: hmx.KernelMatrix("cauchykernel", x32[::-1].copy(), y32, *iv,
+                                              tree_only=True)) == _lib.HM_ERR_NOT_DESCENDING
+    # a Float64 product entry point on a Float32 plan fails before any GPU work
+    assert _lib.lib.hm_matvec_f32(H._h, None, None, 0, 0, 1, 1, 1, 0, 0, 0, 0, None) == \
+        _lib.HM_ERR_INVALID
