@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-n-log2", type=int, default=17)
-    ap.add_argument("--variants", default="hybrid,matrix_free",
+    ap.add_argument("--variants", default="hybrid,matrix_free,f32_stored,f32_hybrid",
                     help="extra variants timed beside the stored product")
     ap.add_argument("--no-peaks", action="store_true", help="skip the torch bandwidth probes")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -166,8 +166,10 @@ def timed(fn, steps, warmup, stream):
 def run_batched(args):
     """256 Cauchy instances, n log-uniform in [2^10, 2^16], rank 16, leaf 64,
     32 right-hand sides; one step = the product over every instance and right-
-    hand side. Variants: stored / hybrid / matrix-free plans, the autotuned
-    choice, and a HIP-graph replay of the tuned product."""
+    hand side. Variants: stored / hybrid / matrix-free plans on the FP64 MFMA
+    passes (the default for 32 right-hand sides) and on the vector units, the
+    FP32 and split-BF16 far field, the autotuned choice, and HIP-graph
+    replays."""
     import numpy as np
     import torch
     import hmx
@@ -199,9 +201,16 @@ def run_batched(args):
                                                       torch.linalg.norm(d))
         err = float(torch.linalg.norm(Y - ref) / torch.linalg.norm(ref))
         info = H.info
+        Yv = torch.empty_like(Y)
+        ms_v = timed(lambda: H._mv(Yv, B, 1, 1, 1, 1, 0, False, False, stream, mfma=False),
+                     args.steps, args.warmup, stream)
         out["variants"][name] = {"ms_per_step": round(ms, 4), "value": round(1000 / ms, 3),
                                  "hbm_bytes_stored": info["bytes_stored"],
-                                 "rel_diff_vs_stored": err}
+                                 "rel_diff_vs_stored": err,
+                                 "vector_units": {"ms_per_step": round(ms_v, 4),
+                                                  "rel_diff_vs_mfma": float(
+                                                      torch.linalg.norm(Yv - Y) / torch.linalg.norm(Y))}}
+        del Yv
         for prec in ("f32", "bf16x3"):
             Ym = torch.empty_like(Y)
             ms_m = timed(lambda: H._mv(Ym, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
@@ -236,7 +245,8 @@ def run_batched(args):
 
 def run_gradient(args):
     """L = 1/2 ||H(x, y) b - t||^2, n = 2^16, leaf 64, rank 16: one step =
-    forward product + pullback to x, y and b."""
+    forward product + pullback to x, y and b, with Float64 tensors (the bench
+    line) and with Float32 tensors (Float32 plans)."""
     import numpy as np
     import torch
     import hmx
@@ -245,37 +255,46 @@ def run_gradient(args):
     stream = torch.cuda.current_stream()
     w = hmx.workloads.config_gradient()
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    x, y, b, t = T(w["x"]), T(w["y"]), T(w["b"]), T(w["t"])
-    x.requires_grad_(True)
-    y.requires_grad_(True)
-    b.requires_grad_(True)
 
-    def step():
-        x.grad = y.grad = b.grad = None
-        u = kernel_matvec("cauchykernel", x, y, b, w["intervals"], 64, 16)
-        L = 0.5 * torch.sum((u - t) ** 2)
-        L.backward()
+    def measure(dtype):
+        x, y, b, t = (T(w[k]).to(dtype) for k in ("x", "y", "b", "t"))
+        x.requires_grad_(True)
+        y.requires_grad_(True)
+        b.requires_grad_(True)
 
-    ms = timed(step, args.steps, args.warmup, stream)
-    # dense FP64 gradient on 256 sampled rows / columns (torch, GPU)
-    n = x.numel()
-    idx = torch.arange(0, n, n // 256, device=dev)
-    with torch.no_grad():
-        xd, yd, bd = x.detach(), y.detach(), b.detach()
-        Kb = dense_rows(lambda d: 1.0 / d, xd, yd, bd[:, None])[:, 0]
-        r = Kb - t
-        gx = r[idx] * dense_rows(lambda d: -1.0 / (d * d), xd[idx], yd, bd[:, None])[:, 0]
-        # column sums: K^T r and K_y^T r at the sampled columns
-        gb = dense_rows(lambda d: 1.0 / d, -yd[idx], -xd, r[:, None])[:, 0]
-        gy = bd[idx] * dense_rows(lambda d: 1.0 / (d * d), -yd[idx], -xd, r[:, None])[:, 0]
-    e = lambda a, c: float(torch.linalg.norm(a - c) / torch.linalg.norm(c))
-    step()
+        def step():
+            x.grad = y.grad = b.grad = None
+            u = kernel_matvec("cauchykernel", x, y, b, w["intervals"], 64, 16)
+            L = 0.5 * torch.sum((u - t) ** 2)
+            L.backward()
+
+        ms = timed(step, args.steps, args.warmup, stream)
+        # dense FP64 gradient of the same (possibly Float32) points on 256
+        # sampled rows / columns (torch, GPU)
+        n = x.numel()
+        idx = torch.arange(0, n, n // 256, device=dev)
+        with torch.no_grad():
+            xd, yd, bd, td = (v.detach().double() for v in (x, y, b, t))
+            Kb = dense_rows(lambda d: 1.0 / d, xd, yd, bd[:, None])[:, 0]
+            r = Kb - td
+            gx = r[idx] * dense_rows(lambda d: -1.0 / (d * d), xd[idx], yd, bd[:, None])[:, 0]
+            # column sums: K^T r and K_y^T r at the sampled columns
+            gb = dense_rows(lambda d: 1.0 / d, -yd[idx], -xd, r[:, None])[:, 0]
+            gy = bd[idx] * dense_rows(lambda d: 1.0 / (d * d), -yd[idx], -xd, r[:, None])[:, 0]
+        e = lambda a, c: float(torch.linalg.norm(a.double() - c) / torch.linalg.norm(c))
+        step()
+        return n, ms, {"grad_x": e(x.grad[idx], gx), "grad_y": e(y.grad[idx], gy),
+                       "grad_b": e(b.grad[idx], gb)}
+
+    n, ms, err = measure(torch.float64)
+    _, ms32, err32 = measure(torch.float32)
     return {"metric": "gradient steps/s (forward + pullback to x, y, b), n=2^16",
             "unit": "steps/s", "value": round(1000 / ms, 3), "ms_per_step": round(ms, 4),
             "dtype": "f64", "steps": args.steps,
             "config": {"workload": "gradient", "n": n, "rank": 16, "leaf_blocksize": 64},
-            "rel_err_vs_dense_fp64": {"grad_x": e(x.grad[idx], gx), "grad_y": e(y.grad[idx], gy),
-                                      "grad_b": e(b.grad[idx], gb)}}
+            "rel_err_vs_dense_fp64": err,
+            "variants": {"f32": {"value": round(1000 / ms32, 3), "ms_per_step": round(ms32, 4),
+                                 "dtype": "f32", "rel_err_vs_dense_fp64": err32}}}
 
 
 def run_scale(args):
@@ -309,6 +328,12 @@ def run_scale(args):
         out["variants"][name] = {"ms_per_step": round(ms, 4), "value": round(1000 / ms, 3),
                                  "build_s": round(build, 3), "hbm_bytes_stored": H.info["bytes_stored"],
                                  "rel_err_vs_dense_4096rows": err}
+        ms_m = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, mfma=True),
+                     args.steps, args.warmup, stream)
+        out["variants"][name]["mfma_fp64"] = {
+            "ms_per_step": round(ms_m, 4), "value": round(1000 / ms_m, 3),
+            "rel_err_vs_dense_4096rows":
+                float(torch.linalg.norm(Y[rows] - dense) / torch.linalg.norm(dense))}
         for prec in ("f32", "bf16x3"):
             ms_m = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
                                        prec), args.steps, args.warmup, stream)
@@ -382,22 +407,35 @@ def main():
     # hybrid (stored A, F; Chebyshev-form basis) and matrix-free variants
     variants = {}
     wanted = [v for v in args.variants.split(",") if v]
-    for name, kw in (("hybrid", dict(hybrid=True)), ("matrix_free", dict(matrix_free=True))):
+    import numpy as np
+    b32 = b.float()
+    for name, kw in (("hybrid", dict(hybrid=True)), ("matrix_free", dict(matrix_free=True)),
+                     ("f32_stored", dict(dtype=np.float32)),
+                     ("f32_hybrid", dict(dtype=np.float32, hybrid=True))):
         if name not in wanted:
             continue
+        is32 = name.startswith("f32")
+        # Float32 plans: the same points (exact in Float32 at this n), the
+        # tree, factors and products in FP32
         Hv = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=BS,
                               rank=RANK, **kw)
-        uv = torch.empty_like(b)
+        bv = b32 if is32 else b
+        uv = torch.empty_like(bv)
 
         def step_v():
-            Hv._mv(uv, b, 1, 1, 1, 1, 0, False, False, stream)
+            Hv._mv(uv, bv, 1, 1, 1, 1, 0, False, False, stream)
 
         for _ in range(args.warmup):
             step_v()
         ms_v = time_steps(step_v, args.steps, stream)
         pv = pass_times(Hv, step_v, args.steps)
+        # a Float32 stored plan reproduces the reference's Float32 update!,
+        # which divides by x - t and gives Inf where a Float32 node equals a
+        # point (many do at this n); the count is reported, not an error
+        err_v = float(torch.linalg.norm(uv.double() - u) / torch.linalg.norm(u))
         variants[name] = dict(ms=ms_v, passes=pv, bytes=Hv.info["bytes_stored"],
-                              err=float(torch.linalg.norm(uv - u) / torch.linalg.norm(u)))
+                              err=err_v if np.isfinite(err_v) else None,
+                              nonfinite=Hv.info["nonfinite"])
         Hv.close()
         del uv
         torch.cuda.empty_cache()
@@ -447,7 +485,8 @@ def main():
                          "ms_per_step": round(v["ms"], 4),
                          "hbm_bytes_per_matvec": v["bytes"],
                          "pass_ms": {q: round(float(t), 4) for q, t in zip(names, v["passes"])},
-                         "rel_diff_vs_stored": v["err"]} for k, v in variants.items()},
+                         "rel_diff_vs_stored": v["err"],
+                         "nonfinite_factors": v["nonfinite"]} for k, v in variants.items()},
         "roofline": roof,
         "measured_hbm_peaks": peaks,
         "cpu_baseline": cpu,

@@ -68,10 +68,13 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
 // kb in {1, 2, 4, 8}: right-hand sides handled by this launch (forward only).
 hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode, int kb,
                              hipStream_t s, const hipEvent_t *ev = nullptr);
-// forward product with the far field on MFMA (prec 0: FP32, 1: split BF16),
-// up to 16 right-hand sides (nq) per launch; mode 1 (stored A, F) or 2 (free)
-hipError_t hmx_launch_matvec_mixed(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                                   hipStream_t s);
+// forward product of up to 16 right-hand sides (nq) on the MFMA units, far
+// field in Chebyshev form: prec 2 FP64, 0 FP32, 1 split BF16 for the far-field
+// products (near field, reduce and translate FP64); mode 0/1 use the stored
+// dense blocks, 2 regenerates them. multi[0..nmulti) lists the column
+// clusters with more than one upward task.
+hipError_t hmx_launch_matvec_m(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                               const int32_t *multi, int nmulti, hipStream_t s);
 // gradients (matrix-free near field, Chebyshev-form far field):
 //   forward (tr = false): y = beta*y + s .* (d/dx_i of (H x)_i)
 //   transpose (tr = true): y = beta*y + H^T x, y2 = beta*y2 + s .* (d/dy_j of (H^T x)_j)

@@ -54,14 +54,14 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
     return hipGetLastError();
 }
 
-hipError_t hmx_launch_matvec_mixed(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                                   hipStream_t s) {
-    if (p.r <= 8) return hmx_matvec_mixed_r8(p, a, mode, prec, nq, s);
-    if (p.r <= 16) return hmx_matvec_mixed_r16(p, a, mode, prec, nq, s);
-    if (p.r <= 24) return hmx_matvec_mixed_r24(p, a, mode, prec, nq, s);
-    if (p.r <= 32) return hmx_matvec_mixed_r32(p, a, mode, prec, nq, s);
-    if (p.r <= 48) return hmx_matvec_mixed_r48(p, a, mode, prec, nq, s);
-    return hmx_matvec_mixed_r64(p, a, mode, prec, nq, s);
+hipError_t hmx_launch_matvec_m(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                               const int32_t *multi, int nmulti, hipStream_t s) {
+    if (p.r <= 8) return hmx_matvec_m_r8(p, a, mode, prec, nq, multi, nmulti, s);
+    if (p.r <= 16) return hmx_matvec_m_r16(p, a, mode, prec, nq, multi, nmulti, s);
+    if (p.r <= 24) return hmx_matvec_m_r24(p, a, mode, prec, nq, multi, nmulti, s);
+    if (p.r <= 32) return hmx_matvec_m_r32(p, a, mode, prec, nq, multi, nmulti, s);
+    if (p.r <= 48) return hmx_matvec_m_r48(p, a, mode, prec, nq, multi, nmulti, s);
+    return hmx_matvec_m_r64(p, a, mode, prec, nq, multi, nmulti, s);
 }
 
 // ------------------------------------------------------------- FP32 plans

@@ -40,6 +40,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace hmxk {
 namespace {  // internal linkage: every translation unit keeps its own copies
 
@@ -314,11 +316,13 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
 // handles row m of right-hand sides q = g, g + G, ... (G = 64 / r groups).
 template <class T, bool CHEB, int KB>
 __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const T *partial,
-                                              const T *chebA, T *w, int r) {
+                                              const T *chebA, T *w, int r,
+                                              const int32_t *ids = nullptr) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= ncl) return;
-    const DCluster C = cl[wid];
+    const int cid = ids ? ids[wid] : wid;  // cluster of this wavefront
+    const DCluster C = cl[cid];
     const int G = WAVE / r;
     const int m = lane % r, g = lane / r;
     const bool act = g < G;
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, cons
             }
             acc = v;
         }
-        if (aq) w[((int64_t)wid * KB + q) * r + m] = acc;
+        if (aq) w[((int64_t)cid * KB + q) * r + m] = acc;
     }
 }
 
@@ -885,48 +889,75 @@ hipError_t grad_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
 }
 
 
-// ------------------------------------------------- mixed-precision far field
-// Far field of the Chebyshev-form passes on MFMA with FP32 accumulation, for
-// 16 right-hand sides per launch. T_m(xh) comes from FP64 point differences
-// and an FP64 recurrence; only the products T * X (upward) and T * C
-// (downward) run in reduced precision:
-//   PREC 0: v_mfma_f32_16x16x4f32 (FP32 operands)
-//   PREC 1: split BF16, x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
-//           hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16
-// Operand layouts (16x16 output tile, lane l): A[i][k] from lane i = l % 16,
-// k = K/4 * (l / 16) + v; B[k][j] from lane j = l % 16, same k; D[i][j] in
-// lane j = l % 16, i = 4 * (l / 16) + v.
+// ------------------------------------------ multi-RHS passes on the MFMA units
+// Forward products of 16 right-hand sides per launch (nq <= 16 of them valid;
+// the others read and write zeros), far field in Chebyshev form, every
+// contraction on the matrix cores:
+//   up         mu = T(points)^T X per task, one wavefront per task; a cluster
+//              with a single task applies A at once (w = A mu), the others
+//              leave partial sums for the reduce
+//   reduce     w = A sum(mu) over the clusters with several tasks
+//   translate  z = sum_b F_b w_b and c = A^T z, one wavefront per row cluster
+//   down       near field A_blk X and far field T(rows) C, one wavefront per
+//              64-row chunk, written once
+// PREC selects the arithmetic of the far-field products T X and T C:
+//   2: FP64 on v_mfma_f64_16x16x4f64
+//   0: FP32 operands on v_mfma_f32_16x16x4f32, FP32 accumulation
+//   1: split BF16 (x = hi + lo, hi = bf16(x), lo = bf16(x - hi); products
+//      hi*hi + hi*lo + lo*hi) on v_mfma_f32_16x16x32_bf16, FP32 accumulation
+// Near field, reduce and translate run in FP64 (v_mfma_f64_16x16x4f64) for
+// every PREC; point differences and the Chebyshev recurrence are FP64.
+// Operand layouts (lane l, j16 = l % 16, g = l / 16): A[i][k] comes from the
+// lane with i = j16 and k = g (K = 4 shapes) or k = 8g + v, v < 8 (K = 32);
+// B[k][j] from the lane with j = j16 and the same k. Results: the FP32 shapes
+// hold D[4g + v][j16] in register v, the FP64 shape holds D[g + 4v][j16].
 typedef float mf4_t __attribute__((ext_vector_type(4)));
+typedef double md4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 mbf8_t __attribute__((ext_vector_type(8)));
-constexpr int MQ = 16;  // right-hand sides per mixed-precision launch
+constexpr int MQ = 16;  // right-hand sides per launch
 
 __device__ __forceinline__ __bf16 bf16_hi(float x) { return (__bf16)x; }
 __device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)(__bf16)x); }
+__device__ __forceinline__ md4_t mfma64(double a, double b, md4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
 
-// Upward pass: moments mu[m][q] = sum_j T_m(xh_j) x[j, q] of one task. Each
-// wavefront forms T for 64 points (one per lane) into LDS, then multiplies.
+// Upward pass, one wavefront (the whole workgroup) per task: the moments
+// mu[m][q] = sum_j T_m(xh_j) x[j, q] of the task's points. The wavefront forms
+// T for PTS points (one per lane, FP64 recurrence) in LDS, then contracts over
+// the points on MFMA.
 template <int RP, int PREC>
-__global__ __launch_bounds__(WG) void k_up_mfma(const DTask *tasks, const DCluster *cl,
-                                               const double2 *cmapv, const double *pts,
-                                               const double *x, int64_t incx, int64_t ldx,
-                                               int nq, double *partial, int r) {
+__global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluster *cl,
+                                              const double2 *cmapv, const double *pts,
+                                              const double *x, int64_t incx, int64_t ldx, int nq,
+                                              const double *chebA, double *partial, double *w,
+                                              int r) {
+    using TL = typename std::conditional<PREC == 2, double, float>::type;
     constexpr int MT = RP / 16;
-    __shared__ float Tl[WPB][RP][WAVE + 1];
-    float (*red)[RP][MQ] = reinterpret_cast<float (*)[RP][MQ]>(&Tl[0][0][0]);  // reused after the loop
-    const DTask T = tasks[blockIdx.x];
-    const DCluster C = cl[T.cl];
-    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
-    const double2 cm = cmapv[T.cl];
-    const int64_t first = C.first + T.off;
-    mf4_t D[MT];
-#pragma unroll
-    for (int t = 0; t < MT; ++t) D[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    constexpr int PTS = (PREC == 2 && RP > 32) ? 32 : 64;  // points per step
+    static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQ * sizeof(double), "mu fits in Tl");
+    __shared__ __align__(16) TL Tl[RP][PTS + 1];
+    const DTask Tk = tasks[blockIdx.x];
+    const DCluster C = cl[Tk.cl];
+    const int lane = threadIdx.x;
     const int j16 = lane % 16, g = lane / 16;
-    for (int base = wv * WAVE; base < T.len; base += WG) {
-        // T_m at this lane's point, FP64 recurrence, stored as FP32
-        {
+    const double2 cm = cmapv[Tk.cl];
+    const int64_t first = C.first + Tk.off;
+    md4_t D64[MT];
+    mf4_t D32[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        D64[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+        D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    }
+    for (int base = 0; base < Tk.len; base += PTS) {
+        if (lane < PTS) {
             const int i = base + lane;
-            const bool ok = i < T.len;
+            const bool ok = i < Tk.len;
             const double xh = ok ? (pts[first + i] - cm.x) * cm.y : 0.0;
             double t0 = 1.0, t1 = xh;
 #pragma unroll
@@ -937,28 +968,29 @@ __global__ __launch_bounds__(WG) void k_up_mfma(const DTask *tasks, const DClust
                     t0 = t1;
                     t1 = tv;
                 }
-                Tl[wv][m][lane] = (ok && m < r) ? (float)tv : 0.f;
+                Tl[m][lane] = (ok && m < r) ? (TL)tv : (TL)0;
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        if constexpr (PREC == 0) {
+        wave_sync();
+        if constexpr (PREC != 1) {
 #pragma unroll 4
-            for (int k0 = 0; k0 < WAVE; k0 += 4) {
+            for (int k0 = 0; k0 < PTS; k0 += 4) {
                 const int i = base + k0 + g;
-                const float b = (i < T.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
+                const double b = (i < Tk.len && j16 < nq) ? x[(first + i) * incx + j16 * ldx] : 0.0;
 #pragma unroll
-                for (int t = 0; t < MT; ++t)
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[wv][t * 16 + j16][k0 + g], b, D[t], 0, 0, 0);
+                for (int t = 0; t < MT; ++t) {
+                    if constexpr (PREC == 2) D64[t] = mfma64(Tl[t * 16 + j16][k0 + g], b, D64[t]);
+                    else D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[t * 16 + j16][k0 + g], (float)b, D32[t], 0, 0, 0);
+                }
             }
         } else {
 #pragma unroll
-            for (int k0 = 0; k0 < WAVE; k0 += 32) {
+            for (int k0 = 0; k0 < PTS; k0 += 32) {
                 mbf8_t bh, bl;
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
                     const int i = base + k0 + 8 * g + v;
-                    const float b = (i < T.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
+                    const float b = (i < Tk.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
                     bh[v] = bf16_hi(b);
                     bl[v] = bf16_lo(b);
                 }
@@ -967,67 +999,154 @@ __global__ __launch_bounds__(WG) void k_up_mfma(const DTask *tasks, const DClust
                     mbf8_t ah, al;
 #pragma unroll
                     for (int v = 0; v < 8; ++v) {
-                        const float tv = Tl[wv][t * 16 + j16][k0 + 8 * g + v];
+                        const float tv = Tl[t * 16 + j16][k0 + 8 * g + v];
                         ah[v] = bf16_hi(tv);
                         al[v] = bf16_lo(tv);
                     }
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D[t], 0, 0, 0);
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D[t], 0, 0, 0);
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        wave_sync();
     }
-    __syncthreads();  // every wavefront is done with Tl before it is reused
+    // mu[m][q] through LDS (Tl is free again)
+    double (*mu)[MQ] = reinterpret_cast<double (*)[MQ]>(&Tl[0][0]);
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) red[wv][t * 16 + 4 * g + v][j16] = D[t][v];
-    __syncthreads();
-    for (int idx = tid; idx < r * MQ; idx += WG) {
-        const int q = idx / r, m = idx % r;
-        if (q < nq) {
-            double sacc = 0.0;
-#pragma unroll
-            for (int w = 0; w < WPB; ++w) sacc += (double)red[w][m][q];
-            partial[(T.slot * MQ + q) * r + m] = sacc;
+        for (int v = 0; v < 4; ++v) {
+            if constexpr (PREC == 2) mu[t * 16 + g + 4 * v][j16] = D64[t][v];
+            else mu[t * 16 + 4 * g + v][j16] = (double)D32[t][v];
+        }
+    wave_sync();
+    if (C.nslots == 1) {
+        // single task: w = A mu (the reduce's Chebyshev step, same order)
+        for (int idx = lane; idx < r * MQ; idx += WAVE) {
+            const int q = idx / r, m = idx - q * r;
+            double v = 0.0;
+            for (int k = 0; k < r; ++k) v = __builtin_fma(chebA[m * r + k], mu[k][q], v);
+            w[((int64_t)Tk.cl * MQ + q) * r + m] = v;
+        }
+    } else {
+        for (int idx = lane; idx < r * MQ; idx += WAVE) {
+            const int q = idx / r, m = idx - q * r;
+            partial[(Tk.slot * MQ + q) * r + m] = mu[m][q];
         }
     }
 }
 
-// Downward pass: near field in FP64 (stored A or regenerated, right-hand
-// sides staged in LDS), far field sum over clusters of T(xh_rows) * C on
-// MFMA in FP32; the two are added in FP64 when the chunk is written.
-template <int RP, int PREC, bool FREE, int KID>
-__global__ __launch_bounds__(WG) void k_down_mfma(DevPlan p, const DChunk *ch, int nch,
-                                                 const DNear *nl, const DFar *fl,
-                                                 const double2 *cmapv, const double *pts_out,
-                                                 const double *pts_in, const double *z, MvArgs a,
-                                                 int nq) {
+// Translation, one wavefront per row cluster: Z = sum over the cluster's far
+// blocks of F_b W_b (F stored, or regenerated from the mapped nodes), then
+// C = A^T Z. The FP64 result layout of Z is the B-operand layout of the
+// second product, so Z feeds it from registers.
+template <int RP, bool FREE, int KID>
+__global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *cl, int ncl,
+                                                   const int32_t *far_list, const double *w,
+                                                   double *z) {
     constexpr int MT = RP / 16;
-    __shared__ float Tl[WPB][WAVE][RP + 1];
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= ncl) return;
+    const int r = p.r;
+    const int j16 = lane % 16, g = lane / 16;
+    const DCluster C = cl[wid];
+    md4_t Z[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) Z[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+    for (int e = C.far_beg; e < C.far_end; ++e) {
+        const DFarBlk b = p.far[far_list[e]];
+        const double *ws = w + (int64_t)b.cc * MQ * r;
+        const double *F = FREE ? nullptr : p.F + b.foff;
+        const double *tx = p.rnodes + (int64_t)b.rc * r;
+        const double *ty = p.cnodes + (int64_t)b.cc * r;
+#pragma unroll
+        for (int k0 = 0; k0 < RP; k0 += 4) {
+            const int l = k0 + g;
+            const bool lok = l < r;
+            const double bv = lok ? ws[j16 * r + l] : 0.0;
+            double tyl = 0.0;
+            if constexpr (FREE) tyl = lok ? ty[l] : 0.0;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                const int m = t * 16 + j16;
+                double av = 0.0;
+                if (lok && m < r) {
+                    if constexpr (FREE) av = kern_fast<double, KID>(tx[m], tyl);
+                    else av = F[m + (int64_t)r * l];
+                }
+                Z[t] = mfma64(av, bv, Z[t]);
+            }
+        }
+    }
+    // C[m][q] = sum_k A[k][m] Z[k][q]; this lane holds Z[16t + g + 4v][j16] in
+    // Z[t][v], which is B[k0 + g][j16] of the K-step k0 = 16t + 4v
+    md4_t Cc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) Cc[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = t * 16 + 4 * v + g;
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt) {
+                const int m = tt * 16 + j16;
+                const double av = (k < r && m < r) ? p.chebA[k * r + m] : 0.0;
+                Cc[tt] = mfma64(av, Z[t][v], Cc[tt]);
+            }
+        }
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = t * 16 + g + 4 * v;
+            if (m < r) z[((int64_t)wid * MQ + j16) * r + m] = Cc[t][v];
+        }
+}
+
+// Downward pass, one wavefront per 64-row chunk. Rows 16t + j16 (t < 4) are
+// this lane's A-operand rows. Near field: per 64-column tile of a dense block
+// the right-hand sides are staged in LDS (xs[col][q]) and the block entries
+// (stored, or regenerated from the points) enter the FP64 MFMA directly. Far
+// field: for each covering cluster, T_k(xh) of the lane's rows by the FP64
+// recurrence, times the cluster's coefficients C[k][q]. The 64 x 16 result
+// goes through LDS to a row-per-lane layout and is written once.
+template <int RP, int PREC, bool FREE, int KID>
+__global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
+                                              const DNear *nl, const DFar *fl,
+                                              const double2 *cmapv, const double *pts_out,
+                                              const double *pts_in, const double *z, MvArgs a,
+                                              int nq) {
+    constexpr int KSEL = PREC == 1 ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     __shared__ double xs_all[WPB][WAVE * MQ];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    const int wv = (threadIdx.x / WAVE) % WPB;
-    double *xs = xs_all[wv];
+    double *xs = xs_all[(threadIdx.x / WAVE) % WPB];
     if (wid >= nch) return;
     const int r = p.r;
     const DChunk c = ch[wid];
-    const bool act = lane < c.np;
-    const double po = act ? pts_out[c.p0 + lane] : 0.0;
     const int j16 = lane % 16, g = lane / 16;
-    double acc[MQ];
+    double po[4];
+    bool rok[4];
 #pragma unroll
-    for (int q = 0; q < MQ; ++q) acc[q] = 0.0;
+    for (int t = 0; t < 4; ++t) {
+        rok[t] = t * 16 + j16 < c.np;
+        po[t] = rok[t] ? pts_out[c.p0 + t * 16 + j16] : 0.0;
+    }
+    md4_t D[4];
+    mf4_t D32[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        D[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+        D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    }
     // ---- near field, FP64
     for (int e = c.near_beg; e < c.near_end; ++e) {
         const DNear s = nl[e];
         const DDense Dd = p.dense[s.blk];
         const double *xin = a.x + Dd.col0 * a.incx;
-        const double *Aj = FREE ? nullptr : p.A + Dd.aoff + tile_index(s.off + lane, 0, Dd.n);
         for (int jt = 0; jt < Dd.n; jt += WAVE) {
             const int nj = min(WAVE, Dd.n - jt);
             if (lane < nj) {
@@ -1035,129 +1154,145 @@ __global__ __launch_bounds__(WG) void k_down_mfma(DevPlan p, const DChunk *ch, i
                 for (int q = 0; q < MQ; ++q)
                     xs[lane * MQ + q] = q < nq ? xin[(int64_t)(jt + lane) * a.incx + q * a.ldx] : 0.0;
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            for (int j = 0; j < nj; ++j) {
-                double f;
-                if constexpr (FREE) f = act ? kern_fast<double, KID>(po, pts_in[Dd.col0 + jt + j]) : 0.0;
-                else f = act ? Aj[(jt + j) * HM_TILE] : 0.0;
+            wave_sync();
+            for (int k0 = 0; k0 < nj; k0 += 4) {
+                const int cc = k0 + g;
+                const bool cok = cc < nj;
+                const double bv = cok ? xs[cc * MQ + j16] : 0.0;
+                double yc = 0.0;
+                if constexpr (FREE) yc = cok ? pts_in[Dd.col0 + jt + cc] : 0.0;
 #pragma unroll
-                for (int q = 0; q < MQ; ++q) acc[q] = __builtin_fma(f, xs[j * MQ + q], acc[q]);
+                for (int t = 0; t < 4; ++t) {
+                    double av = 0.0;
+                    if (rok[t] && cok) {
+                        if constexpr (FREE) av = kern_fast<double, KID>(po[t], yc);
+                        else av = p.A[Dd.aoff + tile_index(s.off + t * 16 + j16, jt + cc, Dd.n)];
+                    }
+                    D[t] = mfma64(av, bv, D[t]);
+                }
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            wave_sync();
         }
     }
-    // ---- far field on MFMA: 4 row tiles of 16
-    mf4_t D[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) D[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    // ---- far field
     for (int e = c.far_beg; e < c.far_end; ++e) {
         const DFar f = fl[e];
         const double2 cm = cmapv[f.cl];
-        const double xh = (po - cm.x) * cm.y;
-        {
+        const double *zc = z + (int64_t)f.cl * MQ * r + (int64_t)j16 * r;  // C[k][q = j16]
+        // this lane's B operands: C[k][j16] for its k of every K-step
+        double bz[KSEL];
+#pragma unroll
+        for (int u = 0; u < KSEL; ++u) {
+            const int k = PREC == 1 ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
+            bz[u] = (k < r && j16 < nq) ? zc[min(k, r - 1)] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double xh = (po[t] - cm.x) * cm.y;
+            double sel[KSEL];
+#pragma unroll
+            for (int u = 0; u < KSEL; ++u) sel[u] = 0.0;
             double t0 = 1.0, t1 = xh;
 #pragma unroll
-            for (int m = 0; m < RP; ++m) {
-                double tv = m == 0 ? t0 : t1;
-                if (m >= 2) {
+            for (int k = 0; k < RP; ++k) {
+                double tv = k == 0 ? t0 : t1;
+                if (k >= 2) {
                     tv = __builtin_fma(xh + xh, t1, -t0);
                     t0 = t1;
                     t1 = tv;
                 }
-                Tl[wv][lane][m] = (act && m < r) ? (float)tv : 0.f;
+                const bool mine = PREC == 1 ? ((k & 31) >> 3) == g : (k & 3) == g;
+                const int u = PREC == 1 ? (k >> 5) * 8 + (k & 7) : (k >> 2);
+                if (mine && rok[t] && k < r) sel[u] = tv;
             }
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        const double *zc = z + (int64_t)f.cl * MQ * r;
-        if constexpr (PREC == 0) {
+            if constexpr (PREC == 2) {
 #pragma unroll
-            for (int k0 = 0; k0 < RP; k0 += 4) {
-                const int m = k0 + g;
-                const float b = (m < r && j16 < nq) ? (float)zc[j16 * r + m] : 0.f;
+                for (int u = 0; u < KSEL; ++u) D[t] = mfma64(sel[u], bz[u], D[t]);
+            } else if constexpr (PREC == 0) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[wv][t * 16 + j16][k0 + g], b, D[t], 0, 0, 0);
-            }
-        } else {
+                for (int u = 0; u < KSEL; ++u)
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[u], D32[t], 0, 0, 0);
+            } else {
 #pragma unroll
-            for (int k0 = 0; k0 < RP; k0 += 32) {
-                mbf8_t bh, bl;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const int m = k0 + 8 * g + v;
-                    const float b = (m < r && j16 < nq) ? (float)zc[j16 * r + min(m, r - 1)] : 0.f;
-                    bh[v] = bf16_hi(b);
-                    bl[v] = bf16_lo(b);
-                }
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    mbf8_t ah, al;
+                for (int s8 = 0; s8 < KSEL / 8; ++s8) {
+                    mbf8_t ah, al, bh, bl;
 #pragma unroll
                     for (int v = 0; v < 8; ++v) {
-                        const int m = k0 + 8 * g + v;
-                        const float tv = m < RP ? Tl[wv][t * 16 + j16][min(m, RP - 1)] : 0.f;
-                        ah[v] = bf16_hi(tv);
-                        al[v] = bf16_lo(tv);
+                        const float av = (float)sel[s8 * 8 + v], bv = (float)bz[s8 * 8 + v];
+                        ah[v] = bf16_hi(av);
+                        al[v] = bf16_lo(av);
+                        bh[v] = bf16_hi(bv);
+                        bl[v] = bf16_lo(bv);
                     }
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D[t], 0, 0, 0);
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D[t], 0, 0, 0);
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
+                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    // D (rows x rhs, FP32) through LDS back to the row-per-lane layout
-    float *Dl = reinterpret_cast<float *>(xs);  // 64 x 16 floats fit in the staging buffer
+    // ---- 64 x 16 result through LDS to one row per lane, written once
+    double *Dl = xs;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Dl[(t * 16 + 4 * g + v) * MQ + j16] = D[t][v];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (act) {
+        for (int v = 0; v < 4; ++v) Dl[(t * 16 + g + 4 * v) * MQ + j16] = D[t][v];
+    if constexpr (PREC != 2) {
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Dl[(t * 16 + 4 * g + v) * MQ + j16] += (double)D32[t][v];
+    }
+    wave_sync();
+    if (lane < c.np) {
 #pragma unroll
         for (int q = 0; q < MQ; ++q) {
             if (q < nq) {
                 double *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
-                *yo = (a.beta ? *yo : 0.0) + acc[q] + (double)Dl[lane * MQ + q];
+                *yo = (a.beta ? *yo : 0.0) + Dl[lane * MQ + q];
             }
         }
     }
 }
 
+// MODE 1: stored dense blocks (stored and hybrid plans); MODE 2: matrix-free
 template <int RP, int PREC, int MODE, int KID>
-hipError_t matvec_mixed_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) {
+hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, const int32_t *multi,
+                      int nmulti, hipStream_t s) {
     constexpr bool FREE = MODE == 2;
     if (p.nfar > 0 && p.nctask > 0) {
-        k_up_mfma<RP, PREC><<<p.nctask, WG, 0, s>>>(p.ctask, p.ccl, p.cmap, p.y, a.x, a.incx,
-                                                     a.ldx, nq, a.partial, p.r);
-        k_reduce<double, true, MQ><<<nblk(p.ncc), WG, 0, s>>>(p.ccl, p.ncc, a.partial, p.chebA, a.w, p.r);
-        k_translate<double, RP, FREE, KID, false, true, MQ><<<nblk(p.nrc), WG, 0, s>>>(
-            p, p.rcl, p.nrc, p.far_by_rc, a.w, a.z);
+        k_up_m<RP, PREC><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.ccl, p.cmap, p.y, a.x, a.incx, a.ldx,
+                                                    nq, p.chebA, a.partial, a.w, p.r);
+        if (nmulti > 0)
+            k_reduce<double, true, MQ><<<nblk(nmulti), WG, 0, s>>>(p.ccl, nmulti, a.partial, p.chebA,
+                                                                  a.w, p.r, multi);
+        k_translate_m<RP, FREE, KID><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc, a.w, a.z);
     }
     if (p.nrch > 0)
-        k_down_mfma<RP, PREC, FREE, KID><<<nblk(p.nrch), WG, 0, s>>>(
+        k_down_m<RP, PREC, FREE, KID><<<nblk(p.nrch), WG, 0, s>>>(
             p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, a.z, a, nq);
     return hipGetLastError();
 }
 
-template <int RP>
-hipError_t matvec_mixed_r(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                          hipStream_t s) {
-    if (mode != 2) {
-        return prec == 0 ? matvec_mixed_t<RP, 0, 1, 0>(p, a, nq, s) : matvec_mixed_t<RP, 1, 1, 0>(p, a, nq, s);
-    }
+template <int RP, int PREC>
+hipError_t matvec_m_p(const DevPlan &p, const MvArgs &a, int mode, int nq, const int32_t *multi,
+                      int nmulti, hipStream_t s) {
+    if (mode != 2) return matvec_m_t<RP, PREC, 1, 0>(p, a, nq, multi, nmulti, s);
     switch (p.kid) {
-    case 0: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 0>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 0>(p, a, nq, s);
-    case 1: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 1>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 1>(p, a, nq, s);
-    case 2: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 2>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 2>(p, a, nq, s);
-    default: return prec == 0 ? matvec_mixed_t<RP, 0, 2, 3>(p, a, nq, s) : matvec_mixed_t<RP, 1, 2, 3>(p, a, nq, s);
+    case 0: return matvec_m_t<RP, PREC, 2, 0>(p, a, nq, multi, nmulti, s);
+    case 1: return matvec_m_t<RP, PREC, 2, 1>(p, a, nq, multi, nmulti, s);
+    case 2: return matvec_m_t<RP, PREC, 2, 2>(p, a, nq, multi, nmulti, s);
+    default: return matvec_m_t<RP, PREC, 2, 3>(p, a, nq, multi, nmulti, s);
     }
+}
+
+template <int RP>
+hipError_t matvec_m_r(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
+                      const int32_t *multi, int nmulti, hipStream_t s) {
+    if (prec == 0) return matvec_m_p<RP, 0>(p, a, mode, nq, multi, nmulti, s);
+    if (prec == 1) return matvec_m_p<RP, 1>(p, a, mode, nq, multi, nmulti, s);
+    return matvec_m_p<RP, 2>(p, a, mode, nq, multi, nmulti, s);
 }
 
 template <class T, int RMAX, int KB>
@@ -1193,8 +1328,8 @@ hipError_t matvec_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, int mode
     hipError_t hmx_grad_r##R(const DevPlan &p, const MvArgs &a, bool tr, double *y2,            \
                              int64_t incy2, const double *sv, int64_t incs, hipStream_t s);     \
     hipError_t hmx_gen_basis_r##R(const DevPlan &p, hipStream_t s);                              \
-    hipError_t hmx_matvec_mixed_r##R(const DevPlan &p, const MvArgs &a, int mode, int prec,      \
-                                     int nq, hipStream_t s);
+    hipError_t hmx_matvec_m_r##R(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,  \
+                                 const int32_t *multi, int nmulti, hipStream_t s);
 #define HMX_DECLARE_BUCKET_F32(R)                                                               \
     hipError_t hmx_matvec_f32_r##R(const DevPlanF &p, const MvArgsF &a, bool tr, int mode,      \
                                    int kb, hipStream_t s, const hipEvent_t *ev);                \

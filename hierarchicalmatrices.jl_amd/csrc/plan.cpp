@@ -394,6 +394,8 @@ struct hm_plan {
     DNear *d_rnear = nullptr, *d_cnear = nullptr;
     DFar *d_rfar = nullptr, *d_cfar = nullptr;
     DTask *d_rtask = nullptr, *d_ctask = nullptr;
+    std::vector<int32_t> multi_c;     // column clusters with more than one upward task
+    int32_t *d_multi_c = nullptr;
     bool profile = false;
     bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
@@ -408,7 +410,7 @@ struct hm_plan {
     ~hm_plan() {
         drop_events();
         void *bufs[] = {d_rcl, d_ccl, d_far, d_far_by_rc, d_far_by_cc, d_dense, d_rch, d_cch,
-                        d_rnear, d_cnear, d_rfar, d_cfar, d_rtask, d_ctask};
+                        d_rnear, d_cnear, d_rfar, d_cfar, d_rtask, d_ctask, d_multi_c};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
         b64.release();
@@ -777,6 +779,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
 
     build_side(nx, P->dense, true, P->rcl, P->rows);
     build_side(ny, P->dense, false, P->ccl, P->cols);
+    for (size_t k = 0; k < P->ccl.size(); ++k)
+        if (P->ccl[k].nslots > 1) P->multi_c.push_back((int32_t)k);
 
     if (flags & HM_PLAN_TREE_ONLY) {
         P->far_by_rc_h = std::move(by_rc);
@@ -800,6 +804,7 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     HM_TRY(upload(&P->d_cfar, P->cols.farl));
     HM_TRY(upload(&P->d_rtask, P->rows.tasks));
     HM_TRY(upload(&P->d_ctask, P->cols.tasks));
+    HM_TRY(upload(&P->d_multi_c, P->multi_c));
     if (f32) HM_TRY(setup_typed<float>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
     else HM_TRY(setup_typed<double>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
 #undef HM_TRY
@@ -1050,6 +1055,11 @@ int hm_plan_blocks(hm_plan_t P, int64_t *ints, double *dbls, int64_t cap, int64_
     return HM_OK;
 }
 
+// right-hand sides per launch of the MFMA passes, and the default number of
+// right-hand sides from which forward products use them
+constexpr int MFMA_KB = 16;
+constexpr int64_t HM_MFMA_MIN_K = 16;
+
 // A product entry point for element type T may only be called on a plan of
 // that type.
 extern "C++" {
@@ -1085,7 +1095,9 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
     if (int rc = check_type<T>(P, "matvec")) return rc;
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
-    if (flags & ~31u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (flags & ~127u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if ((flags & HM_MV_MFMA) && (flags & HM_MV_NO_MFMA))
+        return fail(HM_ERR_INVALID, "matvec: HM_MV_MFMA and HM_MV_NO_MFMA exclude each other");
     const int prec = (flags & HM_MV_FAR_BF16X3) ? 1 : (flags & HM_MV_FAR_F32) ? 0 : -1;
     if ((flags & HM_MV_FAR_BF16X3) && (flags & HM_MV_FAR_F32))
         return fail(HM_ERR_INVALID, "matvec: choose one far-field precision");
@@ -1100,16 +1112,25 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
     int max_kb = P->max_kb;
     if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0 && !P->from_blocks) {
         mode = std::max(mode, P->tuned_mode);
-        max_kb = P->tuned_kb;
+        max_kb = std::min(P->tuned_kb, HM_MAX_KB);
     }
     hipStream_t s = (hipStream_t)stream;
-    if (prec >= 0) {
-        // far field on MFMA in reduced precision: Chebyshev form, 16 rhs per launch
-        if constexpr (std::is_same_v<T, double>) {
-            if (tr || P->from_blocks)
-                return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: forward KernelMatrix products only");
-            const int mmode = std::max(mode, 1);
-            for (int64_t q = 0; q < k; q += 16) {
+    // forward products of a Float64 KernelMatrix plan may run on the MFMA
+    // passes (16 right-hand sides per launch): always for a reduced-precision
+    // far field, by default from HM_MFMA_MIN_K right-hand sides on
+    bool use_m = prec >= 0;
+    if constexpr (std::is_same_v<T, double>) {
+        const bool can_m = !tr && !P->from_blocks;
+        if (prec >= 0 && !can_m)
+            return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: forward KernelMatrix products only");
+        if (prec < 0 && can_m && !(flags & HM_MV_NO_MFMA)) {
+            if (flags & HM_MV_MFMA) use_m = true;
+            else if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0) use_m = P->tuned_kb == MFMA_KB;
+            else use_m = k >= HM_MFMA_MIN_K;
+        }
+        if (use_m) {
+            const int pm = prec >= 0 ? prec : 2;
+            for (int64_t q = 0; q < k; q += MFMA_KB) {
                 MvArgs a = ws_args<double>(P);
                 a.y = y + ioff + q * ldy;
                 a.x = x + joff + q * ldx;
@@ -1118,14 +1139,16 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
                 a.ldy = ldy;
                 a.ldx = ldx;
                 a.beta = beta;
-                const int nq = (int)std::min<int64_t>(16, k - q);
-                hipError_t e = hmx_launch_matvec_mixed(P->b64.dp, a, mmode, prec, nq, s);
+                const int nq = (int)std::min<int64_t>(MFMA_KB, k - q);
+                hipError_t e = hmx_launch_matvec_m(P->b64.dp, a, mode, pm, nq, P->d_multi_c,
+                                                   (int)P->multi_c.size(), s);
                 if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
             }
             return HM_OK;
-        } else {
-            return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: Float64 plans only");
         }
+    } else {
+        if (prec >= 0) return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: Float64 plans only");
+        if (flags & HM_MV_MFMA) return fail(HM_ERR_UNSUPPORTED, "MFMA passes: Float64 plans only");
     }
     for (int64_t q = 0; q < k;) {
         // forward products take up to HM_MAX_KB right-hand sides per launch
@@ -1329,11 +1352,19 @@ static int tune_common(hm_plan_t P, T *y, const T *x, int64_t k, void *stream, i
     float best = 1e30f;
     int best_kb = 1, best_mode = base_mode;
     const int saved_kb = P->max_kb;
+    // candidates: 1, 2, 4, 8 right-hand sides per launch on the vector units,
+    // and 16 per launch on the MFMA passes (Float64 KernelMatrix plans)
+    const bool mfma_ok = std::is_same_v<T, double> && !P->from_blocks;
     for (int mode = base_mode; mode <= 2; ++mode) {
-        const unsigned fl = mode == 2 ? HM_MV_MATRIX_FREE : mode == 1 ? HM_MV_CHEB_BASIS : 0;
-        for (int kb = 1; kb <= HM_MAX_KB; kb *= 2) {
-            if (kb > 1 && kb > k) break;
-            P->max_kb = kb;
+        const unsigned fl0 = mode == 2 ? HM_MV_MATRIX_FREE : mode == 1 ? HM_MV_CHEB_BASIS : 0;
+        for (int kb = 1; kb <= MFMA_KB; kb *= 2) {
+            if (kb == MFMA_KB) {
+                if (!mfma_ok || mode == 0 || k < 2) break;  // MFMA passes: Chebyshev form
+            } else if (kb > 1 && kb > k) {
+                continue;
+            }
+            const unsigned fl = fl0 | (kb == MFMA_KB ? HM_MV_MFMA : HM_MV_NO_MFMA);
+            P->max_kb = std::min(kb, HM_MAX_KB);
             int rc = matvec_common(P, y, x, 0, 0, 1, 1, k, P->nx, P->ny, 0, fl, s, false);
             if (rc != HM_OK) { P->max_kb = saved_kb; return rc; }
             float ms = 0.f;

@@ -27,6 +27,8 @@ HM_MV_CHEB_BASIS = 2
 HM_MV_TUNED = 4
 HM_MV_FAR_F32 = 8
 HM_MV_FAR_BF16X3 = 16
+HM_MV_MFMA = 32
+HM_MV_NO_MFMA = 64
 
 INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",

@@ -156,6 +156,12 @@ class _Vec:
 _FAR = {"f64": 0, "f32": _lib.HM_MV_FAR_F32, "bf16x3": _lib.HM_MV_FAR_BF16X3}
 
 
+def _mfma_flag(mfma):
+    if mfma is None:
+        return 0
+    return _lib.HM_MV_MFMA if mfma else _lib.HM_MV_NO_MFMA
+
+
 class KernelMatrix:
     """KernelMatrix(f, x, y, a, b, c, d), src/KernelMatrix.jl:47-116.
 
@@ -259,7 +265,7 @@ class KernelMatrix:
 
     # -- products
     def _mv(self, u, v, istart, jstart, incy, incx, beta, matrix_free, transpose, stream,
-            cheb_basis=False, tuned=False, far_precision="f64"):
+            cheb_basis=False, tuned=False, far_precision="f64", mfma=None):
         dt = np.float32 if self.f32 else np.float64
         U = _Vec(u, True, dt)
         V = _Vec(v, False, dt)
@@ -281,7 +287,7 @@ class KernelMatrix:
             fn = lib.hm_matvec_t if transpose else lib.hm_matvec
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
             (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
-            _FAR[far_precision]
+            _FAR[far_precision] | _mfma_flag(mfma)
         check(fn(self._h, U.ptr(), V.ptr(), ioff * U.inc, joff * V.inc, incy, incx, U.k,
                  U.ld, V.ld, beta, flags, _stream_ptr(stream)))
         U.writeback()
@@ -296,8 +302,11 @@ class KernelMatrix:
         return self._mv(u, v, istart, jstart, INCY, INCX, 1, matrix_free, True, stream)
 
     def matvec(self, v, *, matrix_free=False, transpose=False, cheb_basis=False, tuned=False,
-               far_precision="f64", stream=None):
-        """H v (or H^T v) into a new vector on the device of v."""
+               far_precision="f64", mfma=None, stream=None):
+        """H v (or H^T v) into a new vector on the device of v. mfma=True runs
+        forward products in groups of 16 right-hand sides on the MFMA passes,
+        False never does, None leaves the choice to the library (MFMA from 16
+        right-hand sides on)."""
         torch = _torch()
         ishost = not isinstance(v, torch.Tensor)
         dt = np.float32 if self.f32 else np.float64
@@ -305,7 +314,7 @@ class KernelMatrix:
         m = self.shape[1] if transpose else self.shape[0]
         out = torch.empty((m,) + tuple(vt.shape[1:]), dtype=self._tdtype(), device=vt.device)
         self._mv(out, vt, 1, 1, 1, 1, 0, matrix_free, transpose, stream, cheb_basis, tuned,
-                 far_precision)
+                 far_precision, mfma)
         if ishost:
             torch.cuda.synchronize()
             return out.cpu().numpy()
@@ -334,7 +343,7 @@ class KernelMatrix:
         return gb, gy
 
     def graph(self, out, v, *, beta=0, matrix_free=False, cheb_basis=False, tuned=False,
-              far_precision="f64"):
+              far_precision="f64", mfma=None):
         """Capture `out = beta*out + H v` (device tensors, 1-D or (n, k)) into an
         executable HIP graph; returns a ProductGraph whose launch() replays it."""
         dt = np.float32 if self.f32 else np.float64
@@ -343,7 +352,7 @@ class KernelMatrix:
             raise TypeError("graphs need device tensors")
         flags = (_lib.HM_MV_MATRIX_FREE if matrix_free else 0) | \
             (_lib.HM_MV_CHEB_BASIS if cheb_basis else 0) | (_lib.HM_MV_TUNED if tuned else 0) | \
-            _FAR[far_precision]
+            _FAR[far_precision] | _mfma_flag(mfma)
         g = ctypes.c_void_p()
         fn = lib.hm_graph_matvec_f32 if self.f32 else lib.hm_graph_matvec
         check(fn(self._h, U.ptr(), V.ptr(), 0, 0, U.inc, V.inc, U.k, U.ld, V.ld, beta, flags,

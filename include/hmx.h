@@ -61,8 +61,13 @@ enum hm_mv_flags {
     HM_MV_TUNED = 4,             /* use the launch shape and mode chosen by hm_plan_tune */
     HM_MV_FAR_F32 = 8,           /* far field (Chebyshev form) on FP32 MFMA, FP32 accumulate;
                                     points, differences, near field and F stay FP64 */
-    HM_MV_FAR_BF16X3 = 16        /* far field on BF16 MFMA with operands split hi + lo
+    HM_MV_FAR_BF16X3 = 16,       /* far field on BF16 MFMA with operands split hi + lo
                                     (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+    HM_MV_MFMA = 32,             /* forward: 16 right-hand sides per launch on the FP64
+                                    MFMA passes (far field in Chebyshev form); the
+                                    default when k >= 16 */
+    HM_MV_NO_MFMA = 64           /* forward: never the MFMA passes (up to 8 right-hand
+                                    sides per launch on the vector units) */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -209,7 +214,8 @@ int hm_graph_destroy(hm_graph_t graph);
 /* Autotuner (no counterpart in the reference): times hm_matvec on this plan
  * for k right-hand sides (device buffers y, x as in hm_matvec with unit
  * stride and ld = n) over the candidate launch shapes (right-hand sides per
- * launch 1, 2, 4, 8) and far-field forms (the plan's own, Chebyshev basis,
+ * launch 1, 2, 4, 8 on the vector units; 16 on the MFMA passes, Float64
+ * KernelMatrix plans) and far-field forms (the plan's own, Chebyshev basis,
  * matrix-free), keeps the fastest for later products that pass HM_MV_TUNED,
  * and writes its choice to out[0] = right-hand sides per launch, out[1] =
  * mode (0 stored, 1 Chebyshev basis, 2 matrix-free), out[2] = its time in
