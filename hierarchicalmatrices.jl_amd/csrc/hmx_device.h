@@ -52,6 +52,7 @@ struct MvArgsT {
     T *partial;  // >= max(nslots) * r * kb
     T *w;        // >= max(nrc, ncc) * r * kb
     T *z;        // >= max(nrc, ncc) * r * kb
+    T *xt;       // >= ny * 16: right-hand sides packed row by row (MFMA passes)
 };
 
 using DevPlan = DevPlanT<double>;

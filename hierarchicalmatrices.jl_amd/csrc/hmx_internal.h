@@ -64,11 +64,13 @@ struct DFar {
 };
 
 struct DTask {
-    int32_t cl;    // cluster id
-    int32_t off;   // first point of the task inside the cluster
-    int32_t len;   // number of points
-    int32_t pad;
-    int64_t slot;  // partial-sum slot
+    int32_t cl;      // cluster id
+    int32_t off;     // first point of the task inside the cluster
+    int32_t len;     // number of points
+    int32_t single;  // 1 if the task is its cluster's only one
+    int64_t slot;    // partial-sum slot
+    int64_t first;   // first point of the task (cluster first + off)
+    double cx, cy;   // the cluster's (center, 1/halfwidth), Float64 plans
 };
 
 struct DDense {

@@ -921,9 +921,34 @@ __device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)
 __device__ __forceinline__ md4_t mfma64(double a, double b, md4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// wave-uniform copies of lane l's value (l uniform)
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ int64_t rl_l(int64_t v, int l) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+    return __longlong_as_double(rl_l(__double_as_longlong(v), l));
+}
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// xt[j][q] = x[j*incx + q*ldx] for q < nq, 0 beyond: the right-hand sides of
+// one launch packed row by row (one 128-byte row per input index), so every
+// MFMA B operand of the upward and downward passes is a coalesced load.
+__global__ __launch_bounds__(WG) void k_pack_x(const double *x, int64_t incx, int64_t ldx, int nq,
+                                              int64_t n, double *xt) {
+    const int64_t j = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (j >= n) return;
+    double v[MQ];
+#pragma unroll
+    for (int q = 0; q < MQ; ++q) v[q] = q < nq ? x[j * incx + q * ldx] : 0.0;
+    double2 *o = reinterpret_cast<double2 *>(xt + j * MQ);
+#pragma unroll
+    for (int q = 0; q < MQ / 2; ++q) o[q] = make_double2(v[2 * q], v[2 * q + 1]);
 }
 
 // Upward pass, one wavefront (the whole workgroup) per task: the moments
@@ -931,9 +956,8 @@ __device__ __forceinline__ void wave_sync() {
 // T for PTS points (one per lane, FP64 recurrence) in LDS, then contracts over
 // the points on MFMA.
 template <int RP, int PREC>
-__global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluster *cl,
-                                              const double2 *cmapv, const double *pts,
-                                              const double *x, int64_t incx, int64_t ldx, int nq,
+__global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double *pts,
+                                              const double *xt, int64_t ldxt, int nq,
                                               const double *chebA, double *partial, double *w,
                                               int r) {
     using TL = typename std::conditional<PREC == 2, double, float>::type;
@@ -942,11 +966,10 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluste
     static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQ * sizeof(double), "mu fits in Tl");
     __shared__ __align__(16) TL Tl[RP][PTS + 1];
     const DTask Tk = tasks[blockIdx.x];
-    const DCluster C = cl[Tk.cl];
     const int lane = threadIdx.x;
     const int j16 = lane % 16, g = lane / 16;
-    const double2 cm = cmapv[Tk.cl];
-    const int64_t first = C.first + Tk.off;
+    const double2 cm = make_double2(Tk.cx, Tk.cy);
+    const int64_t first = Tk.first;
     md4_t D64[MT];
     mf4_t D32[MT];
 #pragma unroll
@@ -955,6 +978,14 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluste
         D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
     }
     for (int base = 0; base < Tk.len; base += PTS) {
+        // this lane's B operands of the step, loaded before the recurrence:
+        // x[point][j16] for the points k0 + g (K = 4) or k0 + 8g + v (K = 32)
+        double bx[PTS / 4];
+#pragma unroll
+        for (int u = 0; u < PTS / 4; ++u) {
+            const int i = base + (PREC == 1 ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g);
+            bx[u] = (i < Tk.len && j16 < nq) ? xt[(first + i) * ldxt + j16] : 0.0;
+        }
         if (lane < PTS) {
             const int i = base + lane;
             const bool ok = i < Tk.len;
@@ -973,10 +1004,9 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluste
         }
         wave_sync();
         if constexpr (PREC != 1) {
-#pragma unroll 4
+#pragma unroll
             for (int k0 = 0; k0 < PTS; k0 += 4) {
-                const int i = base + k0 + g;
-                const double b = (i < Tk.len && j16 < nq) ? x[(first + i) * incx + j16 * ldx] : 0.0;
+                const double b = bx[k0 / 4];
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
                     if constexpr (PREC == 2) D64[t] = mfma64(Tl[t * 16 + j16][k0 + g], b, D64[t]);
@@ -989,8 +1019,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluste
                 mbf8_t bh, bl;
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
-                    const int i = base + k0 + 8 * g + v;
-                    const float b = (i < Tk.len && j16 < nq) ? (float)x[(first + i) * incx + j16 * ldx] : 0.f;
+                    const float b = (float)bx[(k0 / 32) * 8 + v];
                     bh[v] = bf16_hi(b);
                     bl[v] = bf16_lo(b);
                 }
@@ -1021,7 +1050,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const DCluste
             else mu[t * 16 + 4 * g + v][j16] = (double)D32[t][v];
         }
     wave_sync();
-    if (C.nslots == 1) {
+    if (Tk.single) {
         // single task: w = A mu (the reduce's Chebyshev step, same order)
         for (int idx = lane; idx < r * MQ; idx += WAVE) {
             const int q = idx / r, m = idx - q * r;
@@ -1055,28 +1084,48 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
     md4_t Z[MT];
 #pragma unroll
     for (int t = 0; t < MT; ++t) Z[t] = md4_t{0.0, 0.0, 0.0, 0.0};
-    for (int e = C.far_beg; e < C.far_end; ++e) {
-        const DFarBlk b = p.far[far_list[e]];
-        const double *ws = w + (int64_t)b.cc * MQ * r;
-        const double *F = FREE ? nullptr : p.F + b.foff;
-        const double *tx = p.rnodes + (int64_t)b.rc * r;
-        const double *ty = p.cnodes + (int64_t)b.cc * r;
+    for (int e0 = C.far_beg; e0 < C.far_end; e0 += WAVE) {
+        // the far blocks' records, one per lane, read together
+        const int ne = min(WAVE, C.far_end - e0);
+        int b_rc = 0, b_cc = 0;
+        int64_t b_foff = 0;
+        if (lane < ne) {
+            const DFarBlk b = p.far[far_list[e0 + lane]];
+            b_rc = b.rc;
+            b_cc = b.cc;
+            b_foff = b.foff;
+        }
+        for (int e = 0; e < ne; ++e) {
+            const int rc = rl_i(b_rc, e), cc = rl_i(b_cc, e);
+            const double *ws = w + (int64_t)cc * MQ * r;
+            const double *F = FREE ? nullptr : p.F + rl_l(b_foff, e);
+            const double *tx = p.rnodes + (int64_t)rc * r;
+            const double *ty = p.cnodes + (int64_t)cc * r;
 #pragma unroll
-        for (int k0 = 0; k0 < RP; k0 += 4) {
-            const int l = k0 + g;
-            const bool lok = l < r;
-            const double bv = lok ? ws[j16 * r + l] : 0.0;
-            double tyl = 0.0;
-            if constexpr (FREE) tyl = lok ? ty[l] : 0.0;
+            for (int k0 = 0; k0 < RP; k0 += 16) {
+                // four K-steps: operands first, then the products
+                double bv[4], av[4][MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) {
-                const int m = t * 16 + j16;
-                double av = 0.0;
-                if (lok && m < r) {
-                    if constexpr (FREE) av = kern_fast<double, KID>(tx[m], tyl);
-                    else av = F[m + (int64_t)r * l];
+                for (int h = 0; h < 4; ++h) {
+                    const int l = k0 + 4 * h + g;
+                    const bool lok = l < r;
+                    bv[h] = lok ? ws[j16 * r + l] : 0.0;
+                    double tyl = 0.0;
+                    if constexpr (FREE) tyl = lok ? ty[l] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < MT; ++t) {
+                        const int m = t * 16 + j16;
+                        av[h][t] = 0.0;
+                        if (lok && m < r) {
+                            if constexpr (FREE) av[h][t] = kern_fast<double, KID>(tx[m], tyl);
+                            else av[h][t] = F[m + (int64_t)r * l];
+                        }
+                    }
                 }
-                Z[t] = mfma64(av, bv, Z[t]);
+#pragma unroll
+                for (int h = 0; h < 4; ++h)
+#pragma unroll
+                    for (int t = 0; t < MT; ++t) Z[t] = mfma64(av[h][t], bv[h], Z[t]);
             }
         }
     }
@@ -1117,13 +1166,13 @@ template <int RP, int PREC, bool FREE, int KID>
 __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
                                               const DNear *nl, const DFar *fl,
                                               const double2 *cmapv, const double *pts_out,
-                                              const double *pts_in, const double *z, MvArgs a,
-                                              int nq) {
+                                              const double *pts_in, const double *xt,
+                                              int64_t ldxt, const double *z, MvArgs a, int nq) {
     constexpr int KSEL = PREC == 1 ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
-    __shared__ double xs_all[WPB][WAVE * MQ];
+    __shared__ double ds_all[WPB][16 * MQ];  // one 16-row output tile per wavefront
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    double *xs = xs_all[(threadIdx.x / WAVE) % WPB];
+    double *Dl = ds_all[(threadIdx.x / WAVE) % WPB];
     if (wid >= nch) return;
     const int r = p.r;
     const DChunk c = ch[wid];
@@ -1135,6 +1184,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         rok[t] = t * 16 + j16 < c.np;
         po[t] = rok[t] ? pts_out[c.p0 + t * 16 + j16] : 0.0;
     }
+    const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
     md4_t D[4];
     mf4_t D32[4];
 #pragma unroll
@@ -1142,117 +1192,176 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         D[t] = md4_t{0.0, 0.0, 0.0, 0.0};
         D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
     }
-    // ---- near field, FP64
-    for (int e = c.near_beg; e < c.near_end; ++e) {
-        const DNear s = nl[e];
-        const DDense Dd = p.dense[s.blk];
-        const double *xin = a.x + Dd.col0 * a.incx;
-        for (int jt = 0; jt < Dd.n; jt += WAVE) {
-            const int nj = min(WAVE, Dd.n - jt);
-            if (lane < nj) {
+    // ---- near field, FP64: two K-steps (8 columns) per group, operands first;
+    // the chunk's block records are read together, one per lane
+    for (int e0 = c.near_beg; e0 < c.near_end; e0 += WAVE) {
+        const int ne = min(WAVE, c.near_end - e0);
+        int64_t d_aoff = 0, d_col0 = 0;
+        int d_n = 0, d_off = 0;
+        if (lane < ne) {
+            const DNear sn = nl[e0 + lane];
+            const DDense Dd = p.dense[sn.blk];
+            d_aoff = Dd.aoff;
+            d_col0 = Dd.col0;
+            d_n = Dd.n;
+            d_off = sn.off;
+        }
+        for (int e = 0; e < ne; ++e) {
+            const int64_t aoff = rl_l(d_aoff, e), col0 = rl_l(d_col0, e);
+            const int bn = rl_i(d_n, e), boff = rl_i(d_off, e);
+            const double *xtb = xt + col0 * ldxt;
+            // element (i, cc) of the block sits at rowoff(i) + 64 cc
+            int64_t rowoff[4];
 #pragma unroll
-                for (int q = 0; q < MQ; ++q)
-                    xs[lane * MQ + q] = q < nq ? xin[(int64_t)(jt + lane) * a.incx + q * a.ldx] : 0.0;
+            for (int t = 0; t < 4; ++t) {
+                const int i = boff + t * 16 + j16;
+                rowoff[t] = aoff + (int64_t)(i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1));
             }
-            wave_sync();
-            for (int k0 = 0; k0 < nj; k0 += 4) {
-                const int cc = k0 + g;
-                const bool cok = cc < nj;
-                const double bv = cok ? xs[cc * MQ + j16] : 0.0;
-                double yc = 0.0;
-                if constexpr (FREE) yc = cok ? pts_in[Dd.col0 + jt + cc] : 0.0;
+            for (int k0 = 0; k0 < bn; k0 += 8) {
+                double bv[2], av[2][4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    double av = 0.0;
-                    if (rok[t] && cok) {
-                        if constexpr (FREE) av = kern_fast<double, KID>(po[t], yc);
-                        else av = p.A[Dd.aoff + tile_index(s.off + t * 16 + j16, jt + cc, Dd.n)];
+                for (int h = 0; h < 2; ++h) {
+                    const int cc = k0 + 4 * h + g;
+                    const bool cok = cc < bn;
+                    bv[h] = (cok && j16 < nq) ? xtb[(int64_t)cc * ldxt + j16] : 0.0;
+                    double yc = 0.0;
+                    if constexpr (FREE) yc = cok ? pts_in[col0 + cc] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        av[h][t] = 0.0;
+                        if (rok[t] && cok) {
+                            if constexpr (FREE) av[h][t] = kern_fast<double, KID>(po[t], yc);
+                            else av[h][t] = p.A[rowoff[t] + (int64_t)cc * HM_TILE];
+                        }
                     }
-                    D[t] = mfma64(av, bv, D[t]);
                 }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (t < nt) D[t] = mfma64(av[h][t], bv[h], D[t]);
             }
-            wave_sync();
         }
     }
-    // ---- far field
-    for (int e = c.far_beg; e < c.far_end; ++e) {
-        const DFar f = fl[e];
-        const double2 cm = cmapv[f.cl];
-        const double *zc = z + (int64_t)f.cl * MQ * r + (int64_t)j16 * r;  // C[k][q = j16]
-        // this lane's B operands: C[k][j16] for its k of every K-step
-        double bz[KSEL];
+    // ---- far field: the covering clusters' records one per lane; the
+    // coefficients of the next cluster load while this one is applied
+    auto load_bz = [&](double *bz, int cl) {
+        const double *zc = z + (int64_t)cl * MQ * r + (int64_t)j16 * r;  // C[k][q = j16]
 #pragma unroll
         for (int u = 0; u < KSEL; ++u) {
             const int k = PREC == 1 ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
             bz[u] = (k < r && j16 < nq) ? zc[min(k, r - 1)] : 0.0;
         }
+    };
+    for (int e0 = c.far_beg; e0 < c.far_end; e0 += WAVE) {
+        const int ne = min(WAVE, c.far_end - e0);
+        int f_cl = 0;
+        double f_cx = 0.0, f_cy = 0.0;
+        if (lane < ne) {
+            f_cl = fl[e0 + lane].cl;
+            const double2 cmv = cmapv[f_cl];
+            f_cx = cmv.x;
+            f_cy = cmv.y;
+        }
+        double bz[KSEL], bzn[KSEL] = {};
+        load_bz(bz, rl_i(f_cl, 0));
+        for (int e = 0; e < ne; ++e) {
+            const double2 cm = make_double2(rl_d(f_cx, e), rl_d(f_cy, e));
+            if (e + 1 < ne) load_bz(bzn, rl_i(f_cl, e + 1));
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const double xh = (po[t] - cm.x) * cm.y;
-            double sel[KSEL];
+            for (int t = 0; t < 4; ++t) {
+                if (t >= nt) break;
+                const double xh = (po[t] - cm.x) * cm.y;
+                double sel[KSEL];
+                if constexpr (PREC != 1) {
+                    // this lane's T_k, k = g + 4u: T_g and T_{g+4} from the prefix
+                    // T_0..T_7, then T_{k+4} = 2 T_4 T_k - T_{k-4} (stride-4 recurrence)
+                    const double x2 = xh + xh;
+                    const double c2 = __builtin_fma(x2, xh, -1.0), c3 = __builtin_fma(x2, c2, -xh);
+                    const double c4 = __builtin_fma(x2, c3, -c2), c5 = __builtin_fma(x2, c4, -c3);
+                    const double c6 = __builtin_fma(x2, c5, -c4), c7 = __builtin_fma(x2, c6, -c5);
+                    double ta = g == 0 ? 1.0 : g == 1 ? xh : g == 2 ? c2 : c3;  // T_g
+                    double tb = g == 0 ? c4 : g == 1 ? c5 : g == 2 ? c6 : c7;   // T_{g+4}
+                    const double t4x2 = c4 + c4;
 #pragma unroll
-            for (int u = 0; u < KSEL; ++u) sel[u] = 0.0;
-            double t0 = 1.0, t1 = xh;
-#pragma unroll
-            for (int k = 0; k < RP; ++k) {
-                double tv = k == 0 ? t0 : t1;
-                if (k >= 2) {
-                    tv = __builtin_fma(xh + xh, t1, -t0);
-                    t0 = t1;
-                    t1 = tv;
-                }
-                const bool mine = PREC == 1 ? ((k & 31) >> 3) == g : (k & 3) == g;
-                const int u = PREC == 1 ? (k >> 5) * 8 + (k & 7) : (k >> 2);
-                if (mine && rok[t] && k < r) sel[u] = tv;
-            }
-            if constexpr (PREC == 2) {
-#pragma unroll
-                for (int u = 0; u < KSEL; ++u) D[t] = mfma64(sel[u], bz[u], D[t]);
-            } else if constexpr (PREC == 0) {
-#pragma unroll
-                for (int u = 0; u < KSEL; ++u)
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[u], D32[t], 0, 0, 0);
-            } else {
-#pragma unroll
-                for (int s8 = 0; s8 < KSEL / 8; ++s8) {
-                    mbf8_t ah, al, bh, bl;
-#pragma unroll
-                    for (int v = 0; v < 8; ++v) {
-                        const float av = (float)sel[s8 * 8 + v], bv = (float)bz[s8 * 8 + v];
-                        ah[v] = bf16_hi(av);
-                        al[v] = bf16_lo(av);
-                        bh[v] = bf16_hi(bv);
-                        bl[v] = bf16_lo(bv);
+                    for (int u = 0; u < KSEL; ++u) {
+                        sel[u] = (rok[t] && 4 * u + g < r) ? ta : 0.0;
+                        const double tn = __builtin_fma(t4x2, tb, -ta);  // T_{g+4u+8}
+                        ta = tb;
+                        tb = tn;
                     }
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < KSEL; ++u) sel[u] = 0.0;
+                    double t0 = 1.0, t1 = xh;
+#pragma unroll
+                    for (int k = 0; k < RP; ++k) {
+                        double tv = k == 0 ? t0 : t1;
+                        if (k >= 2) {
+                            tv = __builtin_fma(xh + xh, t1, -t0);
+                            t0 = t1;
+                            t1 = tv;
+                        }
+                        const bool mine = ((k & 31) >> 3) == g;
+                        const int u = (k >> 5) * 8 + (k & 7);
+                        if (mine && rok[t] && k < r) sel[u] = tv;
+                    }
+                }
+                if constexpr (PREC == 2) {
+#pragma unroll
+                    for (int u = 0; u < KSEL; ++u) D[t] = mfma64(sel[u], bz[u], D[t]);
+                } else if constexpr (PREC == 0) {
+#pragma unroll
+                    for (int u = 0; u < KSEL; ++u)
+                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[u], D32[t], 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int s8 = 0; s8 < KSEL / 8; ++s8) {
+                        mbf8_t ah, al, bh, bl;
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) {
+                            const float av = (float)sel[s8 * 8 + v], bv = (float)bz[s8 * 8 + v];
+                            ah[v] = bf16_hi(av);
+                            al[v] = bf16_lo(av);
+                            bh[v] = bf16_hi(bv);
+                            bl[v] = bf16_lo(bv);
+                        }
+                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
+                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
+                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < KSEL; ++u) bz[u] = bzn[u];
+        }
+    }
+    // ---- result, one 16-row tile at a time through LDS; a lane then writes
+    // 16 consecutive rows of one right-hand side per store
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (t >= nt) break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Dl[(g + 4 * v) * MQ + j16] = D[t][v];
+        if constexpr (PREC != 2) {
+            wave_sync();
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Dl[(4 * g + v) * MQ + j16] += (double)D32[t][v];
+        }
+        wave_sync();
+        const int row = j16;
+        if (t * 16 + row < c.np) {
+            const int64_t i = c.p0 + t * 16 + row;
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int q = g + 4 * s4;
+                if (q < nq) {
+                    double *yo = a.y + i * a.incy + q * a.ldy;
+                    *yo = (a.beta ? *yo : 0.0) + Dl[row * MQ + q];
                 }
             }
         }
-    }
-    // ---- 64 x 16 result through LDS to one row per lane, written once
-    double *Dl = xs;
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Dl[(t * 16 + g + 4 * v) * MQ + j16] = D[t][v];
-    if constexpr (PREC != 2) {
         wave_sync();
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Dl[(t * 16 + 4 * g + v) * MQ + j16] += (double)D32[t][v];
-    }
-    wave_sync();
-    if (lane < c.np) {
-#pragma unroll
-        for (int q = 0; q < MQ; ++q) {
-            if (q < nq) {
-                double *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
-                *yo = (a.beta ? *yo : 0.0) + Dl[lane * MQ + q];
-            }
-        }
     }
 }
 
@@ -1261,9 +1370,19 @@ template <int RP, int PREC, int MODE, int KID>
 hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, const int32_t *multi,
                       int nmulti, hipStream_t s) {
     constexpr bool FREE = MODE == 2;
+    // rows of x are read in place when the right-hand sides of a row are
+    // contiguous (ldx = 1, as for a row-major n x k array), else packed
+    const double *xr = a.x;
+    int64_t ldxr = a.incx;
+    if (!(a.ldx == 1 && a.incx >= nq)) {
+        if (p.ny > 0)
+            k_pack_x<<<(int)((p.ny + WG - 1) / WG), WG, 0, s>>>(a.x, a.incx, a.ldx, nq, p.ny, a.xt);
+        xr = a.xt;
+        ldxr = MQ;
+    }
     if (p.nfar > 0 && p.nctask > 0) {
-        k_up_m<RP, PREC><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.ccl, p.cmap, p.y, a.x, a.incx, a.ldx,
-                                                    nq, p.chebA, a.partial, a.w, p.r);
+        k_up_m<RP, PREC><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
+                                                    a.partial, a.w, p.r);
         if (nmulti > 0)
             k_reduce<double, true, MQ><<<nblk(nmulti), WG, 0, s>>>(p.ccl, nmulti, a.partial, p.chebA,
                                                                   a.w, p.r, multi);
@@ -1271,7 +1390,7 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, const int32_t *
     }
     if (p.nrch > 0)
         k_down_m<RP, PREC, FREE, KID><<<nblk(p.nrch), WG, 0, s>>>(
-            p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, a.z, a, nq);
+            p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
     return hipGetLastError();
 }
 

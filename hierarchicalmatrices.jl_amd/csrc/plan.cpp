@@ -308,10 +308,17 @@ void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
         C.nslots = 0;
         for (int64_t o = 0; o < C.len; o += HM_TASK_POINTS) {
             const int32_t len = (int32_t)std::min<int64_t>(HM_TASK_POINTS, C.len - o);
-            S.tasks.push_back(DTask{(int32_t)id, (int32_t)o, len, 0, slot++});
+            DTask T{};
+            T.cl = (int32_t)id;
+            T.off = (int32_t)o;
+            T.len = len;
+            T.slot = slot++;
+            T.first = C.first + o;
+            S.tasks.push_back(T);
             C.nslots++;
         }
     }
+    for (DTask &T : S.tasks) T.single = cl[T.cl].nslots == 1 ? 1 : 0;
     S.nslots = slot;
     std::stable_sort(S.tasks.begin(), S.tasks.end(),
                      [](const DTask &u, const DTask &v) { return u.len > v.len; });
@@ -356,10 +363,11 @@ struct PlanBufs {
     V2 *rmap = nullptr, *cmap = nullptr;
     T *A = nullptr, *U = nullptr, *V = nullptr, *F = nullptr;
     T *partial = nullptr, *w = nullptr, *z = nullptr;
+    T *xt = nullptr;  // packed right-hand sides of the MFMA passes (ny x 16)
     DevPlanT<T> dp{};
 
     void release() {
-        void *b[] = {x, y, lam, rnodes, cnodes, chebA, rmap, cmap, A, U, V, F, partial, w, z};
+        void *b[] = {x, y, lam, rnodes, cnodes, chebA, rmap, cmap, A, U, V, F, partial, w, z, xt};
         for (void *q : b)
             if (q) (void)hipFree(q);
     }
@@ -590,6 +598,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
     HM_TRY_T(dalloc(&b.partial, nslots * r * HM_WS_RHS));
     HM_TRY_T(dalloc(&b.w, ncl * r * HM_WS_RHS));
     HM_TRY_T(dalloc(&b.z, ncl * r * HM_WS_RHS));
+    if (std::is_same_v<T, double>) HM_TRY_T(dalloc(&b.xt, P->ny * HM_WS_RHS));
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
     const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
     if (stored) {
@@ -781,6 +790,22 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     build_side(ny, P->dense, false, P->ccl, P->cols);
     for (size_t k = 0; k < P->ccl.size(); ++k)
         if (P->ccl[k].nslots > 1) P->multi_c.push_back((int32_t)k);
+    // the cluster maps the Float64 MFMA upward pass reads from its task
+    // record: (a+b)/2 and 1/((b-a)/2) as the Float64 cluster maps form them
+    if (!f32) {
+        for (DTask &T : P->cols.tasks) {
+            const HCluster &h = B.ccl[T.cl];
+            const double ab2 = 0.5 * (h.lo + h.hi), ba2 = 0.5 * (h.hi - h.lo);
+            T.cx = ab2;
+            T.cy = 1.0 / ba2;
+        }
+        for (DTask &T : P->rows.tasks) {
+            const HCluster &h = B.rcl[T.cl];
+            const double ab2 = 0.5 * (h.lo + h.hi), ba2 = 0.5 * (h.hi - h.lo);
+            T.cx = ab2;
+            T.cy = 1.0 / ba2;
+        }
+    }
 
     if (flags & HM_PLAN_TREE_ONLY) {
         P->far_by_rc_h = std::move(by_rc);
@@ -1081,6 +1106,7 @@ static MvArgsT<T> ws_args(hm_plan_t P) {
     a.partial = b.partial;
     a.w = b.w;
     a.z = b.z;
+    a.xt = b.xt;
     return a;
 }
 }  // extern "C++"
