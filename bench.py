@@ -35,8 +35,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-n-log2", type=int, default=17)
-    ap.add_argument("--variants", default="hybrid,matrix_free,f32_stored,f32_hybrid",
-                    help="extra variants timed beside the stored product")
+    ap.add_argument("--variants", default=None,
+                    help="headline: extra variants timed beside the stored product (default "
+                         "hybrid,matrix_free,f32_stored,f32_hybrid); batched: plan modes to time "
+                         "(default stored,hybrid,matrix_free)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the torch bandwidth probes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
@@ -44,7 +46,11 @@ def parse():
                     choices=["headline", "batched", "gradient", "scale"],
                     help="headline = the n=2^20 product (the bench line); the others time "
                          "the batched, gradient and scale workloads")
-    return ap.parse_args()
+    args = ap.parse_args()
+    args.variants_given = args.variants is not None
+    if args.variants is None:
+        args.variants = "hybrid,matrix_free,f32_stored,f32_hybrid"
+    return args
 
 
 def time_steps(fn, steps, stream):
@@ -185,7 +191,10 @@ def run_batched(args):
                       "points_total": int(sum(len(i["x"]) for i in inst)), "rank": 16,
                       "leaf_blocksize": 64, "rhs": 32}, "variants": {}}
     ref = None
+    wanted = set(v for v in args.variants.split(",") if v) if args.variants_given else None
     for name, kw in (("stored", {}), ("hybrid", dict(hybrid=True)), ("matrix_free", dict(matrix_free=True))):
+        if wanted is not None and name not in wanted:
+            continue
         H = hmx.KernelMatrixBatch("cauchykernel", spec, blocksize=64, rank=16, **kw)
         Y = torch.empty(H.shape[0], 32, dtype=torch.float64, device=dev)
         ms = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream), args.steps,

@@ -1083,7 +1083,7 @@ int hm_plan_blocks(hm_plan_t P, int64_t *ints, double *dbls, int64_t cap, int64_
 // right-hand sides per launch of the MFMA passes, and the default number of
 // right-hand sides from which forward products use them
 constexpr int MFMA_KB = 16;
-constexpr int64_t HM_MFMA_MIN_K = 16;
+constexpr int64_t HM_MFMA_MIN_K = 8;
 
 // A product entry point for element type T may only be called on a plan of
 // that type.

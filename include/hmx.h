@@ -65,7 +65,7 @@ enum hm_mv_flags {
                                     (hi*hi + hi*lo + lo*hi), FP32 accumulate */
     HM_MV_MFMA = 32,             /* forward: 16 right-hand sides per launch on the FP64
                                     MFMA passes (far field in Chebyshev form); the
-                                    default when k >= 16 */
+                                    default when k >= 8 */
     HM_MV_NO_MFMA = 64           /* forward: never the MFMA passes (up to 8 right-hand
                                     sides per launch on the vector units) */
 };

@@ -27,9 +27,12 @@ def test_multi_rhs_matches_single(cuda, mode, k):
                          **mk(mode))
     B = torch.from_numpy(hmx.workloads.normal(31, 4096 * k).reshape(4096, k)).to(cuda)
     Y = H @ B
+    # from 16 right-hand sides on the product runs on the MFMA passes (far
+    # field in Chebyshev form): equal to the single-RHS product up to rounding
+    tol = 1e-13 if k >= 16 else 1e-14
     for q in range(k):
         yq = H @ B[:, q].contiguous()
-        assert rel(Y[:, q].cpu().numpy(), yq.cpu().numpy()) <= 1e-14, q
+        assert rel(Y[:, q].cpu().numpy(), yq.cpu().numpy()) <= tol, q
     # column-major right-hand sides (ld > 1 between RHS, unit stride inside)
     Bt = B.T.contiguous().T
     assert rel((H @ Bt).cpu().numpy(), Y.cpu().numpy()) <= 1e-15
@@ -100,6 +103,9 @@ def test_tuned_products_match(cuda):
     w = hmx.workloads.config_plain(n=8192, blocksize=64, rank=16)
     H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
     kb, mode, us = H.tune(k=16)
-    assert kb in (1, 2, 4, 8) and mode in (0, 1, 2) and us > 0
+    # 1, 2, 4, 8 right-hand sides per launch on the vector units, 16 on MFMA
+    assert kb in (1, 2, 4, 8, 16) and mode in (0, 1, 2) and us > 0
     B = torch.from_numpy(hmx.workloads.normal(2, 8192 * 16).reshape(8192, 16)).to(cuda)
-    assert rel(H.matvec(B, tuned=True).cpu().numpy(), (H @ B).cpu().numpy()) <= 1e-14
+    # the default for 16 right-hand sides is the MFMA passes; any tuned choice
+    # agrees with them to rounding
+    assert rel(H.matvec(B, tuned=True).cpu().numpy(), (H @ B).cpu().numpy()) <= 1e-13
