@@ -209,3 +209,17 @@ def test_f32_defaults_and_errors():
     # a Float64 product entry point on a Float32 plan fails before any GPU work
     assert _lib.lib.hm_matvec_f32(H._h, None, None, 0, 0, 1, 1, 1, 0, 0, 0, 0, None) == \
         _lib.HM_ERR_INVALID
+
+
+def test_wide_intervals_are_reported():
+    """Points in [0.1, 0.9] with intervals (1.5, -0.5): deep in the tree one
+    side's range is empty while the other still splits, so the reference's
+    indsplit reads outside its range. The oracle raises; libhmx returns
+    HM_ERR_DEGENERATE."""
+    n = 1250
+    x = np.repeat((np.arange(n, 0, -1) - 0.25) / n, 2) * 0.8 + 0.1
+    y = np.repeat((np.arange(n, 0, -1) - 0.75) / n, 2) * 0.8 + 0.1
+    iv = (1.5, -0.5, 1.5, -0.5)
+    with pytest.raises(ValueError):
+        O.Tree(O.CAUCHY, x, y, *iv, 40, 12, store=False)
+    assert status_of(lambda: tree("cauchykernel", x, y, iv, 40, 12)) == _lib.HM_ERR_DEGENERATE
