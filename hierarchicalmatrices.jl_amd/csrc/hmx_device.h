@@ -88,6 +88,11 @@ hipError_t hmx_launch_entries(const DevPlan &p, int64_t n, const int64_t *i, con
 hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long long *count,
                                       hipStream_t s);
 
+// in-place diagonal scaling of stored dense blocks and bases (plans built
+// from blocks): side 0 scales columns (A and V rows), 1 rows (A and U rows);
+// b is indexed by the global column (side 0) or row (side 1)
+hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s);
+
 // The same passes for FP32 plans (ranks up to 32; no reduced-precision form).
 hipError_t hmx_launch_compress(const DevPlanF &p, hipStream_t s);
 hipError_t hmx_launch_matvec(const DevPlanF &p, const MvArgsF &a, bool transpose, int mode, int kb,

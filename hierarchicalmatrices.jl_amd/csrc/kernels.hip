@@ -108,3 +108,40 @@ hipError_t hmx_launch_count_nonfinite(const float *v, int64_t n, unsigned long l
     hmxk::k_count_nonfinite<float><<<(int)nb, hmxk::WG, 0, s>>>(v, n, blockcount);
     return hipGetLastError();
 }
+
+// ------------------------------------------------------- diagonal scaling
+
+namespace {
+
+// A[i, j] *= b[col0 + j] (side 0) or b[row0 + i] (side 1), one block per workgroup
+__global__ __launch_bounds__(hmxk::WG) void k_scale_dense(DevPlan p, int side, const double *b) {
+    const DDense D = p.dense[blockIdx.x];
+    const int32_t tot = D.m * D.n;
+    for (int32_t e = threadIdx.x; e < tot; e += hmxk::WG) {
+        const int32_t j = e / D.m, i = e - j * D.m;
+        const double s = side == 0 ? b[D.col0 + j] : b[D.row0 + i];
+        double *a = p.A + D.aoff + tile_index(i, j, D.n);
+        *a = *a * s;
+    }
+}
+
+// B[i, k] *= b[first + i] for every basis row of one cluster
+__global__ __launch_bounds__(hmxk::WG) void k_scale_basis(const DCluster *cl, double *B, int r,
+                                                         const double *b) {
+    const DCluster C = cl[blockIdx.x];
+    const int32_t tot = C.len * r;
+    for (int32_t e = threadIdx.x; e < tot; e += hmxk::WG) {
+        const int32_t k = e / C.len, i = e - k * C.len;
+        double *v = B + C.boff + tile_index(i, k, r);
+        *v = *v * b[C.first + i];
+    }
+}
+
+}  // namespace
+
+hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s) {
+    if (p.ndense > 0 && p.A) k_scale_dense<<<p.ndense, hmxk::WG, 0, s>>>(p, side, b);
+    if (side == 0 && p.ncc > 0 && p.V) k_scale_basis<<<p.ncc, hmxk::WG, 0, s>>>(p.ccl, p.V, p.r, b);
+    if (side == 1 && p.nrc > 0 && p.U) k_scale_basis<<<p.nrc, hmxk::WG, 0, s>>>(p.rcl, p.U, p.r, b);
+    return hipGetLastError();
+}

@@ -988,6 +988,16 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     return HM_OK;
 }
 
+int hm_plan_scale(hm_plan_t P, int side, const double *b, int64_t off, void *stream) {
+    if (!P || !b || off < 0 || (side != 0 && side != 1))
+        return fail(HM_ERR_INVALID, "scale: bad arguments");
+    if (!P->from_blocks)
+        return fail(HM_ERR_UNSUPPORTED, "scale: only plans built from blocks (the reference has no scale! for KernelMatrix leaves)");
+    hipError_t e = hmx_launch_scale(P->b64.dp, side, b + off, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "scale launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
 int hm_plan_batched(int kernel_id, int64_t count, const double *const *x, const int64_t *nx,
                     const double *const *y, const int64_t *ny, const double *intervals,
                     int blocksize, int rank, unsigned flags, void *stream, hm_plan_t *out) {

@@ -9,7 +9,7 @@ from .kernelmatrix import (  # noqa: F401
     cauchykernel, chebyshevbarycentricweights, chebyshevpoints, coulombkernel,
     coulombprimekernel, indsplit, logkernel, mul_, size,
 )
-from .hierarchical import HierarchicalMatrix, LowRankMatrix  # noqa: F401
+from .hierarchical import HierarchicalMatrix, LowRankMatrix, add_col_, lmul_, rmul_  # noqa: F401
 from . import workloads  # noqa: F401
 from . import autodiff  # noqa: F401
 
@@ -17,5 +17,5 @@ __all__ = [
     "BLOCKRANK", "BLOCKSIZE", "Block", "KernelMatrix", "HMError", "blocksize", "size",
     "cauchykernel", "coulombkernel", "coulombprimekernel", "logkernel",
     "chebyshevpoints", "chebyshevbarycentricweights", "indsplit", "mul_", "workloads",
-    "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch",
+    "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch", "rmul_", "lmul_", "add_col_",
 ]

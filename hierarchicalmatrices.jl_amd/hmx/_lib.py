@@ -126,6 +126,8 @@ def _load():
     L.hm_plan_profile_read.argtypes = [_vp, _dp, _ip]
     L.hm_entries.restype = ctypes.c_int
     L.hm_entries.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp]
+    L.hm_plan_scale.restype = ctypes.c_int
+    L.hm_plan_scale.argtypes = [_vp, ctypes.c_int, _vp, _i64, _vp]
     # Float32 plans
     L.hm_plan_from_points_f32.restype = ctypes.c_int
     L.hm_plan_from_points_f32.argtypes = [ctypes.c_int, _fp, _i64, _fp, _i64, ctypes.c_float,

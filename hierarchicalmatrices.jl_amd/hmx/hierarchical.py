@@ -12,6 +12,11 @@
   mul_(y, H, x, istart, jstart, INCX, INCY)
                                   src/HierarchicalMatrix.jl:24-52 with the leaf
                                   loops of src/algebra.jl:37-48,110-131
+  rmul_(H, b), lmul_(b, H)        rmul!(H, Diagonal(b)), lmul!(Diagonal(b), H):
+                                  scale!, src/HierarchicalMatrix.jl:15-16,54-102
+                                  with src/algebra.jl:282-321
+  add_col_(H, u, j[, istart])     add_col!, src/HierarchicalMatrix.jl:21,104-146
+                                  with src/algebra.jl:324-380
 
 The tree is flattened on first use into the leaves' absolute offsets (the
 traversal's istart + p, jstart + q) and handed to hm_plan_from_blocks; the
@@ -156,6 +161,13 @@ class HierarchicalMatrix:
             self._plan = _BlocksPlan(self, stream)
         return self._plan
 
+    def _drop_plans(self):
+        self._plan = None
+        for row in self.blocks:
+            for c in row:
+                if isinstance(c, HierarchicalMatrix):
+                    c._drop_plans()
+
     # products, as KernelMatrix
     def mul(self, u, v, istart=1, jstart=1, INCX=1, INCY=1, **kw):
         return self.plan().mul(u, v, istart, jstart, INCX, INCY, **kw)
@@ -218,3 +230,111 @@ class _BlocksPlan(KernelMatrix):
         self.flags = 0
         self.x = np.zeros(rows)
         self.y = np.zeros(cols)
+
+
+def _scale_host(H, b, start, side):
+    """scale!(H, b, jstart) (side 0) / scale!(b, H, istart) (side 1) on the host
+    leaves, in the reference's loop order (src/HierarchicalMatrix.jl:54-102);
+    start is 1-based."""
+    M, N = H.assigned.shape
+    if side == 0:
+        q = 0
+        for n in range(N):
+            for m in range(M):
+                c = H.blocks[m][n]
+                if isinstance(c, HierarchicalMatrix):
+                    _scale_host(c, b, start + q, 0)
+                elif isinstance(c, LowRankMatrix):  # scale!(C.V, b, A.V, jstart)
+                    c.V *= b[start + q - 1:start + q - 1 + c.V.shape[0], None]
+                elif c is not None:  # C[i, j] = A[i, j] * b[jshift + j]
+                    c *= b[None, start + q - 1:start + q - 1 + c.shape[1]]
+            q += H.blocksize(1, n + 1, 2)
+    else:
+        p = 0
+        for m in range(M):
+            for n in range(N):
+                c = H.blocks[m][n]
+                if isinstance(c, HierarchicalMatrix):
+                    _scale_host(c, b, start + p, 1)
+                elif isinstance(c, LowRankMatrix):  # scale!(C.U, b, A.U, istart)
+                    c.U *= b[start + p - 1:start + p - 1 + c.U.shape[0], None]
+                elif c is not None:  # C[i, j] = A[i, j] * b[ishift + i]
+                    c *= b[start + p - 1:start + p - 1 + c.shape[0], None]
+            p += H.blocksize(m + 1, N, 1)
+
+
+def _scale(H, b, side):
+    b_host = b.detach().cpu().numpy() if hasattr(b, "detach") else np.asarray(b, dtype=np.float64)
+    b_host = np.ascontiguousarray(b_host, dtype=np.float64)
+    n = H.size(2 if side == 0 else 1)
+    if len(b_host) < n:
+        raise ValueError("b is shorter than the scaled dimension of H")
+    _scale_host(H, b_host, 1, side)
+    plan = H._plan
+    for row in H.blocks:  # nested plans would no longer match their leaves
+        for c in row:
+            if isinstance(c, HierarchicalMatrix):
+                c._drop_plans()
+    if plan is not None:  # the device factors are scaled in place, as the host ones
+        from .kernelmatrix import _torch
+        torch = _torch()
+        bd = torch.from_numpy(b_host).cuda()
+        check(lib.hm_plan_scale(plan._h, side, ctypes.c_void_p(bd.data_ptr()), 0,
+                                _stream_ptr(None)))
+        torch.cuda.synchronize()
+    return H
+
+
+def rmul_(H, b):
+    """rmul!(H, Diagonal(b)): H := H * Diagonal(b) (scale!(H, b, 1),
+    src/HierarchicalMatrix.jl:15,54-77): dense leaves scale their columns,
+    LowRank leaves the rows of V. A device plan built for H is scaled in place
+    (hm_plan_scale)."""
+    return _scale(H, b, 0)
+
+
+def lmul_(b, H):
+    """lmul!(Diagonal(b), H): H := Diagonal(b) * H (scale!(b, H, 1),
+    src/HierarchicalMatrix.jl:16,79-102): dense leaves scale their rows,
+    LowRank leaves the rows of U."""
+    return _scale(H, b, 1)
+
+
+def _add_col_lowrank(L, u, istart, j):
+    """add_col(L, u, istart, j) for Float64 (src/algebra.jl:361-380): rank + 1,
+    the new term first: U1 = [u[istart:...] U], S1 = [1; S], V1 = [e_j V]."""
+    m, n = L.shape
+    U1 = np.concatenate([u[istart - 1:istart - 1 + m, None], L.U], axis=1)
+    S1 = np.concatenate([[1.0], L.S])
+    e = np.zeros((n, 1))
+    e[j - 1, 0] = 1.0
+    V1 = np.concatenate([e, L.V], axis=1)
+    return LowRankMatrix(U1, S1, V1)
+
+
+def add_col_(H, u, j, istart=1):
+    """add_col!(H, u, istart, j): column j (1-based) of H gains u[istart:...]
+    (src/HierarchicalMatrix.jl:21,104-146): dense leaves add in place, LowRank
+    leaves are replaced by their rank + 1 form, nested matrices recurse. The
+    device plan is rebuilt on the next product."""
+    u = np.ascontiguousarray(u.detach().cpu().numpy() if hasattr(u, "detach") else u,
+                             dtype=np.float64)
+    M, N = H.assigned.shape
+    n = 0
+    q = H.blocksize(1, 1, 2)
+    while q < j:
+        n += 1
+        q += H.blocksize(1, n + 1, 2)
+    q -= H.blocksize(1, n + 1, 2)
+    p = 0
+    for m in range(M):
+        c = H.blocks[m][n]
+        if isinstance(c, HierarchicalMatrix):
+            add_col_(c, u, j - q, istart + p)
+        elif isinstance(c, LowRankMatrix):
+            H.blocks[m][n] = _add_col_lowrank(c, u, istart + p, j - q)
+        elif c is not None:  # A[i + jm] += u[i + ishift]
+            c[:, j - q - 1] += u[istart + p - 1:istart + p - 1 + c.shape[0]]
+        p += H.blocksize(m + 1, N, 1)
+    H._drop_plans()
+    return H

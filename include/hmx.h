@@ -124,6 +124,19 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
                         const double *const *U, const double *const *S,
                         const double *const *V, void *stream, hm_plan_t *plan);
 
+/* Diagonal scaling of a plan built from blocks, in place on the device
+ * (src/HierarchicalMatrix.jl:15-16,54-102 with the leaf forms of
+ * src/algebra.jl:282-321):
+ *   side = 0: H := H * Diagonal(b[off], ..., b[off + ncols - 1])
+ *             (rmul!, scale!(H, b, jstart) with off = jstart - 1): dense leaves
+ *             A[i, j] *= b[off + col0 + j], LowRank leaves V[j, :] *= b[off + col0 + j];
+ *   side = 1: H := Diagonal(b[off], ..., b[off + nrows - 1]) * H
+ *             (lmul!, scale!(b, H, istart) with off = istart - 1): dense leaves
+ *             A[i, j] *= b[off + row0 + i], LowRank leaves U[i, :] *= b[off + row0 + i].
+ * b is a device vector. Only plans of hm_plan_from_blocks: the reference defines
+ * no scale! for the KernelMatrix leaves (HM_ERR_UNSUPPORTED). */
+int hm_plan_scale(hm_plan_t plan, int side, const double *b, int64_t off, void *stream);
+
 /* Many independent KernelMatrix instances in one plan (no counterpart in the
  * reference, which builds one tree per call of src/KernelMatrix.jl:47).
  * Instance t has points x[t] (nx[t]), y[t] (ny[t]) and intervals

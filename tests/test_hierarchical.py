@@ -117,3 +117,76 @@ def test_gpu_strided_product_vs_oracle(cuda, seed):
     B = rng.standard_normal((cols, 5))
     Y = H.plan().matvec(torch.from_numpy(B).to(cuda)).cpu().numpy()
     assert np.linalg.norm(Y - D @ B) <= 1e-13 * np.linalg.norm(D @ B)
+
+
+# ------------------------------------- scale! and add_col! (src/HierarchicalMatrix.jl)
+
+def test_scale_and_add_col_host_identities():
+    """rmul!/lmul! and add_col! on the host leaves against the dense
+    assembly: H D_b, D_b H and H + u e_j^T (tolerance: rounding of U S V')."""
+    rng = np.random.default_rng(11)
+    H = consistent_tree(rng)
+    rows, cols = H.size()
+    D0 = dense_of(H)
+    b = rng.standard_normal(cols)
+    hmx.rmul_(H, b)
+    assert np.allclose(dense_of(H), D0 * b[None, :], rtol=1e-13, atol=1e-13)
+    c = rng.standard_normal(rows)
+    D1 = dense_of(H)
+    hmx.lmul_(c, H)
+    assert np.allclose(dense_of(H), c[:, None] * D1, rtol=1e-13, atol=1e-13)
+    for j in (1, cols // 3, cols):
+        u = rng.standard_normal(rows)
+        D2 = dense_of(H)
+        ranks = sorted(l.rank() for l, _, _ in H.leaves() if isinstance(l, LowRankMatrix))
+        hmx.add_col_(H, u, j)
+        want = D2.copy()
+        want[:, j - 1] += u
+        assert np.allclose(dense_of(H), want, rtol=1e-13, atol=1e-13)
+        # the LowRank leaves in column j gained one rank, the new term first
+        after = sorted(l.rank() for l, _, _ in H.leaves() if isinstance(l, LowRankMatrix))
+        assert sum(after) >= sum(ranks)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 2])
+def test_gpu_scale_in_place_and_add_col(cuda, seed):
+    """hm_plan_scale on the device: (H D_b) x = H (b .* x) and (D_b H) x =
+    b .* (H x); the scaled plan equals a plan rebuilt from the scaled host
+    leaves bit for bit; after add_col! the product gains u x_j."""
+    import torch
+    rng = np.random.default_rng(seed)
+    H = consistent_tree(rng)
+    rows, cols = H.size()
+    x = torch.from_numpy(rng.standard_normal(cols)).to(cuda)
+    b = torch.from_numpy(rng.standard_normal(cols)).to(cuda)
+    c = torch.from_numpy(rng.standard_normal(rows)).to(cuda)
+    want_r = H.matvec(b * x)                      # plan built here
+    hmx.rmul_(H, b)                               # host leaves and device plan scaled
+    got_r = H.matvec(x)
+    assert torch.linalg.norm(got_r - want_r) <= 1e-14 * torch.linalg.norm(want_r)
+    want_l = c * H.matvec(x)
+    hmx.lmul_(c, H)
+    got_l = H.matvec(x)
+    assert torch.linalg.norm(got_l - want_l) <= 1e-14 * torch.linalg.norm(want_l)
+    # a fresh plan from the scaled host leaves computes the same bits
+    H._drop_plans()
+    assert torch.equal(H.matvec(x), got_l)
+    u = rng.standard_normal(rows)
+    j = int(rng.integers(1, cols + 1))
+    before = H.matvec(x)
+    hmx.add_col_(H, u, j)
+    after = H.matvec(x)
+    want = before + torch.from_numpy(u).to(cuda) * x[j - 1]
+    assert torch.linalg.norm(after - want) <= 1e-13 * torch.linalg.norm(want)
+
+
+@pytest.mark.gpu
+def test_gpu_scale_rejects_kernel_matrix(cuda):
+    import ctypes
+    import torch
+    from hmx._lib import HM_ERR_UNSUPPORTED, lib
+    w = hmx.workloads.config_plain(n=1024, blocksize=64, rank=16)
+    K = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16)
+    b = torch.ones(1024, dtype=torch.float64, device=cuda)
+    assert lib.hm_plan_scale(K._h, 0, ctypes.c_void_p(b.data_ptr()), 0, None) == HM_ERR_UNSUPPORTED
