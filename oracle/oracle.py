@@ -235,16 +235,22 @@ def dense_matvec(kid, x, y, v, rows=None, nthreads=0):
 
 
 def dense_mul(y, A, x, istart, jstart, incx=1, incy=1):
-    A = np.asfortranarray(A, dtype=np.float64)
+    """src/algebra.jl:37-48 in the element type of y (Float64 or Float32)."""
+    f32 = y.dtype == np.float32
+    A = np.asfortranarray(A, dtype=_dt(f32))
+    x = np.ascontiguousarray(x, dtype=_dt(f32))
     m, n = A.shape
-    lib().or_dense_mul(_d(y), A.ctypes.data_as(_dp), m, n, _d(x), istart, jstart, incx, incy)
+    lib(f32).or_dense_mul(_d(y), _d(A), m, n, _d(x), istart, jstart, incx, incy)
     return y
 
 
 def dense_tmul(y, A, x, istart, jstart, incx=1, incy=1):
-    A = np.asfortranarray(A, dtype=np.float64)
+    """src/algebra.jl:52-65 in the element type of y (Float64 or Float32)."""
+    f32 = y.dtype == np.float32
+    A = np.asfortranarray(A, dtype=_dt(f32))
+    x = np.ascontiguousarray(x, dtype=_dt(f32))
     m, n = A.shape
-    lib().or_dense_tmul(_d(y), A.ctypes.data_as(_dp), m, n, _d(x), istart, jstart, incx, incy)
+    lib(f32).or_dense_tmul(_d(y), _d(A), m, n, _d(x), istart, jstart, incx, incy)
     return y
 
 

@@ -175,3 +175,38 @@ def test_f32_traversal_rounds_in_float32():
     diff = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
     assert 1e-9 < diff < 2e-5, diff
     assert np.array_equal(O.Tree(O.CAUCHY, x, y, *iv, 64, 16, f32=True).matvec_omp(b), u32)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_f32_leaf_known_answers(exact):
+    """test/runtests.jl:15-33 at T = Float32: the four offset/stride cases of
+    the dense leaf mul! within eps(Float32)*|ref|, and exact on small
+    integers (the BigFloat twins of :35-52)."""
+    rng = np.random.default_rng(0)
+    if exact:
+        A = rng.integers(-8, 9, (10, 5)).astype(f32)
+        x = rng.integers(-8, 9, 40).astype(f32)
+    else:
+        A = hmx.workloads.uniform(100, 50).reshape(10, 5, order="F").astype(f32)
+        x = hmx.workloads.uniform(101, 40).astype(f32)
+    Ad, xd = A.astype(np.float64), x.astype(np.float64)
+    eps = float(np.finfo(np.float32).eps)
+
+    def check(got, want):
+        if exact:
+            assert np.array_equal(got.astype(np.float64), want)
+        else:
+            assert np.linalg.norm(got.astype(np.float64) - want) <= eps * np.linalg.norm(want)
+
+    y = np.zeros(40, dtype=f32)
+    O.dense_mul(y, A, x, 1, 1)
+    check(y[0:10], Ad @ xd[0:5])
+    y = np.zeros(40, dtype=f32)
+    O.dense_mul(y, A, x, 5, 5, 2, 2)
+    check(y[4:23:2], Ad @ xd[4:14:2])
+    y = np.zeros(40, dtype=f32)
+    O.dense_tmul(y, A, x, 1, 5, 2, 1)
+    check(y[0:5], Ad.T @ xd[4:23:2])
+    y = np.zeros(40, dtype=f32)
+    O.dense_tmul(y, A, x, 6, 3, 1, 3)
+    check(y[5:18:3], Ad.T @ xd[2:12])
