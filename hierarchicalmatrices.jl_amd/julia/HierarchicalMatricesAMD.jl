@@ -1,23 +1,28 @@
 """
     HierarchicalMatricesAMD
 
-Julia front-end of libhmx (C ABI in include/hmx.h): a device-backed
-`KernelMatrix` with the constructor and product methods of HierarchicalMatrices.jl
-(src/KernelMatrix.jl:5-47), implemented by `ccall`.
+Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
 
-Vectors may be host `Vector{Float64}` (staged through device buffers) or any
-array whose `pointer` is a device address (e.g. an AMDGPU.jl `ROCVector`), via
-`mul_device!`.
+  * `DeviceKernelMatrix{T}` (T = Float64 or Float32): the constructor and
+    product methods of `KernelMatrix` (src/KernelMatrix.jl:5-47) on the GPU.
+  * `DeviceHierarchicalMatrix`: a `HierarchicalMatrix` with `LowRankMatrix` and
+    `Matrix` leaves (src/HierarchicalMatrix.jl) flattened into a device plan
+    (`hm_plan_from_blocks`), with `mul!` and the diagonal scalings `rmul!` /
+    `lmul!` (src/HierarchicalMatrix.jl:15-16).
+
+Vectors may be host vectors (staged through device buffers) or device pointers
+(e.g. `pointer` of an AMDGPU.jl `ROCVector`), through `mul_device!`.
 """
 module HierarchicalMatricesAMD
 
 using LinearAlgebra
+import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, blocksize
 
 const libhmx = joinpath(@__DIR__, "..", "hmx", "libhmx.so")
 const libhip = "libamdhip64"
 
-export DeviceKernelMatrix, cauchykernel, coulombkernel, coulombprimekernel, logkernel,
-       mul_device!
+export DeviceKernelMatrix, DeviceHierarchicalMatrix, cauchykernel, coulombkernel,
+       coulombprimekernel, logkernel, mul_device!
 
 # kernels of examples/Kernel.jl:34-37, known to the device code by id
 @inline cauchykernel(x::T, y::T) where T = inv(x - y)
@@ -31,8 +36,13 @@ kernel_id(::typeof(coulombprimekernel)) = Cint(2)
 kernel_id(::typeof(logkernel)) = Cint(3)
 kernel_id(f) = throw(ArgumentError("libhmx knows cauchykernel, coulombkernel, coulombprimekernel and logkernel; got $f"))
 
+# plan and product flags of include/hmx.h
 const HM_PLAN_MATRIX_FREE = Cuint(1)
 const HM_PLAN_HYBRID = Cuint(8)
+const HM_MV_FAR_F32 = Cuint(8)
+const HM_MV_FAR_BF16X3 = Cuint(16)
+const HM_MV_MFMA = Cuint(32)
+const HM_MV_NO_MFMA = Cuint(64)
 
 function check(st::Cint)
     st == 0 && return nothing
@@ -44,77 +54,111 @@ BLOCKRANK(::Type{Float64}) = Int(ccall((:hm_blockrank, libhmx), Cint, (Cint,), 5
 BLOCKRANK(::Type{Float32}) = Int(ccall((:hm_blockrank, libhmx), Cint, (Cint,), 23))
 BLOCKSIZE(T) = 4BLOCKRANK(T)
 
-mutable struct DeviceKernelMatrix <: AbstractMatrix{Float64}
+destroy!(h) = (h.plan != C_NULL && ccall((:hm_plan_destroy, libhmx), Cint, (Ptr{Cvoid},), h.plan); h.plan = C_NULL)
+
+mutable struct DeviceKernelMatrix{T<:Union{Float32,Float64}} <: AbstractMatrix{T}
     plan::Ptr{Cvoid}
     m::Int
     n::Int
-    function DeviceKernelMatrix(plan, m, n)
-        H = new(plan, m, n)
-        finalizer(h -> (h.plan != C_NULL && ccall((:hm_plan_destroy, libhmx), Cint, (Ptr{Cvoid},), h.plan); h.plan = C_NULL), H)
+    function DeviceKernelMatrix{T}(plan, m, n) where T
+        H = new{T}(plan, m, n)
+        finalizer(destroy!, H)
         H
     end
 end
 
 Base.size(H::DeviceKernelMatrix) = (H.m, H.n)
 
+mutable struct DeviceHierarchicalMatrix <: AbstractMatrix{Float64}
+    plan::Ptr{Cvoid}
+    m::Int
+    n::Int
+    function DeviceHierarchicalMatrix(plan, m, n)
+        H = new(plan, m, n)
+        finalizer(destroy!, H)
+        H
+    end
+end
+
+Base.size(H::DeviceHierarchicalMatrix) = (H.m, H.n)
+
 """
     DeviceKernelMatrix(f, x, y, a, b, c, d; blocksize, rank, mode=:stored)
 
 Same arguments as `KernelMatrix(f, x, y, a, b, c, d)` (src/KernelMatrix.jl:47):
-x and y non-increasing, intervals given as (upper, lower). `mode` is `:stored`,
-`:hybrid` or `:free`.
+x and y non-increasing `Vector{T}` (T = Float64 or Float32), intervals given as
+(upper, lower). `mode` is `:stored`, `:hybrid` or `:free`. A Float32 matrix
+splits at Float32 midpoints and forms and applies every factor in Float32, as
+the generic reference code does for T = Float32.
 """
-function DeviceKernelMatrix(f::Function, x::Vector{Float64}, y::Vector{Float64},
-                            a::Float64, b::Float64, c::Float64, d::Float64;
-                            blocksize::Int = BLOCKSIZE(Float64), rank::Int = BLOCKRANK(Float64),
-                            mode::Symbol = :stored)
+function DeviceKernelMatrix(f::Function, x::Vector{T}, y::Vector{T}, a::T, b::T, c::T, d::T;
+                            blocksize::Int = BLOCKSIZE(T), rank::Int = BLOCKRANK(T),
+                            mode::Symbol = :stored) where T<:Union{Float32,Float64}
     flags = mode === :free ? HM_PLAN_MATRIX_FREE : mode === :hybrid ? HM_PLAN_HYBRID : Cuint(0)
     plan = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve x y begin
-        check(ccall((:hm_plan_from_points, libhmx), Cint,
-                    (Cint, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64, Float64, Float64,
-                     Float64, Cint, Cint, Cuint, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
-                    kernel_id(f), x, length(x), y, length(y), a, b, c, d, blocksize, rank,
-                    flags, C_NULL, plan))
+        # ccall needs a constant function name: one call per element type
+        st = if T === Float64
+            ccall((:hm_plan_from_points, libhmx), Cint,
+                  (Cint, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64, Float64, Float64,
+                   Float64, Cint, Cint, Cuint, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                  kernel_id(f), x, length(x), y, length(y), a, b, c, d, blocksize, rank, flags,
+                  C_NULL, plan)
+        else
+            ccall((:hm_plan_from_points_f32, libhmx), Cint,
+                  (Cint, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Float32, Float32, Float32,
+                   Float32, Cint, Cint, Cuint, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                  kernel_id(f), x, length(x), y, length(y), a, b, c, d, blocksize, rank, flags,
+                  C_NULL, plan)
+        end
+        check(st)
     end
-    DeviceKernelMatrix(plan[], length(x), length(y))
+    DeviceKernelMatrix{T}(plan[], length(x), length(y))
 end
 
 """
-    mul_device!(u_dev, H, v_dev, istart, jstart; INCX=1, INCY=1, beta=1)
+    mul_device!(u_dev, H, v_dev, istart, jstart; INCX=1, INCY=1, beta=1, k=1, ldu=0, ldv=0, flags=0)
 
-u[istart + (i-1)INCY] = beta*u[...] + (H v)_i on device pointers (1-based offsets,
-as mul!(u, H, v, istart, jstart), src/KernelMatrix.jl:17).
+u[istart + (i-1)INCY + q ldu] = beta*u[...] + (H v[:, q])_i for q < k on device
+pointers (1-based offsets as mul!(u, H, v, istart, jstart), src/KernelMatrix.jl:17).
+`flags` takes the HM_MV_* values (reduced-precision far field, MFMA passes).
 """
-function mul_device!(u::Ptr{Float64}, H::DeviceKernelMatrix, v::Ptr{Float64}, istart::Int,
-                     jstart::Int; INCX::Int = 1, INCY::Int = 1, beta::Int = 1,
-                     stream::Ptr{Cvoid} = C_NULL)
-    check(ccall((:hm_matvec, libhmx), Cint,
-                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Int64, Int64, Int64,
-                 Int64, Int64, Cint, Cuint, Ptr{Cvoid}),
-                H.plan, u, v, istart - 1, jstart - 1, INCY, INCX, 1, 0, 0, beta, 0, stream))
+function mul_device!(u::Ptr{T}, H::Union{DeviceKernelMatrix{T},DeviceHierarchicalMatrix},
+                     v::Ptr{T}, istart::Int, jstart::Int; INCX::Int = 1, INCY::Int = 1,
+                     beta::Int = 1, k::Int = 1, ldu::Int = 0, ldv::Int = 0, flags::Cuint = Cuint(0),
+                     stream::Ptr{Cvoid} = C_NULL) where T
+    st = if T === Float64
+        ccall((:hm_matvec, libhmx), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Int64, Int64, Int64, Int64,
+               Int64, Cint, Cuint, Ptr{Cvoid}),
+              H.plan, u, v, istart - 1, jstart - 1, INCY, INCX, k, ldu, ldv, beta, flags, stream)
+    else
+        ccall((:hm_matvec_f32, libhmx), Cint,
+              (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Int64, Int64, Int64, Int64, Int64, Int64,
+               Int64, Cint, Cuint, Ptr{Cvoid}),
+              H.plan, u, v, istart - 1, jstart - 1, INCY, INCX, k, ldu, ldv, beta, flags, stream)
+    end
+    check(st)
     u
 end
 
 # host vectors: staged through device buffers
-function devbuf(n)
+function devbuf(nbytes)
     p = Ref{Ptr{Cvoid}}(C_NULL)
-    st = ccall((:hipMalloc, libhip), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, 8n)
+    st = ccall((:hipMalloc, libhip), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, nbytes)
     st == 0 || throw(OutOfMemoryError())
-    Ptr{Float64}(p[])
+    p[]
 end
 devfree(p) = ccall((:hipFree, libhip), Cint, (Ptr{Cvoid},), p)
-h2d(dst, src::Vector{Float64}) = ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), dst, src, 8length(src), 1)
-d2h(dst::Vector{Float64}, src) = ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), dst, src, 8length(dst), 2)
+h2d(dst, src::Vector{T}) where T = ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), dst, src, sizeof(T) * length(src), 1)
+d2h(dst::Vector{T}, src) where T = ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), dst, src, sizeof(T) * length(dst), 2)
 
-# mul!(u, H, v, istart, jstart): u += H v (src/KernelMatrix.jl:17-45)
-function LinearAlgebra.mul!(u::Vector{Float64}, H::DeviceKernelMatrix, v::AbstractVector{Float64},
-                            istart::Int, jstart::Int)
-    vv = Vector{Float64}(v)
-    du, dv = devbuf(length(u)), devbuf(length(vv))
+function staged_mul!(u::Vector{T}, H, v::AbstractVector{T}, istart, jstart, INCX, INCY) where T
+    vv = Vector{T}(v)
+    du, dv = devbuf(sizeof(T) * length(u)), devbuf(sizeof(T) * length(vv))
     try
         h2d(du, u); h2d(dv, vv)
-        mul_device!(du, H, dv, istart, jstart)
+        mul_device!(Ptr{T}(du), H, Ptr{T}(dv), istart, jstart; INCX = INCX, INCY = INCY)
         d2h(u, du)
     finally
         devfree(du); devfree(dv)
@@ -122,11 +166,108 @@ function LinearAlgebra.mul!(u::Vector{Float64}, H::DeviceKernelMatrix, v::Abstra
     u
 end
 
+# mul!(u, H, v, istart, jstart): u += H v (src/KernelMatrix.jl:17-45)
+LinearAlgebra.mul!(u::Vector{T}, H::DeviceKernelMatrix{T}, v::AbstractVector{T}, istart::Int,
+                   jstart::Int) where T = staged_mul!(u, H, v, istart, jstart, 1, 1)
+
 # LinearAlgebra.mul!(u, H, v) = mul!(u, H, v, 1, 1), src/KernelMatrix.jl:14
-LinearAlgebra.mul!(u::Vector{Float64}, H::DeviceKernelMatrix, v::AbstractVector{Float64}) =
+LinearAlgebra.mul!(u::Vector{T}, H::DeviceKernelMatrix{T}, v::AbstractVector{T}) where T =
     mul!(u, H, v, 1, 1)
 
 # H * x, src/KernelMatrix.jl:5-8
-Base.:*(H::DeviceKernelMatrix, x::AbstractVector{Float64}) = mul!(zeros(Float64, size(H, 1)), H, x)
+Base.:*(H::DeviceKernelMatrix{T}, x::AbstractVector{T}) where T = mul!(zeros(T, size(H, 1)), H, x)
+
+# ------------------------------------------------- HierarchicalMatrix leaves
+
+# the leaves in the traversal order of src/HierarchicalMatrix.jl:24-52 with
+# their absolute (0-based) offsets
+function leaves!(out, H::HierarchicalMatrix, i0, j0)
+    M, N = blocksize(H)
+    p = 0
+    for m = 1:M
+        q = 0
+        for n = 1:N
+            code = H.assigned[m, n]
+            if code == 1
+                leaves!(out, H.HierarchicalMatrixblocks[m, n], i0 + p, j0 + q)
+            elseif code == 2
+                push!(out, (H.LowRankMatrixblocks[m, n], i0 + p, j0 + q))
+            elseif code == 3
+                push!(out, (H.Matrixblocks[m, n], i0 + p, j0 + q))
+            end
+            q += blocksize(H, 1, n, 2)
+        end
+        p += blocksize(H, m, N, 1)
+    end
+    out
+end
+
+"""
+    DeviceHierarchicalMatrix(H::HierarchicalMatrix{Float64})
+
+Flattens H (LowRankMatrix and Matrix leaves, nested HierarchicalMatrix nodes)
+into one device plan (`hm_plan_from_blocks`); LowRank ranks up to 64.
+"""
+function DeviceHierarchicalMatrix(H::HierarchicalMatrix{Float64})
+    L = leaves!(Any[], H, 0, 0)
+    nl = length(L)
+    kind = Vector{Int32}(undef, nl)
+    row0, col0, m, n, rk = (Vector{Int64}(undef, nl) for _ in 1:5)
+    A, U, S, V = (fill(Ptr{Float64}(C_NULL), nl) for _ in 1:4)
+    keep = Any[]
+    for (t, (c, i0, j0)) in enumerate(L)
+        row0[t], col0[t] = i0, j0
+        m[t], n[t] = size(c)
+        if c isa LowRankMatrix
+            kind[t] = 2
+            rk[t] = length(c.Σ.diag)
+            u, s, v = Matrix{Float64}(c.U), Vector{Float64}(c.Σ.diag), Matrix{Float64}(c.V)
+            push!(keep, u, s, v)
+            U[t], S[t], V[t] = pointer(u), pointer(s), pointer(v)
+        else
+            kind[t] = 3
+            rk[t] = 0
+            a = Matrix{Float64}(c)
+            push!(keep, a)
+            A[t] = pointer(a)
+        end
+    end
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve keep begin
+        check(ccall((:hm_plan_from_blocks, libhmx), Cint,
+                    (Int64, Int64, Int64, Ptr{Int32}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                     Ptr{Int64}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}},
+                     Ptr{Ptr{Float64}}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                    size(H, 1), size(H, 2), nl, kind, row0, col0, m, n, rk, A, U, S, V, C_NULL, plan))
+    end
+    DeviceHierarchicalMatrix(plan[], size(H, 1), size(H, 2))
+end
+
+# mul!(y, H, x, istart, jstart, INCX, INCY), src/HierarchicalMatrix.jl:24-52
+LinearAlgebra.mul!(y::Vector{Float64}, H::DeviceHierarchicalMatrix, x::AbstractVector{Float64},
+                   istart::Int, jstart::Int, INCX::Int = 1, INCY::Int = 1) =
+    staged_mul!(y, H, x, istart, jstart, INCX, INCY)
+
+LinearAlgebra.mul!(y::Vector{Float64}, H::DeviceHierarchicalMatrix, x::AbstractVector{Float64}) =
+    mul!(y, H, x, 1, 1)
+
+Base.:*(H::DeviceHierarchicalMatrix, x::AbstractVector{Float64}) = mul!(zeros(size(H, 1)), H, x)
+
+# rmul!(H, Diagonal(b)) and lmul!(Diagonal(b), H), src/HierarchicalMatrix.jl:15-16,
+# applied to the device factors in place (hm_plan_scale)
+function device_scale!(H::DeviceHierarchicalMatrix, b::Vector{Float64}, side::Integer)
+    db = devbuf(8length(b))
+    try
+        h2d(db, b)
+        check(ccall((:hm_plan_scale, libhmx), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+                    H.plan, side, db, 0, C_NULL))
+        ccall((:hipDeviceSynchronize, libhip), Cint, ())
+    finally
+        devfree(db)
+    end
+    H
+end
+LinearAlgebra.rmul!(H::DeviceHierarchicalMatrix, b::Diagonal{Float64}) = device_scale!(H, b.diag, 0)
+LinearAlgebra.lmul!(b::Diagonal{Float64}, H::DeviceHierarchicalMatrix) = device_scale!(H, b.diag, 1)
 
 end # module
