@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-n-log2", type=int, default=17)
+    ap.add_argument("--cpu-n-log2", type=int, default=18)
     ap.add_argument("--variants", default=None,
                     help="headline: extra variants timed beside the stored product (default "
                          "hybrid,matrix_free,f32_stored,f32_hybrid); batched: plan modes to time "
@@ -43,9 +43,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "batched", "gradient", "scale"],
+                    choices=["headline", "batched", "gradient", "scale", "even"],
                     help="headline = the n=2^20 product (the bench line); the others time "
-                         "the batched, gradient and scale workloads")
+                         "the batched, gradient, scale and EvenBarycentricMatrix workloads")
     args = ap.parse_args()
     args.variants_given = args.variants is not None
     if args.variants is None:
@@ -115,10 +115,12 @@ def stream_peaks(stream):
             "read_gbs": round(8 * n / (ms_sum * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(threads, n_log2):
+def cpu_baseline(threads, n_log2, min_seconds=10.0):
     """Oracle traversal mul! (ref layout, bit-identical to the single-thread
     restatement) on the n = 2^n_log2 instance of the same family; matvecs/s
-    scaled to n = 2^20 by the ratio of reference-layout bytes."""
+    scaled to n = 2^20 by the ratio of reference-layout bytes. The OpenMP
+    product is repeated until min_seconds of timed work and the median run
+    is reported; the single-thread figure is the median of 3 runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import hmx
@@ -132,14 +134,17 @@ def cpu_baseline(threads, n_log2):
     b_sample = ref_bytes(st, n)
     t.matvec_omp(w["b"], nthreads=threads)
     runs = []
-    for _ in range(5):
+    while sum(runs) < min_seconds and len(runs) < 2000:
         t0 = time.perf_counter()
         t.matvec_omp(w["b"], nthreads=threads)
         runs.append(time.perf_counter() - t0)
     sec = float(np.median(runs))
-    t0 = time.perf_counter()
-    t.matvec(w["b"])
-    sec1 = time.perf_counter() - t0
+    runs1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        t.matvec(w["b"])
+        runs1.append(time.perf_counter() - t0)
+    sec1 = float(np.median(runs1))
     del t
     big = hmx.KernelMatrix("cauchykernel", *hmx.workloads.shifted_grid(N)[:2],
                            *hmx.workloads.shifted_grid(N)[2], blocksize=BS, rank=RANK,
@@ -149,7 +154,8 @@ def cpu_baseline(threads, n_log2):
     return dict(value=(1.0 / sec) * scale, unit="matvecs/s", cores=threads, kind="port",
                 sample=(f"oracle traversal mul! (OpenMP, {threads} threads, ref layout) on the "
                         f"n=2^{n_log2} instance of the same family (leaf 256, rank 24): median "
-                        f"{sec * 1e3:.1f} ms of 5; scaled to n=2^20 by ref-layout bytes "
+                        f"{sec * 1e3:.1f} ms of {len(runs)} runs ({sum(runs):.1f} s timed); "
+                        f"single thread median {sec1 * 1e3:.0f} ms of 3; scaled to n=2^20 by ref-layout bytes "
                         f"{b_sample}/{b_full}"),
                 single_thread_value=(1.0 / sec1) * scale)
 
@@ -358,6 +364,93 @@ def run_scale(args):
     return out
 
 
+def run_even(args):
+    """EvenBarycentricMatrix leaves (src/BarycentricMatrix.jl:5-46, mul!
+    src/algebra.jl:168-239): N = 2^20, the 2046 off-diagonal leaves of a
+    10-level dyadic partition, rank 20, Gaussian f; one step = the product of
+    every leaf with one vector (and with 4 vectors). Error against the dense
+    parity-masked product of f on 256 sampled rows (torch FP64 on the GPU);
+    CPU: the oracle's leaf mul! on the N = 2^16 instance, scaled by bytes."""
+    import numpy as np
+    import torch
+    import hmx
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    w = hmx.workloads.config_even()
+    N, n, levels = w["N"], w["rank"], w["levels"]
+    t0 = time.perf_counter()
+    specs = [(r0, c0, hmx.EvenLeaf(a, b, c, d, hmx.even.even_F(w["f"], a, b, c, d, n)))
+             for (r0, c0, a, b, c, d) in w["leaves"]]
+    t1 = time.perf_counter()
+    H = hmx.EvenBarycentricLeaves(N, N, specs)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    del specs
+    info = H.info
+    v = torch.from_numpy(w["v"]).to(dev)
+    u = torch.empty(N, dtype=torch.float64, device=dev)
+    ms = timed(lambda: H._call(u, v, 0, 0, 0, stream), args.steps, args.warmup, stream)
+    nbytes = H.bytes_per_product(1)
+    # dense check: row I pairs with columns J outside its finest diagonal block
+    # with I + J even
+    s = N >> levels
+    rows = torch.from_numpy(np.sort(np.random.default_rng(7).choice(N, 256, replace=False))).to(dev)
+    J = torch.arange(N, device=dev, dtype=torch.float64)
+    Ji = torch.arange(N, device=dev)
+    dense = torch.empty(256, dtype=torch.float64, device=dev)
+    for q in range(0, 256, 32):
+        I = rows[q:q + 32]
+        keep = (((I[:, None] + Ji[None, :]) % 2) == 0) & ((I[:, None] // s) != (Ji[None, :] // s))
+        M = torch.exp(-((I[:, None].double() - J[None, :]) / N) ** 2) * keep
+        dense[q:q + 32] = M @ v
+        del M, keep
+    err = float(torch.linalg.norm(u[rows] - dense) / torch.linalg.norm(dense))
+    V4 = torch.from_numpy(hmx.workloads.normal(22, 4 * N).reshape(4, N)).to(dev).T
+    U4 = torch.empty(4, N, dtype=torch.float64, device=dev).T
+    ms4 = timed(lambda: H._call(U4, V4, 0, 0, 0, stream), args.steps, args.warmup, stream)
+    err4 = float(torch.linalg.norm(U4[:, 0] - H.matvec(V4[:, 0].contiguous())) /
+                 torch.linalg.norm(U4[:, 0]))
+    out = {"metric": "EvenBarycentricMatrix leaf products/s, N=2^20, 2046 leaves, rank 20",
+           "value": round(1000.0 / ms, 3), "unit": "products/s", "ms_per_step": round(ms, 4),
+           "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+           "data": "synthetic (Gaussian f, v from splitmix64 seed 21)",
+           "config": {"workload": "even_barycentric", "N": N, "levels": levels, "rank": n,
+                      "leaves": info["nleaves"], "rhs": 1},
+           "bytes_per_product": nbytes, "hbm_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+           "plan": info, "f_eval_s": round(t1 - t0, 3), "build_s": round(t2 - t1, 3),
+           "rel_err_vs_dense_256rows": err,
+           "rhs4": {"ms_per_step": round(ms4, 4), "ms_per_rhs": round(ms4 / 4, 4),
+                    "rel_diff_vs_single": err4,
+                    "hbm_gbs": round(H.bytes_per_product(4) / (ms4 * 1e-3) / 1e9, 1)}}
+    H.close()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        Nc = 1 << 16
+        wc = hmx.workloads.config_even(N=Nc, levels=6)
+        facs = []
+        for (r0, c0, a, b, c, d) in wc["leaves"]:
+            _, W = O.even_weights(a, b, n)
+            facs.append((r0, c0, W, hmx.even.even_F(wc["f"], a, b, c, d, n)))
+        cbytes = 8 * sum(W.size + F.size for _, _, W, F in facs) + 8 * 3 * Nc
+        runs = []
+        while sum(runs) < 10.0 and len(runs) < 500:
+            uc = np.zeros(Nc)
+            tc = time.perf_counter()
+            for r0, c0, W, F in facs:
+                O.even_mul(uc, W, F, wc["v"], r0 + 1, c0 + 1)
+            runs.append(time.perf_counter() - tc)
+        sec = float(np.median(runs))
+        out["cpu_baseline"] = {
+            "value": round((cbytes / sec) / nbytes, 4), "unit": "products/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"oracle leaf mul! (src/algebra.jl:168-239 restated, one thread) over the "
+                       f"{len(facs)} leaves of the N=2^16, 6-level instance: median "
+                       f"{sec * 1e3:.1f} ms of {len(runs)}; scaled to N=2^20 by factor bytes "
+                       f"{cbytes}/{nbytes}")}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -380,7 +473,8 @@ def main():
         return
 
     if args.workload != "headline":
-        fn = {"batched": run_batched, "gradient": run_gradient, "scale": run_scale}[args.workload]
+        fn = {"batched": run_batched, "gradient": run_gradient, "scale": run_scale,
+              "even": run_even}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -375,6 +375,12 @@ struct PlanBufs {
 
 }  // namespace
 
+// the thread-local message of hm_last_error, for the other host files
+int hmx_set_error(int code, const char *msg) {
+    g_err = msg;
+    return code;
+}
+
 struct hm_plan {
     int kid = 0, r = 0, bs = 0;
     unsigned flags = 0;

@@ -10,6 +10,7 @@ from .kernelmatrix import (  # noqa: F401
     coulombprimekernel, indsplit, logkernel, mul_, size,
 )
 from .hierarchical import HierarchicalMatrix, LowRankMatrix, add_col_, lmul_, rmul_  # noqa: F401
+from .even import EvenBarycentricLeaves, EvenBarycentricMatrix, EvenLeaf, barycentricmatrix  # noqa: F401
 from . import workloads  # noqa: F401
 from . import autodiff  # noqa: F401
 
@@ -18,4 +19,5 @@ __all__ = [
     "cauchykernel", "coulombkernel", "coulombprimekernel", "logkernel",
     "chebyshevpoints", "chebyshevbarycentricweights", "indsplit", "mul_", "workloads",
     "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch", "rmul_", "lmul_", "add_col_",
+    "EvenBarycentricMatrix", "EvenBarycentricLeaves", "barycentricmatrix",
 ]

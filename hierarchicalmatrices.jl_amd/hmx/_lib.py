@@ -147,6 +147,23 @@ def _load():
     L.hm_graph_matvec_f32.argtypes = L.hm_graph_matvec.argtypes
     L.hm_entries_f32.restype = ctypes.c_int
     L.hm_entries_f32.argtypes = L.hm_entries.argtypes
+    # EvenBarycentricMatrix leaves
+    L.hm_even_nodes.restype = ctypes.c_int
+    L.hm_even_nodes.argtypes = [_i64, _i64, ctypes.c_int, _dp]
+    L.hm_even_weights.restype = ctypes.c_int
+    L.hm_even_weights.argtypes = [_i64, _i64, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    L.hm_even_create.restype = ctypes.c_int
+    L.hm_even_create.argtypes = [_i64, _i64, _i64, _ip, _ip, _ip, ctypes.c_int,
+                                 ctypes.POINTER(_dp), _vp, ctypes.POINTER(_vp)]
+    L.hm_even_matvec.restype = ctypes.c_int
+    L.hm_even_matvec.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                 ctypes.c_int, _vp]
+    L.hm_even_leaf_data.restype = ctypes.c_int
+    L.hm_even_leaf_data.argtypes = [_vp, _i64, _dp, _dp]
+    L.hm_even_info.restype = ctypes.c_int
+    L.hm_even_info.argtypes = [_vp, _ip, ctypes.c_int]
+    L.hm_even_destroy.restype = ctypes.c_int
+    L.hm_even_destroy.argtypes = [_vp]
     return L
 
 

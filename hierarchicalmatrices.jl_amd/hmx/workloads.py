@@ -95,3 +95,34 @@ def config_scale(n=1 << 22, k=8, blocksize=512, rank=32, seed=1):
     x, y, iv = sorted_uniform_points(n, seed)
     return dict(kernel="logkernel", x=x, y=y, intervals=iv, blocksize=blocksize, rank=rank,
                 b=normal(seed + 5, n * k).reshape(n, k), k=k)
+
+
+def even_hodlr_leaves(N, levels):
+    """Off-diagonal blocks of an N x N matrix split dyadically: at level l the
+    blocks (2s, 2s+1) and (2s+1, 2s) of size N / 2^l, l = 1..levels. Each is
+    (row0, col0, a, b, c, d), 0-based placement and the block's 1-based global
+    row and column indices as its integer grids."""
+    out = []
+    for l in range(1, levels + 1):
+        s = N >> l
+        for q in range(0, 1 << l, 2):
+            for bi, bj in ((q, q + 1), (q + 1, q)):
+                r0, c0 = bi * s, bj * s
+                out.append((r0, c0, r0 + 1, r0 + s, c0 + 1, c0 + s))
+    return out
+
+
+def gaussian_even_kernel(N):
+    """f(T, t, j) = exp(-((t - j)/N)^2): entire, so rank-20 interpolation on
+    any sub-interval of 1..N is accurate to round-off."""
+    def f(T, t, j):
+        return np.exp(-((t - j) / N) ** 2)
+    return f
+
+
+def config_even(N=1 << 20, levels=10, rank=20, seed=0):
+    """EvenBarycentricMatrix leaves of an N x N off-diagonal dyadic partition
+    (2046 leaves at N = 2^20, levels 10: 1024-point blocks at the finest
+    level), rank 20 = BLOCKRANK(Float64), Gaussian f, v ~ N(0, 1)."""
+    return dict(N=N, levels=levels, rank=rank, leaves=even_hodlr_leaves(N, levels),
+                f=gaussian_even_kernel(N), v=normal(seed + 21, N))

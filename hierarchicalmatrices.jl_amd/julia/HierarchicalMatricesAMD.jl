@@ -9,6 +9,8 @@ Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
     `Matrix` leaves (src/HierarchicalMatrix.jl) flattened into a device plan
     (`hm_plan_from_blocks`), with `mul!` and the diagonal scalings `rmul!` /
     `lmul!` (src/HierarchicalMatrix.jl:15-16).
+  * `DeviceEvenBarycentric`: EvenBarycentricMatrix leaves (src/BarycentricMatrix.jl:5-46)
+    in one device plan, with their `mul!` (src/algebra.jl:168-239).
 
 Vectors may be host vectors (staged through device buffers) or device pointers
 (e.g. `pointer` of an AMDGPU.jl `ROCVector`), through `mul_device!`.
@@ -21,7 +23,7 @@ import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, blocksize
 const libhmx = joinpath(@__DIR__, "..", "hmx", "libhmx.so")
 const libhip = "libamdhip64"
 
-export DeviceKernelMatrix, DeviceHierarchicalMatrix, cauchykernel, coulombkernel,
+export DeviceKernelMatrix, DeviceHierarchicalMatrix, DeviceEvenBarycentric, cauchykernel, coulombkernel,
        coulombprimekernel, logkernel, mul_device!
 
 # kernels of examples/Kernel.jl:34-37, known to the device code by id
@@ -269,5 +271,74 @@ function device_scale!(H::DeviceHierarchicalMatrix, b::Vector{Float64}, side::In
 end
 LinearAlgebra.rmul!(H::DeviceHierarchicalMatrix, b::Diagonal{Float64}) = device_scale!(H, b.diag, 0)
 LinearAlgebra.lmul!(b::Diagonal{Float64}, H::DeviceHierarchicalMatrix) = device_scale!(H, b.diag, 1)
+
+"""
+    DeviceEvenBarycentric(leaves, nrows, ncols)
+
+A device plan of `EvenBarycentricMatrix{Float64}` leaves (src/BarycentricMatrix.jl:5-46)
+placed at 0-based (row0, col0) in an nrows x ncols matrix: `leaves` is a vector
+of `(row0, col0, B)`. F comes from each B (the caller's f, evaluated by the
+reference's constructor); W is formed on the device. A single leaf at (0, 0)
+gives the leaf's own `mul!(u, B, v, istart, jstart)`.
+"""
+mutable struct DeviceEvenBarycentric
+    plan::Ptr{Cvoid}
+    m::Int
+    n::Int
+end
+Base.size(E::DeviceEvenBarycentric) = (E.m, E.n)
+
+function DeviceEvenBarycentric(leaves::AbstractVector, nrows::Integer, ncols::Integer)
+    nl = length(leaves)
+    nl > 0 || throw(ArgumentError("no leaves"))
+    rank = length(leaves[1][3].x)
+    row0 = Int64[l[1] for l in leaves]
+    col0 = Int64[l[2] for l in leaves]
+    abcd = Int64[]
+    Fs = Matrix{Float64}[]
+    for (_, _, B) in leaves
+        length(B.x) == rank || throw(ArgumentError("all leaves must have one rank"))
+        append!(abcd, (B.a, B.b, B.c, B.d))
+        push!(Fs, Matrix{Float64}(B.F))
+    end
+    Fp = [pointer(F) for F in Fs]
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve Fs begin
+        check(ccall((:hm_even_create, libhmx), Cint,
+                    (Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Cint,
+                     Ptr{Ptr{Float64}}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                    nrows, ncols, nl, row0, col0, abcd, rank, Fp, C_NULL, plan))
+    end
+    E = DeviceEvenBarycentric(plan[], nrows, ncols)
+    finalizer(e -> (e.plan != C_NULL && ccall((:hm_even_destroy, libhmx), Cint, (Ptr{Cvoid},), e.plan);
+                    e.plan = C_NULL), E)
+    E
+end
+
+# mul!(u, B, v, istart, jstart), src/algebra.jl:168-239, for every leaf at once:
+# row I pairs with column J when (istart-1+I) + (jstart-1+J) is even
+function mul_device!(u::Ptr{Float64}, E::DeviceEvenBarycentric, v::Ptr{Float64}, istart::Int,
+                     jstart::Int; beta::Int = 1, k::Int = 1, ldu::Int = 0, ldv::Int = 0,
+                     stream::Ptr{Cvoid} = C_NULL)
+    check(ccall((:hm_even_matvec, libhmx), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Int64, Int64, Int64, Int64,
+                 Int64, Cint, Ptr{Cvoid}),
+                E.plan, u, v, istart - 1, jstart - 1, 1, 1, k, ldu, ldv, beta, stream))
+    u
+end
+
+function LinearAlgebra.mul!(u::Vector{Float64}, E::DeviceEvenBarycentric, v::AbstractVector{Float64},
+                            istart::Int = 1, jstart::Int = 1)
+    vv = Vector{Float64}(v)
+    du, dv = devbuf(8length(u)), devbuf(8length(vv))
+    try
+        h2d(du, u); h2d(dv, vv)
+        mul_device!(Ptr{Float64}(du), E, Ptr{Float64}(dv), istart, jstart)
+        d2h(u, du)
+    finally
+        devfree(du); devfree(dv)
+    end
+    u
+end
 
 end # module

@@ -283,6 +283,57 @@ int hm_graph_matvec_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, 
 int hm_entries_f32(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j, float *out,
                    void *stream);
 
+/* ------------------------------------------------ EvenBarycentricMatrix
+ * EvenBarycentricMatrix(T, f, a, b, c, d) (src/BarycentricMatrix.jl:5-46) is an
+ * m x p block, m = b-a+1 rows on the integer grid a..b and p = d-c+1 columns
+ * on c..d, of rank n: W (n x m) interpolates in the row index at the n
+ * first-kind Chebyshev nodes t_k = (a+b)/2 + (b-a)*x_k/2, and F (p x n) holds
+ * f(T, t_k, j). f is the caller's function, so F comes from the caller; W is
+ * formed on the device. Float64, ranks 1..64. */
+typedef struct hm_even *hm_even_t;
+
+/* t_k, k = 1..n, the arguments at which F[j, k] = f(T, t_k, j) is evaluated
+ * (src/BarycentricMatrix.jl:41). Host output. */
+int hm_even_nodes(int64_t a, int64_t b, int n, double *t);
+
+/* W of the grid a..b into the DEVICE array W, and optionally the sums
+ * w (m values, device, NULL to skip): layout 0 stores W[k + n*(i-a)], the n x m
+ * W of EvenBarycentricMatrix (src/BarycentricMatrix.jl:23-37); layout 1 stores
+ * W[(i-a) + m*k], the m x n W of barycentricmatrix (src/BarycentricMatrix.jl:
+ * 61-89, whose LowRankMatrix(W, I, F) then goes to hm_plan_from_blocks).
+ * Synchronizes `stream`. */
+int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *w, void *stream);
+
+/* A plan of nleaves EvenBarycentricMatrix leaves of rank n placed in one
+ * nrows x ncols matrix: leaf t spans rows row0[t] .. row0[t] + m - 1 and
+ * columns col0[t] .. col0[t] + p - 1, with grids abcd[4t .. 4t+3] = (a, b, c, d)
+ * and F[t] (host, p x n column-major). Leaves may overlap; products add
+ * them all. W of every leaf is formed on the device. */
+int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
+                   const int64_t *col0, const int64_t *abcd, int n, const double *const *F,
+                   void *stream, hm_even_t *plan);
+
+/* Replaces mul!(u, B, v, istart, jstart) (src/algebra.jl:168-239) for every
+ * leaf at once: u[q*ldu + (ioff + I)*incu] (+)= sum over leaves and columns J
+ * with (ioff + I) + (joff + J) even of B_t(I - row0, J - col0) *
+ * v[q*ldv + (joff + J)*incv], for right-hand sides q < k. The parity rule is
+ * the reference's: its iseven(ishift + jshift) branches pair row i with
+ * column j exactly when ishift + i + jshift + j is even. beta = 1 accumulates
+ * (the reference's mul!), beta = 0 overwrites rows 0 .. nrows-1 of each
+ * column. u, v: device. */
+int hm_even_matvec(hm_even_t plan, double *u, const double *v, int64_t ioff, int64_t joff,
+                   int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
+                   void *stream);
+
+/* Host copies of leaf t's W (n x m) and F (p x n); either may be NULL. */
+int hm_even_leaf_data(hm_even_t plan, int64_t leaf, double *W, double *F);
+
+/* info[0..]: nrows, ncols, nleaves, rank, W entries, F entries, upward tasks,
+ * reduced leaves, downward tasks, segment-list entries. */
+enum { HM_EVEN_INFO_COUNT = 10 };
+int hm_even_info(hm_even_t plan, int64_t *info, int n);
+int hm_even_destroy(hm_even_t plan);
+
 #ifdef __cplusplus
 }
 #endif

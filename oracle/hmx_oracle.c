@@ -709,3 +709,111 @@ void or_dense_matvec_rows(int kernel_id, const or_real *x, const or_real *y, int
         out[k] = (or_real)acc;
     }
 }
+
+/* ------------------------------------------------ EvenBarycentricMatrix */
+
+/* src/BarycentricMatrix.jl:41: (a+b)/2 + (b-a)*x[k]/2 -- Int/Int is a Float64
+ * division; (b-a)*x[k] and /2 are in T; the sum promotes to Float64. */
+void or_even_nodes(int64_t a, int64_t b, int n, double *t) {
+    or_real *x = (or_real *)xmalloc(sizeof(or_real) * (size_t)n);
+    or_chebyshevpoints(x, n, 1);
+    double mid = (double)(a + b) / 2.0;
+    for (int k = 0; k < n; ++k) {
+        or_real h = R(b - a) * x[k];
+        h = h / R(2);
+        t[k] = mid + (double)h;
+    }
+    free(x);
+}
+
+/* 2i-a-b-(b-a)*x[k]: Int arithmetic, then Int - T with (b-a)*x[k] in T */
+static or_real even_den(int64_t i, int64_t a, int64_t b, or_real xk) {
+    return R(2 * i - a - b) - R(b - a) * xk;
+}
+
+/* src/BarycentricMatrix.jl:23-37 */
+void or_even_weights(int64_t a, int64_t b, int n, or_real *w, or_real *W) {
+    int64_t m = b - a + 1;
+    or_real *x = (or_real *)xmalloc(sizeof(or_real) * (size_t)n);
+    or_real *lam = (or_real *)xmalloc(sizeof(or_real) * (size_t)n);
+    or_chebyshevpoints(x, n, 1);
+    or_chebyshevbarycentricweights(lam, n, 1);
+    for (int64_t r = 0; r < m; ++r) w[r] = R(0);
+    for (int64_t i = a; i <= b; ++i)
+        for (int k = 0; k < n; ++k) w[i - a] += lam[k] * (R(1) / even_den(i, a, b, x[k]));
+    for (int64_t i = a; i <= b; ++i)
+        for (int k = 0; k < n; ++k)
+            W[k + (int64_t)n * (i - a)] = lam[k] * (R(1) / (even_den(i, a, b, x[k]) * w[i - a]));
+    free(x);
+    free(lam);
+}
+
+/* src/BarycentricMatrix.jl:62-75: the same w, W stored m x n */
+void or_barycentricmatrix_weights(int64_t a, int64_t b, int n, or_real *W) {
+    int64_t m = b - a + 1;
+    or_real *x = (or_real *)xmalloc(sizeof(or_real) * (size_t)n);
+    or_real *lam = (or_real *)xmalloc(sizeof(or_real) * (size_t)n);
+    or_real *w = (or_real *)xmalloc(sizeof(or_real) * (size_t)m);
+    or_chebyshevpoints(x, n, 1);
+    or_chebyshevbarycentricweights(lam, n, 1);
+    for (int64_t r = 0; r < m; ++r) w[r] = R(0);
+    for (int64_t i = a; i <= b; ++i)
+        for (int k = 0; k < n; ++k) w[i - a] += lam[k] * (R(1) / even_den(i, a, b, x[k]));
+    for (int k = 0; k < n; ++k)
+        for (int64_t i = a; i <= b; ++i)
+            W[(i - a) + m * k] = lam[k] * (R(1) / (even_den(i, a, b, x[k]) * w[i - a]));
+    free(x);
+    free(lam);
+    free(w);
+}
+
+/* src/BarycentricMatrix.jl:49-58 */
+or_real or_even_getindex(const or_real *W, const or_real *F, int64_t m, int64_t p, int n,
+                         int64_t i, int64_t j) {
+    or_real ret = R(0);
+    if (((m + p + i + j) & 1) == 0)
+        for (int k = 0; k < n; ++k) ret += F[(j - 1) + p * k] * W[k + (int64_t)n * (i - 1)];
+    return ret;
+}
+
+/* beta_k = sum over j = j1, j1+2, ... <= p of v[jshift+j] F[j,k]
+ * (src/algebra.jl:174-179 and its three twins) */
+static void even_beta(or_real *beta, const or_real *F, int64_t p, int n, const or_real *v,
+                      int64_t jshift, int64_t j1) {
+    for (int k = 0; k < n; ++k) {
+        or_real bk = R(0);
+        for (int64_t j = j1; j <= p; j += 2) bk += v[jshift + j - 1] * F[(j - 1) + p * k];
+        beta[k] = bk;
+    }
+}
+
+/* u[ishift+i] += sum_k beta_k W[k,i] for i = i1, i1+2, ... <= m
+ * (src/algebra.jl:181-187 and its three twins) */
+static void even_apply(or_real *u, const or_real *W, int64_t m, int n, const or_real *beta,
+                       int64_t ishift, int64_t i1) {
+    for (int64_t i = i1; i <= m; i += 2) {
+        or_real ui = R(0);
+        for (int k = 0; k < n; ++k) ui += beta[k] * W[k + (int64_t)n * (i - 1)];
+        u[ishift + i - 1] += ui;
+    }
+}
+
+/* src/algebra.jl:168-239 */
+void or_even_mul(or_real *u, const or_real *W, const or_real *F, int64_t m, int64_t p, int n,
+                 const or_real *v, int64_t istart, int64_t jstart) {
+    int64_t ishift = istart - 1, jshift = jstart - 1;
+    or_real *beta = (or_real *)calloc((size_t)(n > 0 ? n : 1), sizeof(or_real));
+    if (!beta) abort();
+    if (((ishift + jshift) & 1) == 0) {
+        even_beta(beta, F, p, n, v, jshift, 1);
+        even_apply(u, W, m, n, beta, ishift, 1);
+        even_beta(beta, F, p, n, v, jshift, 2);
+        even_apply(u, W, m, n, beta, ishift, 2);
+    } else {
+        even_beta(beta, F, p, n, v, jshift, 2);
+        even_apply(u, W, m, n, beta, ishift, 1);
+        even_beta(beta, F, p, n, v, jshift, 1);
+        even_apply(u, W, m, n, beta, ishift, 2);
+    }
+    free(beta);
+}

@@ -121,6 +121,38 @@ void or_dense_matvec_rows(int kernel_id, const or_real *x, const or_real *y, int
                           const or_real *v, const int64_t *rows, int64_t nrows,
                           or_real *out, int nthreads);
 
+/* ------------------------------------------------ EvenBarycentricMatrix
+ * src/BarycentricMatrix.jl:5-89 and its mul!, src/algebra.jl:168-239. The
+ * rows are the integer grid a..b (m = b-a+1), the columns c..d (p = d-c+1),
+ * n Chebyshev nodes. f is the caller's function: F (p x n, column-major,
+ * F[j,k] = f(T, t_k, j)) is an input here. */
+
+/* Nodes where the reference evaluates f: t_k = (a+b)/2 + (b-a)*x_k/2
+ * (src/BarycentricMatrix.jl:41). (a+b)/2 is an Int/Int division, which Julia
+ * carries out in Float64 for every T, so t is Float64 in both builds. */
+void or_even_nodes(int64_t a, int64_t b, int n, double *t);
+
+/* w (m) and W (n x m column-major, W[k + n*(i-a)]) of
+ * EvenBarycentricMatrix(T, f, a, b, c, d), src/BarycentricMatrix.jl:23-37:
+ * w_i = sum_k lambda_k * inv(2i-a-b-(b-a)x_k), W[k,i] = lambda_k *
+ * inv((2i-a-b-(b-a)x_k) * w_i). */
+void or_even_weights(int64_t a, int64_t b, int n, or_real *w, or_real *W);
+
+/* W of barycentricmatrix(T, f, a, b, c, d), src/BarycentricMatrix.jl:62-75:
+ * the same values, m x n column-major (W[(i-a) + m*k]). */
+void or_barycentricmatrix_weights(int64_t a, int64_t b, int n, or_real *W);
+
+/* getindex(B, i, j), src/BarycentricMatrix.jl:49-58 (1-based i, j): nonzero
+ * only when size(B,1)+size(B,2)+i+j is even. */
+or_real or_even_getindex(const or_real *W, const or_real *F, int64_t m, int64_t p, int n,
+                         int64_t i, int64_t j);
+
+/* mul!(u, B, v, istart, jstart), src/algebra.jl:168-239 (1-based istart,
+ * jstart): the row and column parity classes chosen by iseven(ishift+jshift),
+ * loop for loop. */
+void or_even_mul(or_real *u, const or_real *W, const or_real *F, int64_t m, int64_t p, int n,
+                 const or_real *v, int64_t istart, int64_t jstart);
+
 #ifdef __cplusplus
 }
 #endif

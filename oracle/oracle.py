@@ -73,6 +73,15 @@ def _setup(L, _dp, real):
                               ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64]
     L.or_dense_matvec_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _ip,
                                        ctypes.c_int64, _dp, ctypes.c_int]
+    L.or_even_nodes.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)]
+    L.or_even_weights.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _dp, _dp]
+    L.or_barycentricmatrix_weights.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _dp]
+    L.or_even_getindex.restype = real
+    L.or_even_getindex.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                   ctypes.c_int64, ctypes.c_int64]
+    L.or_even_mul.argtypes = [_dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _dp,
+                              ctypes.c_int64, ctypes.c_int64]
     return L
 
 
@@ -271,4 +280,49 @@ def bary_mul(u, U, F, V, v, istart, jstart):
     lib().or_bary_mul(_d(u), U.ctypes.data_as(_dp), F.ctypes.data_as(_dp),
                       V.ctypes.data_as(_dp), U.shape[0], V.shape[0], F.shape[0], _d(v),
                       istart, jstart)
+    return u
+
+
+# ---------------------------------------------------- EvenBarycentricMatrix
+
+def even_nodes(a, b, n, f32=False):
+    """t_k = (a+b)/2 + (b-a)*x_k/2 where f is evaluated (always Float64)."""
+    t = np.zeros(n, dtype=np.float64)
+    lib(f32).or_even_nodes(a, b, n, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return t
+
+
+def even_weights(a, b, n, f32=False):
+    """(w, W) of EvenBarycentricMatrix: w has b-a+1 entries, W is n x (b-a+1)."""
+    m = b - a + 1
+    w = np.zeros(m, dtype=_dt(f32))
+    W = np.zeros((n, m), dtype=_dt(f32), order="F")
+    lib(f32).or_even_weights(a, b, n, _d(w), _d(W))
+    return w, W
+
+
+def barycentricmatrix_weights(a, b, n, f32=False):
+    """W of barycentricmatrix: (b-a+1) x n."""
+    W = np.zeros((b - a + 1, n), dtype=_dt(f32), order="F")
+    lib(f32).or_barycentricmatrix_weights(a, b, n, _d(W))
+    return W
+
+
+def even_getindex(W, F, i, j):
+    """getindex(B, i, j) with 1-based i, j."""
+    f32 = W.dtype == np.float32
+    W = np.asfortranarray(W)
+    F = np.asfortranarray(F, dtype=W.dtype)
+    n, m = W.shape
+    return lib(f32).or_even_getindex(_d(W), _d(F), m, F.shape[0], n, i, j)
+
+
+def even_mul(u, W, F, v, istart=1, jstart=1):
+    """mul!(u, B, v, istart, jstart) of src/algebra.jl:168-239, in u's type."""
+    f32 = u.dtype == np.float32
+    W = np.asfortranarray(W, dtype=u.dtype)
+    F = np.asfortranarray(F, dtype=u.dtype)
+    v = np.ascontiguousarray(v, dtype=u.dtype)
+    n, m = W.shape
+    lib(f32).or_even_mul(_d(u), _d(W), _d(F), m, F.shape[0], n, _d(v), istart, jstart)
     return u
