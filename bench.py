@@ -43,9 +43,10 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "batched", "gradient", "scale", "even"],
+                    choices=["headline", "batched", "gradient", "scale", "even", "lowrank"],
                     help="headline = the n=2^20 product (the bench line); the others time "
-                         "the batched, gradient, scale and EvenBarycentricMatrix workloads")
+                         "the batched, gradient, scale, EvenBarycentricMatrix and LowRankMatrix "
+                         "workloads")
     args = ap.parse_args()
     args.variants_given = args.variants is not None
     if args.variants is None:
@@ -451,6 +452,93 @@ def run_even(args):
     return out
 
 
+def run_lowrank(args):
+    """LowRankMatrix recompression, L1 + L2 (src/LowRankMatrix.jl:95-102): 2048
+    sums of 512 x 512 rank-16 matrices in one hm_lowrank_op call (QR of the
+    two 512 x 32 stacked factors, SVD of the 32 x 32 core, getrank, Q times
+    the singular vectors). One step = the whole call, workspace and
+    synchronization included. Checked on 8 items against the dense sum of
+    the operands; CPU = the oracle's numpy restatement (LAPACK) on 64 items."""
+    import ctypes
+    import numpy as np
+    import torch
+    import hmx
+    from hmx._lib import check, lib
+    dev = torch.device("cuda:0")
+    count, m, r = 2048, 512, 16
+    g = torch.Generator(device=dev).manual_seed(0)
+    mk = lambda *shape: torch.randn(*shape, generator=g, dtype=torch.float64, device=dev)
+    U1, V1, U2, V2 = mk(count, r, m), mk(count, r, m), mk(count, r, m), mk(count, r, m)
+    S1 = torch.rand(count, r, generator=g, dtype=torch.float64, device=dev) + 0.1
+    S2 = torch.rand(count, r, generator=g, dtype=torch.float64, device=dev) + 0.1
+    Uo = torch.empty(count, 2 * r, m, dtype=torch.float64, device=dev)
+    Vo = torch.empty(count, 2 * r, m, dtype=torch.float64, device=dev)
+    So = torch.empty(count, 2 * r, dtype=torch.float64, device=dev)
+    dims = np.tile(np.array([m, m, 0, r, r], dtype=np.int64), count)
+    inp = (ctypes.c_void_p * (7 * count))()
+    outp = (ctypes.c_void_p * (3 * count))()
+    for t in range(count):
+        inp[7 * t:7 * t + 6] = [U1[t].data_ptr(), S1[t].data_ptr(), V1[t].data_ptr(),
+                                U2[t].data_ptr(), S2[t].data_ptr(), V2[t].data_ptr()]
+        outp[3 * t:3 * t + 3] = [Uo[t].data_ptr(), So[t].data_ptr(), Vo[t].data_ptr()]
+    rank = np.zeros(count, dtype=np.int64)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    call = lambda: check(lib.hm_lowrank_op(0, count, ip(dims), inp, outp, ip(rank), stream))
+    for _ in range(max(1, args.warmup)):
+        call()
+    steps = max(3, min(args.steps, 20))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    sec = (time.perf_counter() - t0) / steps
+    # check on 8 items: U diag(S) V^T against the dense sum of the operands
+    # (torch FP64); random factors give full rank 2r
+    errs = []
+    for t in range(0, count, count // 8):
+        rt = int(rank[t])
+        dense = lambda U, S, V: (U.T * S[None, :]) @ V
+        want = dense(U1[t], S1[t], V1[t]) + dense(U2[t], S2[t], V2[t])
+        got = dense(Uo[t][:rt], So[t][:rt], Vo[t][:rt])
+        errs.append(float(torch.linalg.norm(got - want) / torch.linalg.norm(want)))
+    host = lambda x, t: x[t].cpu().numpy()
+    # algorithmic bytes: read U1, V1, U2, V2 (m x r each) and write U, V (m x 2r)
+    nbytes = count * 8 * (4 * m * r + 2 * m * 2 * r)
+    # flops: two Householder QRs of m x 2r (4 m c^2 - 4/3 c^3 each, c = 2r), the
+    # core (c^3 / 3), two Q applications (4 m c^2 each)
+    c = 2 * r
+    flops = count * (2 * (4 * m * c * c - 4 * c ** 3 / 3) + c ** 3 / 3 + 2 * 4 * m * c * c)
+    out = {"metric": "LowRankMatrix sums recompressed/s (2048 x 512x512 rank 16+16)",
+           "value": round(count / sec, 1), "unit": "sums/s", "ms_per_step": round(sec * 1e3, 3),
+           "steps": steps, "warmup": args.warmup, "dtype": "f64",
+           "data": "synthetic (torch normal factors, seed 0)",
+           "config": {"workload": "lowrank_add", "count": count, "m": m, "n": m, "r1": r, "r2": r},
+           "ranks": sorted(set(int(x) for x in rank)),
+           "rel_err_vs_dense_sum_8items": max(errs),
+           "hbm_gbs_algorithmic": round(nbytes / sec / 1e9, 1),
+           "gflops_model": round(flops / sec / 1e9, 1)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import lowrank as LR
+        sample = 64
+        Ls = [((host(U1, t).T, host(S1, t), host(V1, t).T), (host(U2, t).T, host(S2, t),
+               host(V2, t).T)) for t in range(sample)]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            for L1, L2 in Ls:
+                LR.add(L1, L2)
+            reps += 1
+        per = (time.perf_counter() - t0) / (reps * sample)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / per, 1), "unit": "sums/s", "kind": "port",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None,
+            "sample": (f"oracle numpy restatement (numpy.linalg.qr/svd = LAPACK geqrf/gesdd, "
+                       f"as the reference's qr!/svd!) on {sample} of the items, {reps} passes "
+                       f"in {time.perf_counter() - t0:.1f} s")}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -474,7 +562,7 @@ def main():
 
     if args.workload != "headline":
         fn = {"batched": run_batched, "gradient": run_gradient, "scale": run_scale,
-              "even": run_even}[args.workload]
+              "even": run_even, "lowrank": run_lowrank}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -12,6 +12,7 @@ from .kernelmatrix import (  # noqa: F401
 from .hierarchical import HierarchicalMatrix, LowRankMatrix, add_col_, lmul_, rmul_  # noqa: F401
 from .even import EvenBarycentricLeaves, EvenBarycentricMatrix, EvenLeaf, barycentricmatrix  # noqa: F401
 from . import workloads  # noqa: F401
+from . import lowrank  # noqa: F401
 from . import autodiff  # noqa: F401
 
 __all__ = [

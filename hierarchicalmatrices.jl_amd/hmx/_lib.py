@@ -164,6 +164,10 @@ def _load():
     L.hm_even_info.argtypes = [_vp, _ip, ctypes.c_int]
     L.hm_even_destroy.restype = ctypes.c_int
     L.hm_even_destroy.argtypes = [_vp]
+    # LowRankMatrix algebra
+    L.hm_lowrank_op.restype = ctypes.c_int
+    L.hm_lowrank_op.argtypes = [ctypes.c_int, _i64, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                _ip, _vp]
     return L
 
 

@@ -34,6 +34,8 @@ from .kernelmatrix import Block, KernelMatrix, _stream_ptr
 class LowRankMatrix:
     """U * Diagonal(S) * V' (src/LowRankMatrix.jl:28-38)."""
 
+    __array_ufunc__ = None  # numpy defers `ndarray @ LowRankMatrix` to __rmatmul__
+
     def __init__(self, U, S, V):
         self.U = np.asfortranarray(U, dtype=np.float64)
         self.S = np.ascontiguousarray(np.diag(S) if np.ndim(S) == 2 else S, dtype=np.float64)
@@ -50,6 +52,29 @@ class LowRankMatrix:
 
     def rank(self):
         return len(self.S)
+
+    # recompressing arithmetic on the device, src/LowRankMatrix.jl:95-158
+    def __add__(self, o):
+        from . import lowrank
+        return lowrank.add(self, o) if isinstance(o, LowRankMatrix) else NotImplemented
+
+    def __sub__(self, o):
+        from . import lowrank
+        return lowrank.sub(self, o) if isinstance(o, LowRankMatrix) else NotImplemented
+
+    def __matmul__(self, o):
+        from . import lowrank
+        if isinstance(o, LowRankMatrix):
+            return lowrank.mul(self, o)
+        if isinstance(o, np.ndarray) and o.ndim == 2:
+            return lowrank.muld(self, o)
+        return NotImplemented
+
+    def __rmatmul__(self, o):
+        from . import lowrank
+        if isinstance(o, np.ndarray) and o.ndim == 2:
+            return lowrank.dmul(o, self)
+        return NotImplemented
 
     def __getitem__(self, ij):
         """Entry (i, j), 0-based; sum over k from r down to 1 as

@@ -11,6 +11,8 @@ Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
     `lmul!` (src/HierarchicalMatrix.jl:15-16).
   * `DeviceEvenBarycentric`: EvenBarycentricMatrix leaves (src/BarycentricMatrix.jl:5-46)
     in one device plan, with their `mul!` (src/algebra.jl:168-239).
+  * `device_lowrank`: LowRankMatrix `+`, `-`, `*`, `'*` with recompression
+    (src/LowRankMatrix.jl:95-126) on the device.
 
 Vectors may be host vectors (staged through device buffers) or device pointers
 (e.g. `pointer` of an AMDGPU.jl `ROCVector`), through `mul_device!`.
@@ -23,7 +25,7 @@ import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, blocksize
 const libhmx = joinpath(@__DIR__, "..", "hmx", "libhmx.so")
 const libhip = "libamdhip64"
 
-export DeviceKernelMatrix, DeviceHierarchicalMatrix, DeviceEvenBarycentric, cauchykernel, coulombkernel,
+export DeviceKernelMatrix, DeviceHierarchicalMatrix, DeviceEvenBarycentric, device_lowrank, cauchykernel, coulombkernel,
        coulombprimekernel, logkernel, mul_device!
 
 # kernels of examples/Kernel.jl:34-37, known to the device code by id
@@ -339,6 +341,47 @@ function LinearAlgebra.mul!(u::Vector{Float64}, E::DeviceEvenBarycentric, v::Abs
         devfree(du); devfree(dv)
     end
     u
+end
+
+# LowRankMatrix arithmetic with recompression (src/LowRankMatrix.jl:82-158) on
+# the device: hm_lowrank_op over any number of items; ops of include/hmx.h
+const HM_LR_ADD, HM_LR_SUB, HM_LR_MUL, HM_LR_ADJMUL, HM_LR_DMUL, HM_LR_MULD, HM_LR_SVDTRUNC = 0:6
+
+"""
+    device_lowrank(op, L1, L2) -> LowRankMatrix{Float64}
+
+`L1 + L2` (op = HM_LR_ADD), `L1 - L2` (HM_LR_SUB), `L1 * L2` (HM_LR_MUL) or
+`L1' * L2` (HM_LR_ADJMUL) of two LowRankMatrix{Float64}, computed on the GPU
+(thin QR, SVD of the core, getrank truncation, as the reference's methods).
+"""
+function device_lowrank(op::Integer, L1::LowRankMatrix{Float64}, L2::LowRankMatrix{Float64})
+    U1, S1, V1 = Matrix(L1.U), Vector(L1.Σ.diag), Matrix(L1.V)
+    U2, S2, V2 = Matrix(L2.U), Vector(L2.Σ.diag), Matrix(L2.V)
+    r1, r2 = length(S1), length(S2)
+    m, n, k = op == HM_LR_ADJMUL ? (size(V1, 1), size(V2, 1), size(U1, 1)) :
+              op == HM_LR_MUL ? (size(U1, 1), size(V2, 1), size(V1, 1)) :
+              (size(U1, 1), size(V1, 1), 0)
+    rmax = op in (HM_LR_ADD, HM_LR_SUB) ? r1 + r2 : min(r1, r2)
+    host = (U1, S1, V1, U2, S2, V2)
+    dins = [devbuf(sizeof(a)) for a in host]
+    douts = [devbuf(8m * rmax), devbuf(8rmax), devbuf(8n * rmax)]
+    rank = Int64[0]
+    try
+        for (d, a) in zip(dins, host)
+            h2d(d, vec(a))
+        end
+        ins = Ptr{Cvoid}[dins..., C_NULL]
+        check(ccall((:hm_lowrank_op, libhmx), Cint,
+                    (Cint, Int64, Ptr{Int64}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Cvoid}),
+                    op, 1, Int64[m, n, k, r1, r2], ins, douts, rank, C_NULL))
+        r = rank[1]
+        U, S, V = zeros(m * rmax), zeros(rmax), zeros(n * rmax)
+        d2h(U, douts[1]); d2h(S, douts[2]); d2h(V, douts[3])
+        return LowRankMatrix(reshape(U, m, rmax)[:, 1:r], Diagonal(S[1:r]), reshape(V, n, rmax)[:, 1:r])
+    finally
+        foreach(devfree, dins)
+        foreach(devfree, douts)
+    end
 end
 
 end # module

@@ -334,6 +334,38 @@ enum { HM_EVEN_INFO_COUNT = 10 };
 int hm_even_info(hm_even_t plan, int64_t *info, int n);
 int hm_even_destroy(hm_even_t plan);
 
+/* ------------------------------------------------ LowRankMatrix algebra
+ * Batched recompression arithmetic of src/LowRankMatrix.jl:64-171 on the
+ * device: `count` independent items of one operation. A LowRankMatrix is
+ * U diag(S) V^T with U m x r, V n x r (column-major, leading dimensions m and
+ * n). As in the reference, each result is (Q_U SU)[:, 1:r], S[1:r],
+ * (Q_V SV)[:, 1:r] from a thin QR of the tall factors and an SVD of the small
+ * core, truncated by getrank (src/LowRankMatrix.jl:70-80: r = the largest
+ * index with S[r] > length(S) * eps(S[1])). Singular vectors are determined
+ * up to sign, as LAPACK's are. */
+enum hm_lr_op {
+    HM_LR_ADD = 0,      /* L1 + L2 (src/LowRankMatrix.jl:95-102), both m x n */
+    HM_LR_SUB = 1,      /* L1 - L2 (src/LowRankMatrix.jl:104-111) */
+    HM_LR_MUL = 2,      /* L1 * L2, L1 m x k, L2 k x n (src/LowRankMatrix.jl:113-118) */
+    HM_LR_ADJMUL = 3,   /* L1' * L2, L1 k x m, L2 k x n (src/LowRankMatrix.jl:120-126) */
+    HM_LR_DMUL = 4,     /* A * L2, A m x k dense (src/LowRankMatrix.jl:128-134; a Diagonal
+                           A, :144-150, is passed as its dense form) */
+    HM_LR_MULD = 5,     /* L1 * A, A k x n dense (src/LowRankMatrix.jl:136-142,152-158) */
+    HM_LR_SVDTRUNC = 6  /* convert(LowRankMatrix, A), A m x n dense (src/LowRankMatrix.jl:82-86) */
+};
+
+/* dims: 5 per item, (m, n, k, r1, r2) (k, r1 or r2 unused where the operation
+ * has no such operand). in: 7 DEVICE pointers per item, (U1, S1, V1, U2, S2,
+ * V2, A), NULL where unused. out: 3 DEVICE pointers per item, (U, S, V) with
+ * room for rmax columns: rmax = r1 + r2 (sums), min(r1, r2) (products of two
+ * LowRankMatrix), r2 (A * L2), r1 (L1 * A), min(m, n) (svdtrunc); only the
+ * first rank[t] columns are written. rank: HOST array of count values.
+ * Limits: ranks 1..64; sums need r1 + r2 <= 64 and <= min(m, n); A * L2 needs
+ * m >= r2, L1 * A needs n >= r1; svdtrunc needs min(m, n) <= 64
+ * (HM_ERR_UNSUPPORTED otherwise). Synchronizes `stream`. */
+int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *const *in,
+                  double *const *out, int64_t *rank, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
