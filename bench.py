@@ -43,10 +43,11 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "batched", "gradient", "scale", "even", "lowrank"],
+                    choices=["headline", "batched", "gradient", "scale", "even", "lowrank",
+                             "cholesky"],
                     help="headline = the n=2^20 product (the bench line); the others time "
-                         "the batched, gradient, scale, EvenBarycentricMatrix and LowRankMatrix "
-                         "workloads")
+                         "the batched, gradient, scale, EvenBarycentricMatrix, LowRankMatrix "
+                         "and hierarchical Cholesky workloads")
     args = ap.parse_args()
     args.variants_given = args.variants is not None
     if args.variants is None:
@@ -539,6 +540,85 @@ def run_lowrank(args):
     return out
 
 
+def run_cholesky(args):
+    """Hierarchical Cholesky factorization (src/cholesky.jl:12-95) of an SPD
+    Gaussian-kernel matrix exp(-((x_i - x_j)/0.25)^2) + I, n = 8192 sorted
+    uniform points, stored as the reference's 2 x 2 upper-block tree (leaf
+    256, off-diagonal blocks LowRankMatrix truncated at 1e-13 sigma_1). One
+    step = the whole factorization on the device tree (the host walks the
+    recursion; leaves and updates run in the hm_dense_* and hm_lowrank_op
+    kernels). Error: ||R'R - A|| / ||A||, dense, on the GPU. CPU = the
+    oracle's numpy/LAPACK restatement of the same recursion."""
+    import numpy as np
+    import torch
+    import hmx
+    from hmx import cholesky as HC
+    n, leaf = 8192, 256
+    x = hmx.workloads.spd_points(n, 0)
+    K = hmx.workloads.spd_kernel(x)
+    H = hmx.workloads.spd_hierarchical(K, leaf)
+    D = HC.to_device(H)
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.warmup)):
+        R = HC.hierarchicalcholesky_device(D)
+    steps = max(2, min(args.steps, 10))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R = HC.hierarchicalcholesky_device(D)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    Rd = HC.densify(R)
+    Ad = HC.densify(D)
+    Ad = torch.triu(Ad) + torch.triu(Ad, 1).t()
+    err = float(torch.linalg.norm(Rd.t() @ Rd - Ad) / torch.linalg.norm(Ad))
+    ranks = []
+
+    def walk(T):
+        for row in T.blocks:
+            for c in row:
+                if isinstance(c, HC.DTree):
+                    walk(c)
+                elif isinstance(c, HC.DLowRank):
+                    ranks.append(int(c.S.shape[0]))
+    walk(R)
+    out = {"metric": "hierarchical Cholesky factorizations/s, n=8192 Gaussian-kernel SPD",
+           "value": round(1.0 / sec, 3), "unit": "factorizations/s",
+           "ms_per_step": round(sec * 1e3, 2), "steps": steps, "warmup": args.warmup,
+           "dtype": "f64", "data": "synthetic (sorted uniform points, splitmix64 seed 31)",
+           "config": {"workload": "hierarchical_cholesky", "n": n, "leaf": leaf,
+                      "kernel": "exp(-((x-y)/0.25)^2) + I", "truncation": "1e-13 sigma_1"},
+           "rel_err_RtR_vs_A": err, "lowrank_ranks_of_R": [min(ranks), max(ranks)],
+           "n_lowrank_blocks_of_R": len(ranks)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import cholesky as OC
+
+        def to_oracle(Hh):
+            if isinstance(Hh, hmx.HierarchicalMatrix):
+                T = OC.OTree(*Hh.assigned.shape)
+                for m in range(Hh.assigned.shape[0]):
+                    for q in range(Hh.assigned.shape[1]):
+                        if Hh.assigned[m, q]:
+                            T.set(m, q, to_oracle(Hh.blocks[m][q]))
+                return T
+            if isinstance(Hh, hmx.LowRankMatrix):
+                return (Hh.U, Hh.S, Hh.V)
+            return np.array(Hh)
+        A = to_oracle(H)
+        runs = []
+        while sum(runs) < 10.0 and len(runs) < 20:
+            tc = time.perf_counter()
+            OC.chol(A)
+            runs.append(time.perf_counter() - tc)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / float(np.median(runs)), 3), "unit": "factorizations/s",
+            "kind": "port", "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None,
+            "sample": (f"oracle numpy/LAPACK restatement of the recursion on the same tree: "
+                       f"median {np.median(runs) * 1e3:.0f} ms of {len(runs)}")}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -562,7 +642,7 @@ def main():
 
     if args.workload != "headline":
         fn = {"batched": run_batched, "gradient": run_gradient, "scale": run_scale,
-              "even": run_even, "lowrank": run_lowrank}[args.workload]
+              "even": run_even, "lowrank": run_lowrank, "cholesky": run_cholesky}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -1,0 +1,133 @@
+// dense.cpp -- host side of the batched dense leaf steps (hm_dense_* of
+// include/hmx.h): argument checks, the item table, launches.
+#include "hmx.h"
+#include "hmx_dense.h"
+#include "hmx_even.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <vector>
+
+namespace {
+
+int dfail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return hmx_set_error(code, buf);
+}
+
+// runs fn(items, info) on the stream. One item without an info result goes
+// by value and the call returns without waiting; otherwise the item table is
+// copied to the device and the call synchronizes the stream.
+template <class F>
+int run_items(const std::vector<DenseItem> &items, hipStream_t s, const char *what, F fn,
+              int64_t *info_host = nullptr) {
+    const size_t count = items.size();
+    if (count == 1 && !info_host) {
+        DenseItems its{nullptr, items[0]};
+        const hipError_t e = fn(its, nullptr);
+        if (e != hipSuccess) return dfail(HM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+        return HM_OK;
+    }
+    DenseItem *d = nullptr;
+    int64_t *dinfo = nullptr;
+    hipError_t e = hipSuccess;
+    if (count > 1) e = hipMallocAsync((void **)&d, sizeof(DenseItem) * count, s);
+    if (e == hipSuccess && info_host) e = hipMallocAsync((void **)&dinfo, sizeof(int64_t) * count, s);
+    if (e == hipSuccess && d)
+        e = hipMemcpyAsync(d, items.data(), sizeof(DenseItem) * count, hipMemcpyHostToDevice, s);
+    DenseItems its{d, items[0]};
+    if (e == hipSuccess) e = fn(its, dinfo);
+    if (e == hipSuccess && info_host)
+        e = hipMemcpyAsync(info_host, dinfo, sizeof(int64_t) * count, hipMemcpyDeviceToHost, s);
+    if (d) (void)hipFreeAsync(d, s);
+    if (dinfo) (void)hipFreeAsync(dinfo, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return dfail(HM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hm_dense_potrf(int64_t count, const int64_t *n, double *const *A, const int64_t *lda,
+                   int64_t *info, void *stream) {
+    if (count < 0 || (count > 0 && (!n || !A || !lda || !info)))
+        return dfail(HM_ERR_INVALID, "dense_potrf: bad count or NULL arrays");
+    if (count == 0) return HM_OK;
+    std::vector<DenseItem> items((size_t)count);
+    for (int64_t t = 0; t < count; ++t) {
+        if (n[t] < 0 || n[t] > INT32_MAX || lda[t] < std::max<int64_t>(1, n[t]) || (n[t] > 0 && !A[t]))
+            return dfail(HM_ERR_INVALID, "dense_potrf: item %lld: bad n, lda or A", (long long)t);
+        items[(size_t)t] = DenseItem{(int32_t)n[t], 0, 0, 0, A[t], lda[t], nullptr, 0, nullptr, 0,
+                                     nullptr};
+    }
+    int rc = run_items(items, (hipStream_t)stream, "dense_potrf",
+                       [&](const DenseItems &d, int64_t *di) {
+                           return hmx_dense_potrf(d, count, di, (hipStream_t)stream);
+                       },
+                       info);
+    if (rc) return rc;
+    for (int64_t t = 0; t < count; ++t)
+        if (info[t]) return dfail(HM_ERR_INVALID, "dense_potrf: item %lld is not positive definite (pivot %lld)", (long long)t, (long long)info[t]);
+    return HM_OK;
+}
+
+int hm_dense_trsm_t(int64_t count, const int64_t *n, const int64_t *k, const double *const *R,
+                    const int64_t *ldr, double *const *B, const int64_t *ldb, void *stream) {
+    if (count < 0 || (count > 0 && (!n || !k || !R || !ldr || !B || !ldb)))
+        return dfail(HM_ERR_INVALID, "dense_trsm_t: bad count or NULL arrays");
+    if (count == 0) return HM_OK;
+    std::vector<DenseItem> items((size_t)count);
+    int64_t max_k = 0;
+    for (int64_t t = 0; t < count; ++t) {
+        if (n[t] < 0 || k[t] < 0 || n[t] > INT32_MAX || k[t] > INT32_MAX ||
+            ldr[t] < std::max<int64_t>(1, n[t]) || ldb[t] < std::max<int64_t>(1, n[t]) ||
+            (n[t] > 0 && k[t] > 0 && (!R[t] || !B[t])))
+            return dfail(HM_ERR_INVALID, "dense_trsm_t: item %lld: bad sizes or pointers", (long long)t);
+        items[(size_t)t] = DenseItem{(int32_t)n[t], (int32_t)k[t], 0, 0, B[t], ldb[t], R[t], ldr[t],
+                                     nullptr, 0, nullptr};
+        max_k = std::max(max_k, n[t] > 0 ? k[t] : 0);
+    }
+    return run_items(items, (hipStream_t)stream, "dense_trsm_t",
+                     [&](const DenseItems &d, int64_t *) {
+                         return hmx_dense_trsm_t(d, count, max_k, (hipStream_t)stream);
+                     });
+}
+
+int hm_dense_gemm(int transa, int transb, int64_t count, const int64_t *mnk, double alpha,
+                  const double *const *A, const int64_t *lda, const double *const *D,
+                  const double *const *B, const int64_t *ldb, double beta, double *const *C,
+                  const int64_t *ldc, void *stream) {
+    if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1) || count < 0 ||
+        (count > 0 && (!mnk || !A || !lda || !B || !ldb || !C || !ldc)))
+        return dfail(HM_ERR_INVALID, "dense_gemm: bad arguments");
+    if (count == 0) return HM_OK;
+    std::vector<DenseItem> items((size_t)count);
+    int64_t max_tiles = 0;
+    for (int64_t t = 0; t < count; ++t) {
+        const int64_t m = mnk[3 * t], n = mnk[3 * t + 1], k = mnk[3 * t + 2];
+        const int64_t arows = transa ? k : m, brows = transb ? n : k;
+        if (m < 0 || n < 0 || k < 0 || m > INT32_MAX || n > INT32_MAX || k > INT32_MAX ||
+            ldc[t] < std::max<int64_t>(1, m) || lda[t] < std::max<int64_t>(1, arows) ||
+            ldb[t] < std::max<int64_t>(1, brows) || (m > 0 && n > 0 && !C[t]) ||
+            (m > 0 && n > 0 && k > 0 && (!A[t] || !B[t])))
+            return dfail(HM_ERR_INVALID, "dense_gemm: item %lld: bad sizes or pointers", (long long)t);
+        items[(size_t)t] = DenseItem{(int32_t)m, (int32_t)n, (int32_t)k, 0, C[t], ldc[t], A[t],
+                                     lda[t], B[t], ldb[t], D ? D[t] : nullptr};
+        max_tiles = std::max(max_tiles, ((m + 63) / 64) * ((n + 63) / 64));
+    }
+    return run_items(items, (hipStream_t)stream, "dense_gemm",
+                     [&](const DenseItems &d, int64_t *) {
+                         return hmx_dense_gemm(transa, transb, alpha, beta, d, count, max_tiles,
+                                               (hipStream_t)stream);
+                     });
+}
+
+}  // extern "C"

@@ -13,6 +13,8 @@ from .hierarchical import HierarchicalMatrix, LowRankMatrix, add_col_, lmul_, rm
 from .even import EvenBarycentricLeaves, EvenBarycentricMatrix, EvenLeaf, barycentricmatrix  # noqa: F401
 from . import workloads  # noqa: F401
 from . import lowrank  # noqa: F401
+from . import cholesky  # noqa: F401
+from .cholesky import hierarchicalcholesky, solvetransposed  # noqa: F401
 from . import autodiff  # noqa: F401
 
 __all__ = [

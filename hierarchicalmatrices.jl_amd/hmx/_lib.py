@@ -168,6 +168,16 @@ def _load():
     L.hm_lowrank_op.restype = ctypes.c_int
     L.hm_lowrank_op.argtypes = [ctypes.c_int, _i64, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 _ip, _vp]
+    # dense leaf steps of the hierarchical Cholesky factorization
+    L.hm_dense_potrf.restype = ctypes.c_int
+    L.hm_dense_potrf.argtypes = [_i64, _ip, ctypes.POINTER(_vp), _ip, _ip, _vp]
+    L.hm_dense_trsm_t.restype = ctypes.c_int
+    L.hm_dense_trsm_t.argtypes = [_i64, _ip, _ip, ctypes.POINTER(_vp), _ip, ctypes.POINTER(_vp),
+                                  _ip, _vp]
+    L.hm_dense_gemm.restype = ctypes.c_int
+    L.hm_dense_gemm.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _ip, ctypes.c_double,
+                                ctypes.POINTER(_vp), _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                _ip, ctypes.c_double, ctypes.POINTER(_vp), _ip, _vp]
     return L
 
 

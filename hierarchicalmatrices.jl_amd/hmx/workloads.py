@@ -126,3 +126,46 @@ def config_even(N=1 << 20, levels=10, rank=20, seed=0):
     level), rank 20 = BLOCKRANK(Float64), Gaussian f, v ~ N(0, 1)."""
     return dict(N=N, levels=levels, rank=rank, leaves=even_hodlr_leaves(N, levels),
                 f=gaussian_even_kernel(N), v=normal(seed + 21, N))
+
+
+def spd_points(n, seed=0):
+    """Sorted uniform points on [0, 1] (ascending) for the SPD test matrices."""
+    return np.sort(uniform(seed + 31, n))
+
+
+def spd_kernel(x, ell=0.25, shift=1.0):
+    """K[i, j] = exp(-((x_i - x_j)/ell)^2) + shift * delta_ij: a Gaussian
+    kernel matrix (positive semidefinite) made positive definite by the shift."""
+    d = (x[:, None] - x[None, :]) / ell
+    return np.exp(-d * d) + shift * np.eye(len(x))
+
+
+def spd_hierarchical(K, leaf, tol=1e-13, dense_offdiag=(), dense_diag_at=()):
+    """A 2 x 2 HierarchicalMatrix of the SPD matrix K stored by its upper
+    blocks, as src/cholesky.jl expects: diagonal blocks recurse until they
+    are at most `leaf` wide (then Matrix), the (1, 2) block is a
+    LowRankMatrix from the SVD of K's block truncated at tol * sigma_1 (a
+    Matrix at the depths in dense_offdiag), the (2, 1) block is unassigned.
+    Diagonal blocks at the depths in dense_diag_at stay dense whatever their
+    size."""
+    from .hierarchical import HierarchicalMatrix, LowRankMatrix
+    from .kernelmatrix import Block
+
+    def build(r0, r1, depth):
+        n = r1 - r0
+        if n <= leaf or depth in dense_diag_at:
+            return np.asfortranarray(K[r0:r1, r0:r1])
+        s = r0 + n // 2
+        H = HierarchicalMatrix(2, 2)
+        H[Block(1), Block(1)] = build(r0, s, depth + 1)
+        B = K[r0:s, s:r1]
+        if depth in dense_offdiag:
+            H[Block(1), Block(2)] = np.asfortranarray(B)
+        else:
+            U, S, Vt = np.linalg.svd(B, full_matrices=False)
+            r = max(1, int(np.sum(S > tol * S[0])))
+            H[Block(1), Block(2)] = LowRankMatrix(U[:, :r], S[:r], Vt[:r].T)
+        H[Block(2), Block(2)] = build(s, r1, depth + 1)
+        return H
+
+    return build(0, K.shape[0], 0)

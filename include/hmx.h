@@ -366,6 +366,38 @@ enum hm_lr_op {
 int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *const *in,
                   double *const *out, int64_t *rank, void *stream);
 
+/* ------------------------------------------------ dense leaf steps of the
+ * hierarchical Cholesky factorization (src/cholesky.jl:12-228), batched:
+ * `count` independent items of free sizes, column-major DEVICE arrays with
+ * per-item leading dimensions (host arrays of sizes and pointers).
+ * hm_dense_potrf synchronizes `stream` (it reports the pivot); the other two
+ * are stream-ordered and return at once for count == 1, and synchronize for
+ * count > 1. */
+
+/* Replaces hierarchicalcholesky(A::Matrix) = Matrix(cholesky(Symmetric(A)).U)
+ * (src/cholesky.jl:18-20): A[t] (n x n) is overwritten by the upper factor R,
+ * R^T R = A, read from A's upper triangle; the strict lower triangle is set
+ * to zero. info (host, count): 0, or j + 1 when pivot j is not positive, in
+ * which case the call returns HM_ERR_INVALID (Julia's PosDefException). */
+int hm_dense_potrf(int64_t count, const int64_t *n, double *const *A, const int64_t *lda,
+                   int64_t *info, void *stream);
+
+/* Replaces solvetransposed(R::Matrix, b) = UpperTriangular(R)' \ b and its
+ * Matrix / LowRankMatrix right-hand sides (src/cholesky.jl:116-142):
+ * B[t] (n x k) := R[t]^{-T} B[t], R upper n x n. */
+int hm_dense_trsm_t(int64_t count, const int64_t *n, const int64_t *k, const double *const *R,
+                    const int64_t *ldr, double *const *B, const int64_t *ldb, void *stream);
+
+/* C[t] = beta C[t] + alpha op(A[t]) diag(D[t]) op(B[t]) (op = transpose when
+ * trans = 1; D NULL, or D[t] NULL, for no diagonal), mnk = (m, n, k) per
+ * item: the products of the Schur-complement updates A22 - R12' R12 and of
+ * b2 - R12' x1 (src/cholesky.jl:22-95,150-228), with a LowRankMatrix's
+ * U diag(S) V' applied through D. */
+int hm_dense_gemm(int transa, int transb, int64_t count, const int64_t *mnk, double alpha,
+                  const double *const *A, const int64_t *lda, const double *const *D,
+                  const double *const *B, const int64_t *ldb, double beta, double *const *C,
+                  const int64_t *ldc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
