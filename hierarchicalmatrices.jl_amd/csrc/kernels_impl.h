@@ -354,17 +354,23 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
                                                  T *z) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    if (wid >= ncl) return;
     const int r = p.r;
-    const DCluster C = cl[wid];
     const int G = WAVE / r;
     const int m = lane % r, g = lane / r;
     const bool act = g < G;
-    for (int q0 = 0; q0 < KB; q0 += G) {
-        const int q = q0 + g;
-        const bool aq = act && q < KB;
+    // one right-hand side: the G lane groups of a wavefront take G clusters;
+    // several: one cluster per wavefront, its groups taking the right-hand sides
+    const int ci = KB == 1 ? wid * G + g : wid;
+    if (KB == 1 ? (!act || ci >= ncl) : wid >= ncl) return;
+    const DCluster C = cl[ci];
+    for (int q0 = 0; q0 < KB; q0 += (KB == 1 ? 1 : G)) {
+        const int q = KB == 1 ? 0 : q0 + g;
+        const bool aq = KB == 1 || (act && q < KB);
         const int qa = aq ? q : 0;
         T acc = T(0);
+        // stored F: two blocks' loads in flight; regenerated F: no unrolling
+        constexpr int UNR = FREE ? 1 : 2;
+#pragma unroll UNR
         for (int e = C.far_beg; e < C.far_end; ++e) {
             const DFarBlk b = p.far[far_list[e]];
             const int src = TRANS ? b.rc : b.cc;
@@ -400,7 +406,7 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
             }
             acc = c;
         }
-        if (aq) z[((int64_t)wid * KB + q) * r + m] = acc;
+        if (aq) z[((int64_t)ci * KB + q) * r + m] = acc;
     }
 }
 
@@ -763,6 +769,12 @@ __global__ __launch_bounds__(WG) void k_entries(DevPlanT<T> p, int64_t n, const 
 }
 
 inline int nblk(int64_t waves) { return (int)((waves + WPB - 1) / WPB); }
+// wavefronts of k_translate: one cluster per lane group of r lanes when one
+// right-hand side is translated, one per wavefront otherwise
+inline int64_t tr_waves(int64_t ncl, int r, int kb) {
+    const int G = kb == 1 ? WAVE / r : 1;
+    return (ncl + G - 1) / G;
+}
 
 template <class T, int RMAX>
 hipError_t gen_basis_r(const DevPlanT<T> &p, hipStream_t s) {
@@ -812,11 +824,13 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         mark(2);
         if (tr) {
             if constexpr (KB == 1)
-                k_translate<T, RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_ncl), WG, 0, s>>>(
-                    p, out_cl, out_ncl, out_far_list, a.w, a.z);
+                k_translate<T, RMAX, FREE, KID, true, CHEB, 1>
+                    <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl,
+                                                                   out_far_list, a.w, a.z);
         } else {
-            k_translate<T, RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
+            k_translate<T, RMAX, FREE, KID, false, CHEB, KB>
+                <<<nblk(tr_waves(out_ncl, p.r, KB)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
+                                                                a.w, a.z);
         }
     } else {
         mark(1);
@@ -860,11 +874,13 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
                                                      a.incx, 0, a.partial, p.r);
         k_reduce<T, true, 1><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
         if (tr)
-            k_translate<T, RMAX, true, KID, true, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
+            k_translate<T, RMAX, true, KID, true, true, 1>
+                <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
+                                                               a.w, a.z);
         else
-            k_translate<T, RMAX, true, KID, false, true, 1><<<nblk(out_ncl), WG, 0, s>>>(
-                p, out_cl, out_ncl, out_far_list, a.w, a.z);
+            k_translate<T, RMAX, true, KID, false, true, 1>
+                <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
+                                                               a.w, a.z);
     }
     if (out_n > 0) {
         if (tr)
