@@ -425,6 +425,25 @@ def run_even(args):
                     "rel_diff_vs_single": err4,
                     "hbm_gbs": round(H.bytes_per_product(4) / (ms4 * 1e-3) / 1e9, 1)}}
     H.close()
+    del H
+    # EvenBarycentricMatrix{Float32} of the same leaves: rank BLOCKRANK(Float32)
+    n32 = hmx.BLOCKRANK(np.float32)
+    specs = [(r0, c0, hmx.EvenLeaf(a, b, c, d,
+                                   hmx.even.even_F(w["f"], a, b, c, d, n32, np.float32),
+                                   dtype=np.float32))
+             for (r0, c0, a, b, c, d) in w["leaves"]]
+    H32 = hmx.EvenBarycentricLeaves(N, N, specs)
+    del specs
+    v32 = v.float()
+    u32 = torch.empty(N, dtype=torch.float32, device=dev)
+    ms32 = timed(lambda: H32._call(u32, v32, 0, 0, 0, stream), args.steps, args.warmup, stream)
+    err32 = float(torch.linalg.norm(u32[rows].double() - dense) / torch.linalg.norm(dense))
+    out["f32"] = {"rank": n32, "ms_per_step": round(ms32, 4),
+                  "products_per_s": round(1000.0 / ms32, 3),
+                  "bytes_per_product": H32.bytes_per_product(1),
+                  "hbm_gbs": round(H32.bytes_per_product(1) / (ms32 * 1e-3) / 1e9, 1),
+                  "rel_err_vs_dense_256rows": err32}
+    H32.close()
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O

@@ -1,12 +1,14 @@
 // even.cpp -- host side of the EvenBarycentricMatrix leaves: plan layout and
 // the hm_even_* entry points of include/hmx.h.
 #include "hmx.h"
+#include "hmx_cheb.h"
 #include "hmx_even.h"
 
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <set>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -26,7 +28,9 @@ int efail(int code, const char *fmt, ...) {
         if (e_ != hipSuccess) return efail(HM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-// grid values stay exact in Float64 through 2i - a - b
+// 2i - a - b is formed in 64-bit integers and converted to T once, with
+// round-to-nearest as the reference's Int -> T promotion; the grid bound
+// keeps 2i in range and exact in Float64
 constexpr int64_t GRID_MAX = (int64_t)1 << 50;
 
 int check_grid(int64_t a, int64_t b, const char *what) {
@@ -37,10 +41,12 @@ int check_grid(int64_t a, int64_t b, const char *what) {
     return HM_OK;
 }
 
-ENodes<double> nodes_of(int n) {
-    ENodes<double> nd{};
-    hm_chebyshevpoints(nd.x, n, 1);
-    hm_chebyshevbarycentricweights(nd.lam, n, 1);
+// first-kind nodes and weights in the element type T
+template <class T>
+ENodes<T> nodes_of(int n) {
+    ENodes<T> nd{};
+    cheb_points_t<T>(nd.x, n);
+    cheb_weights_t<T>(nd.lam, n);
     return nd;
 }
 
@@ -82,7 +88,8 @@ struct hm_even {
     ERed *d_red = nullptr;
     EDown *d_down = nullptr;
     int32_t *d_list = nullptr;
-    double *d_W = nullptr, *d_F = nullptr, *d_partial = nullptr, *d_beta = nullptr;
+    bool f32 = false;  // element type of W, F and the products
+    void *d_W = nullptr, *d_F = nullptr, *d_partial = nullptr, *d_beta = nullptr;
 
     ~hm_even() {
         void *bufs[] = {d_leaves, d_up, d_red, d_down, d_list, d_W, d_F, d_partial, d_beta};
@@ -90,8 +97,9 @@ struct hm_even {
             if (b) (void)hipFree(b);
     }
 
-    EvenDev<double> dev() const {
-        EvenDev<double> p{};
+    template <class T>
+    EvenDev<T> dev() const {
+        EvenDev<T> p{};
         p.n = n;
         p.leaves = d_leaves;
         p.up = d_up;
@@ -101,10 +109,10 @@ struct hm_even {
         p.down = d_down;
         p.ndown = (int32_t)down.size();
         p.list = d_list;
-        p.W = d_W;
-        p.F = d_F;
-        p.partial = d_partial;
-        p.beta = d_beta;
+        p.W = (const T *)d_W;
+        p.F = (const T *)d_F;
+        p.partial = (T *)d_partial;
+        p.beta = (T *)d_beta;
         p.nrows = nrows;
         p.covered = covered_rows == nrows;
         return p;
@@ -161,20 +169,25 @@ struct hm_even {
     }
 };
 
-extern "C" {
-
-int hm_even_nodes(int64_t a, int64_t b, int n, double *t) {
+template <class T>
+int even_nodes_t(int64_t a, int64_t b, int n, double *t) {
     if (n < 1 || n > HM_EVEN_RMAX || !t) return efail(HM_ERR_INVALID, "even_nodes: bad n or t");
     if (int rc = check_grid(a, b, "even_nodes")) return rc;
-    double x[HM_EVEN_RMAX];
-    hm_chebyshevpoints(x, n, 1);
-    // (a+b)/2 is a Float64 division of integers; (b-a)*x_k/2 in Float64
+    T x[HM_EVEN_RMAX];
+    cheb_points_t<T>(x, n);
+    // (a+b)/2 is a Float64 division of integers; (b-a)*x_k and /2 are in T;
+    // the sum promotes to Float64 (src/BarycentricMatrix.jl:41)
     const double mid = (double)(a + b) / 2.0;
-    for (int k = 0; k < n; ++k) t[k] = mid + ((double)(b - a) * x[k]) / 2.0;
+    for (int k = 0; k < n; ++k) {
+        T h = (T)(b - a) * x[k];
+        h = h / T(2);
+        t[k] = mid + (double)h;
+    }
     return HM_OK;
 }
 
-int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *w, void *stream) {
+template <class T>
+int even_weights_t(int64_t a, int64_t b, int n, int layout, T *W, T *w, void *stream) {
     if (n < 1 || n > HM_EVEN_RMAX || !W || (layout != 0 && layout != 1))
         return efail(HM_ERR_INVALID, "even_weights: bad n, layout or W");
     if (int rc = check_grid(a, b, "even_weights")) return rc;
@@ -184,16 +197,18 @@ int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *
     ELeaf *d = nullptr;
     if (int rc = dalloc(&d, 1)) return rc;
     hipError_t e = hipMemcpyAsync(d, &L, sizeof(L), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hmx_even_gen_weights(d, 1, m, nodes_of(n), n, layout, W, w, s);
+    if (e == hipSuccess) e = hmx_even_gen_weights(d, 1, m, nodes_of<T>(n), n, layout, W, w, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (e != hipSuccess) return efail(HM_ERR_HIP, "even_weights: %s", hipGetErrorString(e));
     return HM_OK;
 }
 
-int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
-                   const int64_t *col0, const int64_t *abcd, int n, const double *const *F,
-                   void *stream, hm_even_t *out) {
+template <class T>
+int even_create_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
+                  const int64_t *col0, const int64_t *abcd, int n, const T *const *F,
+                  void *stream, hm_even_t *out) {
+    constexpr bool F32 = std::is_same_v<T, float>;
     if (!out) return efail(HM_ERR_INVALID, "even_create: NULL out");
     *out = nullptr;
     if (nrows < 0 || ncols < 0 || nleaves < 0 || n < 1 || n > HM_EVEN_RMAX)
@@ -205,6 +220,7 @@ int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t 
     P->nrows = nrows;
     P->ncols = ncols;
     P->n = n;
+    P->f32 = F32;
     int64_t max_m = 0, max_pn = 0;
     for (int64_t t = 0; t < nleaves; ++t) {
         const int64_t a = abcd[4 * t], b = abcd[4 * t + 1], c = abcd[4 * t + 2], d = abcd[4 * t + 3];
@@ -234,29 +250,33 @@ int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t 
     P->make_down();
     hipStream_t s = (hipStream_t)stream;
     int rc = HM_OK;
+    T *dW = nullptr, *dF = nullptr, *dpart = nullptr, *dbeta = nullptr;
 #define HM_ETRY(expr) do { rc = (expr); if (rc != HM_OK) { delete P; return rc; } } while (0)
     HM_ETRY(upload(&P->d_leaves, P->leaves));
     HM_ETRY(upload(&P->d_up, P->up));
     HM_ETRY(upload(&P->d_red, P->red));
     HM_ETRY(upload(&P->d_down, P->down));
     HM_ETRY(upload(&P->d_list, P->list));
-    HM_ETRY(dalloc(&P->d_W, P->wsize));
-    HM_ETRY(dalloc(&P->d_F, P->fsize));
+    HM_ETRY(dalloc(&dW, P->wsize));
+    P->d_W = dW;
+    HM_ETRY(dalloc(&dF, P->fsize));
+    P->d_F = dF;
     const int64_t stride = (int64_t)HM_EVEN_KB * 2 * n;
-    HM_ETRY(dalloc(&P->d_partial, P->nslots * stride));
-    HM_ETRY(dalloc(&P->d_beta, nleaves * stride));
+    HM_ETRY(dalloc(&dpart, P->nslots * stride));
+    P->d_partial = dpart;
+    HM_ETRY(dalloc(&dbeta, nleaves * stride));
+    P->d_beta = dbeta;
     // F: the caller's column-major arrays into a staging copy, then into tiles
-    double *stage = nullptr;
+    T *stage = nullptr;
     HM_ETRY(dalloc(&stage, P->fentries));
     hipError_t e = hipSuccess;
     for (int64_t t = 0; t < nleaves && e == hipSuccess; ++t) {
         const ELeaf &L = P->leaves[t];
-        e = hipMemcpy(stage + L.soff, F[t], sizeof(double) * (size_t)(L.p * n),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpy(stage + L.soff, F[t], sizeof(T) * (size_t)(L.p * n), hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hmx_even_tile_F(P->d_leaves, nleaves, max_pn, n, stage, P->d_F, s);
+    if (e == hipSuccess) e = hmx_even_tile_F(P->d_leaves, nleaves, max_pn, n, stage, dF, s);
     if (e == hipSuccess)
-        e = hmx_even_gen_weights(P->d_leaves, nleaves, max_m, nodes_of(n), n, 2, P->d_W, nullptr,
+        e = hmx_even_gen_weights(P->d_leaves, nleaves, max_m, nodes_of<T>(n), n, 2, dW, (T *)nullptr,
                                  s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (stage) (void)hipFree(stage);
@@ -269,37 +289,43 @@ int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t 
     return HM_OK;
 }
 
-int hm_even_matvec(hm_even_t P, double *u, const double *v, int64_t ioff, int64_t joff,
-                   int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
-                   void *stream) {
+template <class T>
+int even_matvec_t(hm_even_t P, T *u, const T *v, int64_t ioff, int64_t joff, int64_t incu,
+                  int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta, void *stream) {
     if (!P) return efail(HM_ERR_INVALID, "even_matvec: NULL plan");
+    if (P->f32 != std::is_same_v<T, float>)
+        return efail(HM_ERR_INVALID, "even_matvec: the plan holds %s data; use the %s entry point",
+                     P->f32 ? "Float32" : "Float64", P->f32 ? "_f32" : "Float64");
     if (k < 0 || ioff < 0 || joff < 0 || incu < 1 || incv < 1 || (beta != 0 && beta != 1))
         return efail(HM_ERR_INVALID, "even_matvec: bad offsets, strides, k or beta");
     if (k == 0) return HM_OK;
     if (!u || !v) return efail(HM_ERR_INVALID, "even_matvec: NULL vectors");
     if (k > (int64_t)1 << 30) return efail(HM_ERR_INVALID, "even_matvec: k too large");
-    hipError_t e = hmx_even_matvec(P->dev(), u, v, ioff, joff, incu, incv, ldu, ldv, (int)k,
+    hipError_t e = hmx_even_matvec(P->dev<T>(), u, v, ioff, joff, incu, incv, ldu, ldv, (int)k,
                                    beta, (hipStream_t)stream);
     if (e != hipSuccess) return efail(HM_ERR_HIP, "even_matvec: %s", hipGetErrorString(e));
     return HM_OK;
 }
 
-int hm_even_leaf_data(hm_even_t P, int64_t leaf, double *W, double *F) {
+template <class T>
+int even_leaf_data_t(hm_even_t P, int64_t leaf, T *W, T *F) {
     if (!P || leaf < 0 || leaf >= (int64_t)P->leaves.size())
         return efail(HM_ERR_INVALID, "even_leaf_data: bad plan or leaf");
+    if (P->f32 != std::is_same_v<T, float>)
+        return efail(HM_ERR_INVALID, "even_leaf_data: element type of the plan differs");
     const ELeaf &L = P->leaves[leaf];
     const int n = P->n;
     HM_EHIP(hipDeviceSynchronize());
     if (W) {
-        std::vector<double> tiles((size_t)even_tiled_size(L.m, n));
-        HM_EHIP(hipMemcpy(tiles.data(), P->d_W + L.woff, sizeof(double) * tiles.size(),
+        std::vector<T> tiles((size_t)even_tiled_size(L.m, n));
+        HM_EHIP(hipMemcpy(tiles.data(), (const T *)P->d_W + L.woff, sizeof(T) * tiles.size(),
                           hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < L.m; ++i)
             for (int k = 0; k < n; ++k) W[k + (int64_t)n * i] = tiles[(size_t)even_tile(i, k, n)];
     }
     if (F) {
-        std::vector<double> tiles((size_t)even_tiled_size(L.p, n));
-        HM_EHIP(hipMemcpy(tiles.data(), P->d_F + L.foff, sizeof(double) * tiles.size(),
+        std::vector<T> tiles((size_t)even_tiled_size(L.p, n));
+        HM_EHIP(hipMemcpy(tiles.data(), (const T *)P->d_F + L.foff, sizeof(T) * tiles.size(),
                           hipMemcpyDeviceToHost));
         for (int k = 0; k < n; ++k)
             for (int64_t j = 0; j < L.p; ++j) F[j + L.p * k] = tiles[(size_t)even_tile(j, k, n)];
@@ -307,12 +333,53 @@ int hm_even_leaf_data(hm_even_t P, int64_t leaf, double *W, double *F) {
     return HM_OK;
 }
 
+extern "C" {
+
+int hm_even_nodes(int64_t a, int64_t b, int n, double *t) { return even_nodes_t<double>(a, b, n, t); }
+int hm_even_nodes_f32(int64_t a, int64_t b, int n, double *t) { return even_nodes_t<float>(a, b, n, t); }
+
+int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *w, void *stream) {
+    return even_weights_t<double>(a, b, n, layout, W, w, stream);
+}
+int hm_even_weights_f32(int64_t a, int64_t b, int n, int layout, float *W, float *w, void *stream) {
+    return even_weights_t<float>(a, b, n, layout, W, w, stream);
+}
+
+int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
+                   const int64_t *col0, const int64_t *abcd, int n, const double *const *F,
+                   void *stream, hm_even_t *out) {
+    return even_create_t<double>(nrows, ncols, nleaves, row0, col0, abcd, n, F, stream, out);
+}
+int hm_even_create_f32(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
+                       const int64_t *col0, const int64_t *abcd, int n, const float *const *F,
+                       void *stream, hm_even_t *out) {
+    return even_create_t<float>(nrows, ncols, nleaves, row0, col0, abcd, n, F, stream, out);
+}
+
+int hm_even_matvec(hm_even_t P, double *u, const double *v, int64_t ioff, int64_t joff,
+                   int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
+                   void *stream) {
+    return even_matvec_t<double>(P, u, v, ioff, joff, incu, incv, k, ldu, ldv, beta, stream);
+}
+int hm_even_matvec_f32(hm_even_t P, float *u, const float *v, int64_t ioff, int64_t joff,
+                       int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
+                       void *stream) {
+    return even_matvec_t<float>(P, u, v, ioff, joff, incu, incv, k, ldu, ldv, beta, stream);
+}
+
+int hm_even_leaf_data(hm_even_t P, int64_t leaf, double *W, double *F) {
+    return even_leaf_data_t<double>(P, leaf, W, F);
+}
+int hm_even_leaf_data_f32(hm_even_t P, int64_t leaf, float *W, float *F) {
+    return even_leaf_data_t<float>(P, leaf, W, F);
+}
+
 int hm_even_info(hm_even_t P, int64_t *info, int n) {
     if (!P || !info || n < 0) return efail(HM_ERR_INVALID, "even_info: bad arguments");
     const int64_t v[HM_EVEN_INFO_COUNT] = {
-        P->nrows, P->ncols, (int64_t)P->leaves.size(), P->n, P->wsize, P->fsize,
+        P->nrows, P->ncols, (int64_t)P->leaves.size(), P->n, P->wentries, P->fentries,
         (int64_t)P->up.size(), (int64_t)P->red.size(), (int64_t)P->down.size(),
-        (int64_t)P->list.size()};
+        (int64_t)P->list.size(), P->f32 ? 1 : 0};
     for (int i = 0; i < n && i < HM_EVEN_INFO_COUNT; ++i) info[i] = v[i];
     return HM_OK;
 }

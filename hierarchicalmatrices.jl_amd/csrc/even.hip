@@ -222,41 +222,74 @@ hipError_t even_matvec_t(const EvenDev<T> &p, T *u, const T *v, int64_t ioff, in
     return hipGetLastError();
 }
 
+template <class T>
+hipError_t gen_weights_t(const ELeaf *leaves, int64_t nleaves, int64_t max_m, const ENodes<T> &nodes,
+                         int n, int layout, T *W, T *w, hipStream_t s) {
+    if (nleaves <= 0 || max_m <= 0) return hipSuccess;
+    const int64_t bx = (max_m + WG - 1) / WG;
+    dim3 grid((unsigned)(bx < 1024 ? bx : 1024), (unsigned)(nleaves < 65535 ? nleaves : 65535));
+    k_even_gen_w<T><<<grid, WG, 0, s>>>(leaves, nleaves, nodes, n, layout, W, w);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t tile_F_t(const ELeaf *leaves, int64_t nleaves, int64_t max_pn, int n, const T *stage,
+                    T *F, hipStream_t s) {
+    if (nleaves <= 0 || max_pn <= 0) return hipSuccess;
+    const int64_t bx = (max_pn + WG - 1) / WG;
+    dim3 grid((unsigned)(bx < 4096 ? bx : 4096), (unsigned)(nleaves < 65535 ? nleaves : 65535));
+    k_even_tile_F<T><<<grid, WG, 0, s>>>(leaves, nleaves, n, stage, F);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t matvec_dispatch(const EvenDev<T> &p, T *u, const T *v, int64_t ioff, int64_t joff,
+                           int64_t incu, int64_t incv, int64_t ldu, int64_t ldv, int nq, int beta,
+                           hipStream_t s) {
+    const int n = p.n;
+    if (nq == 1) {  // one right-hand side: the single-column kernels (fewer registers)
+        if (n <= 8) return even_matvec_t<T, 8, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+        if (n <= 16) return even_matvec_t<T, 16, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+        if (n <= 24) return even_matvec_t<T, 24, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+        if (n <= 32) return even_matvec_t<T, 32, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    }
+    if (n <= 8) return even_matvec_t<T, 8, 4>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    if (n <= 16) return even_matvec_t<T, 16, 4>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    if (n <= 24) return even_matvec_t<T, 24, 2>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    if (n <= 32) return even_matvec_t<T, 32, 2>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    if (n <= 48) return even_matvec_t<T, 48, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    return even_matvec_t<T, 64, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+}
+
 }  // namespace
 
 hipError_t hmx_even_gen_weights(const ELeaf *leaves, int64_t nleaves, int64_t max_m,
                                 const ENodes<double> &nodes, int n, int layout, double *W,
                                 double *w, hipStream_t s) {
-    if (nleaves <= 0 || max_m <= 0) return hipSuccess;
-    const int64_t bx = (max_m + WG - 1) / WG;
-    dim3 grid((unsigned)(bx < 1024 ? bx : 1024), (unsigned)(nleaves < 65535 ? nleaves : 65535));
-    k_even_gen_w<double><<<grid, WG, 0, s>>>(leaves, nleaves, nodes, n, layout, W, w);
-    return hipGetLastError();
+    return gen_weights_t<double>(leaves, nleaves, max_m, nodes, n, layout, W, w, s);
+}
+hipError_t hmx_even_gen_weights(const ELeaf *leaves, int64_t nleaves, int64_t max_m,
+                                const ENodes<float> &nodes, int n, int layout, float *W,
+                                float *w, hipStream_t s) {
+    return gen_weights_t<float>(leaves, nleaves, max_m, nodes, n, layout, W, w, s);
 }
 
 hipError_t hmx_even_tile_F(const ELeaf *leaves, int64_t nleaves, int64_t max_pn, int n,
                            const double *stage, double *F, hipStream_t s) {
-    if (nleaves <= 0 || max_pn <= 0) return hipSuccess;
-    const int64_t bx = (max_pn + WG - 1) / WG;
-    dim3 grid((unsigned)(bx < 4096 ? bx : 4096), (unsigned)(nleaves < 65535 ? nleaves : 65535));
-    k_even_tile_F<double><<<grid, WG, 0, s>>>(leaves, nleaves, n, stage, F);
-    return hipGetLastError();
+    return tile_F_t<double>(leaves, nleaves, max_pn, n, stage, F, s);
+}
+hipError_t hmx_even_tile_F(const ELeaf *leaves, int64_t nleaves, int64_t max_pn, int n,
+                           const float *stage, float *F, hipStream_t s) {
+    return tile_F_t<float>(leaves, nleaves, max_pn, n, stage, F, s);
 }
 
 hipError_t hmx_even_matvec(const EvenDev<double> &p, double *u, const double *v, int64_t ioff,
                            int64_t joff, int64_t incu, int64_t incv, int64_t ldu, int64_t ldv,
                            int nq, int beta, hipStream_t s) {
-    const int n = p.n;
-    if (nq == 1) {  // one right-hand side: the single-column kernels (fewer registers)
-        if (n <= 8) return even_matvec_t<double, 8, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-        if (n <= 16) return even_matvec_t<double, 16, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-        if (n <= 24) return even_matvec_t<double, 24, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-        if (n <= 32) return even_matvec_t<double, 32, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    }
-    if (n <= 8) return even_matvec_t<double, 8, 4>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    if (n <= 16) return even_matvec_t<double, 16, 4>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    if (n <= 24) return even_matvec_t<double, 24, 2>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    if (n <= 32) return even_matvec_t<double, 32, 2>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    if (n <= 48) return even_matvec_t<double, 48, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
-    return even_matvec_t<double, 64, 1>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+    return matvec_dispatch<double>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
+}
+hipError_t hmx_even_matvec(const EvenDev<float> &p, float *u, const float *v, int64_t ioff,
+                           int64_t joff, int64_t incu, int64_t incv, int64_t ldu, int64_t ldv,
+                           int nq, int beta, hipStream_t s) {
+    return matvec_dispatch<float>(p, u, v, ioff, joff, incu, incv, ldu, ldv, nq, beta, s);
 }

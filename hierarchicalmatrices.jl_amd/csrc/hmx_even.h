@@ -98,12 +98,20 @@ int hmx_set_error(int code, const char *msg);
 hipError_t hmx_even_gen_weights(const ELeaf *leaves, int64_t nleaves, int64_t max_m,
                                 const ENodes<double> &nodes, int n, int layout, double *W,
                                 double *w, hipStream_t s);
+hipError_t hmx_even_gen_weights(const ELeaf *leaves, int64_t nleaves, int64_t max_m,
+                                const ENodes<float> &nodes, int n, int layout, float *W,
+                                float *w, hipStream_t s);
 // F of the leaves from the column-major staging copy (F[j + p*k] at soff)
 // into the 64-row tiles of the pool (even_tile(j, k, n) at foff)
 hipError_t hmx_even_tile_F(const ELeaf *leaves, int64_t nleaves, int64_t max_pn, int n,
                            const double *stage, double *F, hipStream_t s);
+hipError_t hmx_even_tile_F(const ELeaf *leaves, int64_t nleaves, int64_t max_pn, int n,
+                           const float *stage, float *F, hipStream_t s);
 // u[q*ldu + (ioff + I)*incu] (+)= sum over leaves of the leaf products, q < nq
 // (launched in groups of up to HM_EVEN_KB right-hand sides)
 hipError_t hmx_even_matvec(const EvenDev<double> &p, double *u, const double *v, int64_t ioff,
+                           int64_t joff, int64_t incu, int64_t incv, int64_t ldu, int64_t ldv,
+                           int nq, int beta, hipStream_t s);
+hipError_t hmx_even_matvec(const EvenDev<float> &p, float *u, const float *v, int64_t ioff,
                            int64_t joff, int64_t incu, int64_t incv, int64_t ldu, int64_t ldv,
                            int nq, int beta, hipStream_t s);

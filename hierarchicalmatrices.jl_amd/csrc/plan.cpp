@@ -2,6 +2,7 @@
 // device layout, and the C ABI declared in include/hmx.h.
 #include "hmx.h"
 #include "hmx_device.h"
+#include "hmx_cheb.h"
 
 #include <algorithm>
 #include <cmath>
@@ -490,6 +491,20 @@ int hm_chebyshevbarycentricweights(double *out, int n, int kind) {
     return HM_OK;
 }
 
+int hm_chebyshevpoints_f32(float *out, int n, int kind) {
+    if (!out || n < 1 || (kind != 1 && kind != 2) || (kind == 2 && n < 2))
+        return fail(HM_ERR_INVALID, "chebyshevpoints: bad arguments");
+    cheb_points_t<float>(out, n, kind);
+    return HM_OK;
+}
+
+int hm_chebyshevbarycentricweights_f32(float *out, int n, int kind) {
+    if (!out || n < 1 || (kind != 1 && kind != 2))
+        return fail(HM_ERR_INVALID, "chebyshevbarycentricweights: bad arguments");
+    cheb_weights_t<float>(out, n, kind);
+    return HM_OK;
+}
+
 int hm_indsplit(const double *x, int64_t nx, int64_t first, int64_t len, double a, double b,
                 int64_t out[4]) {
     if (!x || !out || len < 0 || first < 1) return fail(HM_ERR_INVALID, "indsplit: bad arguments");
@@ -514,36 +529,7 @@ struct Inst {
     double a, b, c, d;
 };
 
-// chebyshevpoints(T, n) and chebyshevbarycentricweights(T, n) of the first
-// kind (src/BarycentricMatrix.jl:92-136) in the element type T: the argument
-// of sinpi is formed in T, sin(pi q) is evaluated in long double and rounded
-// once to T.
-extern "C++" {
-template <class T>
-void cheb_points_t(T *x, int n) {
-    const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
-    std::fill(x, x + n, T(0));
-    for (int k = 1; k <= n / 2; ++k) {
-        const T q = ((T)(n - 2 * k) + T(1)) / (T)(2 * n);
-        x[k - 1] = (T)sinl(PI_L * (long double)q);
-    }
-    for (int k = 1; k <= n / 2; ++k) x[n - k] = -x[k - 1];
-}
-}  // extern "C++"
 
-extern "C++" {
-template <class T>
-void cheb_weights_t(T *l, int n) {
-    const long double PI_L = 3.14159265358979323846264338327950288419716939937510L;
-    std::fill(l, l + n, T(0));
-    for (int k = 1; k <= n / 2 + 1 && k <= n; ++k) {
-        const T q = ((T)(2 * k) - T(1)) / (T)(2 * n);
-        l[k - 1] = (T)sinl(PI_L * (long double)q);
-    }
-    for (int k = 1; k <= n / 2; ++k) l[n - k] = l[k - 1];
-    for (int k = 2; k <= n; k += 2) l[k - 1] = -l[k - 1];
-}
-}  // extern "C++"
 
 // The element-typed part of a plan: nodes, weights, cluster maps and the
 // Chebyshev-form matrix in T, the device arrays, the device view, and the

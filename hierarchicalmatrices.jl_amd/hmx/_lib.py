@@ -160,6 +160,21 @@ def _load():
                                  ctypes.c_int, _vp]
     L.hm_even_leaf_data.restype = ctypes.c_int
     L.hm_even_leaf_data.argtypes = [_vp, _i64, _dp, _dp]
+    L.hm_even_nodes_f32.restype = ctypes.c_int
+    L.hm_even_nodes_f32.argtypes = L.hm_even_nodes.argtypes
+    L.hm_even_weights_f32.restype = ctypes.c_int
+    L.hm_even_weights_f32.argtypes = L.hm_even_weights.argtypes
+    L.hm_even_create_f32.restype = ctypes.c_int
+    L.hm_even_create_f32.argtypes = [_i64, _i64, _i64, _ip, _ip, _ip, ctypes.c_int,
+                                     ctypes.POINTER(_fp), _vp, ctypes.POINTER(_vp)]
+    L.hm_even_matvec_f32.restype = ctypes.c_int
+    L.hm_even_matvec_f32.argtypes = L.hm_even_matvec.argtypes
+    L.hm_even_leaf_data_f32.restype = ctypes.c_int
+    L.hm_even_leaf_data_f32.argtypes = [_vp, _i64, _fp, _fp]
+    L.hm_chebyshevpoints_f32.restype = ctypes.c_int
+    L.hm_chebyshevpoints_f32.argtypes = [_fp, ctypes.c_int, ctypes.c_int]
+    L.hm_chebyshevbarycentricweights_f32.restype = ctypes.c_int
+    L.hm_chebyshevbarycentricweights_f32.argtypes = [_fp, ctypes.c_int, ctypes.c_int]
     L.hm_even_info.restype = ctypes.c_int
     L.hm_even_info.argtypes = [_vp, _ip, ctypes.c_int]
     L.hm_even_destroy.restype = ctypes.c_int

@@ -36,10 +36,12 @@ class LowRankMatrix:
 
     __array_ufunc__ = None  # numpy defers `ndarray @ LowRankMatrix` to __rmatmul__
 
-    def __init__(self, U, S, V):
-        self.U = np.asfortranarray(U, dtype=np.float64)
-        self.S = np.ascontiguousarray(np.diag(S) if np.ndim(S) == 2 else S, dtype=np.float64)
-        self.V = np.asfortranarray(V, dtype=np.float64)
+    def __init__(self, U, S, V, dtype=np.float64):
+        """dtype: the element type T (float64, or float32 for the factors of
+        a Float32 barycentricmatrix; the device arithmetic below is Float64)."""
+        self.U = np.asfortranarray(U, dtype=dtype)
+        self.S = np.ascontiguousarray(np.diag(S) if np.ndim(S) == 2 else S, dtype=dtype)
+        self.V = np.asfortranarray(V, dtype=dtype)
         if self.U.shape[1] != len(self.S) or self.V.shape[1] != len(self.S):
             raise ValueError("U, S, V ranks differ")
 

@@ -64,15 +64,20 @@ def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def chebyshevpoints(n, kind=1):
-    out = np.zeros(n)
-    check(lib.hm_chebyshevpoints(_dptr(out), n, kind))
+def chebyshevpoints(n, kind=1, dtype=np.float64):
+    """chebyshevpoints(T, n; kind), src/BarycentricMatrix.jl:92-111."""
+    out = np.zeros(n, dtype=dtype)
+    fn = lib.hm_chebyshevpoints_f32 if out.dtype == np.float32 else lib.hm_chebyshevpoints
+    check(fn(_dptr(out), n, kind))
     return out
 
 
-def chebyshevbarycentricweights(n, kind=1):
-    out = np.zeros(n)
-    check(lib.hm_chebyshevbarycentricweights(_dptr(out), n, kind))
+def chebyshevbarycentricweights(n, kind=1, dtype=np.float64):
+    """chebyshevbarycentricweights(T, n; kind), src/BarycentricMatrix.jl:114-136."""
+    out = np.zeros(n, dtype=dtype)
+    fn = (lib.hm_chebyshevbarycentricweights_f32 if out.dtype == np.float32
+          else lib.hm_chebyshevbarycentricweights)
+    check(fn(_dptr(out), n, kind))
     return out
 
 

@@ -9,7 +9,7 @@ Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
     `Matrix` leaves (src/HierarchicalMatrix.jl) flattened into a device plan
     (`hm_plan_from_blocks`), with `mul!` and the diagonal scalings `rmul!` /
     `lmul!` (src/HierarchicalMatrix.jl:15-16).
-  * `DeviceEvenBarycentric`: EvenBarycentricMatrix leaves (src/BarycentricMatrix.jl:5-46)
+  * `DeviceEvenBarycentric`: EvenBarycentricMatrix{Float64|Float32} leaves (src/BarycentricMatrix.jl:5-46)
     in one device plan, with their `mul!` (src/algebra.jl:168-239).
   * `device_lowrank`: LowRankMatrix `+`, `-`, `*`, `'*` with recompression
     (src/LowRankMatrix.jl:95-126) on the device.
@@ -277,41 +277,62 @@ LinearAlgebra.lmul!(b::Diagonal{Float64}, H::DeviceHierarchicalMatrix) = device_
 """
     DeviceEvenBarycentric(leaves, nrows, ncols)
 
-A device plan of `EvenBarycentricMatrix{Float64}` leaves (src/BarycentricMatrix.jl:5-46)
-placed at 0-based (row0, col0) in an nrows x ncols matrix: `leaves` is a vector
-of `(row0, col0, B)`. F comes from each B (the caller's f, evaluated by the
-reference's constructor); W is formed on the device. A single leaf at (0, 0)
-gives the leaf's own `mul!(u, B, v, istart, jstart)`.
+A device plan of `EvenBarycentricMatrix{T}` leaves, T = Float64 or Float32
+(src/BarycentricMatrix.jl:5-46), placed at 0-based (row0, col0) in an
+nrows x ncols matrix: `leaves` is a vector of `(row0, col0, B)`, all of one T.
+F comes from each B (the caller's f, evaluated by the reference's
+constructor); W is formed on the device in T. A single leaf at (0, 0) gives
+the leaf's own `mul!(u, B, v, istart, jstart)`.
 """
-mutable struct DeviceEvenBarycentric
+mutable struct DeviceEvenBarycentric{T<:Union{Float64,Float32}}
     plan::Ptr{Cvoid}
     m::Int
     n::Int
 end
 Base.size(E::DeviceEvenBarycentric) = (E.m, E.n)
+Base.eltype(::DeviceEvenBarycentric{T}) where {T} = T
+
+for (T, create, matvec) in ((Float64, :hm_even_create, :hm_even_matvec),
+                            (Float32, :hm_even_create_f32, :hm_even_matvec_f32))
+    @eval begin
+        function even_create(::Type{$T}, nrows, ncols, nl, row0, col0, abcd, rank, Fp, plan)
+            ccall(($(QuoteNode(create)), libhmx), Cint,
+                  (Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Cint,
+                   Ptr{Ptr{$T}}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                  nrows, ncols, nl, row0, col0, abcd, rank, Fp, C_NULL, plan)
+        end
+        function even_matvec(E::DeviceEvenBarycentric{$T}, u::Ptr{$T}, v::Ptr{$T}, ioff, joff,
+                             k, ldu, ldv, beta, stream)
+            ccall(($(QuoteNode(matvec)), libhmx), Cint,
+                  (Ptr{Cvoid}, Ptr{$T}, Ptr{$T}, Int64, Int64, Int64, Int64, Int64, Int64,
+                   Int64, Cint, Ptr{Cvoid}),
+                  E.plan, u, v, ioff, joff, 1, 1, k, ldu, ldv, beta, stream)
+        end
+    end
+end
 
 function DeviceEvenBarycentric(leaves::AbstractVector, nrows::Integer, ncols::Integer)
     nl = length(leaves)
     nl > 0 || throw(ArgumentError("no leaves"))
+    T = eltype(leaves[1][3].F)
+    T in (Float64, Float32) || throw(ArgumentError("element type must be Float64 or Float32"))
     rank = length(leaves[1][3].x)
     row0 = Int64[l[1] for l in leaves]
     col0 = Int64[l[2] for l in leaves]
     abcd = Int64[]
-    Fs = Matrix{Float64}[]
+    Fs = Matrix{T}[]
     for (_, _, B) in leaves
         length(B.x) == rank || throw(ArgumentError("all leaves must have one rank"))
+        eltype(B.F) == T || throw(ArgumentError("all leaves must have one element type"))
         append!(abcd, (B.a, B.b, B.c, B.d))
-        push!(Fs, Matrix{Float64}(B.F))
+        push!(Fs, Matrix{T}(B.F))
     end
     Fp = [pointer(F) for F in Fs]
     plan = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve Fs begin
-        check(ccall((:hm_even_create, libhmx), Cint,
-                    (Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Cint,
-                     Ptr{Ptr{Float64}}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
-                    nrows, ncols, nl, row0, col0, abcd, rank, Fp, C_NULL, plan))
+        check(even_create(T, nrows, ncols, nl, row0, col0, abcd, rank, Fp, plan))
     end
-    E = DeviceEvenBarycentric(plan[], nrows, ncols)
+    E = DeviceEvenBarycentric{T}(plan[], nrows, ncols)
     finalizer(e -> (e.plan != C_NULL && ccall((:hm_even_destroy, libhmx), Cint, (Ptr{Cvoid},), e.plan);
                     e.plan = C_NULL), E)
     E
@@ -319,23 +340,20 @@ end
 
 # mul!(u, B, v, istart, jstart), src/algebra.jl:168-239, for every leaf at once:
 # row I pairs with column J when (istart-1+I) + (jstart-1+J) is even
-function mul_device!(u::Ptr{Float64}, E::DeviceEvenBarycentric, v::Ptr{Float64}, istart::Int,
+function mul_device!(u::Ptr{T}, E::DeviceEvenBarycentric{T}, v::Ptr{T}, istart::Int,
                      jstart::Int; beta::Int = 1, k::Int = 1, ldu::Int = 0, ldv::Int = 0,
-                     stream::Ptr{Cvoid} = C_NULL)
-    check(ccall((:hm_even_matvec, libhmx), Cint,
-                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Int64, Int64, Int64, Int64,
-                 Int64, Cint, Ptr{Cvoid}),
-                E.plan, u, v, istart - 1, jstart - 1, 1, 1, k, ldu, ldv, beta, stream))
+                     stream::Ptr{Cvoid} = C_NULL) where {T}
+    check(even_matvec(E, u, v, istart - 1, jstart - 1, k, ldu, ldv, beta, stream))
     u
 end
 
-function LinearAlgebra.mul!(u::Vector{Float64}, E::DeviceEvenBarycentric, v::AbstractVector{Float64},
-                            istart::Int = 1, jstart::Int = 1)
-    vv = Vector{Float64}(v)
-    du, dv = devbuf(8length(u)), devbuf(8length(vv))
+function LinearAlgebra.mul!(u::Vector{T}, E::DeviceEvenBarycentric{T}, v::AbstractVector{T},
+                            istart::Int = 1, jstart::Int = 1) where {T}
+    vv = Vector{T}(v)
+    du, dv = devbuf(sizeof(T) * length(u)), devbuf(sizeof(T) * length(vv))
     try
         h2d(du, u); h2d(dv, vv)
-        mul_device!(Ptr{Float64}(du), E, Ptr{Float64}(dv), istart, jstart)
+        mul_device!(Ptr{T}(du), E, Ptr{T}(dv), istart, jstart)
         d2h(u, du)
     finally
         devfree(du); devfree(dv)

@@ -86,6 +86,10 @@ int hm_chebyshevpoints(double *out, int n, int kind);
 
 /* Replaces chebyshevbarycentricweights(T, n; kind), src/BarycentricMatrix.jl:114-136. */
 int hm_chebyshevbarycentricweights(double *out, int n, int kind);
+/* The same for T = Float32 (the quotient formed in Float32, sinpi rounded
+ * once to Float32). */
+int hm_chebyshevpoints_f32(float *out, int n, int kind);
+int hm_chebyshevbarycentricweights_f32(float *out, int n, int kind);
 
 /* Replaces indsplit(x, ir, a, b), src/BarycentricMatrix.jl:299-307.
  * ir = first:first+len-1 (1-based first, as in Julia); out = {first1, len1,
@@ -289,12 +293,17 @@ int hm_entries_f32(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j
  * on c..d, of rank n: W (n x m) interpolates in the row index at the n
  * first-kind Chebyshev nodes t_k = (a+b)/2 + (b-a)*x_k/2, and F (p x n) holds
  * f(T, t_k, j). f is the caller's function, so F comes from the caller; W is
- * formed on the device. Float64, ranks 1..64. */
+ * formed on the device. Ranks 1..64; Float64 entry points and Float32 twins
+ * (suffix _f32: W, F, u, v in Float32, computed in Float32 as the reference's
+ * EvenBarycentricMatrix{Float32}). A plan is of one element type. */
 typedef struct hm_even *hm_even_t;
 
 /* t_k, k = 1..n, the arguments at which F[j, k] = f(T, t_k, j) is evaluated
  * (src/BarycentricMatrix.jl:41). Host output. */
 int hm_even_nodes(int64_t a, int64_t b, int n, double *t);
+/* The same for T = Float32: x_k and (b-a)*x_k/2 in Float32, (a+b)/2 and the
+ * sum in Float64, as the reference's promotion gives. */
+int hm_even_nodes_f32(int64_t a, int64_t b, int n, double *t);
 
 /* W of the grid a..b into the DEVICE array W, and optionally the sums
  * w (m values, device, NULL to skip): layout 0 stores W[k + n*(i-a)], the n x m
@@ -303,6 +312,7 @@ int hm_even_nodes(int64_t a, int64_t b, int n, double *t);
  * 61-89, whose LowRankMatrix(W, I, F) then goes to hm_plan_from_blocks).
  * Synchronizes `stream`. */
 int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *w, void *stream);
+int hm_even_weights_f32(int64_t a, int64_t b, int n, int layout, float *W, float *w, void *stream);
 
 /* A plan of nleaves EvenBarycentricMatrix leaves of rank n placed in one
  * nrows x ncols matrix: leaf t spans rows row0[t] .. row0[t] + m - 1 and
@@ -312,6 +322,9 @@ int hm_even_weights(int64_t a, int64_t b, int n, int layout, double *W, double *
 int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
                    const int64_t *col0, const int64_t *abcd, int n, const double *const *F,
                    void *stream, hm_even_t *plan);
+int hm_even_create_f32(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
+                       const int64_t *col0, const int64_t *abcd, int n, const float *const *F,
+                       void *stream, hm_even_t *plan);
 
 /* Replaces mul!(u, B, v, istart, jstart) (src/algebra.jl:168-239) for every
  * leaf at once: u[q*ldu + (ioff + I)*incu] (+)= sum over leaves and columns J
@@ -320,17 +333,23 @@ int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t 
  * the reference's: its iseven(ishift + jshift) branches pair row i with
  * column j exactly when ishift + i + jshift + j is even. beta = 1 accumulates
  * (the reference's mul!), beta = 0 overwrites rows 0 .. nrows-1 of each
- * column. u, v: device. */
+ * column. u, v: device. The plan's element type must match the entry point's
+ * (HM_ERR_INVALID otherwise). */
 int hm_even_matvec(hm_even_t plan, double *u, const double *v, int64_t ioff, int64_t joff,
                    int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
                    void *stream);
+int hm_even_matvec_f32(hm_even_t plan, float *u, const float *v, int64_t ioff, int64_t joff,
+                       int64_t incu, int64_t incv, int64_t k, int64_t ldu, int64_t ldv, int beta,
+                       void *stream);
 
 /* Host copies of leaf t's W (n x m) and F (p x n); either may be NULL. */
 int hm_even_leaf_data(hm_even_t plan, int64_t leaf, double *W, double *F);
+int hm_even_leaf_data_f32(hm_even_t plan, int64_t leaf, float *W, float *F);
 
 /* info[0..]: nrows, ncols, nleaves, rank, W entries, F entries, upward tasks,
- * reduced leaves, downward tasks, segment-list entries. */
-enum { HM_EVEN_INFO_COUNT = 10 };
+ * reduced leaves, downward tasks, segment-list entries, element type (0:
+ * Float64, 1: Float32). */
+enum { HM_EVEN_INFO_COUNT = 11 };
 int hm_even_info(hm_even_t plan, int64_t *info, int n);
 int hm_even_destroy(hm_even_t plan);
 

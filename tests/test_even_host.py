@@ -62,3 +62,43 @@ def test_matvec_rejects_bad_arguments():
     assert L.hm_even_matvec(None, None, None, 0, 0, 1, 1, 1, 0, 0, 1, None) == _lib.HM_ERR_INVALID
     assert L.hm_even_weights(1, 10, 8, 2, None, None, None) == _lib.HM_ERR_INVALID
     assert L.hm_even_weights(1, 10, 8, 0, None, None, None) == _lib.HM_ERR_INVALID
+
+
+@pytest.mark.parametrize("a,b,n", [(1, 40, 10), (2, 9, 7), (-3, 1000, 10), (0, 1, 64), (7, 7, 8),
+                                   (-(1 << 30), 1 << 30, 10), (1, 1 << 22, 10)])
+def test_nodes_f32_match_oracle(a, b, n):
+    """Float32 nodes: x_k and (b-a)*x_k/2 in Float32, the rest in Float64."""
+    t = hmx.even.even_nodes(a, b, n, np.float32)
+    assert np.array_equal(t, O.even_nodes(a, b, n, f32=True))
+    if n > 1 and b > a + 8:
+        assert not np.array_equal(t, O.even_nodes(a, b, n))
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 10, 16, 33, 64])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_chebyshev_f32_match_oracle(n, kind):
+    if kind == 2 and n < 2:
+        return
+    x = hmx.chebyshevpoints(n, kind, dtype=np.float32)
+    lam = hmx.chebyshevbarycentricweights(n, kind, dtype=np.float32)
+    assert x.dtype == np.float32 and lam.dtype == np.float32
+    assert np.array_equal(x, O.chebyshevpoints(n, kind, f32=True))
+    assert np.array_equal(lam, O.chebyshevbarycentricweights(n, kind, f32=True))
+
+
+def test_create_f32_rejects_bad_leaves():
+    _fp = ctypes.POINTER(ctypes.c_float)
+    F = np.zeros((10, 8), dtype=np.float32, order="F")
+    r0 = np.array([15], dtype=np.int64)
+    c0 = np.array([0], dtype=np.int64)
+    g = np.array([1, 10, 1, 10], dtype=np.int64)
+    Fp = (_fp * 1)(F.ctypes.data_as(_fp))
+    h = ctypes.c_void_p()
+    rc = L.hm_even_create_f32(20, 20, 1, r0.ctypes.data_as(_ip), c0.ctypes.data_as(_ip),
+                              g.ctypes.data_as(_ip), 8, Fp, None, ctypes.byref(h))
+    assert rc == _lib.HM_ERR_INVALID and not h.value
+    assert b"outside" in L.hm_last_error()
+    assert L.hm_even_matvec_f32(None, None, None, 0, 0, 1, 1, 1, 0, 0, 1, None) == \
+        _lib.HM_ERR_INVALID
+    assert L.hm_even_weights_f32(1, 10, 8, 2, None, None, None) == _lib.HM_ERR_INVALID
+    assert L.hm_even_nodes_f32(1, 10, 65, None) == _lib.HM_ERR_INVALID

@@ -167,3 +167,57 @@ def test_interpolates_a_smooth_kernel_on_the_grid():
         exact = 1.0 / (i + j + s)
         err = np.max(np.abs(full - exact) / np.abs(exact))
         assert lo <= err < hi, (s, err)
+
+
+# ---------------------------------------------------------------- Float32
+def test_chebyshevpoints_f32_are_sinpi_of_the_f32_quotient_rounded_once():
+    """chebyshevpoints(Float32, n): q = (n-2k+1)/2n formed in Float32, then
+    sinpi(q) rounded once to Float32 (mpmath at 50 digits)."""
+    for n in (2, 7, 10, 16, 33, 64):
+        x = O.chebyshevpoints(n, 1, f32=True)
+        lam = O.chebyshevbarycentricweights(n, 1, f32=True)
+        for k in range(1, n // 2 + 1):
+            q = np.float32(np.float32(n - 2 * k) + np.float32(1)) / np.float32(2 * n)
+            assert x[k - 1] == np.float32(float(mp.sin(mp.pi * mpf(float(q)))))
+            assert x[n - k] == -x[k - 1]
+        for k in range(1, n // 2 + 2):
+            if k > n:
+                break
+            q = (np.float32(2 * k) - np.float32(1)) / np.float32(2 * n)
+            want = np.float32(float(mp.sin(mp.pi * mpf(float(q)))))
+            assert abs(lam[k - 1]) == want and np.sign(lam[k - 1]) == (-1) ** (k - 1)
+
+
+def test_nodes_f32_closed_form():
+    """(a+b)/2 exact in Float64, (b-a)*x_k/2 within one Float32 rounding of
+    the Float32 node times (b-a)/2."""
+    for a, b, n in [(1, 40, 10), (2, 9, 7), (-3, 1000, 10), (1, 1 << 22, 10)]:
+        t = O.even_nodes(a, b, n, f32=True)
+        x = O.chebyshevpoints(n, 1, f32=True)
+        for k in range(n):
+            h = float(b - a) * float(x[k]) / 2
+            assert abs((t[k] - (a + b) / 2) - h) <= abs(h) * 2.0 ** -23 + 1e-300, (a, b, k)
+        assert np.all(np.diff(t) < 0)
+
+
+@pytest.mark.parametrize("istart,jstart", [(1, 1), (2, 1), (4, 6)])
+def test_mul_f32_is_the_masked_product(istart, jstart):
+    """Float32 mul! against the masked dense product of the same Float32
+    factors in Float64: within a few eps(Float32) of the sum of |terms|."""
+    a, b, c, d, n = 3, 200, 7, 300, 10
+    _, W = O.even_weights(a, b, n, f32=True)
+    F = smooth_F(a, b, c, d, n).astype(np.float32)
+    m, p = b - a + 1, d - c + 1
+    rng = np.random.default_rng(istart + 5 * jstart)
+    v = rng.standard_normal(p + jstart + 3).astype(np.float32)
+    u0 = rng.standard_normal(m + istart + 3).astype(np.float32)
+    u = O.even_mul(u0.copy(), W, F, v, istart, jstart)
+    assert u.dtype == np.float32
+    M = masked_dense(W.astype(np.float64), F.astype(np.float64), istart - 1, jstart - 1)
+    vs = v[jstart - 1:jstart - 1 + p].astype(np.float64)
+    want = u0.astype(np.float64)
+    want[istart - 1:istart - 1 + m] += M @ vs
+    scale = np.abs(M) @ np.abs(vs) + 1.0
+    err = np.abs(u[istart - 1:istart - 1 + m] - want[istart - 1:istart - 1 + m])
+    assert np.all(err <= 4 * p * 2.0 ** -24 * scale)
+    assert np.array_equal(u[:istart - 1], u0[:istart - 1])
