@@ -85,7 +85,7 @@ int hm_dense_trsm_t(int64_t count, const int64_t *n, const int64_t *k, const dou
         return dfail(HM_ERR_INVALID, "dense_trsm_t: bad count or NULL arrays");
     if (count == 0) return HM_OK;
     std::vector<DenseItem> items((size_t)count);
-    int64_t max_k = 0;
+    int64_t max_k = 0, max_n = 0;
     for (int64_t t = 0; t < count; ++t) {
         if (n[t] < 0 || k[t] < 0 || n[t] > INT32_MAX || k[t] > INT32_MAX ||
             ldr[t] < std::max<int64_t>(1, n[t]) || ldb[t] < std::max<int64_t>(1, n[t]) ||
@@ -94,10 +94,11 @@ int hm_dense_trsm_t(int64_t count, const int64_t *n, const int64_t *k, const dou
         items[(size_t)t] = DenseItem{(int32_t)n[t], (int32_t)k[t], 0, 0, B[t], ldb[t], R[t], ldr[t],
                                      nullptr, 0, nullptr};
         max_k = std::max(max_k, n[t] > 0 ? k[t] : 0);
+        max_n = std::max(max_n, n[t]);
     }
     return run_items(items, (hipStream_t)stream, "dense_trsm_t",
                      [&](const DenseItems &d, int64_t *) {
-                         return hmx_dense_trsm_t(d, count, max_k, (hipStream_t)stream);
+                         return hmx_dense_trsm_t(d, count, max_n, max_k, (hipStream_t)stream);
                      });
 }
 

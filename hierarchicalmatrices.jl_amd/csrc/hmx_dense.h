@@ -24,11 +24,35 @@ struct DenseItems {
     const DenseItem *items;
     DenseItem one;
 #ifdef __HIPCC__
-    __device__ DenseItem get(int t) const { return items ? items[t] : one; }
+    // the item of workgroup-uniform index t, its fields in scalar registers
+    // (the table and the by-value copy live in different address spaces, so
+    // the loaded values are not known to be uniform otherwise)
+    __device__ DenseItem get(int t) const {
+        const DenseItem d = items ? items[t] : one;
+        DenseItem u;
+        u.n = __builtin_amdgcn_readfirstlane(d.n);
+        u.k = __builtin_amdgcn_readfirstlane(d.k);
+        u.kk = __builtin_amdgcn_readfirstlane(d.kk);
+        u.pad = 0;
+        u.A = (double *)uni64((uint64_t)d.A);
+        u.lda = (int64_t)uni64((uint64_t)d.lda);
+        u.R = (const double *)uni64((uint64_t)d.R);
+        u.ldr = (int64_t)uni64((uint64_t)d.ldr);
+        u.B = (const double *)uni64((uint64_t)d.B);
+        u.ldb = (int64_t)uni64((uint64_t)d.ldb);
+        u.D = (const double *)uni64((uint64_t)d.D);
+        return u;
+    }
+    __device__ static uint64_t uni64(uint64_t v) {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        return ((uint64_t)hi << 32) | lo;
+    }
 #endif
 };
 
 hipError_t hmx_dense_potrf(DenseItems its, int64_t count, int64_t *info, hipStream_t s);
-hipError_t hmx_dense_trsm_t(DenseItems its, int64_t count, int64_t max_k, hipStream_t s);
+hipError_t hmx_dense_trsm_t(DenseItems its, int64_t count, int64_t max_n, int64_t max_k,
+                            hipStream_t s);
 hipError_t hmx_dense_gemm(int ta, int tb, double alpha, double beta, DenseItems its,
                           int64_t count, int64_t max_tiles, hipStream_t s);

@@ -76,7 +76,7 @@ def test_solvetransposed_vs_oracle(cuda, kinds):
 
 def test_dense_kernels(cuda):
     rng = np.random.default_rng(3)
-    for n in (1, 7, 64, 300):
+    for n in (1, 7, 33, 64, 255, 256, 300, 600):
         M = rng.standard_normal((n, n))
         A = M @ M.T + n * np.eye(n)
         A_up = np.triu(A) + np.tril(rng.standard_normal((n, n)), -1)  # lower part ignored
@@ -88,9 +88,13 @@ def test_dense_kernels(cuda):
         X = hmx.cholesky.trsm_t(torch.from_numpy(np.asfortranarray(Rc)).cuda(),
                                 torch.from_numpy(B).cuda()).cpu().numpy()
         assert rel(X, sla.solve_triangular(Rc, B, trans="T")) <= 1e-12
+        b = rng.standard_normal((n, 1))
+        x = hmx.cholesky.trsm_t(torch.from_numpy(np.asfortranarray(Rc)).cuda(),
+                                torch.from_numpy(b).cuda()).cpu().numpy()
+        assert rel(x, sla.solve_triangular(Rc, b, trans="T")) <= 1e-12
     for ta in (0, 1):
         for tb in (0, 1):
-            m, n, k = 130, 70, 45
+            m, n, k = (130, 70, 45) if ta == tb else (12, 9, 300)
             A = rng.standard_normal((k, m) if ta else (m, k))
             B = rng.standard_normal((n, k) if tb else (k, n))
             C = rng.standard_normal((m, n))
