@@ -79,6 +79,37 @@ int hm_dense_potrf(int64_t count, const int64_t *n, double *const *A, const int6
     return HM_OK;
 }
 
+int hm_dense_potrf_dev(int64_t count, const int64_t *n, double *const *A, const int64_t *lda,
+                       int64_t *info, void *stream) {
+    if (count < 0 || (count > 0 && (!n || !A || !lda || !info)))
+        return dfail(HM_ERR_INVALID, "dense_potrf_dev: bad count or NULL arrays");
+    if (count == 0) return HM_OK;
+    std::vector<DenseItem> items((size_t)count);
+    for (int64_t t = 0; t < count; ++t) {
+        if (n[t] < 0 || n[t] > INT32_MAX || lda[t] < std::max<int64_t>(1, n[t]) || (n[t] > 0 && !A[t]))
+            return dfail(HM_ERR_INVALID, "dense_potrf_dev: item %lld: bad n, lda or A", (long long)t);
+        items[(size_t)t] = DenseItem{(int32_t)n[t], 0, 0, 0, A[t], lda[t], nullptr, 0, nullptr, 0,
+                                     nullptr};
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipSuccess;
+    if (count == 1) {
+        e = hmx_dense_potrf(DenseItems{nullptr, items[0]}, 1, info, s);
+    } else {
+        // the table goes up with the stream (the copy from pageable host
+        // memory has read it when the call returns) and is freed in order
+        DenseItem *d = nullptr;
+        e = hipMallocAsync((void **)&d, sizeof(DenseItem) * (size_t)count, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d, items.data(), sizeof(DenseItem) * (size_t)count,
+                               hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hmx_dense_potrf(DenseItems{d, items[0]}, count, info, s);
+        if (d) (void)hipFreeAsync(d, s);
+    }
+    if (e != hipSuccess) return dfail(HM_ERR_HIP, "dense_potrf_dev: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+
 int hm_dense_trsm_t(int64_t count, const int64_t *n, const int64_t *k, const double *const *R,
                     const int64_t *ldr, double *const *B, const int64_t *ldb, void *stream) {
     if (count < 0 || (count > 0 && (!n || !k || !R || !ldr || !B || !ldb)))

@@ -186,6 +186,8 @@ def _load():
     # dense leaf steps of the hierarchical Cholesky factorization
     L.hm_dense_potrf.restype = ctypes.c_int
     L.hm_dense_potrf.argtypes = [_i64, _ip, ctypes.POINTER(_vp), _ip, _ip, _vp]
+    L.hm_dense_potrf_dev.restype = ctypes.c_int
+    L.hm_dense_potrf_dev.argtypes = [_i64, _ip, ctypes.POINTER(_vp), _ip, _vp, _vp]
     L.hm_dense_trsm_t.restype = ctypes.c_int
     L.hm_dense_trsm_t.argtypes = [_i64, _ip, _ip, ctypes.POINTER(_vp), _ip, ctypes.POINTER(_vp),
                                   _ip, _vp]

@@ -21,7 +21,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import HM_ERR_INVALID, HMError, check, lib
 from .hierarchical import HierarchicalMatrix, LowRankMatrix
 from .kernelmatrix import Block, _stream_ptr, _torch
 from . import lowrank as _lr
@@ -84,12 +84,22 @@ def _new(m, n):
     return torch.zeros(n, m, dtype=torch.float64, device="cuda").t()  # column-major m x n
 
 
+def _empty(m, n):
+    """An uninitialized column-major m x n tensor (for outputs written whole)."""
+    torch = _torch()
+    return torch.empty(n, m, dtype=torch.float64, device="cuda").t()
+
+
 def _copy(t):
     """A new column-major tensor with the values of t (a matrix or vector)."""
     t2 = t.reshape(t.shape[0], -1) if t.dim() == 1 else t
-    out = _new(*t2.shape)
+    out = _empty(*t2.shape)
     out.copy_(t2)
     return out
+
+
+def _same(a, b):
+    return (a.data_ptr() == b.data_ptr() and a.shape == b.shape and a.stride() == b.stride())
 
 
 def _ld(t):
@@ -123,9 +133,45 @@ def potrf(A):
     return R
 
 
-def trsm_t(R, B):
-    """R^{-T} B for an upper dense R (a new tensor; B n x k)."""
-    X = _copy(B)
+def potrf_async(A, infos):
+    """Upper Cholesky factor of a dense SPD block (a new tensor), without
+    waiting: the pivot report (0 or the failing pivot + 1) is appended to
+    `infos` as a device tensor, to be read once by check_infos."""
+    torch = _torch()
+    R = _copy(A)
+    n = R.shape[0]
+    _, np_ = _i64([n])
+    _, lp = _i64([_ld(R)])
+    info = torch.empty(1, dtype=torch.int64, device="cuda")
+    check(lib.hm_dense_potrf_dev(1, np_, _ptrs([R]), lp, ctypes.c_void_p(info.data_ptr()),
+                                 _stream_ptr(None)))
+    infos.append(info)
+    return R
+
+
+def check_infos(infos):
+    """Raise for the first leaf whose factorization met a non-positive pivot
+    (Julia's PosDefException), in the order the leaves were factored."""
+    if not infos:
+        return
+    torch = _torch()
+    vals = torch.cat(infos).cpu().numpy()
+    bad = np.flatnonzero(vals)
+    if len(bad):
+        t = int(bad[0])
+        raise HMError(HM_ERR_INVALID, f"hierarchicalcholesky: leaf {t} is not positive definite "
+                      f"(pivot {int(vals[t])})")
+
+
+def trsm_t(R, B, out=None):
+    """R^{-T} B for an upper dense R (B n x k): into `out` (column-major,
+    may be B itself) or a new tensor."""
+    if out is None:
+        X = _copy(B)
+    else:
+        X = out
+        if not _same(X, B):
+            X.copy_(B)
     n, k = X.shape
     _, np_ = _i64([n])
     _, kp = _i64([k])
@@ -224,32 +270,36 @@ def _check_case(A, what):
         raise TypeError(f"{what}: no method for the block kinds ({c11}, {c12}, {c22})")
 
 
-def solvetransposed_device(R, B):
+def solvetransposed_device(R, B, out=None):
     """R^{-T} B: src/cholesky.jl:97-228. B: a tensor (vector or n x k) or a
-    DLowRank (its U is solved, S and V kept)."""
+    DLowRank (its U is solved, S and V kept). The result goes to `out` (a
+    column-major n x k tensor, which may be B itself) or a new tensor; the
+    halves of a tree are solved in place in their rows of it."""
     if isinstance(B, DLowRank):
         return DLowRank(solvetransposed_device(R, B.U), B.S, B.V)
     vec = B.dim() == 1
     B2 = B.reshape(B.shape[0], -1)
+    X = out if out is not None else _empty(*B2.shape)
     if not isinstance(R, DTree):
-        X = trsm_t(R, B2)
+        trsm_t(R, B2, out=X)
     else:
         _check_case(R, "solvetransposed")
         n = R.size(2)
         s = R.blocks[0][0].shape[1]
-        x1 = solvetransposed_device(R.blocks[0][0], B2[:s])
-        rhs = _copy(B2[s:n])
+        x1 = X[:s]
+        solvetransposed_device(R.blocks[0][0], B2[:s], out=x1)
+        rhs = X[s:n]
+        if not _same(rhs, B2[s:n]):
+            rhs.copy_(B2[s:n])
         R12 = R.blocks[0][1]
         if isinstance(R12, DLowRank):
             # rhs -= V diag(S) (U' x1)
-            T1 = _new(R12.U.shape[1], x1.shape[1])
+            T1 = _empty(R12.U.shape[1], x1.shape[1])
             gemm(1, 0, 1.0, R12.U, x1, 0.0, T1)
             gemm(0, 0, -1.0, R12.V, T1, 1.0, rhs, D=R12.S)
         else:
             gemm(1, 0, -1.0, R12, x1, 1.0, rhs)
-        x2 = solvetransposed_device(R.blocks[1][1], rhs)
-        torch = _torch()
-        X = _colmajor(torch.cat([x1, x2], dim=0))
+        solvetransposed_device(R.blocks[1][1], rhs, out=rhs)
     return X.reshape(-1) if vec else X
 
 
@@ -323,17 +373,27 @@ def _schur(A22, R12):
     return C
 
 
-def hierarchicalcholesky_device(A):
-    """src/cholesky.jl:12-95 on a device tree (or a dense device block)."""
+def _cholesky_rec(A, infos):
     if not isinstance(A, DTree):
-        return potrf(A)
+        return potrf_async(A, infos)
     _check_case(A, "hierarchicalcholesky")
     R = DTree(2, 2)
-    R11 = hierarchicalcholesky_device(A.blocks[0][0])
+    R11 = _cholesky_rec(A.blocks[0][0], infos)
     R.set(0, 0, R11)
     R12 = solvetransposed_device(R11, A.blocks[0][1])
     R.set(0, 1, R12)
-    R.set(1, 1, hierarchicalcholesky_device(_schur(A.blocks[1][1], R12)))
+    R.set(1, 1, _cholesky_rec(_schur(A.blocks[1][1], R12), infos))
+    return R
+
+
+def hierarchicalcholesky_device(A):
+    """src/cholesky.jl:12-95 on a device tree (or a dense device block). The
+    leaf factorizations do not wait for their pivot reports; those are read
+    once at the end, and a non-positive pivot raises HMError then (the
+    reference raises PosDefException at that leaf)."""
+    infos = []
+    R = _cholesky_rec(A, infos)
+    check_infos(infos)
     return R
 
 

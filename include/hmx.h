@@ -400,6 +400,13 @@ int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *cons
  * which case the call returns HM_ERR_INVALID (Julia's PosDefException). */
 int hm_dense_potrf(int64_t count, const int64_t *n, double *const *A, const int64_t *lda,
                    int64_t *info, void *stream);
+/* The same factorization with info in DEVICE memory (count values, written
+ * by the kernel) and no synchronization: the call is stream-ordered, so a
+ * recursion of many leaves runs without a wait per leaf and the caller reads
+ * the info values once. Factors after a failed pivot hold NaNs; every later
+ * step on them ends normally. */
+int hm_dense_potrf_dev(int64_t count, const int64_t *n, double *const *A, const int64_t *lda,
+                       int64_t *info, void *stream);
 
 /* Replaces solvetransposed(R::Matrix, b) = UpperTriangular(R)' \ b and its
  * Matrix / LowRankMatrix right-hand sides (src/cholesky.jl:116-142):

@@ -110,3 +110,26 @@ def test_not_positive_definite(cuda):
     with pytest.raises(hmx.HMError) as e:
         hmx.cholesky.potrf(torch.tensor([[1.0, 2.0], [2.0, 1.0]], dtype=torch.float64).cuda())
     assert e.value.code == 1 and b"positive definite" in str(e.value).encode()
+
+
+def test_indefinite_leaf_raises_after_the_recursion(cuda):
+    """The leaf factorizations report their pivots without waiting; a tree
+    whose last diagonal leaf is indefinite raises HMError (the reference's
+    PosDefException) once the recursion has ended, and the steps that ran on
+    the NaN factors in between end normally."""
+    K, H = case_matrix(1024, 128)
+    leaf = H
+    while isinstance(leaf.blocks[1][1], hmx.HierarchicalMatrix):
+        leaf = leaf.blocks[1][1]
+    D = np.array(leaf.blocks[1][1])
+    D[0, 0] = -1.0
+    leaf.blocks[1][1] = D
+    with pytest.raises(hmx.HMError) as e:
+        hmx.hierarchicalcholesky(H)
+    assert b"not positive definite" in str(e.value).encode()
+    # the device is usable afterwards
+    K2, H2 = case_matrix(512, 128)
+    Rh = hmx.hierarchicalcholesky(H2)
+    Rd = OC.densify(to_oracle(Rh))
+    Ad = OC.symmetric_upper(OC.densify(to_oracle(H2)))
+    assert np.linalg.norm(Rd.T @ Rd - Ad) <= 1e-12 * np.linalg.norm(Ad)
