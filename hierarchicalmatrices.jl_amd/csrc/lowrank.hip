@@ -315,21 +315,142 @@ __host__ __device__ inline int64_t lr_lds_need(int64_t M, int64_t C) {
     return M * C + 2 * C * (C + 1);
 }
 
-// factors too large for LDS: QR and T in global memory (L2-resident)
-__global__ __launch_bounds__(WG) void k_lr_qr(const LRItem *items, double *tall, double *small) {
+// Householder QR of a factor too large for LDS, in global memory (it stays
+// L2-resident), with the same reflectors and convention as qr_steps but
+// row-parallel: every thread owns rows i = tid, tid + WGR, ... For column j,
+// one pass over the rows forms |x_j[j+1:]|^2 and the dot products
+// x_j[j+1:] . x_k[j+1:] of all trailing columns (KCH at a time, summed over
+// the wavefronts in LDS); then v = [1; x_j[j+1:] / (alpha - beta)] and
+// w_k = tau (x_k[j] + v[j+1:] . x_k[j+1:]) follow, and a second pass scales
+// the tail of x_j and takes x_k -= w_k v from every trailing column.
+constexpr int WGR = 1024, KCH = 8;
+__global__ __launch_bounds__(WGR) void k_lr_qr(const LRItem *items, double *tall, double *small) {
     const LRItem it = items[blockIdx.x];
     const TallView T = tall_view(it, blockIdx.y, tall, small);
     if (T.C == 0 || lr_lds_need(T.M, T.C) <= LR_LDS_DOUBLES) return;
-    __shared__ double red[4 * 8], w[8];
-    qr_steps<WG>(T.X, T.M, T.C, T, red, w);
+    __shared__ double red[(WGR / 64) * KCH], sums[LR_MAX], wk[LR_MAX];
+    double *X = T.X;
+    const int M = T.M, C = T.C;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int j = 0; j < C; ++j) {
+        double *xj = X + (int64_t)j * M;
+        const int nk = C - j;  // q = 0: the norm of x_j's tail; q > 0: column j + q
+        for (int k0 = 0; k0 < nk; k0 += KCH) {
+            double part[KCH];
+#pragma unroll
+            for (int q = 0; q < KCH; ++q) part[q] = 0.0;
+            for (int i = j + 1 + tid; i < M; i += WGR) {
+                const double xi = xj[i];
+#pragma unroll
+                for (int q = 0; q < KCH; ++q)
+                    if (k0 + q < nk) part[q] += xi * X[(int64_t)(j + k0 + q) * M + i];
+            }
+#pragma unroll
+            for (int q = 0; q < KCH; ++q) {
+                const double sq = wave_sum(part[q]);
+                if (lane == 0) red[wv * KCH + q] = sq;
+            }
+            __syncthreads();
+            if (tid < KCH && k0 + tid < nk) {
+                double sq = 0.0;
+                for (int w = 0; w < WGR / 64; ++w) sq += red[w * KCH + tid];
+                sums[k0 + tid] = sq;
+            }
+            __syncthreads();
+        }
+        const double xn2 = sums[0], alpha = xj[j];
+        double beta = alpha, t = 0.0, scal = 0.0;
+        if (xn2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            t = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        if (tid > 0 && tid < nk) wk[tid] = t * (X[(int64_t)(j + tid) * M + j] + scal * sums[tid]);
+        __syncthreads();
+        if (t != 0.0) {
+            for (int i = j + tid; i < M; i += WGR) {
+                const double vi = i == j ? 1.0 : xj[i] * scal;
+                if (i > j) xj[i] = vi;
+                int q = 1;
+                for (; q + 3 < nk; q += 4) {
+                    double *x0 = X + (int64_t)(j + q) * M + i;
+                    const double a0 = x0[0], a1 = x0[M], a2 = x0[2 * (int64_t)M],
+                                 a3 = x0[3 * (int64_t)M];
+                    x0[0] = a0 - wk[q] * vi;
+                    x0[M] = a1 - wk[q + 1] * vi;
+                    x0[2 * (int64_t)M] = a2 - wk[q + 2] * vi;
+                    x0[3 * (int64_t)M] = a3 - wk[q + 3] * vi;
+                }
+                for (; q < nk; ++q) X[(int64_t)(j + q) * M + i] -= wk[q] * vi;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            xj[j] = beta;
+            T.tau[j] = t;
+            T.R[j + LR_MAX * j] = beta;
+        }
+        if (tid > 0 && tid < nk) T.R[j + LR_MAX * (j + tid)] = X[(int64_t)(j + tid) * M + j];
+        __syncthreads();
+    }
+    for (int e = tid; e < C * C; e += WGR) {
+        const int a = e % C, b = e / C;
+        if (a > b) T.R[a + LR_MAX * b] = 0.0;
+    }
 }
 
-__global__ __launch_bounds__(WG) void k_lr_wy(const LRItem *items, double *tall, double *small) {
+// T of the compact WY form for a factor too large for LDS: the Gram entries
+// G[a][b] = v_a . v_b (a < b; v_b is 1 at row b and zero above) row-parallel
+// as in k_lr_qr, one pass over the rows below b per KCH columns a, then the
+// dlarft recurrence of wy_steps in LDS.
+__global__ __launch_bounds__(WGR) void k_lr_wy(const LRItem *items, double *tall, double *small) {
     const LRItem it = items[blockIdx.x];
     const TallView T = tall_view(it, blockIdx.y, tall, small);
     if (T.C == 0 || lr_lds_need(T.M, T.C) <= LR_LDS_DOUBLES) return;
-    __shared__ double G[LR_MAX * LD], Ts[LR_MAX * LD];
-    wy_steps<WG>(T.X, T.M, T.C, T, G, Ts, LD);
+    __shared__ double G[LR_MAX * LD], Ts[LR_MAX * LD], red[(WGR / 64) * KCH];
+    const double *X = T.X;
+    const int M = T.M, C = T.C;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int b = 1; b < C; ++b) {
+        const double *vb = X + (int64_t)b * M;
+        for (int a0 = 0; a0 < b; a0 += KCH) {
+            double part[KCH];
+#pragma unroll
+            for (int q = 0; q < KCH; ++q) part[q] = 0.0;
+            for (int i = b + 1 + tid; i < M; i += WGR) {
+                const double x = vb[i];
+#pragma unroll
+                for (int q = 0; q < KCH; ++q)
+                    if (a0 + q < b) part[q] += X[(int64_t)(a0 + q) * M + i] * x;
+            }
+#pragma unroll
+            for (int q = 0; q < KCH; ++q) {
+                const double sq = wave_sum(part[q]);
+                if (lane == 0) red[wv * KCH + q] = sq;
+            }
+            __syncthreads();
+            if (tid < KCH && a0 + tid < b) {
+                double sq = X[(int64_t)(a0 + tid) * M + b];  // v_a at row b (v_b[b] = 1)
+                for (int w = 0; w < WGR / 64; ++w) sq += red[w * KCH + tid];
+                G[(a0 + tid) + LD * b] = sq;
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = 0; j < C; ++j) {
+        const double t = T.tau[j];
+        if (tid < j) {
+            double sacc = 0.0;
+            for (int l = tid; l < j; ++l) sacc += Ts[tid + LD * l] * G[l + LD * j];
+            Ts[tid + LD * j] = -t * sacc;
+        }
+        if (tid == j) Ts[j + LD * j] = t;
+        __syncthreads();
+    }
+    for (int e = tid; e < C * C; e += WGR) {
+        const int a = e % C, b = e / C;
+        T.T[a + LR_MAX * b] = a <= b ? Ts[a + LD * b] : 0.0;
+    }
 }
 
 // factors that fit: copy to LDS, QR and T there, copy the reflectors back.
@@ -585,8 +706,8 @@ hipError_t hmx_lowrank_run(int op, const LRItem *items, int64_t count, double *t
             k_lr_qr_lds<<<dim3(n, 2), WGQ, bytes, s>>>(items, tall, small);
         }
         if (big_qr) {
-            k_lr_qr<<<dim3(n, 2), WG, 0, s>>>(items, tall, small);
-            k_lr_wy<<<dim3(n, 2), WG, 0, s>>>(items, tall, small);
+            k_lr_qr<<<dim3(n, 2), WGR, 0, s>>>(items, tall, small);
+            k_lr_wy<<<dim3(n, 2), WGR, 0, s>>>(items, tall, small);
         }
     }
     k_lr_svd<<<n, WG, sizeof(double) * (size_t)lds_svd, s>>>(op, items, small);
