@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <set>
 #include <type_traits>
 #include <vector>
@@ -80,9 +81,11 @@ struct hm_even {
     std::vector<ERed> red;
     std::vector<EDown> down;
     std::vector<int32_t> list;
-    // wsize, fsize: the tiled pools; wentries, fentries: m*n and p*n summed
+    // wsize, fsize: the tiled pools; wentries, fentries: m*n over the
+    // distinct W blocks and p*n over the leaves
     int64_t wsize = 0, wentries = 0, fsize = 0, fentries = 0, nslots = 0;
     int64_t covered_rows = 0;  // rows inside some segment
+    int64_t nwblocks = 0;      // distinct W blocks (one per leaf row count)
     ELeaf *d_leaves = nullptr;
     EUp *d_up = nullptr;
     ERed *d_red = nullptr;
@@ -222,6 +225,11 @@ int even_create_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *
     P->n = n;
     P->f32 = F32;
     int64_t max_m = 0, max_pn = 0;
+    // W depends on the rows only through i - a and b - a (2i - a - b =
+    // 2(i - a) - (b - a)), so leaves with the same row count share one
+    // bit-identical W: one block per distinct m, formed once (reps)
+    std::map<int64_t, int64_t> w_of_m;
+    std::vector<ELeaf> reps;
     for (int64_t t = 0; t < nleaves; ++t) {
         const int64_t a = abcd[4 * t], b = abcd[4 * t + 1], c = abcd[4 * t + 2], d = abcd[4 * t + 3];
         int rc = check_grid(a, b, "even_create rows");
@@ -230,7 +238,7 @@ int even_create_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *
             delete P;
             return rc;
         }
-        ELeaf L{row0[t], col0[t], b - a + 1, d - c + 1, P->wsize, P->fsize, a, b, P->fentries};
+        ELeaf L{row0[t], col0[t], b - a + 1, d - c + 1, 0, P->fsize, a, b, P->fentries};
         if (L.row0 < 0 || L.col0 < 0 || L.row0 + L.m > nrows || L.col0 + L.p > ncols || !F[t]) {
             delete P;
             return efail(HM_ERR_INVALID, "even_create: leaf %lld (%lld x %lld at %lld, %lld) "
@@ -238,14 +246,23 @@ int even_create_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *
                          (long long)L.m, (long long)L.p, (long long)L.row0, (long long)L.col0,
                          (long long)nrows, (long long)ncols);
         }
-        P->wsize += even_tiled_size(L.m, n);
-        P->wentries += L.m * n;
+        const auto w = w_of_m.find(L.m);
+        if (w != w_of_m.end()) {
+            L.woff = w->second;
+        } else {
+            L.woff = P->wsize;
+            w_of_m.emplace(L.m, L.woff);
+            reps.push_back(L);
+            P->wsize += even_tiled_size(L.m, n);
+            P->wentries += L.m * n;
+        }
         P->fsize += even_tiled_size(L.p, n);
         P->fentries += L.p * n;
         max_m = std::max(max_m, L.m);
         max_pn = std::max(max_pn, L.p * n);
         P->leaves.push_back(L);
     }
+    P->nwblocks = (int64_t)reps.size();
     P->make_up();
     P->make_down();
     hipStream_t s = (hipStream_t)stream;
@@ -275,11 +292,18 @@ int even_create_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *
         e = hipMemcpy(stage + L.soff, F[t], sizeof(T) * (size_t)(L.p * n), hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hmx_even_tile_F(P->d_leaves, nleaves, max_pn, n, stage, dF, s);
-    if (e == hipSuccess)
-        e = hmx_even_gen_weights(P->d_leaves, nleaves, max_m, nodes_of<T>(n), n, 2, dW, (T *)nullptr,
-                                 s);
+    ELeaf *d_reps = nullptr;
+    if (e == hipSuccess && !reps.empty()) {
+        if (dalloc(&d_reps, (int64_t)reps.size()) != HM_OK) e = hipErrorOutOfMemory;
+        if (e == hipSuccess)
+            e = hipMemcpy(d_reps, reps.data(), sizeof(ELeaf) * reps.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hmx_even_gen_weights(d_reps, (int64_t)reps.size(), max_m, nodes_of<T>(n), n, 2, dW,
+                                     (T *)nullptr, s);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (stage) (void)hipFree(stage);
+    if (d_reps) (void)hipFree(d_reps);
     if (e != hipSuccess) {
         delete P;
         return efail(HM_ERR_HIP, "even_create: %s", hipGetErrorString(e));
@@ -379,7 +403,7 @@ int hm_even_info(hm_even_t P, int64_t *info, int n) {
     const int64_t v[HM_EVEN_INFO_COUNT] = {
         P->nrows, P->ncols, (int64_t)P->leaves.size(), P->n, P->wentries, P->fentries,
         (int64_t)P->up.size(), (int64_t)P->red.size(), (int64_t)P->down.size(),
-        (int64_t)P->list.size(), P->f32 ? 1 : 0};
+        (int64_t)P->list.size(), P->f32 ? 1 : 0, P->nwblocks};
     for (int i = 0; i < n && i < HM_EVEN_INFO_COUNT; ++i) info[i] = v[i];
     return HM_OK;
 }

@@ -265,15 +265,20 @@ class EvenBarycentricLeaves:
     @property
     def info(self):
         keys = ["nrows", "ncols", "nleaves", "rank", "w_entries", "f_entries", "n_up",
-                "n_reduce", "n_down", "list_entries", "f32"]
+                "n_reduce", "n_down", "list_entries", "f32", "w_blocks"]
         out = np.zeros(len(keys), dtype=np.int64)
         check(lib.hm_even_info(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                len(keys)))
         return dict(zip(keys, (int(v) for v in out)))
 
+    def leaf_w_entries(self):
+        """m*n summed over the leaves: the W entries the leaves would hold
+        each on their own (the plan stores w_entries, one W per row count)."""
+        return sum(m * self.rank for m, _ in self._shapes)
+
     def bytes_per_product(self, k=1):
-        """Algorithmic HBM bytes of one product: W and F once, v read and u
-        read and written, per right-hand side for the vectors."""
+        """Algorithmic HBM bytes of one product: the stored W and F once, v
+        read and u read and written, per right-hand side for the vectors."""
         i = self.info
         e = self.dtype.itemsize
         return e * (i["w_entries"] + i["f_entries"]) + e * k * (self.ncols + 2 * self.nrows)

@@ -318,7 +318,8 @@ int hm_even_weights_f32(int64_t a, int64_t b, int n, int layout, float *W, float
  * nrows x ncols matrix: leaf t spans rows row0[t] .. row0[t] + m - 1 and
  * columns col0[t] .. col0[t] + p - 1, with grids abcd[4t .. 4t+3] = (a, b, c, d)
  * and F[t] (host, p x n column-major). Leaves may overlap; products add
- * them all. W of every leaf is formed on the device. */
+ * them all. W is formed on the device, once per distinct leaf row count
+ * (the values depend on the rows only through i - a and b - a). */
 int hm_even_create(int64_t nrows, int64_t ncols, int64_t nleaves, const int64_t *row0,
                    const int64_t *col0, const int64_t *abcd, int n, const double *const *F,
                    void *stream, hm_even_t *plan);
@@ -346,10 +347,11 @@ int hm_even_matvec_f32(hm_even_t plan, float *u, const float *v, int64_t ioff, i
 int hm_even_leaf_data(hm_even_t plan, int64_t leaf, double *W, double *F);
 int hm_even_leaf_data_f32(hm_even_t plan, int64_t leaf, float *W, float *F);
 
-/* info[0..]: nrows, ncols, nleaves, rank, W entries, F entries, upward tasks,
+/* info[0..]: nrows, ncols, nleaves, rank, W entries (stored: leaves with the
+ * same row count share one bit-identical W), F entries, upward tasks,
  * reduced leaves, downward tasks, segment-list entries, element type (0:
- * Float64, 1: Float32). */
-enum { HM_EVEN_INFO_COUNT = 11 };
+ * Float64, 1: Float32), distinct W blocks. */
+enum { HM_EVEN_INFO_COUNT = 12 };
 int hm_even_info(hm_even_t plan, int64_t *info, int n);
 int hm_even_destroy(hm_even_t plan);
 

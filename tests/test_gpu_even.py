@@ -140,9 +140,14 @@ def test_leaves_vs_oracle(cuda, n):
     want = oracle_leaves(np.zeros(N), leaves, v)
     got = H.matvec(torch.from_numpy(v).to(cuda)).cpu().numpy()
     assert rel(got, want) <= 1e-14, rel(got, want)
-    # leaf data as stored on the device
-    W, F = H.leaf_data(3)
-    assert np.array_equal(W, leaves[3][2].W) and np.array_equal(F, leaves[3][2].F)
+    # one stored W per leaf size (the values depend on the rows only through
+    # i - a and b - a), bit-identical to each leaf's own W formed apart
+    assert info["w_blocks"] == levels
+    assert info["w_entries"] == sum((N >> l) * n for l in range(1, levels + 1))
+    assert H.leaf_w_entries() == sum(B.shape[0] * n for _, _, B in leaves)
+    for t in range(len(leaves)):
+        W, F = H.leaf_data(t)
+        assert np.array_equal(W, leaves[t][2].W) and np.array_equal(F, leaves[t][2].F), t
 
 
 def test_offsets_strides_and_many_rhs(cuda):
