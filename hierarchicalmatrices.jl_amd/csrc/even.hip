@@ -80,13 +80,14 @@ __global__ __launch_bounds__(64) void k_even_up(EvenDev<T> p, const T *v, int64_
     for (int64_t j = tk.j0 + lane; j < jend; j += 64) {
         // all loads of the column first, then the products: each load is
         // guarded by k < n, and a guard around load and use would wait on
-        // every load in turn
+        // every load in turn. F is read once per product: non-temporal
+        // loads (W, shared by the leaves of one size, stays cached instead)
         T x[KB], f[RB];
 #pragma unroll
         for (int q = 0; q < KB; ++q) x[q] = q < nq ? v[q * ldv + (jg + j) * incv] : T(0);
         const T *Fj = F + even_tile(j, 0, n);
 #pragma unroll
-        for (int k = 0; k < RB; ++k) f[k] = k < n ? Fj[k * EVEN_TILE] : T(0);
+        for (int k = 0; k < RB; ++k) f[k] = k < n ? __builtin_nontemporal_load(Fj + k * EVEN_TILE) : T(0);
 #pragma unroll
         for (int k = 0; k < RB; ++k)
 #pragma unroll
