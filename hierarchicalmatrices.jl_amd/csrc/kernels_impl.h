@@ -442,10 +442,11 @@ __device__ __forceinline__ T near_free(T p, const T *qs, const T *v,
 
 // Downward pass. Forward: output = rows; near blocks read by rows; far basis U
 // (or its Chebyshev form). Transpose (KB = 1 only): output = columns; near
-// blocks read by columns; far basis V. The stored blocks and bases are read
-// once per product, with non-temporal loads (the streaming hint keeps them
-// from displacing the vectors and coefficients the caches hold; it took this
-// pass from 6.15 to 6.77 TB/s at n = 2^20).
+// blocks read by columns; far basis V. With one right-hand side the stored
+// blocks and bases are read with non-temporal loads (the streaming hint keeps
+// them from displacing the vectors and coefficients the caches hold; it took
+// this pass from 6.15 to 6.77 TB/s at n = 2^20); with several per launch the
+// hint measured slower and the loads are plain.
 template <class T, int RMAX, bool FREE, int KID, bool TRANS, bool CHEB, int KB>
 __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, int nch,
                                             const DNear *nl, const DFar *fl,
@@ -507,8 +508,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
                         for (; j + UN <= nj; j += UN) {
                             T av[UN];
 #pragma unroll
-                            for (int u = 0; u < UN; ++u)
-                                av[u] = act ? __builtin_nontemporal_load(Aj + (jt + j + u) * HM_TILE) : T(0);
+                            for (int u = 0; u < UN; ++u) av[u] = act ? Aj[(jt + j + u) * HM_TILE] : T(0);
 #pragma unroll
                             for (int u = 0; u < UN; ++u)
 #pragma unroll
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
                                     acc[q] = fmaT(av[u], xs[(j + u) * KB + q], acc[q]);
                         }
                         for (; j < nj; ++j) {
-                            const T av = act ? __builtin_nontemporal_load(Aj + (jt + j) * HM_TILE) : T(0);
+                            const T av = act ? Aj[(jt + j) * HM_TILE] : T(0);
 #pragma unroll
                             for (int q = 0; q < KB; ++q) acc[q] = fmaT(av, xs[j * KB + q], acc[q]);
                         }
@@ -1254,7 +1254,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                         av[h][t] = 0.0;
                         if (rok[t] && cok) {
                             if constexpr (FREE) av[h][t] = kern_fast<double, KID>(po[t], yc);
-                            else av[h][t] = p.A[rowoff[t] + (int64_t)cc * HM_TILE];
+                            else av[h][t] = __builtin_nontemporal_load(p.A + rowoff[t] + (int64_t)cc * HM_TILE);
                         }
                     }
                 }
