@@ -13,6 +13,9 @@ Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
     in one device plan, with their `mul!` (src/algebra.jl:168-239).
   * `device_lowrank`: LowRankMatrix `+`, `-`, `*`, `'*` with recompression
     (src/LowRankMatrix.jl:95-126) on the device.
+  * `kernel_matvec(f, x, y, b, a, bb, c, d)`: `KernelMatrix(f, x, y, a, bb, c, d) * b`
+    with a ChainRulesCore `rrule` whose pullback gives the cotangents of x, y
+    and b on the device (`kernelmatrix_pullback`).
 
 Vectors may be host vectors (staged through device buffers) or device pointers
 (e.g. `pointer` of an AMDGPU.jl `ROCVector`), through `mul_device!`.
@@ -21,12 +24,13 @@ module HierarchicalMatricesAMD
 
 using LinearAlgebra
 import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, blocksize
+import ChainRulesCore
 
 const libhmx = joinpath(@__DIR__, "..", "hmx", "libhmx.so")
 const libhip = "libamdhip64"
 
 export DeviceKernelMatrix, DeviceHierarchicalMatrix, DeviceEvenBarycentric, device_lowrank, cauchykernel, coulombkernel,
-       coulombprimekernel, logkernel, mul_device!
+       coulombprimekernel, logkernel, mul_device!, kernel_matvec, kernelmatrix_pullback
 
 # kernels of examples/Kernel.jl:34-37, known to the device code by id
 @inline cauchykernel(x::T, y::T) where T = inv(x - y)
@@ -180,6 +184,74 @@ LinearAlgebra.mul!(u::Vector{T}, H::DeviceKernelMatrix{T}, v::AbstractVector{T})
 
 # H * x, src/KernelMatrix.jl:5-8
 Base.:*(H::DeviceKernelMatrix{T}, x::AbstractVector{T}) where T = mul!(zeros(T, size(H, 1)), H, x)
+
+# ------------------------------------------------- differentiation
+
+"""
+    kernelmatrix_pullback(K, b, ū) -> (x̄, ȳ, b̄)
+
+Cotangents of u = K(x, y) b for the output cotangent ū (host vectors):
+b̄ = Kᵀ ū and ȳ_j = b_j ∂(Kᵀ ū)_j/∂y_j in one adjoint pass
+(`hm_matvec_t_grad`), x̄_i = ū_i ∂(K b)_i/∂x_i (`hm_matvec_grad_x`): the exact
+derivatives of the computed product (near field regenerated with the
+kernel's derivative, far field through its Chebyshev-form basis).
+"""
+function kernelmatrix_pullback(K::DeviceKernelMatrix{T}, b::AbstractVector{T},
+                               ū::AbstractVector{T}) where T
+    m, n = size(K)
+    bb, uu = Vector{T}(b), Vector{T}(ū)
+    db, dr = devbuf(sizeof(T) * n), devbuf(sizeof(T) * m)
+    dgb, dgy, dgx = devbuf(sizeof(T) * n), devbuf(sizeof(T) * n), devbuf(sizeof(T) * m)
+    b̄, ȳ, x̄ = Vector{T}(undef, n), Vector{T}(undef, n), Vector{T}(undef, m)
+    try
+        h2d(db, bb); h2d(dr, uu)
+        if T === Float64
+            check(ccall((:hm_matvec_t_grad, libhmx), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64,
+                         Int64, Int64, Cint, Ptr{Cvoid}),
+                        K.plan, dgb, dgy, dr, db, 1, 1, 1, 1, 0, C_NULL))
+            check(ccall((:hm_matvec_grad_x, libhmx), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Int64,
+                         Int64, Cint, Ptr{Cvoid}),
+                        K.plan, dgx, db, dr, 0, 0, 1, 1, 1, 0, C_NULL))
+        else
+            check(ccall((:hm_matvec_t_grad_f32, libhmx), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64,
+                         Int64, Int64, Cint, Ptr{Cvoid}),
+                        K.plan, dgb, dgy, dr, db, 1, 1, 1, 1, 0, C_NULL))
+            check(ccall((:hm_matvec_grad_x_f32, libhmx), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int64, Int64,
+                         Int64, Cint, Ptr{Cvoid}),
+                        K.plan, dgx, db, dr, 0, 0, 1, 1, 1, 0, C_NULL))
+        end
+        d2h(b̄, dgb); d2h(ȳ, dgy); d2h(x̄, dgx)
+    finally
+        devfree(db); devfree(dr); devfree(dgb); devfree(dgy); devfree(dgx)
+    end
+    x̄, ȳ, b̄
+end
+
+"""
+    kernel_matvec(f, x, y, b, a, bb, c, d; kw...)
+
+`DeviceKernelMatrix(f, x, y, a, bb, c, d; kw...) * b`, differentiable in x, y
+and b (the ChainRulesCore `rrule` below); the intervals only select the
+tree, so they get no tangent.
+"""
+kernel_matvec(f, x::Vector{T}, y::Vector{T}, b::AbstractVector{T}, a::T, bb::T, c::T, d::T;
+              kw...) where T = DeviceKernelMatrix(f, x, y, a, bb, c, d; kw...) * b
+
+function ChainRulesCore.rrule(::typeof(kernel_matvec), f, x::Vector{T}, y::Vector{T},
+                              b::AbstractVector{T}, a::T, bb::T, c::T, d::T; kw...) where T
+    K = DeviceKernelMatrix(f, x, y, a, bb, c, d; kw...)
+    u = K * b
+    function kernel_matvec_pullback(ū)
+        x̄, ȳ, b̄ = kernelmatrix_pullback(K, b, ChainRulesCore.unthunk(ū))
+        no = ChainRulesCore.NoTangent()
+        return (no, no, x̄, ȳ, b̄, no, no, no, no)
+    end
+    return u, kernel_matvec_pullback
+end
 
 # ------------------------------------------------- HierarchicalMatrix leaves
 
