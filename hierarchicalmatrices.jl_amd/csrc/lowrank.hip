@@ -25,24 +25,6 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// workgroup sums of KC values per thread into out[0..KC) (LDS); every thread
-// must call it; ends with a barrier
-template <int KC, int NT = WG>
-__device__ inline void block_sum(const double (&v)[KC], double *red, double *out) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int kk = 0; kk < KC; ++kk) {
-        const double s = wave_sum(v[kk]);
-        if (lane == 0) red[wv * KC + kk] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < KC) {
-        double s = 0.0;
-        for (int q = 0; q < NT / 64; ++q) s += red[q * KC + threadIdx.x];
-        out[threadIdx.x] = s;
-    }
-    __syncthreads();
-}
 
 struct TallView {
     double *X;    // M x C, column-major, ld M
@@ -221,54 +203,76 @@ __global__ __launch_bounds__(WG) void k_lr_gram(int op, const LRItem *items, dou
     }
 }
 
-// Householder QR of X (M x C, M >= C) in place: v below the diagonal, R and
-// tau in the small slot. Per column j: the norm of X[j+1:, j] by the whole
-// workgroup; then the trailing columns are updated with NT/32 threads per
-// column (group g takes columns j+1+g, j+1+g+32, ..., its lanes rows j+l,
-// j+l+NT/32, ...). X is the global factor, or its copy in LDS.
+// Householder QR in place (LAPACK dlarfg convention: beta = -sign(alpha) *
+// norm, tau = (beta - alpha) / beta, v = [1; x[j+1:] / (alpha - beta)]), R
+// (upper C x C, ld LR_MAX) and tau into the small slot. Group g of NT/32
+// threads takes the columns j + g, j + g + 32, ... of step j; a step has two
+// barriers. First every group forms the dot product of the tail of x_j
+// with its column's tail by shuffles (q = 0 is |x_j[j+1:]|^2). Then every
+// thread knows beta and tau, and the groups take x_k -= w_k v with
+// w_k = tau (x_k[j] + v[j+1:] . x_k[j+1:]), v read as the unscaled tail
+// times 1/(alpha - beta), each writing its R[j, k]. The tail of x_j is scaled
+// (and beta stored) during the next step, which does not read column j.
 template <int NT>
-__device__ inline void qr_steps(double *X, int M, int C, const TallView &T, double *red,
-                                double *w) {
-    constexpr int GS = NT / 32;  // threads per trailing column
-    const int tid = threadIdx.x, grp = tid / GS, l8 = tid % GS;
+__device__ inline void qr_steps(double *X, int M, int C, const TallView &T, double *sums) {
+    constexpr int GS = NT / 32, NG = 32;  // threads per column, groups
+    const int tid = threadIdx.x, grp = tid / GS, l = tid % GS;
+    double pscal = 1.0, pbeta = 0.0;  // the previous column's pending scale and beta
     for (int j = 0; j < C; ++j) {
-        double *xj = X + (int64_t)j * M;
-        double part[1] = {0.0};
-        for (int i = j + 1 + tid; i < M; i += NT) part[0] += xj[i] * xj[i];
-        block_sum<1, NT>(part, red, w);
-        const double xn2 = w[0];
-        const double alpha = xj[j];
-        double beta = alpha, t = 0.0;
+        const double *xj = X + (int64_t)j * M;
+        if (j > 0) {
+            double *xp = X + (int64_t)(j - 1) * M;
+            for (int i = j + tid; i < M; i += NT) xp[i] *= pscal;
+            if (tid == 0) xp[j - 1] = pbeta;
+        }
+        const int nk = C - j;
+        for (int q = grp; q < nk; q += NG) {
+            const double *xk = X + (int64_t)(j + q) * M;
+            double sq = 0.0;
+            for (int i = j + 1 + l; i < M; i += GS) sq += xj[i] * xk[i];
+            for (int o = 1; o < GS; o <<= 1) sq += __shfl_xor(sq, o, 64);
+            if (l == 0) sums[q] = sq;
+        }
+        __syncthreads();
+        const double xn2 = sums[0], alpha = xj[j];
+        double beta = alpha, t = 0.0, scal = 1.0;
         if (xn2 > 0.0) {
             beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
             t = (beta - alpha) / beta;
-            const double scal = 1.0 / (alpha - beta);
-            for (int i = j + 1 + tid; i < M; i += NT) xj[i] *= scal;
+            scal = 1.0 / (alpha - beta);
         }
-        __syncthreads();
+        for (int q = 1 + grp; q < nk; q += NG) {
+            double *xk = X + (int64_t)(j + q) * M;
+            if (t != 0.0) {
+                const double wk = t * (xk[j] + scal * sums[q]);
+                for (int i = j + 1 + l; i < M; i += GS) xk[i] -= wk * (scal * xj[i]);
+                if (l == 0) {
+                    const double r = xk[j] - wk;
+                    xk[j] = r;
+                    T.R[j + LR_MAX * (j + q)] = r;
+                }
+            } else if (l == 0) {
+                T.R[j + LR_MAX * (j + q)] = xk[j];
+            }
+        }
         if (tid == 0) {
-            xj[j] = beta;
             T.tau[j] = t;
             T.R[j + LR_MAX * j] = beta;
         }
-        if (t != 0.0) {
-            for (int k = j + 1 + grp; k < C; k += 32) {
-                double *xk = X + (int64_t)k * M;
-                double sacc = 0.0;
-                for (int i = j + l8; i < M; i += GS) sacc += (i == j ? 1.0 : xj[i]) * xk[i];
-                for (int o = 1; o < GS; o <<= 1) sacc += __shfl_xor(sacc, o, 64);
-                const double wk = t * sacc;
-                for (int i = j + l8; i < M; i += GS) xk[i] -= wk * (i == j ? 1.0 : xj[i]);
-            }
-        }
+        pscal = t != 0.0 ? scal : 1.0;
+        pbeta = beta;
         __syncthreads();
-        for (int k = j + 1 + tid; k < C; k += NT) T.R[j + LR_MAX * k] = X[(int64_t)k * M + j];
-        __syncthreads();
+    }
+    if (C > 0) {
+        double *xp = X + (int64_t)(C - 1) * M;
+        for (int i = C + tid; i < M; i += NT) xp[i] *= pscal;
+        if (tid == 0) xp[C - 1] = pbeta;
     }
     for (int e = tid; e < C * C; e += NT) {
         const int a = e % C, b = e / C;
         if (a > b) T.R[a + LR_MAX * b] = 0.0;
     }
+    __syncthreads();
 }
 
 // T of Q = H_0 ... H_{C-1} = I - V T V^T (dlarft, forward, columnwise):
@@ -294,16 +298,22 @@ __device__ inline void wy_steps(const double *X, int M, int C, const TallView &T
         if (e < npair && a < b && l8 == 0) G[a + gld * b] = sacc;
     }
     __syncthreads();
-    for (int j = 0; j < C; ++j) {
-        const double t = T.tau[j];
-        if (tid < j) {
-            double sacc = 0.0;
-            for (int l = tid; l < j; ++l) sacc += Ts[tid + gld * l] * G[l + gld * j];
-            Ts[tid + gld * j] = -t * sacc;
+    // the recurrence by one wavefront (C <= 64 lanes): its LDS reads and
+    // writes stay in program order, so no workgroup barrier per column
+    if (tid < 64) {
+        for (int j = 0; j < C; ++j) {
+            const double t = T.tau[j];
+            if (tid < j) {
+                double sacc = 0.0;
+                for (int l = tid; l < j; ++l) sacc += Ts[tid + gld * l] * G[l + gld * j];
+                Ts[tid + gld * j] = -t * sacc;
+            }
+            if (tid == j) Ts[j + gld * j] = t;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         }
-        if (tid == j) Ts[j + gld * j] = t;
-        __syncthreads();
     }
+    __syncthreads();
     for (int e = tid; e < C * C; e += NT) {
         const int a = e % C, b = e / C;
         T.T[a + LR_MAX * b] = a <= b ? Ts[a + gld * b] : 0.0;
@@ -463,14 +473,14 @@ __global__ __launch_bounds__(WGQ) void k_lr_qr_lds(const LRItem *items, double *
     const TallView T = tall_view(it, blockIdx.y, tall, small);
     if (T.C == 0 || lr_lds_need(T.M, T.C) > LR_LDS_DOUBLES) return;
     extern __shared__ double dyn[];
-    __shared__ double red[16 * 1], w[8];
+    __shared__ double sums[LR_MAX];
     const int M = T.M, C = T.C;
     const int64_t cnt = (int64_t)M * C;
     double *xs = dyn;
     double *G = dyn + cnt, *Ts = G + C * (C + 1);
     for (int64_t e = threadIdx.x; e < cnt; e += WGQ) xs[e] = T.X[e];
     __syncthreads();
-    qr_steps<WGQ>(xs, M, C, T, red, w);
+    qr_steps<WGQ>(xs, M, C, T, sums);
     __syncthreads();
     wy_steps<WGQ>(xs, M, C, T, G, Ts, C + 1);
     for (int64_t e = threadIdx.x; e < cnt; e += WGQ) T.X[e] = xs[e];
