@@ -655,6 +655,40 @@ __global__ __launch_bounds__(WG) void k_lr_wz(const LRItem *items, double *tall,
 
 // out[i, :] = [B; 0][i, :] - V[i, :] W, one thread per row i; W (C x r) is
 // read at wave-uniform addresses.
+// one row of out = [S_; 0] - V W with RB accumulators (RB >= rank R): the
+// reflector entries of four columns are loaded before their products
+template <int RB>
+__device__ inline void applyq_row(const double *X, int M, int C, int R, int i, const double *Wg,
+                                  const double *B, double *out) {
+    double acc[RB];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) acc[k] = 0.0;
+    for (int t0 = 0; t0 < C; t0 += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            // V[i][t]: 0 above the diagonal, 1 on it, stored below
+            const int t = t0 + u;
+            const bool below = t < C && i > t && i < M;
+            const double x = X[below ? (int64_t)t * M + i : 0];
+            v[u] = below ? x : (t < C && i == t ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (t0 + u < C) {
+                const double *wr = Wg + LR_MAX * (t0 + u);
+#pragma unroll
+                for (int k = 0; k < RB; ++k) acc[k] += v[u] * wr[k];
+            }
+        }
+    }
+    if (i < M) {
+#pragma unroll
+        for (int k = 0; k < RB; ++k)
+            if (k < R) out[i + (int64_t)M * k] = (i < C ? B[i + LR_MAX * k] : 0.0) - acc[k];
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_lr_applyq(const LRItem *items, double *tall,
                                                   double *small) {
     const LRItem it = items[blockIdx.x];
@@ -667,25 +701,12 @@ __global__ __launch_bounds__(WG) void k_lr_applyq(const LRItem *items, double *t
     double *out = side ? it.Vo : it.Uo;
     const int M = T.M, C = T.C;
     const int R = (int)sl[LR_RANK];
-    const double *X = T.X;
     const int i = blockIdx.z * WG + threadIdx.x;
     if (blockIdx.z * WG >= M) return;
-    double acc[LR_MAX];
-#pragma unroll
-    for (int k = 0; k < LR_MAX; ++k) acc[k] = 0.0;
-    for (int t = 0; t < C; ++t) {
-        // V[i][t]: 0 above the diagonal, 1 on it, stored below
-        const double v = i < t || i >= M ? 0.0 : (i == t ? 1.0 : X[(int64_t)t * M + i]);
-        const double *wr = Wg + LR_MAX * t;
-#pragma unroll
-        for (int k = 0; k < LR_MAX; ++k)
-            if (k < R) acc[k] += v * wr[k];
-    }
-    if (i < M) {
-#pragma unroll
-        for (int k = 0; k < LR_MAX; ++k)
-            if (k < R) out[i + (int64_t)M * k] = (i < C ? B[i + LR_MAX * k] : 0.0) - acc[k];
-    }
+    // W's columns past R hold finite values (or zeros) and are not stored
+    if (R <= 16) applyq_row<16>(T.X, M, C, R, i, Wg, B, out);
+    else if (R <= 32) applyq_row<32>(T.X, M, C, R, i, Wg, B, out);
+    else applyq_row<LR_MAX>(T.X, M, C, R, i, Wg, B, out);
 }
 
 __global__ void k_lr_ranks(const LRItem *items, int64_t count, const double *small,
