@@ -653,10 +653,9 @@ __global__ __launch_bounds__(WG) void k_lr_wz(const LRItem *items, double *tall,
     }
 }
 
-// out[i, :] = [B; 0][i, :] - V[i, :] W, one thread per row i; W (C x r) is
-// read at wave-uniform addresses.
-// one row of out = [S_; 0] - V W with RB accumulators (RB >= rank R): the
-// reflector entries of four columns are loaded before their products
+// out[i, :] = [B; 0][i, :] - V[i, :] W, one thread per row i, with RB
+// accumulators (RB >= rank R); W (C x r) is read at wave-uniform addresses,
+// and the reflector entries of four columns are loaded before their products
 template <int RB>
 __device__ inline void applyq_row(const double *X, int M, int C, int R, int i, const double *Wg,
                                   const double *B, double *out) {
