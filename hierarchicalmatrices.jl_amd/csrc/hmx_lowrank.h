@@ -44,7 +44,8 @@ struct LRItem {
 // op codes: the enum hm_lr_op of include/hmx.h. rank (device, count values)
 // receives each result's rank; max_rows = the largest m or n of the items;
 // lds_qr = the largest in-LDS factorization (doubles, 0: none), big_qr = some
-// factor does not fit; lds_svd = the largest Jacobi workspace (doubles).
+// factor does not fit; lds_svd = the largest Jacobi workspace (doubles);
+// max_c = the largest factor width (cu, cv) of the items.
 hipError_t hmx_lowrank_run(int op, const LRItem *items, int64_t count, double *tall,
                            double *small, int64_t *rank, int64_t max_rows, int64_t lds_qr,
-                           bool big_qr, int64_t lds_svd, hipStream_t s);
+                           bool big_qr, int64_t lds_svd, int64_t max_c, hipStream_t s);

@@ -37,7 +37,7 @@ int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *cons
     if (count == 0) return HM_OK;
     if (count >= (int64_t)1 << 31) return lfail(HM_ERR_INVALID, "lowrank_op: too many items");
     std::vector<LRItem> items((size_t)count);
-    int64_t tall = 0, max_rows = 0, lds_qr = 0, lds_svd = 0;
+    int64_t tall = 0, max_rows = 0, lds_qr = 0, lds_svd = 0, max_c = 1;
     bool big_qr = false;
     for (int64_t t = 0; t < count; ++t) {
         const int64_t *d = dims + 5 * t;
@@ -96,6 +96,7 @@ int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *cons
         }
         it.xu = tall;
         tall += (int64_t)it.cu * m;
+        max_c = std::max<int64_t>(max_c, std::max(it.cu, it.cv));
         it.xv = tall;
         tall += (int64_t)it.cv * n;
         it.small = t * LR_SMALL;
@@ -122,7 +123,7 @@ int hm_lowrank_op(int op, int64_t count, const int64_t *dims, const double *cons
         e = hipMemcpyAsync(d_items, items.data(), sizeof(LRItem) * (size_t)count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hmx_lowrank_run(op, d_items, count, d_tall, d_small, d_rank, max_rows, lds_qr, big_qr,
-                            lds_svd, s);
+                            lds_svd, max_c, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(rank, d_rank, sizeof(int64_t) * (size_t)count, hipMemcpyDeviceToHost, s);
     hipError_t e2 = hipSuccess;

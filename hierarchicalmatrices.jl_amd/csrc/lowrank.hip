@@ -626,7 +626,8 @@ __global__ __launch_bounds__(WG) void k_lr_svd(int op, const LRItem *items, doub
 // first r columns of SU or SV (C x r) and V unit lower triangular in its top
 // C rows. W goes to the RU / RV place of the small slot (the SVD has used
 // R by now), where k_lr_applyq reads it with wave-uniform addresses.
-__global__ __launch_bounds__(WG) void k_lr_wz(const LRItem *items, double *tall, double *small) {
+__global__ __launch_bounds__(WG) void k_lr_wz(const LRItem *items, double *tall, double *small,
+                                              int ldw) {
     const LRItem it = items[blockIdx.x];
     const int side = blockIdx.y;
     const TallView T = tall_view(it, side, tall, small);
@@ -637,18 +638,33 @@ __global__ __launch_bounds__(WG) void k_lr_wz(const LRItem *items, double *tall,
     const int M = T.M, C = T.C, tid = threadIdx.x;
     const int R = (int)sl[LR_RANK];
     const double *X = T.X;
-    __shared__ double Z[LR_MAX * LD];
+    // the top C rows of V, B's first r columns and T staged in LDS (one
+    // batch of loads each) before the products; ldw = the call's largest C
+    // plus one (dynamic LDS, 4 * C * ldw doubles)
+    extern __shared__ double wzs[];
+    const int LW = ldw, CW = ldw - 1;
+    double *Z = wzs, *Vt = Z + CW * LW, *Bs = Vt + CW * LW, *Ts = Bs + CW * LW;
+    for (int e = tid; e < C * C; e += WG) {
+        const int i = e % C, a = e / C;
+        Vt[i + LW * a] = X[(int64_t)a * M + i];
+        Ts[i + LW * a] = T.T[i + LR_MAX * a];
+    }
+    for (int e = tid; e < C * R; e += WG) {
+        const int i = e % C, k = e / C;
+        Bs[i + LW * k] = B[i + LR_MAX * k];
+    }
+    __syncthreads();
     for (int e = tid; e < C * R; e += WG) {
         const int a = e % C, k = e / C;
-        double z = B[a + LR_MAX * k];  // V[a][a] = 1, V[i][a] stored below
-        for (int i = a + 1; i < C; ++i) z += X[(int64_t)a * M + i] * B[i + LR_MAX * k];
-        Z[a + LD * k] = z;
+        double z = Bs[a + LW * k];  // V[a][a] = 1, V[i][a] stored below
+        for (int i = a + 1; i < C; ++i) z += Vt[i + LW * a] * Bs[i + LW * k];
+        Z[a + LW * k] = z;
     }
     __syncthreads();
     for (int e = tid; e < C * R; e += WG) {
         const int a = e % C, k = e / C;
         double wv = 0.0;
-        for (int l = a; l < C; ++l) wv += T.T[a + LR_MAX * l] * Z[l + LD * k];
+        for (int l = a; l < C; ++l) wv += Ts[a + LW * l] * Z[l + LW * k];
         Wg[k + LR_MAX * a] = wv;  // row a contiguous: W[a][0..r)
     }
 }
@@ -718,7 +734,7 @@ __global__ void k_lr_ranks(const LRItem *items, int64_t count, const double *sma
 
 hipError_t hmx_lowrank_run(int op, const LRItem *items, int64_t count, double *tall,
                            double *small, int64_t *rank, int64_t max_rows, int64_t lds_qr,
-                           bool big_qr, int64_t lds_svd, hipStream_t s) {
+                           bool big_qr, int64_t lds_svd, int64_t max_c, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     const unsigned n = (unsigned)count;
     const unsigned rowblk = (unsigned)((max_rows + WG - 1) / WG);
@@ -742,7 +758,12 @@ hipError_t hmx_lowrank_run(int op, const LRItem *items, int64_t count, double *t
     }
     k_lr_svd<<<n, WG, sizeof(double) * (size_t)lds_svd, s>>>(op, items, small);
     if (op != HM_LR_MUL && op != HM_LR_ADJMUL) {
-        k_lr_wz<<<dim3(n, 2), WG, 0, s>>>(items, tall, small);
+        const int ldw = (int)max_c + 1;
+        const size_t wz_bytes = sizeof(double) * 4 * (size_t)max_c * ldw;
+        if (wz_bytes > 65536)
+            (void)hipFuncSetAttribute((const void *)k_lr_wz,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)wz_bytes);
+        k_lr_wz<<<dim3(n, 2), WG, wz_bytes, s>>>(items, tall, small, ldw);
         k_lr_applyq<<<dim3(n, 2, rowblk), WG, 0, s>>>(items, tall, small);
     }
     k_lr_gemm<LR_MAX><<<dim3(n, 2, rowblk), WG, 0, s>>>(op, 1, items, tall, small);
