@@ -133,3 +133,58 @@ def test_indefinite_leaf_raises_after_the_recursion(cuda):
     Rd = OC.densify(to_oracle(Rh))
     Ad = OC.symmetric_upper(OC.densify(to_oracle(H2)))
     assert np.linalg.norm(Rd.T @ Rd - Ad) <= 1e-12 * np.linalg.norm(Ad)
+
+
+def _dev_cm(a):
+    return torch.from_numpy(np.asfortranarray(a)).cuda().t().contiguous().t()
+
+
+def test_batched_dense_entry_points(cuda):
+    """Several items per call: hm_dense_potrf_dev (pivot reports in device
+    memory, no wait; one item indefinite) and hm_dense_trsm_t with sizes on
+    both sides of the LDS path's limit (256 rows)."""
+    import ctypes
+    from hmx import _lib
+    L = _lib.lib
+    rng = np.random.default_rng(11)
+    ns = [40, 256, 100, 3]
+    As, Rc = [], []
+    for t, n in enumerate(ns):
+        M = rng.standard_normal((n, n))
+        A = M @ M.T + n * np.eye(n)
+        if t == 3:
+            A = -np.eye(n)  # first pivot negative
+        else:
+            Rc.append(np.linalg.cholesky(A).T)
+        As.append(_dev_cm(A))
+    p64 = lambda v: np.asarray(v, dtype=np.int64)
+    n_ = p64(ns)
+    ld = p64([a.stride(1) for a in As])
+    ptrs = (ctypes.c_void_p * len(As))(*[a.data_ptr() for a in As])
+    info = torch.full((len(ns),), -7, dtype=torch.int64, device="cuda")
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    assert L.hm_dense_potrf_dev(len(ns), ip(n_), ptrs, ip(ld), ctypes.c_void_p(info.data_ptr()),
+                                None) == 0
+    got = info.cpu().numpy()
+    assert list(got[:3]) == [0, 0, 0] and got[3] == 1
+    for t in range(3):
+        R = As[t].cpu().numpy()
+        assert np.allclose(R, Rc[t], rtol=0, atol=1e-12 * np.abs(Rc[t]).max()), t
+    for sizes in ([(40, 3), (256, 20), (100, 1)], [(40, 3), (300, 17), (7, 2)]):
+        Rs, Bs, want = [], [], []
+        for n, k in sizes:
+            M = rng.standard_normal((n, n))
+            R = np.linalg.cholesky(M @ M.T + n * np.eye(n)).T
+            B = rng.standard_normal((n, k))
+            Rs.append(_dev_cm(R))
+            Bs.append(_dev_cm(B))
+            want.append(sla.solve_triangular(R, B, trans="T"))
+        nn = p64([n for n, _ in sizes])
+        kk = p64([k for _, k in sizes])
+        lr = p64([r.stride(1) for r in Rs])
+        lb = p64([b.stride(1) if b.shape[1] > 1 else b.shape[0] for b in Bs])
+        rp = (ctypes.c_void_p * 3)(*[r.data_ptr() for r in Rs])
+        bp = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in Bs])
+        assert L.hm_dense_trsm_t(3, ip(nn), ip(kk), rp, ip(lr), bp, ip(lb), None) == 0
+        for b, w in zip(Bs, want):
+            assert rel(b.cpu().numpy(), w) <= 1e-12
