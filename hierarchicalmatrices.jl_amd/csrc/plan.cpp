@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <unordered_map>
@@ -219,6 +220,17 @@ struct Builder {
     }
 };
 
+// Stable counting sort of the indices 0 .. n-1 by an integer key in
+// [0, nkeys): the order std::stable_sort by that key gives, in O(n + nkeys).
+template <class Key>
+std::vector<int32_t> counting_order(size_t n, int64_t nkeys, Key key) {
+    std::vector<int32_t> cnt((size_t)nkeys + 1, 0), out(n);
+    for (size_t k = 0; k < n; ++k) cnt[(size_t)key(k) + 1]++;
+    for (int64_t q = 0; q < nkeys; ++q) cnt[q + 1] += cnt[q];
+    for (size_t k = 0; k < n; ++k) out[cnt[(size_t)key(k)]++] = (int32_t)k;
+    return out;
+}
+
 // Chunks over one side: boundaries of every block and cluster, split to
 // HM_CHUNK; then CSR lists of the dense blocks and clusters covering each.
 struct Side {
@@ -232,20 +244,35 @@ struct Side {
 void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
                 std::vector<DCluster> &cl, Side &S) {
     std::vector<int64_t> bnd;
-    bnd.reserve(2 * (dense.size() + cl.size()) + 2);
-    bnd.push_back(0);
-    bnd.push_back(n);
-    for (const DDense &D : dense) {
-        const int64_t s = rows ? D.row0 : D.col0, l = rows ? D.m : D.n;
-        bnd.push_back(s);
-        bnd.push_back(s + l);
+    const size_t nb = 2 * (dense.size() + cl.size()) + 2;
+    const bool marked = (size_t)n + 1 <= 64 * nb;
+    if (marked) {
+        // sorted distinct boundaries from a mark per point: linear in n
+        std::vector<uint8_t> mark((size_t)n + 1, 0);
+        mark[0] = mark[n] = 1;
+        for (const DDense &D : dense) {
+            const int64_t s = rows ? D.row0 : D.col0, l = rows ? D.m : D.n;
+            mark[s] = mark[s + l] = 1;
+        }
+        for (const DCluster &C : cl) mark[C.first] = mark[C.first + C.len] = 1;
+        for (int64_t i = 0; i <= n; ++i)
+            if (mark[i]) bnd.push_back(i);
+    } else {
+        bnd.reserve(nb);
+        bnd.push_back(0);
+        bnd.push_back(n);
+        for (const DDense &D : dense) {
+            const int64_t s = rows ? D.row0 : D.col0, l = rows ? D.m : D.n;
+            bnd.push_back(s);
+            bnd.push_back(s + l);
+        }
+        for (const DCluster &C : cl) {
+            bnd.push_back(C.first);
+            bnd.push_back(C.first + C.len);
+        }
+        std::sort(bnd.begin(), bnd.end());
+        bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
     }
-    for (const DCluster &C : cl) {
-        bnd.push_back(C.first);
-        bnd.push_back(C.first + C.len);
-    }
-    std::sort(bnd.begin(), bnd.end());
-    bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
     std::vector<int64_t> p0;
     for (size_t k = 0; k + 1 < bnd.size(); ++k)
         for (int64_t s = bnd[k]; s < bnd[k + 1]; s += HM_CHUNK) p0.push_back(s);
@@ -256,7 +283,21 @@ void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
         const int64_t e = c + 1 < nch ? p0[c + 1] : n;
         S.ch[c].np = (int32_t)(e - p0[c]);
     }
+    // every block and cluster starts and ends on a boundary, and every
+    // boundary but n starts a chunk: index the chunks by their first point
+    // (only boundary entries are written or read)
+    std::unique_ptr<int32_t[]> pos;
+    if (marked) {
+        pos.reset(new int32_t[(size_t)n + 1]);
+        for (int64_t c = 0; c < nch; ++c) pos[p0[c]] = (int32_t)c;
+        pos[n] = (int32_t)nch;
+    }
     auto range = [&](int64_t s, int64_t l, int64_t &c0, int64_t &c1) {
+        if (pos) {
+            c0 = pos[s];
+            c1 = pos[s + l];
+            return;
+        }
         c0 = std::lower_bound(p0.begin(), p0.end(), s) - p0.begin();
         c1 = std::lower_bound(p0.begin(), p0.end(), s + l) - p0.begin();
     };
@@ -267,10 +308,10 @@ void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
         for (int64_t c = c0; c < c1; ++c) ncount[c + 1]++;
     }
     // clusters in order of depth so every chunk lists coarse levels first
-    std::vector<int32_t> order(cl.size());
-    for (size_t k = 0; k < cl.size(); ++k) order[k] = (int32_t)k;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return cl[a].depth < cl[b].depth; });
+    int32_t maxd = 0;
+    for (const DCluster &C : cl) maxd = std::max(maxd, (int32_t)C.depth);
+    const std::vector<int32_t> order =
+        counting_order(cl.size(), (int64_t)maxd + 1, [&](size_t k) { return cl[k].depth; });
     for (int32_t id : order) {
         const DCluster &C = cl[id];
         int64_t c0, c1;
@@ -321,8 +362,14 @@ void build_side(int64_t n, const std::vector<DDense> &dense, bool rows,
     }
     for (DTask &T : S.tasks) T.single = cl[T.cl].nslots == 1 ? 1 : 0;
     S.nslots = slot;
-    std::stable_sort(S.tasks.begin(), S.tasks.end(),
-                     [](const DTask &u, const DTask &v) { return u.len > v.len; });
+    {
+        // largest first, stable: key HM_TASK_POINTS - len for len in 1 .. HM_TASK_POINTS
+        const std::vector<int32_t> by_len = counting_order(
+            S.tasks.size(), HM_TASK_POINTS, [&](size_t k) { return HM_TASK_POINTS - S.tasks[k].len; });
+        std::vector<DTask> sorted(S.tasks.size());
+        for (size_t k = 0; k < by_len.size(); ++k) sorted[k] = S.tasks[by_len[k]];
+        S.tasks.swap(sorted);
+    }
     for (int64_t c = 0; c < nch; ++c) {
         S.ch[c].near_beg = ncount[c];
         S.ch[c].near_end = ncount[c + 1];
@@ -354,6 +401,70 @@ int dalloc(T **dst, int64_t n) {
     return HM_OK;
 }
 
+// Packs a plan's tables into one device allocation: copied tables first, filled
+// by one synchronous copy from a host staging buffer, then uninitialised
+// workspace. A plan then pays one hipMalloc, one copy and one hipFree for all
+// of them instead of one of each per table (hipFree waits for the device, so
+// per-table frees serialise a rebuild-per-step caller). Offsets are 256-byte
+// aligned; the pointers stored through add()/reserve() are interior pointers
+// into the arena and must never be freed on their own.
+struct Staging {
+    struct Item {
+        void **dst;
+        const void *src;  // nullptr for workspace
+        size_t bytes;
+    };
+    std::vector<Item> items;
+
+    template <class T>
+    void add(T **dst, const std::vector<T> &src) {
+        *dst = nullptr;
+        if (!src.empty()) items.push_back({(void **)dst, src.data(), sizeof(T) * src.size()});
+    }
+    template <class T>
+    void reserve(T **dst, int64_t n) {
+        *dst = nullptr;
+        if (n > 0) items.push_back({(void **)dst, nullptr, sizeof(T) * (size_t)n});
+    }
+    // sources must stay alive until commit; *arena owns the memory afterwards
+    int commit(void **arena) {
+        *arena = nullptr;
+        const auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+        size_t copied = 0, total = 0;
+        for (const Item &I : items)
+            if (I.src) copied += up(I.bytes);
+        total = copied;
+        for (const Item &I : items)
+            if (!I.src) total += up(I.bytes);
+        if (!total) return HM_OK;
+        std::vector<unsigned char> host(copied);
+        std::vector<size_t> off(items.size());
+        size_t c = 0, w = copied;
+        for (size_t q = 0; q < items.size(); ++q) {
+            const Item &I = items[q];
+            if (I.src) {
+                off[q] = c;
+                std::memcpy(host.data() + c, I.src, I.bytes);
+                c += up(I.bytes);
+            } else {
+                off[q] = w;
+                w += up(I.bytes);
+            }
+        }
+        void *d = nullptr;
+        const hipError_t e = hipMalloc(&d, total);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(HM_ERR_OOM, "hipMalloc of %zu bytes of plan tables failed: %s", total,
+                        hipGetErrorString(e));
+        }
+        *arena = d;
+        if (copied) HM_HIP(hipMemcpy(d, host.data(), copied, hipMemcpyHostToDevice));
+        for (size_t q = 0; q < items.size(); ++q) *items[q].dst = (unsigned char *)d + off[q];
+        return HM_OK;
+    }
+};
+
 // Device arrays whose element type is the plan's (double or float), and the
 // device view built from them.
 template <class T>
@@ -365,10 +476,13 @@ struct PlanBufs {
     T *A = nullptr, *U = nullptr, *V = nullptr, *F = nullptr;
     T *partial = nullptr, *w = nullptr, *z = nullptr;
     T *xt = nullptr;  // packed right-hand sides of the MFMA passes (ny x 16)
+    // x .. chebA and the workspace partial .. xt live in this one allocation
+    // (Staging); A, U, V and F are allocated on their own
+    void *arena = nullptr;
     DevPlanT<T> dp{};
 
     void release() {
-        void *b[] = {x, y, lam, rnodes, cnodes, chebA, rmap, cmap, A, U, V, F, partial, w, z, xt};
+        void *b[] = {arena, A, U, V, F};
         for (void *q : b)
             if (q) (void)hipFree(q);
     }
@@ -411,6 +525,7 @@ struct hm_plan {
     DTask *d_rtask = nullptr, *d_ctask = nullptr;
     std::vector<int32_t> multi_c;     // column clusters with more than one upward task
     int32_t *d_multi_c = nullptr;
+    void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_multi_c (Staging)
     bool profile = false;
     bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
@@ -424,10 +539,7 @@ struct hm_plan {
 
     ~hm_plan() {
         drop_events();
-        void *bufs[] = {d_rcl, d_ccl, d_far, d_far_by_rc, d_far_by_cc, d_dense, d_rch, d_cch,
-                        d_rnear, d_cnear, d_rfar, d_cfar, d_rtask, d_ctask, d_multi_c};
-        for (void *b : bufs)
-            if (b) (void)hipFree(b);
+        if (d_tables) (void)hipFree(d_tables);
         b64.release();
         b32.release();
     }
@@ -577,20 +689,24 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
     const std::vector<T> xv(x, x + P->nx), yv(y, y + P->ny);
     int rc;
 #define HM_TRY_T(expr) do { rc = (expr); if (rc != HM_OK) return rc; } while (0)
-    HM_TRY_T(upload(&b.x, xv));
-    HM_TRY_T(upload(&b.y, yv));
-    HM_TRY_T(upload(&b.lam, lam));
-    HM_TRY_T(upload(&b.rnodes, rn));
-    HM_TRY_T(upload(&b.cnodes, cn));
-    HM_TRY_T(upload(&b.rmap, rmap));
-    HM_TRY_T(upload(&b.cmap, cmap));
-    HM_TRY_T(upload(&b.chebA, chebA));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY_T(dalloc(&b.partial, nslots * r * HM_WS_RHS));
-    HM_TRY_T(dalloc(&b.w, ncl * r * HM_WS_RHS));
-    HM_TRY_T(dalloc(&b.z, ncl * r * HM_WS_RHS));
-    if (std::is_same_v<T, double>) HM_TRY_T(dalloc(&b.xt, P->ny * HM_WS_RHS));
+    {
+        Staging st;
+        st.add(&b.x, xv);
+        st.add(&b.y, yv);
+        st.add(&b.lam, lam);
+        st.add(&b.rnodes, rn);
+        st.add(&b.cnodes, cn);
+        st.add(&b.rmap, rmap);
+        st.add(&b.cmap, cmap);
+        st.add(&b.chebA, chebA);
+        st.reserve(&b.partial, nslots * r * HM_WS_RHS);
+        st.reserve(&b.w, ncl * r * HM_WS_RHS);
+        st.reserve(&b.z, ncl * r * HM_WS_RHS);
+        if (std::is_same_v<T, double>) st.reserve(&b.xt, P->ny * HM_WS_RHS);
+        HM_TRY_T(st.commit(&b.arena));
+    }
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
     const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
     if (stored) {
@@ -707,6 +823,19 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         Builder Bt{};
         Bt.x = I.x; Bt.y = I.y; Bt.nx = I.nx; Bt.ny = I.ny; Bt.bs = blocksize; Bt.r = rank;
         Bt.f32 = f32;
+        {
+            // a banded tree holds about 18 leaves, 12 far blocks and 4 clusters
+            // per side for every blocksize points
+            const size_t q = (size_t)(std::max(I.nx, I.ny) / std::max(blocksize, 1)) + 16;
+            Bt.leaves.reserve(20 * q);
+            Bt.far.reserve(14 * q);
+            Bt.far_depth.reserve(14 * q);
+            Bt.dense.reserve(8 * q);
+            Bt.rcl.reserve(5 * q);
+            Bt.ccl.reserve(5 * q);
+            Bt.rmap.reserve(5 * q);
+            Bt.cmap.reserve(5 * q);
+        }
         Bt.node(0, 0, 0, I.nx, 0, I.ny, I.a, I.b, I.c, I.d);
         if (Bt.err) {
             return count == 1 ? fail(Bt.err, "%s", Bt.msg.c_str())
@@ -715,6 +844,10 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         const int64_t ro = row_off[t], co = col_off[t];
         row_off[t + 1] = ro + I.nx;
         col_off[t + 1] = co + I.ny;
+        if (count == 1) {  // offsets are zero: take the tree as built
+            B = std::move(Bt);
+            break;
+        }
         const int32_t rb = (int32_t)B.rcl.size(), cb = (int32_t)B.ccl.size();
         for (Leaf L : Bt.leaves) { L.row0 += ro; L.col0 += co; B.leaves.push_back(L); }
         for (DDense D : Bt.dense) { D.row0 += ro; D.col0 += co; B.dense.push_back(D); }
@@ -761,10 +894,10 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     const int64_t nF = (int64_t)P->far.size() * r * r;
 
     // far blocks grouped by row / column cluster
-    std::vector<int32_t> by_rc(P->far.size()), by_cc(P->far.size());
-    for (size_t k = 0; k < P->far.size(); ++k) by_rc[k] = by_cc[k] = (int32_t)k;
-    std::stable_sort(by_rc.begin(), by_rc.end(), [&](int32_t u, int32_t v) { return P->far[u].rc < P->far[v].rc; });
-    std::stable_sort(by_cc.begin(), by_cc.end(), [&](int32_t u, int32_t v) { return P->far[u].cc < P->far[v].cc; });
+    std::vector<int32_t> by_rc = counting_order(P->far.size(), (int64_t)P->rcl.size(),
+                                                [&](size_t k) { return P->far[k].rc; });
+    std::vector<int32_t> by_cc = counting_order(P->far.size(), (int64_t)P->ccl.size(),
+                                                [&](size_t k) { return P->far[k].cc; });
     for (auto &C : P->rcl) C.far_beg = C.far_end = 0;
     for (auto &C : P->ccl) C.far_beg = C.far_end = 0;
     for (size_t k = 0; k < by_rc.size(); ++k) {
@@ -806,22 +939,26 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     }
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
-    HM_TRY(upload(&P->d_rcl, P->rcl));
-    HM_TRY(upload(&P->d_ccl, P->ccl));
-    HM_TRY(upload(&P->d_far, P->far));
-    HM_TRY(upload(&P->d_far_by_rc, by_rc));
-    P->far_by_rc_h = by_rc;
-    HM_TRY(upload(&P->d_far_by_cc, by_cc));
-    HM_TRY(upload(&P->d_dense, P->dense));
-    HM_TRY(upload(&P->d_rch, P->rows.ch));
-    HM_TRY(upload(&P->d_cch, P->cols.ch));
-    HM_TRY(upload(&P->d_rnear, P->rows.near));
-    HM_TRY(upload(&P->d_cnear, P->cols.near));
-    HM_TRY(upload(&P->d_rfar, P->rows.farl));
-    HM_TRY(upload(&P->d_cfar, P->cols.farl));
-    HM_TRY(upload(&P->d_rtask, P->rows.tasks));
-    HM_TRY(upload(&P->d_ctask, P->cols.tasks));
-    HM_TRY(upload(&P->d_multi_c, P->multi_c));
+    {
+        Staging st;
+        st.add(&P->d_rcl, P->rcl);
+        st.add(&P->d_ccl, P->ccl);
+        st.add(&P->d_far, P->far);
+        st.add(&P->d_far_by_rc, by_rc);
+        st.add(&P->d_far_by_cc, by_cc);
+        st.add(&P->d_dense, P->dense);
+        st.add(&P->d_rch, P->rows.ch);
+        st.add(&P->d_cch, P->cols.ch);
+        st.add(&P->d_rnear, P->rows.near);
+        st.add(&P->d_cnear, P->cols.near);
+        st.add(&P->d_rfar, P->rows.farl);
+        st.add(&P->d_cfar, P->cols.farl);
+        st.add(&P->d_rtask, P->rows.tasks);
+        st.add(&P->d_ctask, P->cols.tasks);
+        st.add(&P->d_multi_c, P->multi_c);
+        HM_TRY(st.commit(&P->d_tables));
+    }
+    P->far_by_rc_h = std::move(by_rc);
     if (f32) HM_TRY(setup_typed<float>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
     else HM_TRY(setup_typed<double>(P, x, y, B.rcl, B.ccl, flags, nA, nU, nV, nF, (hipStream_t)stream));
 #undef HM_TRY
@@ -946,21 +1083,25 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     }
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
-    HM_TRY(upload(&P->d_rcl, P->rcl));
-    HM_TRY(upload(&P->d_ccl, P->ccl));
-    HM_TRY(upload(&P->d_far, P->far));
-    HM_TRY(upload(&P->d_far_by_rc, by_rc));
-    HM_TRY(upload(&P->d_far_by_cc, by_cc));
-    P->far_by_rc_h = by_rc;
-    HM_TRY(upload(&P->d_dense, P->dense));
-    HM_TRY(upload(&P->d_rch, P->rows.ch));
-    HM_TRY(upload(&P->d_cch, P->cols.ch));
-    HM_TRY(upload(&P->d_rnear, P->rows.near));
-    HM_TRY(upload(&P->d_cnear, P->cols.near));
-    HM_TRY(upload(&P->d_rfar, P->rows.farl));
-    HM_TRY(upload(&P->d_cfar, P->cols.farl));
-    HM_TRY(upload(&P->d_rtask, P->rows.tasks));
-    HM_TRY(upload(&P->d_ctask, P->cols.tasks));
+    {
+        Staging st;
+        st.add(&P->d_rcl, P->rcl);
+        st.add(&P->d_ccl, P->ccl);
+        st.add(&P->d_far, P->far);
+        st.add(&P->d_far_by_rc, by_rc);
+        st.add(&P->d_far_by_cc, by_cc);
+        st.add(&P->d_dense, P->dense);
+        st.add(&P->d_rch, P->rows.ch);
+        st.add(&P->d_cch, P->cols.ch);
+        st.add(&P->d_rnear, P->rows.near);
+        st.add(&P->d_cnear, P->cols.near);
+        st.add(&P->d_rfar, P->rows.farl);
+        st.add(&P->d_cfar, P->cols.farl);
+        st.add(&P->d_rtask, P->rows.tasks);
+        st.add(&P->d_ctask, P->cols.tasks);
+        HM_TRY(st.commit(&P->d_tables));
+    }
+    P->far_by_rc_h = std::move(by_rc);
     PlanBufs<double> &b = P->b64;
     HM_TRY(upload(&b.A, hA));
     HM_TRY(upload(&b.U, hU));
@@ -968,9 +1109,13 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     HM_TRY(upload(&b.F, hF));
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
     const int64_t ncl = std::max(P->rcl.size(), P->ccl.size());
-    HM_TRY(dalloc(&b.partial, std::max<int64_t>(nslots, 1) * r * HM_WS_RHS));
-    HM_TRY(dalloc(&b.w, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
-    HM_TRY(dalloc(&b.z, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS));
+    {
+        Staging st;
+        st.reserve(&b.partial, std::max<int64_t>(nslots, 1) * r * HM_WS_RHS);
+        st.reserve(&b.w, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS);
+        st.reserve(&b.z, std::max<int64_t>(ncl, 1) * r * HM_WS_RHS);
+        HM_TRY(st.commit(&b.arena));
+    }
 #undef HM_TRY
     DevPlan &dp = b.dp;
     P->fill_structure(dp);
