@@ -473,7 +473,8 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
     for (int q = 0; q < KB; ++q) acc[q] = T(0);
 
     // ---- near field
-    for (int e = c.near_beg; e < c.near_end; ++e) {
+    const int near_end = a.part == 2 ? c.near_beg : c.near_end;
+    for (int e = c.near_beg; e < near_end; ++e) {
         const DNear s = nl[e];
         const DDense D = p.dense[s.blk];
         if constexpr (!TRANS) {
@@ -574,7 +575,8 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
     }
 
     // ---- far field: acc += B[chunk rows, :] z[cluster]
-    for (int e = c.far_beg; e < c.far_end; ++e) {
+    const int far_end = a.part == 1 ? c.far_beg : c.far_end;
+    for (int e = c.far_beg; e < far_end; ++e) {
         const DFar f = fl[e];
         const T *zc = z + (int64_t)f.cl * KB * r;
         if constexpr (!CHEB) {
@@ -649,10 +651,11 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
         }
     }
     if (act) {
+        const bool keep = a.beta || a.part == 2;
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             T *yo = a.y + (c.p0 + lane) * a.incy + q * a.ldy;
-            *yo = (a.beta ? *yo : T(0)) + acc[q];
+            *yo = (keep ? *yo : T(0)) + acc[q];
         }
     }
 }
@@ -821,6 +824,29 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
     const T *out_pts = tr ? p.y : p.x;
     const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
 
+    auto down = [&](const MvArgsT<T> &da, hipStream_t ds) {
+        if (tr) {
+            if constexpr (KB == 1)
+                k_down<T, RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_n), WG, 0, ds>>>(
+                    p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, da);
+        } else {
+            k_down<T, RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_n), WG, 0, ds>>>(
+                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, da);
+        }
+    };
+    // the near field depends on x alone: with a side stream it runs beside the
+    // far-field chain (up, reduce, translate) and the far field adds onto it
+    const bool split = a.side && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
+    if (split) {
+        hipError_t e = hipEventRecord(a.fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a.side, a.fork, 0);
+        if (e != hipSuccess) return e;
+        MvArgsT<T> an = a;
+        an.part = 1;
+        down(an, a.side);
+        e = hipEventRecord(a.join, a.side);
+        if (e != hipSuccess) return e;
+    }
     mark(0);
     if (p.nfar > 0 && in_ntask > 0) {
         k_up<T, RMAX, CHEB, KB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x,
@@ -844,15 +870,14 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         mark(2);
     }
     mark(3);
-    if (out_n > 0) {
-        if (tr) {
-            if constexpr (KB == 1)
-                k_down<T, RMAX, FREE, KID, true, CHEB, 1><<<nblk(out_n), WG, 0, s>>>(
-                    p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
-        } else {
-            k_down<T, RMAX, FREE, KID, false, CHEB, KB><<<nblk(out_n), WG, 0, s>>>(
-                p, out_ch, out_n, out_near, out_far, out_cl, out_B, out_map, out_pts, in_pts, a.z, a);
-        }
+    if (split) {
+        const hipError_t e = hipStreamWaitEvent(s, a.join, 0);
+        if (e != hipSuccess) return e;
+        MvArgsT<T> af = a;
+        af.part = 2;
+        down(af, s);
+    } else if (out_n > 0) {
+        down(a, s);
     }
     mark(4);
     return hipGetLastError();

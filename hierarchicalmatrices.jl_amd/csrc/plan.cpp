@@ -526,6 +526,11 @@ struct hm_plan {
     std::vector<int32_t> multi_c;     // column clusters with more than one upward task
     int32_t *d_multi_c = nullptr;
     void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_multi_c (Staging)
+    // side stream and fork / join events of products whose near field runs
+    // beside the far-field chain; made at the first such product that is not
+    // being captured into a graph
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     bool profile = false;
     bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
@@ -539,6 +544,9 @@ struct hm_plan {
 
     ~hm_plan() {
         drop_events();
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (d_tables) (void)hipFree(d_tables);
         b64.release();
         b32.release();
@@ -1246,6 +1254,41 @@ static int check_type(hm_plan_t P, const char *what) {
 }  // extern "C++"
 
 extern "C++" {
+// Hybrid and matrix-free products whose near field holds at least this many
+// entries run it on a side stream beside the far-field chain; below it the
+// fork and join cost more than the overlap returns.
+constexpr int64_t HM_SPLIT_MIN_ENTRIES = 1LL << 24;
+
+// Whether this product may split its near field onto the plan's side stream:
+// a large near field, no pass events, and a launch stream that is not being
+// captured (a captured product keeps one stream). Makes the side stream and
+// its events on first use; any failure leaves the product on one stream.
+static bool split_near_field(hm_plan_t P, hipStream_t s) {
+    if (P->profile || P->dense_entries < HM_SPLIT_MIN_ENTRIES) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (cs != hipStreamCaptureStatusNone) return false;
+    if (!P->side) {
+        hipStream_t st = nullptr;
+        hipEvent_t f = nullptr, j = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            if (f) (void)hipEventDestroy(f);
+            if (st) (void)hipStreamDestroy(st);
+            return false;
+        }
+        P->side = st;
+        P->fork = f;
+        P->join = j;
+    }
+    return true;
+}
+
 template <class T>
 static MvArgsT<T> ws_args(hm_plan_t P) {
     MvArgsT<T> a{};
@@ -1323,6 +1366,10 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
         if (prec >= 0) return fail(HM_ERR_UNSUPPORTED, "reduced-precision far field: Float64 plans only");
         if (flags & HM_MV_MFMA) return fail(HM_ERR_UNSUPPORTED, "MFMA passes: Float64 plans only");
     }
+    // a streamed near field (stored form) competes with the upward pass for
+    // HBM and gains nothing on a side stream; the computed ones (hybrid's
+    // Clenshaw far field, matrix-free kernel values) overlap the chain
+    const bool use_side = mode != 0 && split_near_field(P, s);
     for (int64_t q = 0; q < k;) {
         // forward products take up to HM_MAX_KB right-hand sides per launch
         int kb = 1;
@@ -1336,6 +1383,11 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
         a.ldy = ldy;
         a.ldx = ldx;
         a.beta = beta;
+        if (use_side) {
+            a.side = P->side;
+            a.fork = P->fork;
+            a.join = P->join;
+        }
         const hipEvent_t *ev = nullptr;
         if (P->profile) {
             const size_t at = P->prof_events.size();

@@ -191,3 +191,43 @@ def test_example_families(cuda, family, kernel, N):
         assert H.rank == 20 and H.blocksize_ == 80
         u = H.matvec(b)
         assert rel(u, ref) <= TOL, (mode, rel(u, ref))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_side_stream_near_field_matches_one_stream(cuda, mode):
+    """Hybrid and matrix-free plans with at least 2^24 near-field entries run the
+    near field on a side stream beside the far-field chain and add the far field
+    afterwards (stored plans stay on one stream; their case checks that). Pass
+    profiling keeps a product on one stream, so the same plan gives both forms:
+    forward, transpose, accumulate (beta = 1) and four right-hand sides per
+    launch, on a stream other than the default, over repeated products."""
+    import ctypes
+    from hmx._lib import lib
+    w = hmx.workloads.config_plain(n=1 << 16, blocksize=256, rank=24)
+    print(f"case: side-stream split n=2^16 leaf 256 mode={mode}", flush=True)
+    H = plan("cauchykernel", w, 256, 24, mode)
+    assert H.info["dense_entries"] >= 1 << 24
+    b = torch.from_numpy(w["b"]).to(cuda)
+    B4 = torch.from_numpy(np.random.default_rng(4).standard_normal((1 << 16, 4))).to(cuda)
+    u0 = torch.from_numpy(hmx.workloads.normal(9, 1 << 16)).to(cuda)
+    s = torch.cuda.Stream()
+
+    def products():
+        with torch.cuda.stream(s):
+            out = [H.matvec(b, stream=s), H.matvec(b, transpose=True, stream=s),
+                   H.mul(u0.clone(), b, stream=s), H.matvec(B4, mfma=False, stream=s)]
+            out += [H.matvec(b, stream=s) for _ in range(3)]
+        s.synchronize()
+        return [o.cpu().numpy() for o in out]
+
+    split = products()
+    assert lib.hm_plan_profile(H._h, 1) == 0
+    one = products()
+    assert lib.hm_plan_profile(H._h, 0) == 0
+    H.close()
+    for a, e in zip(split, one):
+        assert np.all(np.isfinite(a))
+        assert rel(a, e) <= 1e-14
+    # repeated products on the side stream are identical
+    for q in (5, 6):
+        assert np.array_equal(split[q], split[4])
