@@ -186,7 +186,10 @@ int hm_plan_blocks(hm_plan_t plan, int64_t *ints, double *dbls, int64_t cap, int
  * for q < k right-hand sides, beta in {0, 1}. beta = 1 is the reference's
  * accumulating mul!(u, H, v, istart, jstart) (src/KernelMatrix.jl:14-45, the
  * leaf loops src/algebra.jl:34-48 and :243-277); beta = 0 is H*v
- * (src/KernelMatrix.jl:5-8). */
+ * (src/KernelMatrix.jl:5-8). Hybrid and matrix-free products with at least
+ * 2^24 near-field entries run the near field on a stream the plan owns,
+ * forked from and joined back to `stream` by events; a product stays on
+ * `stream` while it is being captured or pass profiling is on. */
 int hm_matvec(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_t joff,
               int64_t incy, int64_t incx, int64_t k, int64_t ldy, int64_t ldx, int beta,
               unsigned flags, void *stream);
