@@ -231,3 +231,33 @@ def test_side_stream_near_field_matches_one_stream(cuda, mode):
     # repeated products on the side stream are identical
     for q in (5, 6):
         assert np.array_equal(split[q], split[4])
+
+
+def test_side_stream_plan_captured_on_one_stream(cuda):
+    """A matrix-free plan whose products split onto its side stream, captured
+    into a graph after an eager product: the captured product stays on one
+    stream, so every replay equals the one-stream eager product bit for bit
+    (pass profiling keeps an eager product on one stream)."""
+    from hmx._lib import lib
+    w = hmx.workloads.config_plain(n=1 << 16, blocksize=256, rank=24)
+    print("case: side-stream plan under graph capture, n=2^16 leaf 256 matrix-free", flush=True)
+    H = plan("cauchykernel", w, 256, 24, "free")
+    assert H.info["dense_entries"] >= 1 << 24
+    b = torch.from_numpy(w["b"]).to(cuda)
+    split = H @ b  # makes the side stream
+    out = torch.zeros_like(b)
+    g = H.graph(out, b)
+    for rep in range(3):
+        b.mul_(1.25).add_(0.5)  # eager work between replays
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert lib.hm_plan_profile(H._h, 1) == 0
+        one = H @ b
+        torch.cuda.synchronize()
+        assert lib.hm_plan_profile(H._h, 0) == 0
+        assert torch.equal(out, one), rep
+        assert rel((H @ b).cpu().numpy(), one.cpu().numpy()) <= 1e-14
+    assert torch.all(torch.isfinite(split))
+    g.close()
+    H.close()
