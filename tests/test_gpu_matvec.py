@@ -200,7 +200,9 @@ def test_side_stream_near_field_matches_one_stream(cuda, mode):
     afterwards (stored plans stay on one stream; their case checks that). Pass
     profiling keeps a product on one stream, so the same plan gives both forms:
     forward, transpose, accumulate (beta = 1) and four right-hand sides per
-    launch, on a stream other than the default, over repeated products."""
+    launch on the vector units, on a stream other than the default, over
+    repeated products. 20 right-hand sides on the MFMA passes stay on one
+    stream (both halves would compete for the matrix cores) and must agree."""
     import ctypes
     from hmx._lib import lib
     w = hmx.workloads.config_plain(n=1 << 16, blocksize=256, rank=24)
@@ -209,13 +211,16 @@ def test_side_stream_near_field_matches_one_stream(cuda, mode):
     assert H.info["dense_entries"] >= 1 << 24
     b = torch.from_numpy(w["b"]).to(cuda)
     B4 = torch.from_numpy(np.random.default_rng(4).standard_normal((1 << 16, 4))).to(cuda)
+    # column-major (20 columns of n): the MFMA passes pack each group of rows
+    B20 = torch.from_numpy(np.random.default_rng(6).standard_normal((20, 1 << 16))).to(cuda).t()
     u0 = torch.from_numpy(hmx.workloads.normal(9, 1 << 16)).to(cuda)
     s = torch.cuda.Stream()
 
     def products():
         with torch.cuda.stream(s):
             out = [H.matvec(b, stream=s), H.matvec(b, transpose=True, stream=s),
-                   H.mul(u0.clone(), b, stream=s), H.matvec(B4, mfma=False, stream=s)]
+                   H.mul(u0.clone(), b, stream=s), H.matvec(B4, mfma=False, stream=s),
+                   H.matvec(B20, mfma=True, stream=s)]
             out += [H.matvec(b, stream=s) for _ in range(3)]
         s.synchronize()
         return [o.cpu().numpy() for o in out]
@@ -229,8 +234,8 @@ def test_side_stream_near_field_matches_one_stream(cuda, mode):
         assert np.all(np.isfinite(a))
         assert rel(a, e) <= 1e-14
     # repeated products on the side stream are identical
-    for q in (5, 6):
-        assert np.array_equal(split[q], split[4])
+    for q in (6, 7):
+        assert np.array_equal(split[q], split[5])
 
 
 def test_side_stream_plan_captured_on_one_stream(cuda):
