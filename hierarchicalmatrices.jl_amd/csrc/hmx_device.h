@@ -30,6 +30,9 @@ struct DevPlanT {
     int32_t nfar;
     const int32_t *far_by_rc;  // far ids grouped by row cluster (CSR in DCluster.far_*)
     const int32_t *far_by_cc;  // far ids grouped by col cluster
+    // the far records themselves in far_by_rc / far_by_cc order, so a pass
+    // walking a cluster's list reads each record with one load
+    const DFarBlk *far_rc, *far_cc;
     const DDense *dense;
     int32_t ndense;
     const DChunk *rch, *cch;   // row chunks, col chunks

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
         constexpr int UNR = FREE ? 1 : 2;
 #pragma unroll UNR
         for (int e = C.far_beg; e < C.far_end; ++e) {
-            const DFarBlk b = p.far[far_list[e]];
+            const DFarBlk b = (TRANS ? p.far_cc : p.far_rc)[e];
             const int src = TRANS ? b.rc : b.cc;
             const T *ws = w + ((int64_t)src * KB + qa) * r;
             T fv[RMAX];

@@ -518,6 +518,7 @@ struct hm_plan {
     DCluster *d_rcl = nullptr, *d_ccl = nullptr;
     DFarBlk *d_far = nullptr;
     int32_t *d_far_by_rc = nullptr, *d_far_by_cc = nullptr;
+    DFarBlk *d_far_rc = nullptr, *d_far_cc = nullptr;  // records in by_rc / by_cc order
     DDense *d_dense = nullptr;
     DChunk *d_rch = nullptr, *d_cch = nullptr;
     DNear *d_rnear = nullptr, *d_cnear = nullptr;
@@ -566,6 +567,7 @@ struct hm_plan {
         dp.nrc = (int32_t)rcl.size(); dp.ncc = (int32_t)ccl.size();
         dp.far = d_far; dp.nfar = (int32_t)far.size();
         dp.far_by_rc = d_far_by_rc; dp.far_by_cc = d_far_by_cc;
+        dp.far_rc = d_far_rc; dp.far_cc = d_far_cc;
         dp.dense = d_dense; dp.ndense = (int32_t)dense.size();
         dp.rch = d_rch; dp.cch = d_cch;
         dp.nrch = (int32_t)rows.ch.size(); dp.ncch = (int32_t)cols.ch.size();
@@ -945,6 +947,9 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         *out = P;
         return HM_OK;
     }
+    std::vector<DFarBlk> far_rc(by_rc.size()), far_cc(by_cc.size());
+    for (size_t e = 0; e < by_rc.size(); ++e) far_rc[e] = P->far[by_rc[e]];
+    for (size_t e = 0; e < by_cc.size(); ++e) far_cc[e] = P->far[by_cc[e]];
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
     {
@@ -954,6 +959,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         st.add(&P->d_far, P->far);
         st.add(&P->d_far_by_rc, by_rc);
         st.add(&P->d_far_by_cc, by_cc);
+        st.add(&P->d_far_rc, far_rc);
+        st.add(&P->d_far_cc, far_cc);
         st.add(&P->d_dense, P->dense);
         st.add(&P->d_rch, P->rows.ch);
         st.add(&P->d_cch, P->cols.ch);
@@ -1089,6 +1096,9 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
             }
         }
     }
+    std::vector<DFarBlk> far_rc(by_rc.size()), far_cc(by_cc.size());
+    for (size_t e = 0; e < by_rc.size(); ++e) far_rc[e] = P->far[by_rc[e]];
+    for (size_t e = 0; e < by_cc.size(); ++e) far_cc[e] = P->far[by_cc[e]];
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
     {
@@ -1098,6 +1108,8 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
         st.add(&P->d_far, P->far);
         st.add(&P->d_far_by_rc, by_rc);
         st.add(&P->d_far_by_cc, by_cc);
+        st.add(&P->d_far_rc, far_rc);
+        st.add(&P->d_far_cc, far_cc);
         st.add(&P->d_dense, P->dense);
         st.add(&P->d_rch, P->rows.ch);
         st.add(&P->d_cch, P->cols.ch);
