@@ -121,12 +121,21 @@ __global__ __launch_bounds__(64) void k_even_up(EvenDev<T> p, const T *v, int64_
 
 // beta of a leaf with several tasks: the sum of its partials; thread
 // (g, e) sums slots g, g + G, ... of entry e, then the G partial sums are
-// added in g order
+// added in g order. With more than 256 entries (2 parities x kb right-hand
+// sides x rank n) each thread sums whole entries e, e + 256, ... in slot order.
 template <class T>
 __global__ __launch_bounds__(256) void k_even_reduce(EvenDev<T> p, int kb) {
     const ERed r = p.red[blockIdx.x];
     const int64_t stride = (int64_t)HM_EVEN_KB * 2 * p.n;
     const int E = 2 * kb * p.n;
+    if (E > 256) {  // block-uniform: no thread reaches the barrier below
+        for (int e = threadIdx.x; e < E; e += 256) {
+            T s = T(0);
+            for (int t = 0; t < r.ntask; ++t) s += p.partial[(r.slot0 + t) * stride + e];
+            p.beta[(int64_t)r.leaf * stride + e] = s;
+        }
+        return;
+    }
     const int G = 256 / E;
     const int e = threadIdx.x % E, g = threadIdx.x / E;
     __shared__ T part[256];

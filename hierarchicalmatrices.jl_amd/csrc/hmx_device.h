@@ -104,6 +104,21 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
 // b is indexed by the global column (side 0) or row (side 1)
 hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s);
 
+// add_col! on a plan built from blocks: per job (one leaf holding the column),
+// a dense leaf adds u[row0 + i] to its column jl; a low-rank leaf writes
+// u[row0 + i] into column c of its U, 1 into row jl of column c of its V and
+// 1 into F[c, c] (u is offset to the matrix's first row).
+struct AddColJob {
+    int32_t kind, m, n, c, jl, far;
+    int64_t row0, aoff, uoff, voff, foff;
+};
+#define HM_ADD_COL_JOBS 24  // per launch (the jobs go by value, about 1.3 KB)
+struct AddColJobs {
+    AddColJob job[HM_ADD_COL_JOBS];
+    int32_t count;
+};
+hipError_t hmx_launch_add_col(const DevPlan &p, const AddColJobs &J, const double *u, hipStream_t s);
+
 // The same passes for FP32 plans (ranks up to 32; no reduced-precision form).
 hipError_t hmx_launch_compress(const DevPlanF &p, hipStream_t s);
 hipError_t hmx_launch_matvec(const DevPlanF &p, const MvArgsF &a, bool transpose, int mode, int kb,

@@ -1,11 +1,14 @@
 // kernels.hip -- public launchers of the gfx950 kernels: dispatch on the rank
 // bucket to the instantiations in inst_r*.hip (FP64 plans: 8, 16, 24, 32, 48,
 // 64) and inst_f32_r*.hip (FP32 plans: 8, 16, 32).
+#include "hmx.h"
 #include "kernels_impl.h"
 
 hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s) {
-    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<double><<<p.ndense, hmxk::WG, 0, s>>>(p);
-    if (p.nfar > 0 && p.F) hmxk::k_gen_F<double><<<p.nfar, hmxk::WG, 0, s>>>(p);
+    // HM_USER plans take A and F from the caller (hm_plan_set_leaf_data)
+    const bool gen = p.kid != HM_USER;
+    if (gen && p.ndense > 0 && p.A) hmxk::k_gen_dense<double><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    if (gen && p.nfar > 0 && p.F) hmxk::k_gen_F<double><<<p.nfar, hmxk::WG, 0, s>>>(p);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || (!p.U && !p.V)) return err;
     if (p.r <= 8) return hmx_gen_basis_r8(p, s);
@@ -67,8 +70,9 @@ hipError_t hmx_launch_matvec_m(const DevPlan &p, const MvArgs &a, int mode, int 
 // ------------------------------------------------------------- FP32 plans
 
 hipError_t hmx_launch_compress(const DevPlanF &p, hipStream_t s) {
-    if (p.ndense > 0 && p.A) hmxk::k_gen_dense<float><<<p.ndense, hmxk::WG, 0, s>>>(p);
-    if (p.nfar > 0 && p.F) hmxk::k_gen_F<float><<<p.nfar, hmxk::WG, 0, s>>>(p);
+    const bool gen = p.kid != HM_USER;
+    if (gen && p.ndense > 0 && p.A) hmxk::k_gen_dense<float><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    if (gen && p.nfar > 0 && p.F) hmxk::k_gen_F<float><<<p.nfar, hmxk::WG, 0, s>>>(p);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || (!p.U && !p.V)) return err;
     if (p.r <= 8) return hmx_gen_basis_f32_r8(p, s);
@@ -137,7 +141,30 @@ __global__ __launch_bounds__(hmxk::WG) void k_scale_basis(const DCluster *cl, do
     }
 }
 
+// one workgroup per job of hm_plan_add_col
+__global__ __launch_bounds__(hmxk::WG) void k_add_col(DevPlan p, AddColJobs J, const double *u) {
+    const AddColJob q = J.job[blockIdx.x];
+    if (q.kind == 3) {
+        for (int32_t i = threadIdx.x; i < q.m; i += hmxk::WG) {
+            double *a = p.A + q.aoff + tile_index(i, q.jl, q.n);
+            *a = *a + u[q.row0 + i];
+        }
+    } else {
+        for (int32_t i = threadIdx.x; i < q.m; i += hmxk::WG)
+            p.U[q.uoff + tile_index(i, q.c, p.r)] = u[q.row0 + i];
+        if (threadIdx.x == 0) {
+            p.V[q.voff + tile_index(q.jl, q.c, p.r)] = 1.0;
+            p.F[q.foff + q.c + (int64_t)p.r * q.c] = 1.0;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t hmx_launch_add_col(const DevPlan &p, const AddColJobs &J, const double *u, hipStream_t s) {
+    if (J.count > 0) k_add_col<<<J.count, hmxk::WG, 0, s>>>(p, J, u);
+    return hipGetLastError();
+}
 
 hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s) {
     if (p.ndense > 0 && p.A) k_scale_dense<<<p.ndense, hmxk::WG, 0, s>>>(p, side, b);

@@ -302,8 +302,9 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
         for (int i = 0; i < LEFT; ++i) sh[wv][q][base + i] = acc[i];
     }
     __syncthreads();
-    if (tid < KB * r) {
-        const int qq = tid / r, m = tid % r;
+    // KB * r reaches 512 (8 right-hand sides at rank 64): threads loop over it
+    for (int e = tid; e < KB * r; e += WG) {
+        const int qq = e / r, m = e % r;
         T s = T(0);
 #pragma unroll
         for (int w = 0; w < WPB; ++w) s += sh[w][qq][m];
@@ -953,8 +954,16 @@ hipError_t grad_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
 //   0: FP32 operands on v_mfma_f32_16x16x4f32, FP32 accumulation
 //   1: split BF16 (x = hi + lo, hi = bf16(x), lo = bf16(x - hi); products
 //      hi*hi + hi*lo + lo*hi) on v_mfma_f32_16x16x32_bf16, FP32 accumulation
-// Near field, reduce and translate run in FP64 (v_mfma_f64_16x16x4f64) for
-// every PREC; point differences and the Chebyshev recurrence are FP64.
+//   3: split BF16 in three (x = hi + mid + lo, each the bf16 rounding of what
+//      the previous pieces leave; products hi*hi + hi*mid + mid*hi + hi*lo +
+//      lo*hi + mid*mid, the terms down to 2^-16 of hi*hi), FP32 accumulation:
+//      operands are carried to FP32's 24 bits, where two pieces carry 16
+// The FP32 accumulators of the reduced-precision products hold one step's
+// sum only (64 points upward, one cluster's series downward) and are added
+// into FP64 sums after every step, so FP32 rounding does not grow with the
+// number of points or clusters. Near field, reduce and translate run in FP64
+// (v_mfma_f64_16x16x4f64) for every PREC; point differences and the
+// Chebyshev recurrence are FP64.
 // Operand layouts (lane l, j16 = l % 16, g = l / 16): A[i][k] comes from the
 // lane with i = j16 and k = g (K = 4 shapes) or k = 8g + v, v < 8 (K = 32);
 // B[k][j] from the lane with j = j16 and the same k. Results: the FP32 shapes
@@ -966,6 +975,52 @@ constexpr int MQ = 16;  // right-hand sides per launch
 
 __device__ __forceinline__ __bf16 bf16_hi(float x) { return (__bf16)x; }
 __device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)(__bf16)x); }
+// three-way split: x - hi and x - hi - mid are exact in FP32 (each drops the
+// leading 8 significant bits of a value with at most 24)
+__device__ __forceinline__ void bf16_split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+    h = (__bf16)x;
+    const float r1 = x - (float)h;
+    m = (__bf16)r1;
+    l = (__bf16)(r1 - (float)m);
+}
+// D += A B over split operands: PREC 1 three products (hi*hi, hi*lo, lo*hi),
+// PREC 3 six (down to mid*mid)
+template <int PREC>
+__device__ __forceinline__ mf4_t mfma_split(const mbf8_t &ah, const mbf8_t &am, const mbf8_t &al,
+                                            const mbf8_t &bh, const mbf8_t &bm, const mbf8_t &bl,
+                                            mf4_t d) {
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, d, 0, 0, 0);
+    if constexpr (PREC == 1) {
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, d, 0, 0, 0);
+    } else {
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, d, 0, 0, 0);
+    }
+    return d;
+}
+// the 8 values of a split operand: PREC 1 uses (hi, lo) with mid unused,
+// PREC 3 (hi, mid, lo)
+template <int PREC>
+__device__ __forceinline__ void split8(const float *v, mbf8_t &h, mbf8_t &m, mbf8_t &l) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if constexpr (PREC == 1) {
+            h[q] = bf16_hi(v[q]);
+            l[q] = bf16_lo(v[q]);
+            m[q] = (__bf16)0.f;
+        } else {
+            __bf16 a, b, c;
+            bf16_split3(v[q], a, b, c);
+            h[q] = a;
+            m[q] = b;
+            l[q] = c;
+        }
+    }
+}
 __device__ __forceinline__ md4_t mfma64(double a, double b, md4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -1009,6 +1064,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
                                               const double *chebA, double *partial, double *w,
                                               int r) {
     using TL = typename std::conditional<PREC == 2, double, float>::type;
+    constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int MT = RP / 16;
     constexpr int PTS = (PREC == 2 && RP > 32) ? 32 : 64;  // points per step
     static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQ * sizeof(double), "mu fits in Tl");
@@ -1020,10 +1076,13 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
     const int64_t first = Tk.first;
     md4_t D64[MT];
     mf4_t D32[MT];
+    double DF[MT][4];  // FP64 sums of the reduced-precision steps
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
         D64[t] = md4_t{0.0, 0.0, 0.0, 0.0};
         D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) DF[t][v] = 0.0;
     }
     for (int base = 0; base < Tk.len; base += PTS) {
         // this lane's B operands of the step, loaded before the recurrence:
@@ -1031,7 +1090,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
         double bx[PTS / 4];
 #pragma unroll
         for (int u = 0; u < PTS / 4; ++u) {
-            const int i = base + (PREC == 1 ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g);
+            const int i = base + (BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g);
             bx[u] = (i < Tk.len && j16 < nq) ? xt[(first + i) * ldxt + j16] : 0.0;
         }
         if (lane < PTS) {
@@ -1051,7 +1110,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
             }
         }
         wave_sync();
-        if constexpr (PREC != 1) {
+        if constexpr (!BF) {
 #pragma unroll
             for (int k0 = 0; k0 < PTS; k0 += 4) {
                 const double b = bx[k0 / 4];
@@ -1064,26 +1123,28 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
         } else {
 #pragma unroll
             for (int k0 = 0; k0 < PTS; k0 += 32) {
-                mbf8_t bh, bl;
+                float bv[8];
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const float b = (float)bx[(k0 / 32) * 8 + v];
-                    bh[v] = bf16_hi(b);
-                    bl[v] = bf16_lo(b);
-                }
+                for (int v = 0; v < 8; ++v) bv[v] = (float)bx[(k0 / 32) * 8 + v];
+                mbf8_t bh, bm, bl;
+                split8<PREC>(bv, bh, bm, bl);
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
-                    mbf8_t ah, al;
+                    float tv[8];
 #pragma unroll
-                    for (int v = 0; v < 8; ++v) {
-                        const float tv = Tl[t * 16 + j16][k0 + 8 * g + v];
-                        ah[v] = bf16_hi(tv);
-                        al[v] = bf16_lo(tv);
-                    }
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
-                    D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
+                    for (int v = 0; v < 8; ++v) tv[v] = Tl[t * 16 + j16][k0 + 8 * g + v];
+                    mbf8_t ah, am, al;
+                    split8<PREC>(tv, ah, am, al);
+                    D32[t] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[t]);
                 }
+            }
+        }
+        if constexpr (PREC != 2) {  // this step's FP32 sums into the FP64 ones
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) DF[t][v] += (double)D32[t][v];
+                D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
             }
         }
         wave_sync();
@@ -1095,7 +1156,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             if constexpr (PREC == 2) mu[t * 16 + g + 4 * v][j16] = D64[t][v];
-            else mu[t * 16 + 4 * g + v][j16] = (double)D32[t][v];
+            else mu[t * 16 + 4 * g + v][j16] = DF[t][v];
         }
     wave_sync();
     if (Tk.single) {
@@ -1216,7 +1277,8 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                                               const double2 *cmapv, const double *pts_out,
                                               const double *pts_in, const double *xt,
                                               int64_t ldxt, const double *z, MvArgs a, int nq) {
-    constexpr int KSEL = PREC == 1 ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
+    constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
+    constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     __shared__ double ds_all[WPB][16 * MQ];  // one 16-row output tile per wavefront
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
@@ -1235,10 +1297,13 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
     md4_t D[4];
     mf4_t D32[4];
+    double DF[4][4];  // FP64 sums of the reduced-precision far field, per cluster
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         D[t] = md4_t{0.0, 0.0, 0.0, 0.0};
         D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) DF[t][v] = 0.0;
     }
     // ---- near field, FP64: two K-steps (8 columns) per group, operands first;
     // the chunk's block records are read together, one per lane
@@ -1297,7 +1362,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         const double *zc = z + (int64_t)cl * MQ * r + (int64_t)j16 * r;  // C[k][q = j16]
 #pragma unroll
         for (int u = 0; u < KSEL; ++u) {
-            const int k = PREC == 1 ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
+            const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
             bz[u] = (k < r && j16 < nq) ? zc[min(k, r - 1)] : 0.0;
         }
     };
@@ -1321,7 +1386,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                 if (t >= nt) break;
                 const double xh = (po[t] - cm.x) * cm.y;
                 double sel[KSEL];
-                if constexpr (PREC != 1) {
+                if constexpr (!BF) {
                     // this lane's T_k, k = g + 4u: T_g and T_{g+4} from the prefix
                     // T_0..T_7, then T_{k+4} = 2 T_4 T_k - T_{k-4} (stride-4 recurrence)
                     const double x2 = xh + xh;
@@ -1365,19 +1430,22 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                 } else {
 #pragma unroll
                     for (int s8 = 0; s8 < KSEL / 8; ++s8) {
-                        mbf8_t ah, al, bh, bl;
+                        float av[8], bv[8];
 #pragma unroll
                         for (int v = 0; v < 8; ++v) {
-                            const float av = (float)sel[s8 * 8 + v], bv = (float)bz[s8 * 8 + v];
-                            ah[v] = bf16_hi(av);
-                            al[v] = bf16_lo(av);
-                            bh[v] = bf16_hi(bv);
-                            bl[v] = bf16_lo(bv);
+                            av[v] = (float)sel[s8 * 8 + v];
+                            bv[v] = (float)bz[s8 * 8 + v];
                         }
-                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, D32[t], 0, 0, 0);
-                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, D32[t], 0, 0, 0);
-                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, D32[t], 0, 0, 0);
+                        mbf8_t ah, am, al, bh, bm, bl;
+                        split8<PREC>(av, ah, am, al);
+                        split8<PREC>(bv, bh, bm, bl);
+                        D32[t] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[t]);
                     }
+                }
+                if constexpr (PREC != 2) {  // this cluster's FP32 sums into the FP64 ones
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) DF[t][v] += (double)D32[t][v];
+                    D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
                 }
             }
 #pragma unroll
@@ -1394,7 +1462,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         if constexpr (PREC != 2) {
             wave_sync();
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Dl[(4 * g + v) * MQ + j16] += (double)D32[t][v];
+            for (int v = 0; v < 4; ++v) Dl[(4 * g + v) * MQ + j16] += DF[t][v];
         }
         wave_sync();
         const int row = j16;
@@ -1459,6 +1527,7 @@ hipError_t matvec_m_r(const DevPlan &p, const MvArgs &a, int mode, int prec, int
                       const int32_t *multi, int nmulti, hipStream_t s) {
     if (prec == 0) return matvec_m_p<RP, 0>(p, a, mode, nq, multi, nmulti, s);
     if (prec == 1) return matvec_m_p<RP, 1>(p, a, mode, nq, multi, nmulti, s);
+    if (prec == 3) return matvec_m_p<RP, 3>(p, a, mode, nq, multi, nmulti, s);
     return matvec_m_p<RP, 2>(p, a, mode, nq, multi, nmulti, s);
 }
 

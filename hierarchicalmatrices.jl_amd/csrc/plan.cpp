@@ -511,6 +511,7 @@ struct hm_plan {
     int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
     int maxdepth = 0;
     int64_t nonfinite = 0;
+    int64_t nonfinite_basis = 0;  // of the generated factors alone (HM_USER plans: U and V)
     // device: element-typed arrays (b64 for Float64 plans, b32 for Float32)
     bool f32 = false;
     PlanBufs<double> b64;
@@ -534,6 +535,10 @@ struct hm_plan {
     hipEvent_t fork = nullptr, join = nullptr;
     bool profile = false;
     bool from_blocks = false;  // built by hm_plan_from_blocks: stored form only
+    // plans built from blocks: per leaf, its dense or far block id (-1: an
+    // empty leaf, or a rank-0 one), and per far block its active rank (the
+    // columns of its U, V and F in use; the rest, up to r, are zero)
+    std::vector<int32_t> leaf_blk, far_rank;
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
     int tuned_kb = HM_MAX_KB, tuned_mode = -1;  // hm_plan_tune's choice
     std::vector<hipEvent_t> prof_events;  // 5 per product
@@ -741,10 +746,11 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         for (const DDense &D : P->dense) padded |= (D.m % HM_TILE) != 0;
         for (const DCluster &C : P->rcl) padded |= (C.len % HM_TILE) != 0;
         for (const DCluster &C : P->ccl) padded |= (C.len % HM_TILE) != 0;
-        if (padded) {
-            T *arrs[] = {b.A, b.U, b.V};
-            const int64_t lens[] = {nA, nU, nV};
-            for (int q = 0; q < 3; ++q)
+        const bool user = P->kid == HM_USER;  // A and F come from hm_plan_set_leaf_data
+        if (padded || user) {
+            T *arrs[] = {b.A, b.U, b.V, user ? b.F : nullptr};
+            const int64_t lens[] = {nA, nU, nV, nF};
+            for (int q = 0; q < 4; ++q)
                 if (lens[q] > 0 && arrs[q]) {
                     hipError_t e0 = hipMemsetAsync((void *)arrs[q], 0, sizeof(T) * lens[q], s);
                     if (e0 != hipSuccess) return fail(HM_ERR_HIP, "memset: %s", hipGetErrorString(e0));
@@ -758,7 +764,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         if (e == hipSuccess) {
             std::vector<unsigned long long> h(1024);
             int64_t total = 0;
-            const T *arrs[] = {b.A, b.U, b.V, b.F};
+            const T *arrs[] = {user ? nullptr : b.A, b.U, b.V, user ? nullptr : b.F};
             const int64_t lens[] = {nA, nU, nV, nF};
             for (int q = 0; q < 4 && e == hipSuccess; ++q) {
                 if (lens[q] <= 0 || !arrs[q]) continue;
@@ -770,6 +776,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
             }
             (void)hipFree(d_cnt);
             P->nonfinite = total;
+            P->nonfinite_basis = total;
         }
         if (e != hipSuccess) return fail(HM_ERR_HIP, "compression: %s", hipGetErrorString(e));
     }
@@ -787,7 +794,9 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
                      unsigned flags, void *stream, hm_plan_t *out, bool f32 = false) {
     if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
     *out = nullptr;
-    if (kernel_id < HM_CAUCHY || kernel_id > HM_LOG) return fail(HM_ERR_INVALID, "unknown kernel id %d", kernel_id);
+    if (kernel_id < HM_CAUCHY || kernel_id > HM_USER) return fail(HM_ERR_INVALID, "unknown kernel id %d", kernel_id);
+    if (kernel_id == HM_USER && (flags & HM_PLAN_MATRIX_FREE))
+        return fail(HM_ERR_UNSUPPORTED, "HM_USER: f is the caller's host function, so the factors cannot be regenerated on the device (matrix-free)");
     if (count < 1) return fail(HM_ERR_INVALID, "no instances");
     if (blocksize < 1) return fail(HM_ERR_INVALID, "blocksize must be >= 1");
     if (rank < 1 || rank > 64) return fail(HM_ERR_UNSUPPORTED, "rank %d outside 1..64", rank);
@@ -1001,11 +1010,20 @@ int hm_plan_from_points_f32(int kernel_id, const float *x, int64_t nx, const flo
     return make_plan(kernel_id, &I, 1, blocksize, rank, flags, stream, out, true);
 }
 
-int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
-                        const int64_t *row0, const int64_t *col0, const int64_t *m,
-                        const int64_t *n, const int64_t *rank, const double *const *A,
-                        const double *const *U, const double *const *S,
-                        const double *const *V, void *stream, hm_plan_t *out) {
+// A plan from leaves (hm_plan_from_blocks and its Float32 twin). Leaf kinds:
+// 3 dense, 2 low rank U diag(S) V', 4 barycentric U F V' (src/algebra.jl:
+// 243-277) with the rank x rank core F passed in S[t]. Low-rank and barycentric
+// leaves become far blocks with private row and column clusters; their cores
+// sit in the plan's r x r F blocks (r = the largest leaf rank, the rest zero).
+extern "C++" {
+template <class T>
+static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                         const int64_t *row0, const int64_t *col0, const int64_t *m,
+                         const int64_t *n, const int64_t *rank, const T *const *A,
+                         const T *const *U, const T *const *S, const T *const *V,
+                         void *stream, hm_plan_t *out) {
+    constexpr bool F32 = std::is_same_v<T, float>;
+    constexpr int RMAX = F32 ? 32 : 64;
     if (!out) return fail(HM_ERR_INVALID, "plan pointer is NULL");
     *out = nullptr;
     if (nrows < 1 || ncols < 1 || nleaves < 0 || (nleaves > 0 && (!kind || !row0 || !col0 || !m || !n)))
@@ -1013,14 +1031,16 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     if (nrows >= (1LL << 40) || ncols >= (1LL << 40)) return fail(HM_ERR_INVALID, "too many rows");
     int R = 1;
     for (int64_t t = 0; t < nleaves; ++t) {
-        if (kind[t] != 2 && kind[t] != 3) return fail(HM_ERR_INVALID, "leaf %lld: kind must be 2 or 3", (long long)t);
+        if (kind[t] != 2 && kind[t] != 3 && kind[t] != 4)
+            return fail(HM_ERR_INVALID, "leaf %lld: kind must be 2, 3 or 4", (long long)t);
         if (m[t] < 0 || n[t] < 0 || row0[t] < 0 || col0[t] < 0 || row0[t] + m[t] > nrows || col0[t] + n[t] > ncols)
             return fail(HM_ERR_INVALID, "leaf %lld out of range", (long long)t);
         if (m[t] >= (1LL << 31) || n[t] >= (1LL << 31)) return fail(HM_ERR_INVALID, "leaf %lld too large", (long long)t);
         if (kind[t] == 3 && m[t] * n[t] > 0 && (!A || !A[t])) return fail(HM_ERR_INVALID, "leaf %lld: no data", (long long)t);
-        if (kind[t] == 2) {
+        if (kind[t] != 3) {
             if (!rank || rank[t] < 0) return fail(HM_ERR_INVALID, "leaf %lld: bad rank", (long long)t);
-            if (rank[t] > 64) return fail(HM_ERR_UNSUPPORTED, "leaf %lld: rank %lld > 64", (long long)t, (long long)rank[t]);
+            if (rank[t] > RMAX)
+                return fail(HM_ERR_UNSUPPORTED, "leaf %lld: rank %lld > %d", (long long)t, (long long)rank[t], RMAX);
             if (rank[t] > 0 && (!U || !S || !V || !U[t] || !S[t] || !V[t]))
                 return fail(HM_ERR_INVALID, "leaf %lld: no factors", (long long)t);
             R = std::max<int>(R, (int)rank[t]);
@@ -1028,6 +1048,7 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     }
     hm_plan *P = new hm_plan();
     P->from_blocks = true;
+    P->f32 = F32;
     P->kid = 0; P->r = R; P->bs = 0; P->flags = 0;
     P->nx = nrows; P->ny = ncols;
     P->row_off = {0, nrows};
@@ -1035,13 +1056,17 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     std::vector<HCluster> hr, hc;
     std::vector<int64_t> lr_leaf;  // leaf index of each far block
     for (int64_t t = 0; t < nleaves; ++t) {
-        P->leaves.push_back({kind[t], 1, row0[t], m[t], col0[t], n[t], 0, 0, 0, 0});
+        P->leaves.push_back({kind[t] == 3 ? 3 : 2, 1, row0[t], m[t], col0[t], n[t], 0, 0, 0, 0});
+        P->leaf_blk.push_back(-1);
         if (m[t] == 0 || n[t] == 0) continue;
         if (kind[t] == 3) {
+            P->leaf_blk.back() = (int32_t)P->dense.size();
             P->dense.push_back({row0[t], col0[t], (int32_t)m[t], (int32_t)n[t], 0});
             P->dense_entries += m[t] * n[t];
         } else if (rank[t] > 0) {
             const int32_t rc = (int32_t)hr.size(), cc = (int32_t)hc.size();
+            P->leaf_blk.back() = (int32_t)P->far.size();
+            P->far_rank.push_back((int32_t)rank[t]);
             hr.push_back({1, row0[t], m[t], 0, 0});
             hc.push_back({1, col0[t], n[t], 0, 0});
             P->far.push_back({rc, cc, 0});
@@ -1077,7 +1102,7 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
     build_side(nrows, P->dense, true, P->rcl, P->rows);
     build_side(ncols, P->dense, false, P->ccl, P->cols);
     // host-side tiling of the data
-    std::vector<double> hA(nA, 0.0), hU(nU, 0.0), hV(nV, 0.0), hF(nF, 0.0);
+    std::vector<T> hA(nA, T(0)), hU(nU, T(0)), hV(nV, T(0)), hF(nF, T(0));
     {
         size_t d = 0;
         for (int64_t t = 0; t < nleaves; ++t) {
@@ -1092,7 +1117,10 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
             for (int64_t c = 0; c < k; ++c) {
                 for (int64_t i = 0; i < m[t]; ++i) hU[Cr.boff + tile_index(i, c, r)] = U[t][i + m[t] * c];
                 for (int64_t j = 0; j < n[t]; ++j) hV[Cc.boff + tile_index(j, c, r)] = V[t][j + n[t] * c];
-                hF[(int64_t)f * r * r + c + (int64_t)r * c] = S[t][c];
+                if (kind[t] == 2) hF[(int64_t)f * r * r + c + (int64_t)r * c] = S[t][c];
+                else
+                    for (int64_t c2 = 0; c2 < k; ++c2)
+                        hF[(int64_t)f * r * r + c2 + (int64_t)r * c] = S[t][c2 + k * c];
             }
         }
     }
@@ -1122,7 +1150,7 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
         HM_TRY(st.commit(&P->d_tables));
     }
     P->far_by_rc_h = std::move(by_rc);
-    PlanBufs<double> &b = P->b64;
+    PlanBufs<T> &b = P->bufs<T>();
     HM_TRY(upload(&b.A, hA));
     HM_TRY(upload(&b.U, hU));
     HM_TRY(upload(&b.V, hV));
@@ -1137,11 +1165,138 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
         HM_TRY(st.commit(&b.arena));
     }
 #undef HM_TRY
-    DevPlan &dp = b.dp;
+    DevPlanT<T> &dp = b.dp;
     P->fill_structure(dp);
     dp.A = b.A; dp.U = b.U; dp.V = b.V; dp.F = b.F;
     (void)stream;
     *out = P;
+    return HM_OK;
+}
+}  // extern "C++"
+
+int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                        const int64_t *row0, const int64_t *col0, const int64_t *m,
+                        const int64_t *n, const int64_t *rank, const double *const *A,
+                        const double *const *U, const double *const *S,
+                        const double *const *V, void *stream, hm_plan_t *out) {
+    return from_blocks_t<double>(nrows, ncols, nleaves, kind, row0, col0, m, n, rank, A, U, S, V,
+                                 stream, out);
+}
+
+int hm_plan_from_blocks_f32(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                            const int64_t *row0, const int64_t *col0, const int64_t *m,
+                            const int64_t *n, const int64_t *rank, const float *const *A,
+                            const float *const *U, const float *const *S,
+                            const float *const *V, void *stream, hm_plan_t *out) {
+    return from_blocks_t<float>(nrows, ncols, nleaves, kind, row0, col0, m, n, rank, A, U, S, V,
+                                stream, out);
+}
+
+// Dense leaves and F of an HM_USER plan from the caller's values of f.
+extern "C++" {
+template <class T>
+static int check_type(hm_plan_t P, const char *what);
+
+template <class T>
+static int set_leaf_data_t(hm_plan_t P, const T *A, int64_t nA, const T *F, int64_t nF,
+                           void *stream) {
+    if (!P) return fail(HM_ERR_INVALID, "set_leaf_data: NULL plan");
+    if (P->kid != HM_USER || P->from_blocks)
+        return fail(HM_ERR_INVALID, "set_leaf_data: only plans built with kernel HM_USER");
+    if (P->flags & (HM_PLAN_TREE_ONLY | HM_PLAN_MATRIX_FREE))
+        return fail(HM_ERR_INVALID, "set_leaf_data: the plan stores no dense blocks and F");
+    if (int rc = check_type<T>(P, "set_leaf_data")) return rc;
+    const int64_t r = P->r;
+    int64_t wantA = 0;
+    for (const DDense &D : P->dense) wantA += (int64_t)D.m * D.n;
+    const int64_t wantF = (int64_t)P->far.size() * r * r;
+    if (nA != wantA || nF != wantF)
+        return fail(HM_ERR_INVALID, "set_leaf_data: %lld dense and %lld far values given, the plan has %lld and %lld",
+                    (long long)nA, (long long)nF, (long long)wantA, (long long)wantF);
+    if ((nA > 0 && !A) || (nF > 0 && !F)) return fail(HM_ERR_INVALID, "set_leaf_data: NULL data");
+    PlanBufs<T> &b = P->bufs<T>();
+    int64_t tot = 0;
+    for (const DDense &D : P->dense) tot = std::max<int64_t>(tot, D.aoff + tiled_size(D.m, D.n));
+    std::vector<T> hA((size_t)tot, T(0));
+    int64_t nonfinite = 0;
+    const T *src = A;
+    for (const DDense &D : P->dense) {
+        for (int64_t j = 0; j < D.n; ++j)
+            for (int64_t i = 0; i < D.m; ++i) {
+                const T v = src[i + (int64_t)D.m * j];
+                nonfinite += std::isfinite(v) ? 0 : 1;
+                hA[(size_t)(D.aoff + tile_index(i, j, D.n))] = v;
+            }
+        src += (int64_t)D.m * D.n;
+    }
+    for (int64_t q = 0; q < nF; ++q) nonfinite += std::isfinite(F[q]) ? 0 : 1;
+    hipStream_t s = (hipStream_t)stream;
+    if (tot > 0) HM_HIP(hipMemcpyAsync(b.A, hA.data(), sizeof(T) * (size_t)tot, hipMemcpyHostToDevice, s));
+    if (nF > 0) HM_HIP(hipMemcpyAsync(b.F, F, sizeof(T) * (size_t)nF, hipMemcpyHostToDevice, s));
+    HM_HIP(hipStreamSynchronize(s));  // the host staging is released on return
+    P->nonfinite = P->nonfinite_basis + nonfinite;
+    return HM_OK;
+}
+}  // extern "C++"
+
+int hm_plan_set_leaf_data(hm_plan_t P, const double *A, int64_t nA, const double *F, int64_t nF,
+                          void *stream) {
+    return set_leaf_data_t(P, A, nA, F, nF, stream);
+}
+
+int hm_plan_set_leaf_data_f32(hm_plan_t P, const float *A, int64_t nA, const float *F,
+                              int64_t nF, void *stream) {
+    return set_leaf_data_t(P, A, nA, F, nF, stream);
+}
+
+int hm_plan_add_col(hm_plan_t P, const double *u, int64_t ioff, int64_t j, void *stream) {
+    if (!P || !u || ioff < 0) return fail(HM_ERR_INVALID, "add_col: bad arguments");
+    if (!P->from_blocks)
+        return fail(HM_ERR_UNSUPPORTED, "add_col: only plans built from blocks (the reference defines add_col! for HierarchicalMatrix)");
+    if (P->f32) return fail(HM_ERR_INVALID, "add_col: a Float32 plan (Float64 entry point)");
+    if (j < 0 || j >= P->ny) return fail(HM_ERR_INVALID, "add_col: column %lld outside 0..%lld", (long long)j, (long long)P->ny - 1);
+    // the leaves holding column j: dense ones add u, low-rank ones take it as
+    // one more rank term (u in U, e_j in V, 1 on F's diagonal) in their next
+    // free column; all are checked before anything changes
+    std::vector<AddColJob> jobs;
+    for (size_t t = 0; t < P->leaves.size(); ++t) {
+        const Leaf &L = P->leaves[t];
+        if (L.m == 0 || L.n == 0 || j < L.col0 || j >= L.col0 + L.n) continue;
+        const int32_t id = P->leaf_blk[t];
+        AddColJob q{};
+        q.m = (int32_t)L.m;
+        q.jl = (int32_t)(j - L.col0);
+        q.row0 = L.row0;
+        if (L.code == 3) {
+            const DDense &D = P->dense[id];
+            q.kind = 3;
+            q.n = D.n;
+            q.aoff = D.aoff;
+        } else {
+            if (id < 0 || P->far_rank[id] >= P->r)
+                return fail(HM_ERR_UNSUPPORTED,
+                            "add_col: leaf %zu has no free rank column (the plan's rank is %d); rebuild the plan from the leaves",
+                            t, P->r);
+            const DFarBlk &F = P->far[id];
+            q.kind = 2;
+            q.c = P->far_rank[id];
+            q.uoff = P->rcl[F.rc].boff;
+            q.voff = P->ccl[F.cc].boff;
+            q.foff = F.foff;
+            q.far = id;
+        }
+        jobs.push_back(q);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t b = 0; b < jobs.size(); b += HM_ADD_COL_JOBS) {
+        AddColJobs J{};
+        J.count = (int32_t)std::min<size_t>(HM_ADD_COL_JOBS, jobs.size() - b);
+        for (int q = 0; q < J.count; ++q) J.job[q] = jobs[b + q];
+        const hipError_t e = hmx_launch_add_col(P->b64.dp, J, u + ioff, s);
+        if (e != hipSuccess) return fail(HM_ERR_HIP, "add_col launch: %s", hipGetErrorString(e));
+    }
+    for (const AddColJob &q : jobs)
+        if (q.kind == 2) ++P->far_rank[q.far];
     return HM_OK;
 }
 
@@ -1150,6 +1305,7 @@ int hm_plan_scale(hm_plan_t P, int side, const double *b, int64_t off, void *str
         return fail(HM_ERR_INVALID, "scale: bad arguments");
     if (!P->from_blocks)
         return fail(HM_ERR_UNSUPPORTED, "scale: only plans built from blocks (the reference has no scale! for KernelMatrix leaves)");
+    if (P->f32) return fail(HM_ERR_INVALID, "scale: a Float32 plan (Float64 entry point)");
     hipError_t e = hmx_launch_scale(P->b64.dp, side, b + off, (hipStream_t)stream);
     if (e != hipSuccess) return fail(HM_ERR_HIP, "scale launch: %s", hipGetErrorString(e));
     return HM_OK;
@@ -1323,11 +1479,12 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
     if (int rc = check_type<T>(P, "matvec")) return rc;
     if (k < 0 || incx < 1 || incy < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "matvec: bad offsets, strides, k or beta");
-    if (flags & ~127u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
+    if (flags & ~255u) return fail(HM_ERR_INVALID, "matvec: unknown flags 0x%x", flags);
     if ((flags & HM_MV_MFMA) && (flags & HM_MV_NO_MFMA))
         return fail(HM_ERR_INVALID, "matvec: HM_MV_MFMA and HM_MV_NO_MFMA exclude each other");
-    const int prec = (flags & HM_MV_FAR_BF16X3) ? 1 : (flags & HM_MV_FAR_F32) ? 0 : -1;
-    if ((flags & HM_MV_FAR_BF16X3) && (flags & HM_MV_FAR_F32))
+    const int prec = (flags & HM_MV_FAR_BF16X6) ? 3 : (flags & HM_MV_FAR_BF16X3) ? 1
+                   : (flags & HM_MV_FAR_F32) ? 0 : -1;
+    if (__builtin_popcount(flags & (HM_MV_FAR_BF16X3 | HM_MV_FAR_F32 | HM_MV_FAR_BF16X6)) > 1)
         return fail(HM_ERR_INVALID, "matvec: choose one far-field precision");
     if (k == 0) return HM_OK;
     if (!y || !x) return fail(HM_ERR_INVALID, "matvec: NULL vector");
@@ -1337,9 +1494,11 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
     else if ((P->flags & HM_PLAN_HYBRID) || (flags & HM_MV_CHEB_BASIS)) mode = 1;
     if (P->from_blocks && (mode != 0 || (flags & (HM_MV_MATRIX_FREE | HM_MV_CHEB_BASIS))))
         return fail(HM_ERR_UNSUPPORTED, "a plan built from blocks has no points: stored form only");
+    if (P->kid == HM_USER && mode == 2)
+        return fail(HM_ERR_UNSUPPORTED, "HM_USER plan: f is a host function, no matrix-free product");
     int max_kb = P->max_kb;
-    if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0 && !P->from_blocks) {
-        mode = std::max(mode, P->tuned_mode);
+    if ((flags & HM_MV_TUNED) && P->tuned_mode >= 0) {
+        if (!P->from_blocks) mode = std::max(mode, P->tuned_mode);
         max_kb = std::min(P->tuned_kb, HM_MAX_KB);
     }
     hipStream_t s = (hipStream_t)stream;
@@ -1449,6 +1608,7 @@ static int grad_x_common(hm_plan_t P, T *g, const T *b, const T *s, int64_t ioff
     if (!P || !g || !b || !s) return fail(HM_ERR_INVALID, "grad_x: NULL argument");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "grad_x: tree-only plan");
     if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "grad_x: a plan built from blocks has no points");
+    if (P->kid == HM_USER) return fail(HM_ERR_UNSUPPORTED, "grad_x: f is a host function (no derivative on the device)");
     if (int rc = check_type<T>(P, "grad_x")) return rc;
     if (incg < 1 || incb < 1 || incs < 1 || ioff < 0 || joff < 0 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "grad_x: bad offsets, strides or beta");
@@ -1471,6 +1631,7 @@ static int t_grad_common(hm_plan_t P, T *gb, T *gy, const T *r, const T *s, int6
     if (!P || !gb || !gy || !r || !s) return fail(HM_ERR_INVALID, "t_grad: NULL argument");
     if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "t_grad: tree-only plan");
     if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "t_grad: a plan built from blocks has no points");
+    if (P->kid == HM_USER) return fail(HM_ERR_UNSUPPORTED, "t_grad: f is a host function (no derivative on the device)");
     if (int rc = check_type<T>(P, "t_grad")) return rc;
     if (incgb < 1 || incgy < 1 || incr < 1 || incs < 1 || (beta != 0 && beta != 1))
         return fail(HM_ERR_INVALID, "t_grad: bad strides or beta");
@@ -1582,17 +1743,31 @@ extern "C++" {
 template <class T>
 static int tune_common(hm_plan_t P, T *y, const T *x, int64_t k, void *stream, int64_t out[3]) {
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t e0, e1;
-    HM_HIP(hipEventCreate(&e0));
-    HM_HIP(hipEventCreate(&e1));
+    struct Events {  // destroyed on every exit path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    HM_HIP(hipEventCreate(&ev.e0));
+    HM_HIP(hipEventCreate(&ev.e1));
     const int base_mode = (P->flags & HM_PLAN_MATRIX_FREE) ? 2 : (P->flags & HM_PLAN_HYBRID) ? 1 : 0;
+    // a plan built from blocks has no points: its stored form is the only one
+    // nor is f on the device for HM_USER plans: no matrix-free candidate
+    const int last_mode = P->from_blocks ? 0 : P->kid == HM_USER ? 1 : 2;
     float best = 1e30f;
     int best_kb = 1, best_mode = base_mode;
     const int saved_kb = P->max_kb;
+    struct Restore {  // the plan's launch shape is restored on every exit path
+        hm_plan_t P;
+        int kb;
+        ~Restore() { P->max_kb = kb; }
+    } restore{P, saved_kb};
     // candidates: 1, 2, 4, 8 right-hand sides per launch on the vector units,
     // and 16 per launch on the MFMA passes (Float64 KernelMatrix plans)
     const bool mfma_ok = std::is_same_v<T, double> && !P->from_blocks;
-    for (int mode = base_mode; mode <= 2; ++mode) {
+    for (int mode = base_mode; mode <= last_mode; ++mode) {
         const unsigned fl0 = mode == 2 ? HM_MV_MATRIX_FREE : mode == 1 ? HM_MV_CHEB_BASIS : 0;
         for (int kb = 1; kb <= MFMA_KB; kb *= 2) {
             if (kb == MFMA_KB) {
@@ -1600,27 +1775,24 @@ static int tune_common(hm_plan_t P, T *y, const T *x, int64_t k, void *stream, i
             } else if (kb > 1 && kb > k) {
                 continue;
             }
-            const unsigned fl = fl0 | (kb == MFMA_KB ? HM_MV_MFMA : HM_MV_NO_MFMA);
+            const unsigned fl = fl0 | (kb == MFMA_KB ? HM_MV_MFMA : (mfma_ok ? HM_MV_NO_MFMA : 0));
             P->max_kb = std::min(kb, HM_MAX_KB);
             int rc = matvec_common(P, y, x, 0, 0, 1, 1, k, P->nx, P->ny, 0, fl, s, false);
-            if (rc != HM_OK) { P->max_kb = saved_kb; return rc; }
+            if (rc != HM_OK) return rc;
             float ms = 0.f;
-            HM_HIP(hipEventRecord(e0, s));
+            HM_HIP(hipEventRecord(ev.e0, s));
             for (int rep = 0; rep < 3; ++rep) {
                 rc = matvec_common(P, y, x, 0, 0, 1, 1, k, P->nx, P->ny, 0, fl, s, false);
-                if (rc != HM_OK) { P->max_kb = saved_kb; return rc; }
+                if (rc != HM_OK) return rc;
             }
-            HM_HIP(hipEventRecord(e1, s));
-            HM_HIP(hipEventSynchronize(e1));
-            HM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            HM_HIP(hipEventRecord(ev.e1, s));
+            HM_HIP(hipEventSynchronize(ev.e1));
+            HM_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
             if (ms < best) { best = ms; best_kb = kb; best_mode = mode; }
         }
     }
-    P->max_kb = saved_kb;
     P->tuned_kb = best_kb;
     P->tuned_mode = best_mode;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (out) {
         out[0] = best_kb;
         out[1] = best_mode;

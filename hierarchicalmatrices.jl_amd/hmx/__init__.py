@@ -9,7 +9,12 @@ from .kernelmatrix import (  # noqa: F401
     cauchykernel, chebyshevbarycentricweights, chebyshevpoints, coulombkernel,
     coulombprimekernel, indsplit, logkernel, mul_, size,
 )
-from .hierarchical import HierarchicalMatrix, LowRankMatrix, add_col_, lmul_, rmul_  # noqa: F401
+from .barycentric import (  # noqa: F401
+    BarycentricFactors, BarycentricMatrix2D, BarycentricPoly2D, evaluate, update_,
+)
+from .hierarchical import (  # noqa: F401
+    HierarchicalMatrix, LowRankMatrix, add_col_, blocks_plan, lmul_, rmul_,
+)
 from .even import EvenBarycentricLeaves, EvenBarycentricMatrix, EvenLeaf, barycentricmatrix  # noqa: F401
 from . import workloads  # noqa: F401
 from . import lowrank  # noqa: F401
@@ -23,4 +28,6 @@ __all__ = [
     "chebyshevpoints", "chebyshevbarycentricweights", "indsplit", "mul_", "workloads",
     "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch", "rmul_", "lmul_", "add_col_",
     "EvenBarycentricMatrix", "EvenBarycentricLeaves", "barycentricmatrix",
+    "BarycentricPoly2D", "BarycentricMatrix2D", "BarycentricFactors", "evaluate", "update_",
+    "blocks_plan",
 ]

@@ -17,6 +17,8 @@ HM_ERR_OOM = 4
 HM_ERR_HIP = 5
 HM_ERR_UNSUPPORTED = 6
 
+HM_USER = 4  # kernel id of a host function f (hm_plan_set_leaf_data)
+
 HM_PLAN_STORED = 0
 HM_PLAN_MATRIX_FREE = 1
 HM_PLAN_FAR_F32 = 2
@@ -29,6 +31,7 @@ HM_MV_FAR_F32 = 8
 HM_MV_FAR_BF16X3 = 16
 HM_MV_MFMA = 32
 HM_MV_NO_MFMA = 64
+HM_MV_FAR_BF16X6 = 128
 
 INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
@@ -88,6 +91,23 @@ def _load():
                                       _ip, _ip, _ip, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                       ctypes.POINTER(_dp), ctypes.POINTER(_dp), _vp,
                                       ctypes.POINTER(_vp)]
+    L.hm_plan_from_blocks_f32.restype = ctypes.c_int
+    L.hm_plan_from_blocks_f32.argtypes = [_i64, _i64, _i64, ctypes.POINTER(ctypes.c_int32), _ip,
+                                          _ip, _ip, _ip, _ip, ctypes.POINTER(_fp),
+                                          ctypes.POINTER(_fp), ctypes.POINTER(_fp),
+                                          ctypes.POINTER(_fp), _vp, ctypes.POINTER(_vp)]
+    L.hm_plan_set_leaf_data.restype = ctypes.c_int
+    L.hm_plan_set_leaf_data.argtypes = [_vp, _dp, _i64, _dp, _i64, _vp]
+    L.hm_plan_set_leaf_data_f32.restype = ctypes.c_int
+    L.hm_plan_set_leaf_data_f32.argtypes = [_vp, _fp, _i64, _fp, _i64, _vp]
+    L.hm_bary_evaluate.restype = ctypes.c_int
+    L.hm_bary_evaluate.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]
+    L.hm_bary_evaluate_f32.restype = ctypes.c_int
+    L.hm_bary_evaluate_f32.argtypes = L.hm_bary_evaluate.argtypes
+    L.hm_bary_basis.restype = ctypes.c_int
+    L.hm_bary_basis.argtypes = [ctypes.c_int, _vp, _vp, _i64, _vp, _vp, _i64, _vp]
+    L.hm_bary_basis_f32.restype = ctypes.c_int
+    L.hm_bary_basis_f32.argtypes = L.hm_bary_basis.argtypes
     L.hm_plan_batched.restype = ctypes.c_int
     L.hm_plan_batched.argtypes = [ctypes.c_int, _i64, ctypes.POINTER(_dp), _ip,
                                   ctypes.POINTER(_dp), _ip, _dp, ctypes.c_int, ctypes.c_int,
@@ -126,6 +146,8 @@ def _load():
     L.hm_plan_profile_read.argtypes = [_vp, _dp, _ip]
     L.hm_entries.restype = ctypes.c_int
     L.hm_entries.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp]
+    L.hm_plan_add_col.restype = ctypes.c_int
+    L.hm_plan_add_col.argtypes = [_vp, _vp, _i64, _i64, _vp]
     L.hm_plan_scale.restype = ctypes.c_int
     L.hm_plan_scale.argtypes = [_vp, ctypes.c_int, _vp, _i64, _vp]
     # Float32 plans

@@ -28,6 +28,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
+from .barycentric import BarycentricFactors, BarycentricMatrix2D
 from .kernelmatrix import Block, KernelMatrix, _stream_ptr
 
 
@@ -129,7 +130,9 @@ class HierarchicalMatrix:
             code = 1
         elif isinstance(A, LowRankMatrix):
             code = 2
-        elif isinstance(A, np.ndarray) and A.ndim == 2 and A.dtype == np.float64:
+        elif isinstance(A, (BarycentricMatrix2D, BarycentricFactors)):
+            code = 2  # a far leaf: the LowRank slot of the reference's tree types
+        elif isinstance(A, np.ndarray) and A.ndim == 2 and A.dtype in (np.float64, np.float32):
             code = 3
             A = np.asfortranarray(A)
         else:  # no matching field: the reference leaves the block unassigned
@@ -160,9 +163,9 @@ class HierarchicalMatrix:
         c = self.blocks[m][n]
         if c is None:
             return 0.0
-        if isinstance(c, (HierarchicalMatrix, LowRankMatrix)):
-            return c[i, j] if isinstance(c, LowRankMatrix) else c.getindex(i, j)
-        return float(c[i, j])
+        if isinstance(c, HierarchicalMatrix):
+            return c.getindex(i, j)
+        return c[i, j] if not isinstance(c, np.ndarray) else float(c[i, j])
 
     def leaves(self, istart=0, jstart=0):
         """Leaves in the traversal order of src/HierarchicalMatrix.jl:24-52,
@@ -209,54 +212,97 @@ class HierarchicalMatrix:
 
 
 class _BlocksPlan(KernelMatrix):
-    """hm_plan_from_blocks over the flattened leaves of a HierarchicalMatrix."""
+    """hm_plan_from_blocks (or its Float32 twin) over the flattened leaves of
+    a HierarchicalMatrix. The element type is that of the leaves (all float64
+    or all float32)."""
 
     def __init__(self, H, stream=None):
         rows, cols = H.size()
         leaves = H.leaves()
         nl = len(leaves)
+        dts = {_leaf_dtype(c) for c, _, _ in leaves}
+        if len(dts) > 1:
+            raise TypeError("leaves of one HierarchicalMatrix must share one element type")
+        dt = dts.pop() if dts else np.float64
+        self.dtype = dt
         kind = np.zeros(max(nl, 1), dtype=np.int32)
         r0 = np.zeros(max(nl, 1), dtype=np.int64)
         c0 = np.zeros_like(r0)
         mm = np.zeros_like(r0)
         nn = np.zeros_like(r0)
         rk = np.zeros_like(r0)
-        DP = ctypes.POINTER(ctypes.c_double)
+        DP = ctypes.POINTER(ctypes.c_float if dt == np.float32 else ctypes.c_double)
         A = (DP * max(nl, 1))()
         U = (DP * max(nl, 1))()
         S = (DP * max(nl, 1))()
         V = (DP * max(nl, 1))()
         keep = []
+
+        def put(arr, dst, t):
+            a = np.ascontiguousarray(np.asarray(arr).ravel(order="F"), dtype=dt)
+            keep.append(a)
+            dst[t] = a.ctypes.data_as(DP)
+
         for t, (c, i0, j0) in enumerate(leaves):
             r0[t], c0[t] = i0, j0
+            mm[t], nn[t] = c.shape
             if isinstance(c, LowRankMatrix):
                 kind[t] = 2
-                mm[t], nn[t] = c.shape
                 rk[t] = c.rank()
-                for arr, dst in ((c.U, U), (c.S, S), (c.V, V)):
-                    a = np.ascontiguousarray(arr.ravel(order="F"))
-                    keep.append(a)
-                    dst[t] = a.ctypes.data_as(DP)
+                put(c.U, U, t)
+                put(c.S, S, t)
+                put(c.V, V, t)
+            elif isinstance(c, (BarycentricMatrix2D, BarycentricFactors)):
+                kind[t] = 4  # U F V' with the core F in the S slot
+                rk[t] = c.rank()
+                put(c.U, U, t)
+                put(c.F, S, t)
+                put(c.V, V, t)
             else:
                 kind[t] = 3
-                mm[t], nn[t] = c.shape
-                a = np.ascontiguousarray(c.ravel(order="F"))
-                keep.append(a)
-                A[t] = a.ctypes.data_as(DP)
+                put(c, A, t)
         IP = ctypes.POINTER(ctypes.c_int64)
         h = ctypes.c_void_p()
-        check(lib.hm_plan_from_blocks(rows, cols, nl, kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                      r0.ctypes.data_as(IP), c0.ctypes.data_as(IP),
-                                      mm.ctypes.data_as(IP), nn.ctypes.data_as(IP),
-                                      rk.ctypes.data_as(IP), A, U, S, V, _stream_ptr(stream),
-                                      ctypes.byref(h)))
+        ctor = lib.hm_plan_from_blocks_f32 if dt == np.float32 else lib.hm_plan_from_blocks
+        check(ctor(rows, cols, nl, kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                   r0.ctypes.data_as(IP), c0.ctypes.data_as(IP), mm.ctypes.data_as(IP),
+                   nn.ctypes.data_as(IP), rk.ctypes.data_as(IP), A, U, S, V,
+                   _stream_ptr(stream), ctypes.byref(h)))
         self._h = h
         self.f = None
         self.rank = int(rk.max()) if nl else 0
         self.blocksize_ = 0
         self.flags = 0
-        self.x = np.zeros(rows)
-        self.y = np.zeros(cols)
+        self.x = np.zeros(rows, dtype=dt)
+        self.y = np.zeros(cols, dtype=dt)
+
+
+def _leaf_dtype(c):
+    if isinstance(c, (LowRankMatrix, BarycentricMatrix2D, BarycentricFactors)):
+        return np.dtype(c.U.dtype).type
+    return np.dtype(c.dtype).type
+
+
+class _FlatLeaves:
+    """A leaf list with the two methods _BlocksPlan reads."""
+
+    def __init__(self, rows, cols, leaves):
+        self._shape = (int(rows), int(cols))
+        self._leaves = [(c, int(i0), int(j0)) for c, i0, j0 in leaves]
+
+    def size(self):
+        return self._shape
+
+    def leaves(self):
+        return self._leaves
+
+
+def blocks_plan(rows, cols, leaves, stream=None):
+    """A device plan of a rows x cols matrix given as leaves (block, row0,
+    col0) with 0-based offsets: dense numpy blocks, LowRankMatrix,
+    BarycentricMatrix2D or BarycentricFactors (hm_plan_from_blocks). The
+    product adds every leaf, as the reference's traversal does."""
+    return _BlocksPlan(_FlatLeaves(rows, cols, leaves), stream)
 
 
 def _scale_host(H, b, start, side):
@@ -328,24 +374,20 @@ def lmul_(b, H):
 
 
 def _add_col_lowrank(L, u, istart, j):
-    """add_col(L, u, istart, j) for Float64 (src/algebra.jl:361-380): rank + 1,
-    the new term first: U1 = [u[istart:...] U], S1 = [1; S], V1 = [e_j V]."""
+    """add_col(L, u, istart, j) (src/algebra.jl:352-360): rank + 1 with the
+    new term last, U1 = [U u[istart:...]], S1 = [S; 1], V1 = [V e_j]. (The
+    Float64 method of :362-380 puts the term first but calls Vector{T}(r+1),
+    which Julia no longer has; the matrix is the same.)"""
     m, n = L.shape
-    U1 = np.concatenate([u[istart - 1:istart - 1 + m, None], L.U], axis=1)
-    S1 = np.concatenate([[1.0], L.S])
-    e = np.zeros((n, 1))
-    e[j - 1, 0] = 1.0
-    V1 = np.concatenate([e, L.V], axis=1)
+    U1 = np.concatenate([L.U, u[istart - 1:istart - 1 + m, None]], axis=1)
+    S1 = np.concatenate([L.S, [1.0]])
+    V1 = np.concatenate([L.V, np.zeros((n, 1))], axis=1)
+    V1[j - 1, -1] = 1.0
     return LowRankMatrix(U1, S1, V1)
 
 
-def add_col_(H, u, j, istart=1):
-    """add_col!(H, u, istart, j): column j (1-based) of H gains u[istart:...]
-    (src/HierarchicalMatrix.jl:21,104-146): dense leaves add in place, LowRank
-    leaves are replaced by their rank + 1 form, nested matrices recurse. The
-    device plan is rebuilt on the next product."""
-    u = np.ascontiguousarray(u.detach().cpu().numpy() if hasattr(u, "detach") else u,
-                             dtype=np.float64)
+def _add_col_host(H, u, j, istart):
+    """add_col! on the host leaves (src/HierarchicalMatrix.jl:104-146)."""
     M, N = H.assigned.shape
     n = 0
     q = H.blocksize(1, 1, 2)
@@ -357,11 +399,43 @@ def add_col_(H, u, j, istart=1):
     for m in range(M):
         c = H.blocks[m][n]
         if isinstance(c, HierarchicalMatrix):
-            add_col_(c, u, j - q, istart + p)
+            _add_col_host(c, u, j - q, istart + p)
         elif isinstance(c, LowRankMatrix):
             H.blocks[m][n] = _add_col_lowrank(c, u, istart + p, j - q)
-        elif c is not None:  # A[i + jm] += u[i + ishift]
+        elif isinstance(c, np.ndarray):  # A[i + jm] += u[i + ishift]
             c[:, j - q - 1] += u[istart + p - 1:istart + p - 1 + c.shape[0]]
+        elif c is not None:
+            raise TypeError(f"add_col!: no method for a {type(c).__name__} leaf")
         p += H.blocksize(m + 1, N, 1)
-    H._drop_plans()
+
+
+def add_col_(H, u, j, istart=1):
+    """add_col!(H, u, istart, j): column j (1-based) of H gains u[istart:...]
+    (src/HierarchicalMatrix.jl:21,104-146): dense leaves add in place, LowRank
+    leaves become their rank + 1 form, nested matrices recurse. A device plan
+    built for H is updated in place (hm_plan_add_col) when every LowRank leaf
+    holding column j has a free rank column (its rank below the plan's);
+    otherwise it is rebuilt on the next product."""
+    u_host = np.ascontiguousarray(u.detach().cpu().numpy() if hasattr(u, "detach") else u,
+                                  dtype=np.float64)
+    rows = H.size(1)
+    if len(u_host) < istart - 1 + rows:
+        raise ValueError("u is shorter than istart - 1 + size(H, 1)")
+    _add_col_host(H, u_host, j, istart)
+    plan = H._plan
+    for row in H.blocks:  # nested plans would no longer match their leaves
+        for c in row:
+            if isinstance(c, HierarchicalMatrix):
+                c._drop_plans()
+    if plan is not None:
+        from .kernelmatrix import _torch
+        torch = _torch()
+        ud = torch.from_numpy(u_host).cuda()
+        rc = lib.hm_plan_add_col(plan._h, ctypes.c_void_p(ud.data_ptr()), istart - 1, j - 1,
+                                 _stream_ptr(None))
+        torch.cuda.synchronize()
+        if rc == _lib.HM_ERR_UNSUPPORTED:
+            H._plan = None  # no free rank column: rebuilt from the leaves when next used
+        else:
+            check(rc)
     return H

@@ -10,12 +10,16 @@ Names, argument meaning and accumulate semantics follow the reference:
                                            src/BarycentricMatrix.jl:92-136,299-307
   blocksize(H), size(H), H[i, j]           src/hierarchical.jl:31-47,120-147
 
-Differences a caller sees: the kernel f is one of the named kernels of
-examples/Kernel.jl:34-37 (cauchykernel, coulombkernel, coulombprimekernel,
-logkernel) instead of an arbitrary Julia function; element access H[i, j] uses
-0-based Python indices; vectors may be float64 torch tensors on the GPU (used
-in place, stream-ordered on torch's current stream) or numpy arrays (copied to
-and from the device).
+f is either one of the named kernels of examples/Kernel.jl:34-37
+(cauchykernel, coulombkernel, coulombprimekernel, logkernel), which the device
+evaluates itself (every plan form, gradients), or any host function of two
+scalars (the reference's arbitrary f): then the device builds the tree and the
+bases U, V, which do not depend on f, while f is evaluated on the host at the
+dense leaves' point pairs and the far blocks' node pairs and handed over
+(kernel HM_USER, hm_plan_set_leaf_data); such a matrix has the stored and
+hybrid forms only. Element access H[i, j] uses 0-based Python indices; vectors
+may be torch tensors on the GPU (used in place, stream-ordered on torch's
+current stream) or numpy arrays (copied to and from the device).
 """
 import ctypes
 
@@ -158,7 +162,8 @@ class _Vec:
             self.host[...] = self.t.cpu().numpy()
 
 
-_FAR = {"f64": 0, "f32": _lib.HM_MV_FAR_F32, "bf16x3": _lib.HM_MV_FAR_BF16X3}
+_FAR = {"f64": 0, "f32": _lib.HM_MV_FAR_F32, "bf16x3": _lib.HM_MV_FAR_BF16X3,
+        "bf16x6": _lib.HM_MV_FAR_BF16X6}
 
 
 def _mfma_flag(mfma):
@@ -185,9 +190,12 @@ class KernelMatrix:
                  hybrid=False, tree_only=False, stream=None, dtype=None):
         if isinstance(f, str):
             f = KERNELS[f]
-        if not isinstance(f, NamedKernel):
-            raise TypeError("f must be one of cauchykernel, coulombkernel, coulombprimekernel, "
-                            "logkernel")
+        if not isinstance(f, NamedKernel) and not callable(f):
+            raise TypeError("f must be a named kernel or a function of two scalars")
+        user = not isinstance(f, NamedKernel)
+        if user and matrix_free:
+            raise TypeError("a host function f cannot be regenerated on the device: "
+                            "stored or hybrid form only")
         self.f = f
         if dtype is None:
             dtype = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
@@ -212,10 +220,43 @@ class KernelMatrix:
         ctor = lib.hm_plan_from_points_f32 if self.f32 else lib.hm_plan_from_points
         if self.f32:  # the intervals are Float32 values too
             a, b, c, d = (float(np.float32(v)) for v in (a, b, c, d))
-        check(ctor(f.kid, _dptr(self.x), len(self.x), _dptr(self.y), len(self.y), float(a),
-                   float(b), float(c), float(d), self.blocksize_, self.rank, flags, sp,
-                   ctypes.byref(h)))
+        check(ctor(_lib.HM_USER if user else f.kid, _dptr(self.x), len(self.x), _dptr(self.y),
+                   len(self.y), float(a), float(b), float(c), float(d), self.blocksize_,
+                   self.rank, flags, sp, ctypes.byref(h)))
         self._h = h
+        if user and not tree_only:
+            self._set_user_leaves(stream)
+
+    def _set_user_leaves(self, stream=None):
+        """f on the host at every dense leaf's (x_i, y_j) and every far leaf's
+        mapped node pairs (t_m, s_n) (src/KernelMatrix.jl:57-60,82,104;
+        src/BarycentricMatrix.jl:147-178), in traversal order, into the plan."""
+        from .barycentric import eval_pairs, mapped_nodes
+        dt = self.dtype
+        ints, dbls = self.blocks()
+        dense, far = [], []
+        r = self.rank
+        for (code, _, i0, j0, m, n), (a, b, c, d) in zip(ints, dbls):
+            if m == 0 or n == 0:
+                continue
+            if code == 3:
+                blk = eval_pairs(self.f, self.x[i0:i0 + m, None], self.y[None, j0:j0 + n], dt)
+                dense.append(np.asarray(blk, dtype=dt).ravel(order="F"))
+            else:
+                tx, ty = mapped_nodes(a, b, r, dt), mapped_nodes(c, d, r, dt)
+                far.append((tx, ty))
+        A = np.concatenate(dense) if dense else np.zeros(0, dtype=dt)
+        if far:
+            TX = np.stack([t for t, _ in far])
+            TY = np.stack([t for _, t in far])
+            Fb = eval_pairs(self.f, TX[:, :, None], TY[:, None, :], dt)  # (nb, m, n)
+            F = np.ascontiguousarray(np.transpose(Fb, (0, 2, 1))).ravel()  # F[m + r n] per block
+        else:
+            F = np.zeros(0, dtype=dt)
+        A = np.ascontiguousarray(A, dtype=dt)
+        F = np.ascontiguousarray(F, dtype=dt)
+        fn = lib.hm_plan_set_leaf_data_f32 if self.f32 else lib.hm_plan_set_leaf_data
+        check(fn(self._h, _dptr(A), len(A), _dptr(F), len(F), _stream_ptr(stream)))
 
     @property
     def f32(self):
@@ -325,10 +366,26 @@ class KernelMatrix:
             return out.cpu().numpy()
         return out
 
-    def grad_x(self, b, s, stream=None):
-        """g_i = s_i * d/dx_i (H(x, y) b)_i (device tensors, 1-D)."""
+    def _grad_vec(self, t, n, name):
+        """A 1-D device tensor of the plan's element type with at least n
+        entries at its stride (the gradient entry points read n of them)."""
         torch = _torch()
-        g = torch.empty(self.shape[0], dtype=self._tdtype(), device=b.device)
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != self._tdtype():
+            raise TypeError(f"{name} must be a {np.dtype(self.dtype).name} CUDA tensor (the eltype of H)")
+        if t.dim() != 1:
+            raise ValueError(f"{name} must be 1-D")
+        if t.shape[0] < n:
+            raise ValueError(f"{name} has {t.shape[0]} entries, H needs {n}")
+        return t
+
+    def grad_x(self, b, s, stream=None):
+        """g_i = s_i * d/dx_i (H(x, y) b)_i (device tensors, 1-D: b of length
+        size(H, 2), s of length size(H, 1))."""
+        torch = _torch()
+        m, n = self.shape
+        b = self._grad_vec(b, n, "b")
+        s = self._grad_vec(s, m, "s")
+        g = torch.empty(m, dtype=self._tdtype(), device=b.device)
         fn = lib.hm_matvec_grad_x_f32 if self.f32 else lib.hm_matvec_grad_x
         check(fn(self._h, ctypes.c_void_p(g.data_ptr()),
                                    ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(s.data_ptr()),
@@ -338,7 +395,10 @@ class KernelMatrix:
     def tgrad(self, r, s, stream=None):
         """(H^T r, s .* d/dy_j (H^T r)_j) for device tensors r (rows) and s (cols)."""
         torch = _torch()
-        gb = torch.empty(self.shape[1], dtype=self._tdtype(), device=r.device)
+        m, n = self.shape
+        r = self._grad_vec(r, m, "r")
+        s = self._grad_vec(s, n, "s")
+        gb = torch.empty(n, dtype=self._tdtype(), device=r.device)
         gy = torch.empty_like(gb)
         fn = lib.hm_matvec_t_grad_f32 if self.f32 else lib.hm_matvec_t_grad
         check(fn(self._h, ctypes.c_void_p(gb.data_ptr()),

@@ -4,7 +4,9 @@
 Julia front-end of libhmx (C ABI in include/hmx.h), implemented by `ccall`:
 
   * `DeviceKernelMatrix{T}` (T = Float64 or Float32): the constructor and
-    product methods of `KernelMatrix` (src/KernelMatrix.jl:5-47) on the GPU.
+    product methods of `KernelMatrix` (src/KernelMatrix.jl:5-47) on the GPU,
+    for the named kernels (every form, gradients) or any Julia f (stored and
+    hybrid forms; also from a reference-built `KernelMatrix`).
   * `DeviceHierarchicalMatrix`: a `HierarchicalMatrix` with `LowRankMatrix` and
     `Matrix` leaves (src/HierarchicalMatrix.jl) flattened into a device plan
     (`hm_plan_from_blocks`), with `mul!` and the diagonal scalings `rmul!` /
@@ -23,7 +25,8 @@ Vectors may be host vectors (staged through device buffers) or device pointers
 module HierarchicalMatricesAMD
 
 using LinearAlgebra
-import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, blocksize
+import HierarchicalMatrices: HierarchicalMatrix, LowRankMatrix, KernelMatrix, blocksize,
+                             chebyshevpoints
 import ChainRulesCore
 
 const libhmx = joinpath(@__DIR__, "..", "hmx", "libhmx.so")
@@ -42,7 +45,11 @@ kernel_id(::typeof(cauchykernel)) = Cint(0)
 kernel_id(::typeof(coulombkernel)) = Cint(1)
 kernel_id(::typeof(coulombprimekernel)) = Cint(2)
 kernel_id(::typeof(logkernel)) = Cint(3)
-kernel_id(f) = throw(ArgumentError("libhmx knows cauchykernel, coulombkernel, coulombprimekernel and logkernel; got $f"))
+# any other f is a host function (HM_USER): the device builds the tree and the
+# bases U, V, and f's values at the dense leaves and far node pairs are
+# handed over by set_leaf_data! (stored and hybrid forms only)
+const HM_USER = Cint(4)
+kernel_id(f) = HM_USER
 
 # plan and product flags of include/hmx.h
 const HM_PLAN_MATRIX_FREE = Cuint(1)
@@ -51,6 +58,7 @@ const HM_MV_FAR_F32 = Cuint(8)
 const HM_MV_FAR_BF16X3 = Cuint(16)
 const HM_MV_MFMA = Cuint(32)
 const HM_MV_NO_MFMA = Cuint(64)
+const HM_MV_FAR_BF16X6 = Cuint(128)
 
 function check(st::Cint)
     st == 0 && return nothing
@@ -121,7 +129,141 @@ function DeviceKernelMatrix(f::Function, x::Vector{T}, y::Vector{T}, a::T, b::T,
         end
         check(st)
     end
-    DeviceKernelMatrix{T}(plan[], length(x), length(y))
+    H = DeviceKernelMatrix{T}(plan[], length(x), length(y))
+    if kernel_id(f) == HM_USER
+        mode === :free && throw(ArgumentError("a host function f has no matrix-free form"))
+        set_leaf_data!(H, user_leaf_data(f, H, x, y, rank)...)
+    end
+    H
+end
+
+# leaf list of a plan in traversal order: (code, depth, row0, col0, m, n) and
+# (a, b, c, d) per leaf (hm_plan_blocks)
+function plan_blocks(H)
+    cnt = Ref{Int64}(0)
+    check(ccall((:hm_plan_blocks, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Float64}, Int64,
+                                                  Ref{Int64}), H.plan, C_NULL, C_NULL, 0, cnt))
+    ints = Vector{Int64}(undef, 6cnt[])
+    dbls = Vector{Float64}(undef, 4cnt[])
+    check(ccall((:hm_plan_blocks, libhmx), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Float64}, Int64,
+                                                  Ref{Int64}), H.plan, ints, dbls, cnt[], cnt))
+    reshape(ints, 6, :), reshape(dbls, 4, :)
+end
+
+# f at every dense leaf's (x_i, y_j) and at every far leaf's mapped node pairs
+# F[m, n] = f(t_m, s_n), t = (a+b)/2 + (b-a)/2 x_hat in T, as
+# BarycentricPoly2D forms them (src/BarycentricMatrix.jl:147-178)
+function user_leaf_data(f, H::DeviceKernelMatrix{T}, x::Vector{T}, y::Vector{T}, r) where T
+    ints, dbls = plan_blocks(H)
+    A, F = T[], T[]
+    xh = chebyshevpoints(T, r)
+    for t in 1:size(ints, 2)
+        code, i0, j0, m, n = ints[1, t], ints[3, t], ints[4, t], ints[5, t], ints[6, t]
+        (m == 0 || n == 0) && continue
+        if code == 3
+            for j in 1:n, i in 1:m
+                push!(A, f(x[i0 + i], y[j0 + j]))
+            end
+        else
+            a, b, c, d = T(dbls[1, t]), T(dbls[2, t]), T(dbls[3, t]), T(dbls[4, t])
+            tx = [T(0.5) * (a + b) + T(0.5) * (b - a) * xh[p] for p in 1:r]
+            ty = [T(0.5) * (c + d) + T(0.5) * (d - c) * xh[q] for q in 1:r]
+            for q in 1:r, p in 1:r
+                push!(F, f(tx[p], ty[q]))
+            end
+        end
+    end
+    A, F
+end
+
+"""
+    set_leaf_data!(H, A, F)
+
+Dense leaves and far-field cores of a DeviceKernelMatrix of a host function
+(`hm_plan_set_leaf_data`): A every dense leaf (column-major, traversal order),
+F every far leaf's r x r core.
+"""
+function set_leaf_data!(H::DeviceKernelMatrix{T}, A::Vector{T}, F::Vector{T}) where T
+    GC.@preserve A F begin
+        st = if T === Float64
+            ccall((:hm_plan_set_leaf_data, libhmx), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid}),
+                  H.plan, A, length(A), F, length(F), C_NULL)
+        else
+            ccall((:hm_plan_set_leaf_data_f32, libhmx), Cint,
+                  (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid}),
+                  H.plan, A, length(A), F, length(F), C_NULL)
+        end
+        check(st)
+    end
+    H
+end
+
+# the leaves of a reference-built KernelMatrix in its traversal order
+# (src/KernelMatrix.jl:17-45): codes 1 KernelMatrix, 2 BarycentricMatrix2D,
+# 3 Matrix (@hierarchical KernelMatrix BarycentricMatrix2D Matrix)
+function kernel_leaves!(out, K::KernelMatrix)
+    M, N = blocksize(K)
+    for m = 1:M, n = 1:N
+        code = K.assigned[m, n]
+        if code == 1
+            kernel_leaves!(out, K.KernelMatrixblocks[m, n])
+        elseif code == 2
+            push!(out, K.BarycentricMatrix2Dblocks[m, n])
+        elseif code == 3
+            push!(out, K.Matrixblocks[m, n])
+        end
+    end
+    out
+end
+
+"""
+    DeviceKernelMatrix(K::KernelMatrix{T}, x, y, a, b, c, d; blocksize, rank, mode=:stored)
+
+A KernelMatrix the reference built from an arbitrary f (src/KernelMatrix.jl:
+47-116) on the device: the plan rebuilds the same tree from the points and
+intervals K was built from, forms U and V (which do not depend on f) on the
+device, and takes K's own dense leaves and far cores B.F. The leaf sizes are
+checked against the plan's, leaf for leaf.
+"""
+function DeviceKernelMatrix(K::KernelMatrix{T}, x::Vector{T}, y::Vector{T}, a::T, b::T, c::T,
+                            d::T; blocksize::Int = BLOCKSIZE(T), rank::Int = BLOCKRANK(T),
+                            mode::Symbol = :stored) where T<:Union{Float32,Float64}
+    mode === :free && throw(ArgumentError("a host function f has no matrix-free form"))
+    flags = mode === :hybrid ? HM_PLAN_HYBRID : Cuint(0)
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve x y begin
+        st = if T === Float64
+            ccall((:hm_plan_from_points, libhmx), Cint,
+                  (Cint, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64, Float64, Float64,
+                   Float64, Cint, Cint, Cuint, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                  HM_USER, x, length(x), y, length(y), a, b, c, d, blocksize, rank, flags,
+                  C_NULL, plan)
+        else
+            ccall((:hm_plan_from_points_f32, libhmx), Cint,
+                  (Cint, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Float32, Float32, Float32,
+                   Float32, Cint, Cint, Cuint, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                  HM_USER, x, length(x), y, length(y), a, b, c, d, blocksize, rank, flags,
+                  C_NULL, plan)
+        end
+        check(st)
+    end
+    H = DeviceKernelMatrix{T}(plan[], length(x), length(y))
+    ints, _ = plan_blocks(H)
+    L = kernel_leaves!(Any[], K)
+    length(L) == size(ints, 2) || throw(ArgumentError("K's tree differs from the plan's"))
+    A, F = T[], T[]
+    for (t, leaf) in enumerate(L)
+        size(leaf) == (ints[5, t], ints[6, t]) || throw(ArgumentError("leaf $t differs in size"))
+        (ints[5, t] == 0 || ints[6, t] == 0) && continue
+        if ints[1, t] == 3
+            append!(A, vec(Matrix{T}(leaf)))
+        else
+            size(leaf.B.F) == (rank, rank) || throw(ArgumentError("leaf $t: rank differs"))
+            append!(F, vec(leaf.B.F))
+        end
+    end
+    set_leaf_data!(H, A, F)
 end
 
 """

@@ -38,8 +38,14 @@ enum hm_status {
     HM_ERR_UNSUPPORTED = 6
 };
 
-/* kernels of examples/Kernel.jl:34-37 */
-enum hm_kernel { HM_CAUCHY = 0, HM_COULOMB = 1, HM_COULOMB3 = 2, HM_LOG = 3 };
+/* kernels of examples/Kernel.jl:34-37, and HM_USER: an arbitrary host
+ * function f (the reference's KernelMatrix(f, ...) takes any Julia f,
+ * src/KernelMatrix.jl:47). A plan of HM_USER builds the tree and the bases U, V
+ * (which do not depend on f) on the device; the caller evaluates f on the
+ * dense leaves and at the far blocks' node pairs and hands the values over
+ * with hm_plan_set_leaf_data. Such plans have no matrix-free form and no
+ * gradients (HM_ERR_UNSUPPORTED). */
+enum hm_kernel { HM_CAUCHY = 0, HM_COULOMB = 1, HM_COULOMB3 = 2, HM_LOG = 3, HM_USER = 4 };
 
 /* plan flags (hm_plan_from_points) */
 enum hm_plan_flags {
@@ -59,15 +65,21 @@ enum hm_mv_flags {
     HM_MV_CHEB_BASIS = 2,        /* apply the far-field basis in Chebyshev form even if
                                     the plan stores U and V */
     HM_MV_TUNED = 4,             /* use the launch shape and mode chosen by hm_plan_tune */
-    HM_MV_FAR_F32 = 8,           /* far field (Chebyshev form) on FP32 MFMA, FP32 accumulate;
-                                    points, differences, near field and F stay FP64 */
+    HM_MV_FAR_F32 = 8,           /* far field (Chebyshev form) on FP32 MFMA, FP32 accumulate
+                                    per step (64 points upward, one cluster downward) and
+                                    FP64 across steps; points, differences, near field,
+                                    reduce, F and translation stay FP64 */
     HM_MV_FAR_BF16X3 = 16,       /* far field on BF16 MFMA with operands split hi + lo
-                                    (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+                                    (hi*hi + hi*lo + lo*hi), FP32 accumulate: operands
+                                    carry 16 bits */
     HM_MV_MFMA = 32,             /* forward: 16 right-hand sides per launch on the FP64
                                     MFMA passes (far field in Chebyshev form); the
                                     default when k >= 8 */
-    HM_MV_NO_MFMA = 64           /* forward: never the MFMA passes (up to 8 right-hand
+    HM_MV_NO_MFMA = 64,          /* forward: never the MFMA passes (up to 8 right-hand
                                     sides per launch on the vector units) */
+    HM_MV_FAR_BF16X6 = 128       /* far field on BF16 MFMA with operands split in three,
+                                    hi + mid + lo (six products, down to mid*mid), FP32
+                                    accumulate: operands carry FP32's 24 bits */
 };
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -112,21 +124,49 @@ int hm_plan_from_points(int kernel_id, const double *x, int64_t nx, const double
                         double a, double b, double c, double d, int blocksize, int rank,
                         unsigned flags, void *stream, hm_plan_t *plan);
 
+/* The dense leaves and far-field cores of a plan built with kernel HM_USER
+ * (stored or hybrid form), in the order of hm_plan_blocks (the reference's
+ * traversal, src/KernelMatrix.jl:17-45), leaves with an empty side skipped:
+ * A holds every dense leaf's f(x[i], y[j]) (m x n column-major, one after the
+ * other, nA values in all), F every far leaf's F[m, n] = f(t_m, s_n) at its
+ * mapped Chebyshev nodes (src/BarycentricMatrix.jl:147-178; r x r column-major,
+ * nF = far leaves x r x r). Host arrays; synchronizes `stream`. */
+int hm_plan_set_leaf_data(hm_plan_t plan, const double *A, int64_t nA, const double *F,
+                          int64_t nF, void *stream);
+int hm_plan_set_leaf_data_f32(hm_plan_t plan, const float *A, int64_t nA, const float *F,
+                              int64_t nF, void *stream);
+
 /* A HierarchicalMatrix given by its leaves (src/HierarchicalMatrix.jl:1-52 with
- * LowRankMatrix and Matrix leaves, src/LowRankMatrix.jl:28-38): leaf t sits at
- * (row0[t], col0[t]) with size m[t] x n[t] in an nrows x ncols matrix, placed
- * where the reference's traversal offsets put it (src/HierarchicalMatrix.jl:45,47).
- * kind[t] = 3: dense, A[t] is m x n column-major. kind[t] = 2: low rank
- * U Diagonal(S) V' with U[t] m x rank[t], S[t] rank[t], V[t] n x rank[t]
- * (column-major). Leaves may be missing (unassigned blocks are zero). Host
- * pointers; the data is copied into the plan. Products: hm_matvec /
- * hm_matvec_t in stored form (the leaf loops of src/algebra.jl:37-48,110-131
- * and their transposes :52-65,137-159). Ranks up to 64. */
+ * LowRankMatrix and Matrix leaves, src/LowRankMatrix.jl:28-38), or the leaves
+ * of a KernelMatrix built by the reference from an arbitrary Julia f
+ * (src/KernelMatrix.jl:47-116, whose far leaves are BarycentricMatrix2D,
+ * src/BarycentricMatrix.jl:208-218): leaf t sits at (row0[t], col0[t]) with size
+ * m[t] x n[t] in an nrows x ncols matrix, placed where the reference's traversal
+ * offsets put it (src/HierarchicalMatrix.jl:45,47; src/KernelMatrix.jl:27-29).
+ *   kind[t] = 3: dense, A[t] is m x n column-major.
+ *   kind[t] = 2: low rank U Diagonal(S) V' with U[t] m x rank[t], S[t] rank[t],
+ *                V[t] n x rank[t] (column-major).
+ *   kind[t] = 4: barycentric U F V' (the leaf mul! of src/algebra.jl:243-277:
+ *                t1 = V'v, t2 = F t1, u += U t2) with U[t] m x rank[t],
+ *                V[t] n x rank[t] and the rank[t] x rank[t] core F passed in
+ *                S[t] (column-major, F[i + rank*j]).
+ * Leaves may be missing (unassigned blocks are zero). Host pointers; the data
+ * is copied into the plan. Products: hm_matvec / hm_matvec_t in stored form
+ * (the leaf loops of src/algebra.jl:37-48,110-131,243-277 and their
+ * transposes :52-65,137-159). Ranks up to 64. */
 int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
                         const int64_t *row0, const int64_t *col0, const int64_t *m,
                         const int64_t *n, const int64_t *rank, const double *const *A,
                         const double *const *U, const double *const *S,
                         const double *const *V, void *stream, hm_plan_t *plan);
+/* The same for T = Float32 (the reference's leaf loops are generic in T,
+ * src/algebra.jl:37-48,110-131): every leaf and every product in FP32;
+ * products through hm_matvec_f32 / hm_matvec_t_f32. Ranks up to 32. */
+int hm_plan_from_blocks_f32(int64_t nrows, int64_t ncols, int64_t nleaves, const int32_t *kind,
+                            const int64_t *row0, const int64_t *col0, const int64_t *m,
+                            const int64_t *n, const int64_t *rank, const float *const *A,
+                            const float *const *U, const float *const *S,
+                            const float *const *V, void *stream, hm_plan_t *plan);
 
 /* Diagonal scaling of a plan built from blocks, in place on the device
  * (src/HierarchicalMatrix.jl:15-16,54-102 with the leaf forms of
@@ -137,9 +177,22 @@ int hm_plan_from_blocks(int64_t nrows, int64_t ncols, int64_t nleaves, const int
  *   side = 1: H := Diagonal(b[off], ..., b[off + nrows - 1]) * H
  *             (lmul!, scale!(b, H, istart) with off = istart - 1): dense leaves
  *             A[i, j] *= b[off + row0 + i], LowRank leaves U[i, :] *= b[off + row0 + i].
- * b is a device vector. Only plans of hm_plan_from_blocks: the reference defines
- * no scale! for the KernelMatrix leaves (HM_ERR_UNSUPPORTED). */
+ * b is a device vector. Only Float64 plans of hm_plan_from_blocks: the reference
+ * defines no scale! for the KernelMatrix leaves (HM_ERR_UNSUPPORTED). A
+ * barycentric leaf (kind 4) scales its U or V rows as a low-rank leaf does. */
 int hm_plan_scale(hm_plan_t plan, int side, const double *b, int64_t off, void *stream);
+
+/* add_col!(H, u, istart, j) (src/HierarchicalMatrix.jl:104-146 with the leaf
+ * forms of src/algebra.jl:324-380) in place on the device, for a Float64 plan
+ * of hm_plan_from_blocks: column j (0-based) of H gains u[ioff + i], i < nrows
+ * (ioff = istart - 1). Dense leaves holding column j add their part of u to it
+ * (add_col!); low-rank and barycentric leaves take u e_j' as one more rank
+ * term (the reference's add_col: U1 = [U u], S1 = [S; 1], V1 = [V e_j]) in
+ * their next free column: a leaf whose rank already equals the plan's rank
+ * has none, and the call then changes nothing and returns HM_ERR_UNSUPPORTED
+ * (rebuild the plan from the updated leaves). u is a device vector;
+ * stream-ordered, allocates nothing. */
+int hm_plan_add_col(hm_plan_t plan, const double *u, int64_t ioff, int64_t j, void *stream);
 
 /* Many independent KernelMatrix instances in one plan (no counterpart in the
  * reference, which builds one tree per call of src/KernelMatrix.jl:47).
@@ -289,6 +342,29 @@ int hm_graph_matvec_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, 
                         unsigned flags, hm_graph_t *graph);
 int hm_entries_f32(hm_plan_t plan, int64_t n, const int64_t *i, const int64_t *j, float *out,
                    void *stream);
+
+/* ------------------------------------------------ barycentric pieces
+ * BarycentricPoly2D and BarycentricMatrix2D (src/BarycentricMatrix.jl:139-297)
+ * outside a KernelMatrix plan. All arrays are DEVICE arrays; stream-ordered.
+ * The reference's operation order with IEEE division. */
+
+/* out[t] = evaluate(B, px[t], py[t]) for t < n (src/BarycentricMatrix.jl:180-205):
+ * the 2-D barycentric interpolant of F (r x r column-major, F[m + r*n] =
+ * f(tx_m, ty_n)) on the mapped nodes tx, ty with weights lamx, lamy. */
+int hm_bary_evaluate(int r, const double *tx, const double *ty, const double *lamx,
+                     const double *lamy, const double *F, int64_t n, const double *px,
+                     const double *py, double *out, void *stream);
+int hm_bary_evaluate_f32(int r, const float *tx, const float *ty, const float *lamx,
+                         const float *lamy, const float *F, int64_t n, const float *px,
+                         const float *py, float *out, void *stream);
+
+/* The update! basis of one side of a BarycentricMatrix2D
+ * (src/BarycentricMatrix.jl:248-297): B[i + ldb*m] = (lam_m / (pts_i - nodes_m))
+ * / sum_k lam_k / (pts_i - nodes_k) for i < n, m < r (r <= 64). */
+int hm_bary_basis(int r, const double *nodes, const double *lam, int64_t n, const double *pts,
+                  double *B, int64_t ldb, void *stream);
+int hm_bary_basis_f32(int r, const float *nodes, const float *lam, int64_t n, const float *pts,
+                      float *B, int64_t ldb, void *stream);
 
 /* ------------------------------------------------ EvenBarycentricMatrix
  * EvenBarycentricMatrix(T, f, a, b, c, d) (src/BarycentricMatrix.jl:5-46) is an

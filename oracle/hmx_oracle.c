@@ -497,6 +497,23 @@ void or_bary_mul(or_real *u, const or_real *U, const or_real *F, const or_real *
     free(t2);
 }
 
+/* evaluate(B::BarycentricPoly2D, x, y), src/BarycentricMatrix.jl:180-205:
+ * tx, ty the mapped nodes, lx, ly the weights, F r x r column-major. */
+or_real or_bary_evaluate(int r, const or_real *tx, const or_real *ty, const or_real *lx,
+                         const or_real *ly, const or_real *F, or_real x, or_real y) {
+    or_real temp1 = R(0), temp2 = R(0), temp3, ret = R(0);
+    for (int p = 0; p < r; ++p) temp1 += lx[p] * (R(1) / (x - tx[p]));
+    for (int q = 0; q < r; ++q) temp2 += ly[q] * (R(1) / (y - ty[q]));
+    for (int n = 0; n < r; ++n) {
+        temp3 = R(0);
+        for (int m = 0; m < r; ++m) temp3 += lx[m] * F[m + r * n] * (R(1) / (x - tx[m]));
+        temp3 *= R(1) / temp1;
+        ret += ly[n] * temp3 * (R(1) / (y - ty[n]));
+    }
+    ret *= R(1) / temp2;
+    return ret;
+}
+
 /* --------------------------------------------------------------- matvecs */
 
 int or_tree_matvec(or_tree *t, or_real *u, const or_real *v) {

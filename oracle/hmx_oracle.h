@@ -115,6 +115,12 @@ void or_lowrank_adjmul(or_real *y, const or_real *U, const or_real *S, const or_
 void or_bary_mul(or_real *u, const or_real *U, const or_real *F, const or_real *V, int64_t m,
                  int64_t n, int64_t r, const or_real *v, int64_t istart, int64_t jstart);
 
+/* evaluate(B::BarycentricPoly2D, x, y), src/BarycentricMatrix.jl:180-205: the
+ * 2-D barycentric interpolant of F (r x r column-major, F[m + r*n] =
+ * f(tx_m, ty_n)) on the mapped nodes tx, ty with weights lx, ly. */
+or_real or_bary_evaluate(int r, const or_real *tx, const or_real *ty, const or_real *lx,
+                         const or_real *ly, const or_real *F, or_real x, or_real y);
+
 /* Dense oracle: out[k] = sum_j f(x[rows[k]], y[j]) * v[j] accumulated in long
  * or_real (rows==NULL means all rows 0..nrows-1). OpenMP over rows. */
 void or_dense_matvec_rows(int kernel_id, const or_real *x, const or_real *y, int64_t ny,

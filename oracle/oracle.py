@@ -71,6 +71,8 @@ def _setup(L, _dp, real):
                                      ctypes.c_int64, ctypes.c_int64]
     L.or_bary_mul.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
                               ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64]
+    L.or_bary_evaluate.restype = real
+    L.or_bary_evaluate.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, real, real]
     L.or_dense_matvec_rows.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp, _ip,
                                        ctypes.c_int64, _dp, ctypes.c_int]
     L.or_even_nodes.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
@@ -281,6 +283,19 @@ def bary_mul(u, U, F, V, v, istart, jstart):
                       V.ctypes.data_as(_dp), U.shape[0], V.shape[0], F.shape[0], _d(v),
                       istart, jstart)
     return u
+
+
+def bary_evaluate(tx, ty, lx, ly, F, x, y, f32=False):
+    """evaluate(B::BarycentricPoly2D, x, y) at each pair (x[t], y[t])."""
+    dt = _dt(f32)
+    tx, ty, lx, ly = (np.ascontiguousarray(a, dtype=dt) for a in (tx, ty, lx, ly))
+    F = np.asfortranarray(F, dtype=dt)
+    fn = lib(f32).or_bary_evaluate
+    xs = np.atleast_1d(np.asarray(x, dtype=dt))
+    ys = np.atleast_1d(np.asarray(y, dtype=dt))
+    out = np.array([fn(len(tx), _d(tx), _d(ty), _d(lx), _d(ly), _d(F), float(a), float(b))
+                    for a, b in zip(xs, ys)], dtype=dt)
+    return out
 
 
 # ---------------------------------------------------- EvenBarycentricMatrix

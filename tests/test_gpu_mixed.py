@@ -1,8 +1,10 @@
-"""Reduced-precision far field (HM_MV_FAR_F32, HM_MV_FAR_BF16X3): far-field
-products on MFMA with FP32 accumulation, FP64 for point differences, the
-Chebyshev recurrence, the near field and F. Stated bounds against the FP64
-product: FP32 <= 1e-6 relative 2-norm, split BF16 (hi*hi + hi*lo + lo*hi,
-about 16 significant bits per operand) <= 1e-4."""
+"""Reduced-precision far field (HM_MV_FAR_F32, HM_MV_FAR_BF16X3,
+HM_MV_FAR_BF16X6): far-field products on MFMA with FP32 accumulation per
+step and FP64 across steps, FP64 for point differences, the Chebyshev
+recurrence, the near field and F. Stated bound against the FP64 product and
+the oracle: <= 1e-6 relative 2-norm for every form on these Cauchy matrices
+(the near field, exact in FP64, carries most of each row; the log kernel's
+far-field-dominated bounds are in test_gpu_fullsize.py)."""
 import numpy as np
 import pytest
 import torch
@@ -11,14 +13,14 @@ import hmx
 import oracle as O
 
 pytestmark = pytest.mark.gpu
-BOUND = {"f32": 1e-6, "bf16x3": 1e-4}
+BOUND = {"f32": 1e-6, "bf16x3": 1e-6, "bf16x6": 1e-6}
 
 
 def rel(a, b):
     return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
 @pytest.mark.parametrize("mode", ["stored", "hybrid", "free"])
 @pytest.mark.parametrize("k", [1, 7, 16, 32])
 def test_mixed_vs_fp64(cuda, prec, mode, k):
@@ -33,7 +35,7 @@ def test_mixed_vs_fp64(cuda, prec, mode, k):
     assert e <= BOUND[prec]
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3", "bf16x6"])
 @pytest.mark.parametrize("rank,bs", [(16, 64), (24, 96), (32, 128)])
 def test_mixed_ranks_batched_vs_oracle(cuda, prec, rank, bs):
     w = hmx.workloads.config_batched(count=6, nmin_log2=11, nmax_log2=13, k=16, rank=rank,

@@ -148,37 +148,76 @@ def test_scale_and_add_col_host_identities():
         assert sum(after) >= sum(ranks)
 
 
+def ranked_tree(rng, free):
+    """2x2 of 2x2; LowRank leaves of ranks 2-5, and (free=True) one leaf of
+    rank 8 so that every other leaf has free rank columns in the plan."""
+    def leaf(mm, nn, r):
+        return LowRankMatrix(rng.standard_normal((mm, r)), rng.standard_normal(r),
+                             rng.standard_normal((nn, r)))
+    rows, cols = [37, 29, 33, 41], [31, 40, 26, 38]
+    top = HierarchicalMatrix(2, 2)
+    for m in range(2):
+        for n in range(2):
+            H = HierarchicalMatrix(2, 2)
+            for a in range(2):
+                for b in range(2):
+                    mm, nn = rows[2 * m + a], cols[2 * n + b]
+                    if (m, n, a, b) == (0, 1, 0, 0):
+                        H[Block(a + 1), Block(b + 1)] = leaf(mm, nn, 8 if free else 5)
+                    elif (a + b) % 2:
+                        H[Block(a + 1), Block(b + 1)] = leaf(mm, nn, int(rng.integers(2, 6)))
+                    else:
+                        H[Block(a + 1), Block(b + 1)] = rng.standard_normal((mm, nn))
+            top[Block(m + 1), Block(n + 1)] = H
+    return top
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("seed", [1, 2])
-def test_gpu_scale_in_place_and_add_col(cuda, seed):
-    """hm_plan_scale on the device: (H D_b) x = H (b .* x) and (D_b H) x =
-    b .* (H x); the scaled plan equals a plan rebuilt from the scaled host
-    leaves bit for bit; after add_col! the product gains u x_j."""
+@pytest.mark.parametrize("free", [True, False])
+def test_gpu_scale_and_add_col_vs_oracle(cuda, seed, free):
+    """rmul!/lmul! (hm_plan_scale) and add_col! (hm_plan_add_col) applied in
+    place to the device plan, against the oracle's restatement of scale! and
+    add_col! (oracle/hierarchical.py, pinned by dense assembly) and its
+    traversal mul!: products within 1e-13. With free rank columns add_col!
+    updates the plan in place; without (free=False: every LowRank leaf of the
+    column at the plan's rank), the plan is rebuilt and the product agrees
+    all the same."""
     import torch
+    import hierarchical as OH
+    from hmx._lib import lib
     rng = np.random.default_rng(seed)
-    H = consistent_tree(rng)
+    H = ranked_tree(rng, free)
+    T = OH.from_tree(H)
     rows, cols = H.size()
-    x = torch.from_numpy(rng.standard_normal(cols)).to(cuda)
-    b = torch.from_numpy(rng.standard_normal(cols)).to(cuda)
-    c = torch.from_numpy(rng.standard_normal(rows)).to(cuda)
-    want_r = H.matvec(b * x)                      # plan built here
-    hmx.rmul_(H, b)                               # host leaves and device plan scaled
-    got_r = H.matvec(x)
-    assert torch.linalg.norm(got_r - want_r) <= 1e-14 * torch.linalg.norm(want_r)
-    want_l = c * H.matvec(x)
+    x = rng.standard_normal(cols)
+    xd = torch.from_numpy(x).to(cuda)
+
+    def check_product(T):
+        want = OH.mul(T, np.zeros(rows), x, 1, 1)
+        got = H.matvec(xd).cpu().numpy()
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
+    check_product(T)                      # builds the plan
+    plan = H._plan
+    b = rng.standard_normal(cols)
+    hmx.rmul_(H, torch.from_numpy(b).to(cuda))
+    T = OH.scale_right(T, b, 1)
+    assert H._plan is plan                # scaled in place
+    check_product(T)
+    c = rng.standard_normal(rows)
     hmx.lmul_(c, H)
-    got_l = H.matvec(x)
-    assert torch.linalg.norm(got_l - want_l) <= 1e-14 * torch.linalg.norm(want_l)
-    # a fresh plan from the scaled host leaves computes the same bits
-    H._drop_plans()
-    assert torch.equal(H.matvec(x), got_l)
-    u = rng.standard_normal(rows)
-    j = int(rng.integers(1, cols + 1))
-    before = H.matvec(x)
-    hmx.add_col_(H, u, j)
-    after = H.matvec(x)
-    want = before + torch.from_numpy(u).to(cuda) * x[j - 1]
-    assert torch.linalg.norm(after - want) <= 1e-13 * torch.linalg.norm(want)
+    T = OH.scale_left(c, T, 1)
+    check_product(T)
+    for j in (3, cols // 2 + 1, cols - 2):
+        u = rng.standard_normal(rows + 3)
+        hmx.add_col_(H, u, j, istart=2)
+        T = OH.add_col(T, u, 2, j)
+        if free:
+            assert H._plan is plan        # updated in place on the device
+        check_product(T)
+    # a NULL vector is refused
+    assert lib.hm_plan_add_col(H.plan()._h, None, 0, 0, None) != 0
 
 
 @pytest.mark.gpu
