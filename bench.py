@@ -33,12 +33,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-n-log2", type=int, default=18)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU leg threads (default: the affinity set, capped by the cgroup quota)")
+    ap.add_argument("--cpu-n-log2", type=int, default=None,
+                    help="CPU leg instance size (default: n=2^20 when 24 GB of RAM are free)")
     ap.add_argument("--variants", default=None,
                     help="headline: extra variants timed beside the stored product (default "
-                         "hybrid,matrix_free,f32_stored,f32_hybrid); batched: plan modes to time "
-                         "(default stored,hybrid,matrix_free)")
+                         "hybrid,matrix_free,f32_hybrid; f32_stored on request); batched: plan "
+                         "modes to time (default stored,hybrid,matrix_free)")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="headline: skip the batched / gradient / scale summaries")
     ap.add_argument("--no-peaks", action="store_true", help="skip the torch bandwidth probes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run torch copies of 2^27 doubles (for PMC byte calibration)")
@@ -51,7 +55,7 @@ def parse():
     args = ap.parse_args()
     args.variants_given = args.variants is not None
     if args.variants is None:
-        args.variants = "hybrid,matrix_free,f32_stored,f32_hybrid"
+        args.variants = "hybrid,matrix_free,f32_hybrid"
     return args
 
 
@@ -117,19 +121,67 @@ def stream_peaks(stream):
             "read_gbs": round(8 * n / (ms_sum * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(threads, n_log2, min_seconds=10.0):
+def host_facts():
+    """The host the CPU leg runs on: CPU model, logical CPUs, the affinity set,
+    the cgroup CPU quota (cores), total and available RAM."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    mem = {}
+    try:
+        for line in open("/proc/meminfo"):
+            k, v = line.split(":", 1)
+            if k in ("MemTotal", "MemAvailable"):
+                mem[k] = int(v.split()[0]) * 1024
+    except OSError:
+        pass
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "cgroup_cpu_quota": quota, "ram_total_gb": round(mem.get("MemTotal", 0) / 1e9, 1),
+            "ram_available_gb": round(mem.get("MemAvailable", 0) / 1e9, 1)}
+
+
+def cpu_baseline(threads=0, n_log2=None, min_seconds=10.0):
     """Oracle traversal mul! (ref layout, bit-identical to the single-thread
-    restatement) on the n = 2^n_log2 instance of the same family; matvecs/s
-    scaled to n = 2^20 by the ratio of reference-layout bytes. The OpenMP
-    product is repeated until min_seconds of timed work and the median run
-    is reported; the single-thread figure is the median of 3 runs."""
+    restatement) on the n = 2^20 headline instance itself when the host has
+    room for its 16.75 GB of reference-layout factors (24 GB available),
+    else on the largest 2^k instance of the same family that fits, scaled to
+    n = 2^20 by the ratio of reference-layout bytes. Threads: every CPU of the
+    affinity set, capped by the cgroup CPU quota when there is one. The OpenMP
+    product is repeated until min_seconds of timed work and the median run is
+    reported; the single-thread figure is the median of 3 runs. Tree
+    construction is timed apart."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import hmx
     import oracle as O
+    facts = host_facts()
+    if not threads:
+        threads = facts["affinity_cpus"]
+        if facts["cgroup_cpu_quota"]:
+            threads = max(1, min(threads, int(facts["cgroup_cpu_quota"])))
+    if n_log2 is None:
+        avail = facts["ram_available_gb"]
+        n_log2 = 20
+        while n_log2 > 14 and avail < 24.0 / (1 << (20 - n_log2)):
+            n_log2 -= 1
     n = 1 << n_log2
     w = hmx.workloads.config_plain(n=n, blocksize=BS, rank=RANK)
+    t0 = time.perf_counter()
     t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], BS, RANK, store=True)
+    build = time.perf_counter() - t0
     st = t.stats()
     ref_bytes = lambda s, nn: 8 * (s["dense_entries"] + RANK * (s["far_rows"] + s["far_cols"])
                                    + s["far"] * RANK * RANK) + 16 * nn
@@ -148,18 +200,23 @@ def cpu_baseline(threads, n_log2, min_seconds=10.0):
         runs1.append(time.perf_counter() - t0)
     sec1 = float(np.median(runs1))
     del t
-    big = hmx.KernelMatrix("cauchykernel", *hmx.workloads.shifted_grid(N)[:2],
-                           *hmx.workloads.shifted_grid(N)[2], blocksize=BS, rank=RANK,
-                           tree_only=True).info
-    b_full = big["bytes_ref_layout"]
+    if n_log2 == 20:
+        b_full = b_sample
+    else:
+        b_full = hmx.KernelMatrix("cauchykernel", *hmx.workloads.shifted_grid(N)[:2],
+                                  *hmx.workloads.shifted_grid(N)[2], blocksize=BS, rank=RANK,
+                                  tree_only=True).info["bytes_ref_layout"]
     scale = b_sample / b_full
+    how = ("the n=2^20 instance itself" if n_log2 == 20 else
+           f"the n=2^{n_log2} instance of the same family, scaled to n=2^20 by ref-layout bytes "
+           f"{b_sample}/{b_full}")
     return dict(value=(1.0 / sec) * scale, unit="matvecs/s", cores=threads, kind="port",
-                sample=(f"oracle traversal mul! (OpenMP, {threads} threads, ref layout) on the "
-                        f"n=2^{n_log2} instance of the same family (leaf 256, rank 24): median "
-                        f"{sec * 1e3:.1f} ms of {len(runs)} runs ({sum(runs):.1f} s timed); "
-                        f"single thread median {sec1 * 1e3:.0f} ms of 3; scaled to n=2^20 by ref-layout bytes "
-                        f"{b_sample}/{b_full}"),
-                single_thread_value=(1.0 / sec1) * scale)
+                sample=(f"oracle traversal mul! (OpenMP, {threads} threads, ref layout, "
+                        f"{b_sample / 1e9:.2f} GB of factors) on {how} (leaf 256, rank 24): "
+                        f"median {sec * 1e3:.1f} ms of {len(runs)} runs ({sum(runs):.1f} s "
+                        f"timed); single thread median {sec1 * 1e3:.0f} ms of 3; tree and "
+                        f"factors built in {build:.1f} s"),
+                single_thread_value=(1.0 / sec1) * scale, n_log2=n_log2, host=facts)
 
 
 def dense_rows(f, xr, y, B, chunk=1 << 16):
@@ -175,6 +232,143 @@ def timed(fn, steps, warmup, stream):
     for _ in range(warmup):
         fn()
     return time_steps(fn, steps, stream)
+
+
+def rel_err(a, b):
+    import torch
+    return float(torch.linalg.norm(a.double() - b.double()) / torch.linalg.norm(b.double()))
+
+
+def summary_batched(steps, warmup, stream):
+    """Config 2 in the headline run: 256 Cauchy instances (n log-uniform in
+    [2^10, 2^16]), rank 16, leaf 64, 32 right-hand sides, hybrid plan; the
+    plain FP64 MFMA product against FP32, split-BF16 hi+lo and hi+mid+lo far
+    fields (ms, batched products/s, speed-up and error against the FP64 GPU
+    product, error of instance 0 against its dense FP64 product)."""
+    import numpy as np
+    import torch
+    import hmx
+    dev = torch.device("cuda:0")
+    w = hmx.workloads.config_batched()
+    inst = w["instances"]
+    t0 = time.perf_counter()
+    H = hmx.KernelMatrixBatch("cauchykernel", [(i["x"], i["y"], i["intervals"]) for i in inst],
+                              blocksize=64, rank=16, hybrid=True)
+    torch.cuda.synchronize()
+    build = time.perf_counter() - t0
+    B = torch.from_numpy(np.concatenate([i["b"] for i in inst])).to(dev)
+    Y = torch.empty(H.shape[0], 32, dtype=torch.float64, device=dev)
+    n0 = len(inst[0]["x"])
+    d0 = dense_rows(lambda d: 1.0 / d, torch.from_numpy(inst[0]["x"]).to(dev),
+                    torch.from_numpy(inst[0]["y"]).to(dev), B[:len(inst[0]["y"])])
+    out = {"plan": "hybrid", "instances": len(inst), "points_total": int(H.shape[0]), "rhs": 32,
+           "rank": 16, "leaf_blocksize": 64, "build_s": round(build, 3)}
+    ref = None
+    for prec in ("f64", "f32", "bf16x3", "bf16x6"):
+        ms = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, False, False, prec),
+                   steps, warmup, stream)
+        if ref is None:
+            ref, ms64 = Y.clone(), ms
+        out[prec] = {"ms_per_step": round(ms, 4), "value": round(1000.0 / ms, 3),
+                     "unit": "batched products/s",
+                     "speedup_vs_fp64": round(ms64 / ms, 3),
+                     "rel_err_vs_fp64_gpu": rel_err(Y, ref),
+                     "rel_err_instance0_vs_dense_fp64": rel_err(Y[:n0], d0)}
+    H.close()
+    return out
+
+
+def summary_gradient(steps, warmup, stream):
+    """Config 3 in the headline run: L = 1/2 ||H(x, y) b - t||^2, n = 2^16,
+    leaf 64, rank 16; one step = forward product + pullback to x, y and b
+    (plans reused and refitted on the device from step to step), Float64 and
+    Float32, against the plain FP64 forward product of the same matrix-free
+    plan; errors against the dense FP64 gradient on 256 sampled entries."""
+    import numpy as np
+    import torch
+    import hmx
+    from hmx.autodiff import kernel_matvec
+    dev = torch.device("cuda:0")
+    w = hmx.workloads.config_gradient()
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    n = len(w["x"])
+    H = hmx.KernelMatrix("cauchykernel", w["x"], w["y"], *w["intervals"], blocksize=64, rank=16,
+                         matrix_free=True)
+    b64 = T(w["b"])
+    u = torch.empty_like(b64)
+    ms_plain = timed(lambda: H._mv(u, b64, 1, 1, 1, 1, 0, False, False, stream), steps, warmup,
+                     stream)
+    H.close()
+    out = {"n": n, "rank": 16, "leaf_blocksize": 64,
+           "plain_fp64_forward_ms": round(ms_plain, 4)}
+    idx = torch.arange(0, n, n // 256, device=dev)
+    for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        x, y, b, t = (T(w[k]).to(dtype) for k in ("x", "y", "b", "t"))
+        x.requires_grad_(True)
+        y.requires_grad_(True)
+        b.requires_grad_(True)
+
+        def step():
+            x.grad = y.grad = b.grad = None
+            uu = kernel_matvec("cauchykernel", x, y, b, w["intervals"], 64, 16)
+            L = 0.5 * torch.sum((uu - t) ** 2)
+            L.backward()
+
+        ms = timed(step, steps, warmup, stream)
+        with torch.no_grad():
+            xd, yd, bd, td = (v.detach().double() for v in (x, y, b, t))
+            Kb = dense_rows(lambda d: 1.0 / d, xd, yd, bd[:, None])[:, 0]
+            r = Kb - td
+            gx = r[idx] * dense_rows(lambda d: -1.0 / (d * d), xd[idx], yd, bd[:, None])[:, 0]
+            gb = dense_rows(lambda d: 1.0 / d, -yd[idx], -xd, r[:, None])[:, 0]
+            gy = bd[idx] * dense_rows(lambda d: 1.0 / (d * d), -yd[idx], -xd, r[:, None])[:, 0]
+        step()
+        out[name] = {"ms_per_step": round(ms, 4), "value": round(1000.0 / ms, 3),
+                     "unit": "gradient steps/s",
+                     "step_vs_plain_forward": round(ms / ms_plain, 3),
+                     "rel_err_vs_dense_fp64": {"grad_x": rel_err(x.grad[idx], gx),
+                                               "grad_y": rel_err(y.grad[idx], gy),
+                                               "grad_b": rel_err(b.grad[idx], gb)}}
+    return out
+
+
+def summary_scale(steps, warmup, stream):
+    """Config 4 in the headline run: log|x - y|, n = 2^22 sorted uniform points
+    (seed 1), y = x + 0.5/n, leaf 512, rank 32, 8 right-hand sides, hybrid
+    plan: FP64 MFMA against FP32, split-BF16 hi+mid+lo and hi+lo far fields;
+    errors against 4096 dense FP64 rows (torch on the GPU)."""
+    import numpy as np
+    import torch
+    import hmx
+    dev = torch.device("cuda:0")
+    w = hmx.workloads.config_scale()
+    B = torch.from_numpy(w["b"]).to(dev)
+    n = len(w["x"])
+    rows = torch.from_numpy(np.sort(np.random.default_rng(5).choice(n, 4096, replace=False))).to(dev)
+    dense = dense_rows(lambda d: torch.log(torch.abs(d)), torch.from_numpy(w["x"]).to(dev)[rows],
+                       torch.from_numpy(w["y"]).to(dev), B)
+    t0 = time.perf_counter()
+    H = hmx.KernelMatrix("logkernel", w["x"], w["y"], *w["intervals"], blocksize=512, rank=32,
+                         hybrid=True)
+    torch.cuda.synchronize()
+    build = time.perf_counter() - t0
+    Y = torch.empty(n, 8, dtype=torch.float64, device=dev)
+    out = {"plan": "hybrid", "n": n, "rank": 32, "leaf_blocksize": 512, "rhs": 8,
+           "build_s": round(build, 3), "hbm_bytes_stored": H.info["bytes_stored"]}
+    ref = None
+    for prec in ("f64", "f32", "bf16x6", "bf16x3"):
+        ms = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, False, False, prec),
+                   steps, warmup, stream)
+        if ref is None:
+            ref, ms64 = Y.clone(), ms
+        out[prec] = {"ms_per_step": round(ms, 4), "value": round(1000.0 / ms, 3),
+                     "unit": "products/s (8 RHS)", "speedup_vs_fp64": round(ms64 / ms, 3),
+                     "rel_err_vs_fp64_gpu": rel_err(Y, ref),
+                     "rel_err_vs_dense_fp64_4096rows": rel_err(Y[rows], dense)}
+    H.close()
+    del Y, ref
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_batched(args):
@@ -228,7 +422,7 @@ def run_batched(args):
                                                   "rel_diff_vs_mfma": float(
                                                       torch.linalg.norm(Yv - Y) / torch.linalg.norm(Y))}}
         del Yv
-        for prec in ("f32", "bf16x3"):
+        for prec in ("f32", "bf16x3", "bf16x6"):
             Ym = torch.empty_like(Y)
             ms_m = timed(lambda: H._mv(Ym, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
                                        prec), args.steps, args.warmup, stream)
@@ -351,7 +545,7 @@ def run_scale(args):
             "ms_per_step": round(ms_m, 4), "value": round(1000 / ms_m, 3),
             "rel_err_vs_dense_4096rows":
                 float(torch.linalg.norm(Y[rows] - dense) / torch.linalg.norm(dense))}
-        for prec in ("f32", "bf16x3"):
+        for prec in ("f32", "bf16x3", "bf16x6"):
             ms_m = timed(lambda: H._mv(Y, B, 1, 1, 1, 1, 0, False, False, stream, False, False,
                                        prec), args.steps, args.warmup, stream)
             out["variants"][name][f"far_{prec}"] = {
@@ -747,11 +941,16 @@ def main():
                 up_pass=dict(achieved=round(bytes_up / (passes[0] * 1e-3) / 1e9, 1),
                              avg_ms=round(float(passes[0]), 4)))
 
+    workloads = None
+    if not args.no_workloads:
+        wsteps, wwarm = max(3, min(args.steps, 10)), max(1, min(args.warmup, 2))
+        workloads = {"batched": summary_batched(wsteps, wwarm, stream),
+                     "gradient": summary_gradient(wsteps, wwarm, stream),
+                     "scale": summary_scale(max(2, wsteps // 2), 1, stream)}
     peaks = None if args.no_peaks else stream_peaks(stream)
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        cpu = cpu_baseline(threads, args.cpu_n_log2)
+        cpu = cpu_baseline(args.cpu_threads, args.cpu_n_log2)
 
     out = {
         "metric": "Cauchy H-matvecs/s + HBM GB/s, n=2^20 fp64",
@@ -775,13 +974,18 @@ def main():
         "pass_ms": {k: round(float(v), 4) for k, v in zip(names, passes)},
         "rel_err_vs_dense_64rows": err_dense,
         "build_s": round(build_s, 3),
-        "variants": {k: {"value": round(1000.0 / v["ms"], 3), "unit": "matvecs/s",
+        "variants": {k: {"value": (round(1000.0 / v["ms"], 3) if not v["nonfinite"] else None),
+                         "invalid": (f"{v['nonfinite']} non-finite factors: Float32 nodes "
+                                     f"coincide with points (the reference's Float32 update! "
+                                     f"divides by zero there)" if v["nonfinite"] else None),
+                         "unit": "matvecs/s",
                          "ms_per_step": round(v["ms"], 4),
                          "hbm_bytes_per_matvec": v["bytes"],
                          "pass_ms": {q: round(float(t), 4) for q, t in zip(names, v["passes"])},
                          "rel_diff_vs_stored": v["err"],
                          "nonfinite_factors": v["nonfinite"]} for k, v in variants.items()},
         "roofline": roof,
+        "workloads": workloads,
         "measured_hbm_peaks": peaks,
         "cpu_baseline": cpu,
     }

@@ -30,9 +30,9 @@ struct DevPlanT {
     int32_t nfar;
     const int32_t *far_by_rc;  // far ids grouped by row cluster (CSR in DCluster.far_*)
     const int32_t *far_by_cc;  // far ids grouped by col cluster
-    // the far records themselves in far_by_rc / far_by_cc order, so a pass
+    // the translation records in far_by_rc / far_by_cc order, so a pass
     // walking a cluster's list reads each record with one load
-    const DFarBlk *far_rc, *far_cc;
+    const DTrRec *tr_rc, *tr_cc;
     const DDense *dense;
     int32_t ndense;
     const DChunk *rch, *cch;   // row chunks, col chunks
@@ -43,6 +43,10 @@ struct DevPlanT {
     int32_t nrtask, nctask;
     int64_t nslots_r, nslots_c;
     T *A, *U, *V, *F;          // stored factors (null when matrix-free)
+    // the row / column clusters with more than one upward task (the others'
+    // tasks write w themselves)
+    const int32_t *multi_r, *multi_c;
+    int32_t nmulti_r, nmulti_c;
 };
 
 template <class T>
@@ -80,13 +84,13 @@ hipError_t hmx_launch_compress(const DevPlan &p, hipStream_t s);
 // kb in {1, 2, 4, 8}: right-hand sides handled by this launch (forward only).
 hipError_t hmx_launch_matvec(const DevPlan &p, const MvArgs &a, bool transpose, int mode, int kb,
                              hipStream_t s, const hipEvent_t *ev = nullptr);
-// forward product of up to 16 right-hand sides (nq) on the MFMA units, far
+// forward product of up to 32 right-hand sides (nq; two groups of 16 when
+// nq > 16) on the MFMA units, far
 // field in Chebyshev form: prec 2 FP64, 0 FP32, 1 split BF16 for the far-field
 // products (near field, reduce and translate FP64); mode 0/1 use the stored
-// dense blocks, 2 regenerates them. multi[0..nmulti) lists the column
-// clusters with more than one upward task.
+// dense blocks, 2 regenerates them.
 hipError_t hmx_launch_matvec_m(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                               const int32_t *multi, int nmulti, hipStream_t s);
+                               hipStream_t s);
 // gradients (matrix-free near field, Chebyshev-form far field):
 //   forward (tr = false): y = beta*y + s .* (d/dx_i of (H x)_i)
 //   transpose (tr = true): y = beta*y + H^T x, y2 = beta*y2 + s .* (d/dy_j of (H^T x)_j)
@@ -103,6 +107,17 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
 // from blocks): side 0 scales columns (A and V rows), 1 rows (A and U rows);
 // b is indexed by the global column (side 0) or row (side 1)
 hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s);
+
+// hm_plan_update_points: *status = 0 when the points x (nx) and y (ny) are
+// finite, non-increasing and split as the tree's recorded splits do, else 1
+// (the status word is cleared by a kernel first, so the launch is capturable);
+// refit re-forms the dense blocks and the stored bases from the plan's points
+template <class T>
+hipError_t hmx_launch_check_points(const T *x, int64_t nx, const DSplit *xs, int64_t nxs,
+                                   const T *y, int64_t ny, const DSplit *ys, int64_t nys,
+                                   int32_t *status, hipStream_t s);
+hipError_t hmx_launch_refit(const DevPlanT<double> &p, hipStream_t s);
+hipError_t hmx_launch_refit(const DevPlanT<float> &p, hipStream_t s);
 
 // add_col! on a plan built from blocks: per job (one leaf holding the column),
 // a dense leaf adds u[row0 + i] to its column jl; a low-rank leaf writes

@@ -26,7 +26,7 @@
 #define HM_CHUNK 64          // rows per chunk = wavefront width
 #define HM_TASK_POINTS 8192  // points per upward-pass task
 #define HM_MAX_KB 8          // right-hand sides per forward launch (FP64 passes)
-#define HM_WS_RHS 16         // right-hand sides the workspace holds (mixed precision: 16)
+#define HM_WS_RHS 32         // right-hand sides the workspace holds (an MFMA launch: 32)
 
 // Stored matrices (dense blocks m x n, bases len x r) are kept in 64-row
 // tiles, column-major inside a tile: element (i, k) of a matrix with `cols`
@@ -92,4 +92,25 @@ struct DCluster {
 struct DFarBlk {
     int32_t rc, cc;  // row cluster, col cluster
     int64_t foff;    // offset of F
+};
+
+// A far block as the translation of one side reads it (the records of a
+// cluster's list in order): F's offset and the source cluster (column cluster
+// forward, row cluster transposed) with its task count and first partial-sum
+// slot, so a source with several upward tasks is summed from its slots.
+struct DTrRec {
+    int64_t foff;
+    int64_t pbase;
+    int32_t src;
+    int32_t nslots;
+};
+
+// One indsplit of the tree: the range [first, first + len) of one side split
+// at index `at` by the midpoint `mid` (points at or above it first). For other
+// points the split falls at the same index exactly when p[at - 1] >= mid (if
+// at > first) and p[at] < mid (if at < first + len); an empty range needs
+// p[first] < mid, as the reference reads the point after it.
+struct DSplit {
+    int64_t first, len, at;
+    double mid;
 };

@@ -14,6 +14,6 @@ hipError_t hmx_grad_r48(const DevPlan &p, const MvArgs &a, bool tr, double *y2, 
 hipError_t hmx_gen_basis_r48(const DevPlan &p, hipStream_t s) { return hmxk::gen_basis_r<double, 48>(p, s); }
 
 hipError_t hmx_matvec_m_r48(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                            const int32_t *multi, int nmulti, hipStream_t s) {
-    return hmxk::matvec_m_r<48>(p, a, mode, prec, nq, multi, nmulti, s);
+                            hipStream_t s) {
+    return hmxk::matvec_m_r<48>(p, a, mode, prec, nq, s);
 }

@@ -58,13 +58,13 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
 }
 
 hipError_t hmx_launch_matvec_m(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                               const int32_t *multi, int nmulti, hipStream_t s) {
-    if (p.r <= 8) return hmx_matvec_m_r8(p, a, mode, prec, nq, multi, nmulti, s);
-    if (p.r <= 16) return hmx_matvec_m_r16(p, a, mode, prec, nq, multi, nmulti, s);
-    if (p.r <= 24) return hmx_matvec_m_r24(p, a, mode, prec, nq, multi, nmulti, s);
-    if (p.r <= 32) return hmx_matvec_m_r32(p, a, mode, prec, nq, multi, nmulti, s);
-    if (p.r <= 48) return hmx_matvec_m_r48(p, a, mode, prec, nq, multi, nmulti, s);
-    return hmx_matvec_m_r64(p, a, mode, prec, nq, multi, nmulti, s);
+                               hipStream_t s) {
+    if (p.r <= 8) return hmx_matvec_m_r8(p, a, mode, prec, nq, s);
+    if (p.r <= 16) return hmx_matvec_m_r16(p, a, mode, prec, nq, s);
+    if (p.r <= 24) return hmx_matvec_m_r24(p, a, mode, prec, nq, s);
+    if (p.r <= 32) return hmx_matvec_m_r32(p, a, mode, prec, nq, s);
+    if (p.r <= 48) return hmx_matvec_m_r48(p, a, mode, prec, nq, s);
+    return hmx_matvec_m_r64(p, a, mode, prec, nq, s);
 }
 
 // ------------------------------------------------------------- FP32 plans
@@ -141,6 +141,30 @@ __global__ __launch_bounds__(hmxk::WG) void k_scale_basis(const DCluster *cl, do
     }
 }
 
+__global__ void k_zero_status(int32_t *status) { *status = 0; }
+
+// monotone and finite points (t < n - 1 compares t and t + 1), then the
+// splits; any failing thread stores 1
+template <class T>
+__global__ __launch_bounds__(hmxk::WG) void k_check_points(const T *p, int64_t n, const DSplit *sp,
+                                                          int64_t nsp, int32_t *status) {
+    const int64_t t = (int64_t)blockIdx.x * hmxk::WG + threadIdx.x;
+    bool bad = false;
+    if (t < n) {
+        const double v = (double)p[t];
+        bad = !isfinite(v) || (t + 1 < n && !(v >= (double)p[t + 1]));
+    } else if (t < n + nsp) {
+        const DSplit q = sp[t - n];
+        if (q.len == 0) {
+            bad = !(q.first < n && (double)p[q.first] < q.mid);
+        } else {
+            if (q.at > q.first) bad |= !((double)p[q.at - 1] >= q.mid);
+            if (q.at < q.first + q.len) bad |= !((double)p[q.at] < q.mid);
+        }
+    }
+    if (bad) *status = 1;
+}
+
 // one workgroup per job of hm_plan_add_col
 __global__ __launch_bounds__(hmxk::WG) void k_add_col(DevPlan p, AddColJobs J, const double *u) {
     const AddColJob q = J.job[blockIdx.x];
@@ -160,6 +184,45 @@ __global__ __launch_bounds__(hmxk::WG) void k_add_col(DevPlan p, AddColJobs J, c
 }
 
 }  // namespace
+
+template <class T>
+hipError_t hmx_launch_check_points(const T *x, int64_t nx, const DSplit *xs, int64_t nxs,
+                                   const T *y, int64_t ny, const DSplit *ys, int64_t nys,
+                                   int32_t *status, hipStream_t s) {
+    k_zero_status<<<1, 1, 0, s>>>(status);
+    const int64_t bx = (nx + nxs + hmxk::WG - 1) / hmxk::WG, by = (ny + nys + hmxk::WG - 1) / hmxk::WG;
+    if (bx > 0) k_check_points<T><<<(unsigned)bx, hmxk::WG, 0, s>>>(x, nx, xs, nxs, status);
+    if (by > 0) k_check_points<T><<<(unsigned)by, hmxk::WG, 0, s>>>(y, ny, ys, nys, status);
+    return hipGetLastError();
+}
+template hipError_t hmx_launch_check_points<double>(const double *, int64_t, const DSplit *,
+                                                    int64_t, const double *, int64_t,
+                                                    const DSplit *, int64_t, int32_t *, hipStream_t);
+template hipError_t hmx_launch_check_points<float>(const float *, int64_t, const DSplit *, int64_t,
+                                                   const float *, int64_t, const DSplit *, int64_t,
+                                                   int32_t *, hipStream_t);
+
+hipError_t hmx_launch_refit(const DevPlan &p, hipStream_t s) {
+    if (p.kid != HM_USER && p.ndense > 0 && p.A) hmxk::k_gen_dense<double><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || (!p.U && !p.V)) return err;
+    if (p.r <= 8) return hmx_gen_basis_r8(p, s);
+    if (p.r <= 16) return hmx_gen_basis_r16(p, s);
+    if (p.r <= 24) return hmx_gen_basis_r24(p, s);
+    if (p.r <= 32) return hmx_gen_basis_r32(p, s);
+    if (p.r <= 48) return hmx_gen_basis_r48(p, s);
+    return hmx_gen_basis_r64(p, s);
+}
+
+hipError_t hmx_launch_refit(const DevPlanF &p, hipStream_t s) {
+    if (p.kid != HM_USER && p.ndense > 0 && p.A) hmxk::k_gen_dense<float><<<p.ndense, hmxk::WG, 0, s>>>(p);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || (!p.U && !p.V)) return err;
+    if (p.r <= 8) return hmx_gen_basis_f32_r8(p, s);
+    if (p.r <= 16) return hmx_gen_basis_f32_r16(p, s);
+    if (p.r <= 32) return hmx_gen_basis_f32_r32(p, s);
+    return hipErrorInvalidValue;
+}
 
 hipError_t hmx_launch_add_col(const DevPlan &p, const AddColJobs &J, const double *u, hipStream_t s) {
     if (J.count > 0) k_add_col<<<J.count, hmxk::WG, 0, s>>>(p, J, u);

@@ -5,8 +5,10 @@
 //   up         per task (a column cluster and <= 8192 of its columns), one
 //              workgroup: w_task = B[cols, :]^T x[cols] reduced in registers,
 //              across the wavefront by recursive halving and across the four
-//              wavefronts in LDS; each task owns one partial slot (no atomics)
-//   reduce     per column cluster: w = sum of its task slots
+//              wavefronts in LDS; a cluster's only task writes w, with
+//              several each task writes its own partial slot (no atomics)
+//   reduce     per column cluster with several tasks: w = sum of its slots
+//              (folded into the translation of single right-hand sides)
 //   translate  per row cluster: z = sum over its far blocks of F w
 //   down       per row chunk (one wavefront, one lane per row): near field of
 //              the chunk's dense blocks plus U z of every row cluster covering
@@ -55,6 +57,17 @@ __device__ __forceinline__ int wave_id() {
     return uniform((int)(blockIdx.x * WPB + threadIdx.x / WAVE));
 }
 
+// wave-uniform copies of lane l's value (l uniform)
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ int64_t rl_l(int64_t v, int l) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+    return __longlong_as_double(rl_l(__double_as_longlong(v), l));
+}
+
 // Scalar helpers for the two element types (FP64 plans and FP32 plans).
 template <class T> struct V2Of;
 template <> struct V2Of<double> { using type = double2; };
@@ -67,6 +80,43 @@ __device__ __forceinline__ double logT(double v) { return log(v); }
 __device__ __forceinline__ float logT(float v) { return logf(v); }
 __device__ __forceinline__ double fabsT(double v) { return fabs(v); }
 __device__ __forceinline__ float fabsT(float v) { return fabsf(v); }
+
+// sum over n slots p[t * stride], t = 0, 1, ... in that order; the loads of
+// eight slots are issued before their additions
+template <class T>
+__device__ __forceinline__ T sum_slots(const T *p, int64_t stride, int n) {
+    T s = T(0);
+    int t = 0;
+    for (; t + 8 <= n; t += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(t + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; t < n; ++t) s += p[(int64_t)t * stride];
+    return s;
+}
+
+// v_m = sum_k A[m][k] mu_k, k ascending, where lane base + k holds mu_k: the
+// node values of one cluster from its moments (A's row loaded eight entries
+// at a time, so the loads do not wait on one another)
+template <class T>
+__device__ __forceinline__ T cheb_nodes(const T *chebA, T mu, int r, int m, int base) {
+    const int mm = m < r ? m : 0;
+    T v = T(0);
+    for (int k0 = 0; k0 < r; k0 += 8) {
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = chebA[mm * r + min(k0 + u, r - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T mk = __shfl(mu, (base + min(k0 + u, r - 1)) & (WAVE - 1));
+            if (k0 + u < r) v = fmaT(a[u], mk, v);
+        }
+    }
+    return v;
+}
 
 // Reciprocal: hardware estimate refined by Newton-Raphson steps
 // r <- r + r(1 - d r) (Markstein's FMA formulation), two in FP64, one in
@@ -248,12 +298,17 @@ __global__ __launch_bounds__(WG) void k_count_nonfinite(const T *v, int64_t n,
 // V (forward) or U (transpose) read from HBM, or, in Chebyshev form, the
 // moments sum_j T_m(xh_j) x_j with xh = (p - center) * inv_halfwidth. A lane
 // works on one (point, right-hand side) pair: lane = point*KB + q.
+// A cluster's only task writes its cluster's w itself (in Chebyshev form
+// w = A mu, the reduce's arithmetic in the reduce's order); the reduce then
+// runs over the clusters with several tasks only.
 template <class T, int RMAX, bool CHEB, int KB>
 __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *cl, const T *B,
                                           const V2<T> *cmapv, const T *pts, const T *x,
-                                          int64_t incx, int64_t ldx, T *partial, int r) {
+                                          int64_t incx, int64_t ldx, T *partial, T *w,
+                                          const T *chebA, int r) {
     constexpr int PP = WAVE / KB;  // points per wavefront step
     __shared__ T sh[WPB][KB][RMAX];
+    __shared__ T mus[KB][RMAX];
     const DTask Tk = tasks[blockIdx.x];
     const DCluster C = cl[Tk.cl];
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
@@ -303,26 +358,40 @@ __global__ __launch_bounds__(WG) void k_up(const DTask *tasks, const DCluster *c
     }
     __syncthreads();
     // KB * r reaches 512 (8 right-hand sides at rank 64): threads loop over it
+    const bool single = Tk.single != 0;  // uniform over the workgroup
     for (int e = tid; e < KB * r; e += WG) {
         const int qq = e / r, m = e % r;
         T s = T(0);
 #pragma unroll
-        for (int w = 0; w < WPB; ++w) s += sh[w][qq][m];
-        partial[(Tk.slot * KB + qq) * r + m] = s;
+        for (int wi = 0; wi < WPB; ++wi) s += sh[wi][qq][m];
+        if (!single) partial[(Tk.slot * KB + qq) * r + m] = s;
+        else if constexpr (CHEB) mus[qq][m] = s;
+        else w[((int64_t)Tk.cl * KB + qq) * r + m] = s;
+    }
+    if constexpr (CHEB) {
+        if (single) {
+            __syncthreads();
+            for (int e = tid; e < KB * r; e += WG) {
+                const int qq = e / r, m = e % r;
+                T v = T(0);
+                for (int k = 0; k < r; ++k) v = fmaT(chebA[m * r + k], mus[qq][k], v);
+                w[((int64_t)Tk.cl * KB + qq) * r + m] = v;
+            }
+        }
     }
 }
 
-// w[(c, q)] = sum over the task slots of cluster c; in Chebyshev form the
-// slots hold moments and w = A mu. One wavefront per cluster: lane = g*r + m
-// handles row m of right-hand sides q = g, g + G, ... (G = 64 / r groups).
+// w[(c, q)] = sum over the task slots of cluster c, in slot order; in
+// Chebyshev form the slots hold moments and w = A mu. One wavefront per
+// cluster of the list ids: lane = g*r + m handles row m of right-hand sides
+// q = g, g + G, ... (G = 64 / r groups).
 template <class T, bool CHEB, int KB>
-__global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, const T *partial,
-                                              const T *chebA, T *w, int r,
-                                              const int32_t *ids = nullptr) {
+__global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, const int32_t *ids, int n,
+                                              const T *partial, const T *chebA, T *w, int r) {
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    if (wid >= ncl) return;
-    const int cid = ids ? ids[wid] : wid;  // cluster of this wavefront
+    if (wid >= n) return;
+    const int cid = ids[wid];
     const DCluster C = cl[cid];
     const int G = WAVE / r;
     const int m = lane % r, g = lane / r;
@@ -331,16 +400,8 @@ __global__ __launch_bounds__(WG) void k_reduce(const DCluster *cl, int ncl, cons
         const int q = q0 + g;
         const bool aq = act && q < KB;
         T acc = T(0);
-        if (aq)
-            for (int t = 0; t < C.nslots; ++t) acc += partial[((C.pbase + t) * KB + q) * r + m];
-        if constexpr (CHEB) {
-            T v = T(0);
-            for (int k = 0; k < r; ++k) {
-                const T mu = __shfl(acc, (g * r + k) & (WAVE - 1));
-                v = fmaT(chebA[m * r + k], mu, v);
-            }
-            acc = v;
-        }
+        if (aq) acc = sum_slots(partial + (C.pbase * KB + q) * r + m, (int64_t)KB * r, C.nslots);
+        if constexpr (CHEB) acc = cheb_nodes(chebA, acc, r, m, g * r);
         if (aq) w[((int64_t)cid * KB + q) * r + m] = acc;
     }
 }
@@ -373,8 +434,8 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
         constexpr int UNR = FREE ? 1 : 2;
 #pragma unroll UNR
         for (int e = C.far_beg; e < C.far_end; ++e) {
-            const DFarBlk b = (TRANS ? p.far_cc : p.far_rc)[e];
-            const int src = TRANS ? b.rc : b.cc;
+            const DTrRec b = (TRANS ? p.tr_cc : p.tr_rc)[e];
+            const int src = b.src;
             const T *ws = w + ((int64_t)src * KB + qa) * r;
             T fv[RMAX];
             if constexpr (!FREE) {
@@ -386,8 +447,8 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
                 }
             } else {
                 // forward: F[m, l] = f(tx_m, ty_l); transpose: lane row m is F's column
-                const T *own = TRANS ? p.cnodes + (int64_t)b.cc * r : p.rnodes + (int64_t)b.rc * r;
-                const T *oth = TRANS ? p.rnodes + (int64_t)b.rc * r : p.cnodes + (int64_t)b.cc * r;
+                const T *own = (TRANS ? p.cnodes : p.rnodes) + (int64_t)ci * r;
+                const T *oth = (TRANS ? p.rnodes : p.cnodes) + (int64_t)src * r;
                 const T to = own[m];
 #pragma unroll
                 for (int l = 0; l < RMAX; ++l) {
@@ -409,6 +470,103 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
         }
         if (aq) z[((int64_t)ci * KB + q) * r + m] = acc;
     }
+}
+
+// Translation of one right-hand side with stored F, one wavefront per
+// cluster: lane (m, h) = (lane % MW, lane / MW) sums F[m, l] w_l over the h-th
+// part of the block's columns l (NH parts of LH), the parts added with a
+// shuffle, then (Chebyshev form) c = A^T z. The cluster's block records are
+// read first, one per lane, so the F loads of a block issue without waiting
+// on its record; two blocks' loads are in flight, each lane holding LH of a
+// block's F values (12 at rank 24) and lane l its source cluster's w_l (shared
+// by shuffles). A source cluster with several upward tasks is summed here from
+// its partial slots in slot order (and, in Chebyshev form, mapped by A), the
+// reduce's arithmetic in the reduce's order, so no reduce runs before this
+// pass. Few registers keep many wavefronts resident to cover the memory
+// latency.
+template <class T, int RMAX, bool TRANS, bool CHEB>
+__global__ __launch_bounds__(WG) void k_translate1(DevPlanT<T> p, const DCluster *cl, int ncl,
+                                                  const T *partial, const T *w, T *z) {
+    constexpr int NH = RMAX <= 32 ? 2 : 1;  // column parts
+    constexpr int MW = WAVE / NH;           // lanes per part
+    constexpr int LHMAX = (RMAX + NH - 1) / NH;
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= ncl) return;
+    const int r = p.r;
+    const int m = lane % MW, h = lane / MW;
+    const int LH = (r + NH - 1) / NH;
+    const int l0 = h * LH, l1 = min(r, l0 + LH);
+    const bool act = m < r;
+    const int mm = act ? m : 0;
+    const int lw = min(lane, r - 1);
+    const DCluster C = cl[wid];
+    const DTrRec *recs = TRANS ? p.tr_cc : p.tr_rc;
+    T acc = T(0);
+    for (int e0 = C.far_beg; e0 < C.far_end; e0 += WAVE) {
+        const int ne = min(WAVE, C.far_end - e0);
+        int64_t b_foff = 0, b_pbase = 0;
+        int b_src = 0, b_ns = 1;
+        if (lane < ne) {
+            const DTrRec b = recs[e0 + lane];
+            b_foff = b.foff;
+            b_pbase = b.pbase;
+            b_src = b.src;
+            b_ns = b.nslots;
+        }
+        auto load = [&](int e, T *fv, T &wl) {
+            const T *F = p.F + rl_l(b_foff, e);
+            const int ns = rl_i(b_ns, e);
+            if (ns > 1) {  // a source with several tasks (wave-uniform branch)
+                wl = sum_slots(partial + rl_l(b_pbase, e) * r + lw, (int64_t)r, ns);
+                if constexpr (CHEB) wl = cheb_nodes(p.chebA, wl, r, lw, 0);
+            } else {
+                wl = w[(int64_t)rl_i(b_src, e) * r + lw];
+            }
+#pragma unroll
+            for (int j = 0; j < LHMAX; ++j) {
+                const int l = min(l0 + j, r - 1);  // in bounds; the extra terms are dropped
+                fv[j] = __builtin_nontemporal_load(TRANS ? F + (l + r * mm) : F + (mm + r * l));
+            }
+        };
+        auto apply = [&](const T *fv, T wl) {
+#pragma unroll
+            for (int j = 0; j < LHMAX; ++j) {
+                const T wv = __shfl(wl, min(l0 + j, r - 1));
+                if (l0 + j < l1) acc = fmaT(fv[j], wv, acc);
+            }
+        };
+        int e = 0;
+        for (; e + 2 <= ne; e += 2) {
+            T fa[LHMAX], fb[LHMAX], wa, wb;
+            load(e, fa, wa);
+            load(e + 1, fb, wb);
+            apply(fa, wa);
+            apply(fb, wb);
+        }
+        if (e < ne) {
+            T fa[LHMAX], wa;
+            load(e, fa, wa);
+            apply(fa, wa);
+        }
+    }
+    if constexpr (NH == 2) acc += __shfl_xor(acc, MW);
+    if constexpr (CHEB) {  // coefficients c = A^T z, A's column loaded eight rows at a time
+        T c = T(0);
+#pragma unroll
+        for (int k0 = 0; k0 < RMAX; k0 += 8) {
+            T a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = p.chebA[min(k0 + u, r - 1) * r + mm];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const T zk = __shfl(acc, k0 + u);
+                if (k0 + u < r) c = fmaT(a[u], zk, c);
+            }
+        }
+        acc = c;
+    }
+    if (act && h == 0) z[(int64_t)wid * r + m] = acc;
 }
 
 // Matrix-free near field of one lane and one right-hand side: sum_j f(p, q_j)
@@ -810,7 +968,8 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
     const DTask *in_task = tr ? p.rtask : p.ctask;
     const int in_ntask = tr ? p.nrtask : p.nctask;
     const DCluster *in_cl = tr ? p.rcl : p.ccl;
-    const int in_ncl = tr ? p.nrc : p.ncc;
+    const int32_t *in_multi = tr ? p.multi_r : p.multi_c;  // clusters with several tasks
+    const int in_nmulti = tr ? p.nmulti_r : p.nmulti_c;
     const T *in_B = tr ? p.U : p.V;
     const V2<T> *in_map = tr ? p.rmap : p.cmap;
     const T *in_pts = tr ? p.x : p.y;
@@ -851,12 +1010,24 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
     mark(0);
     if (p.nfar > 0 && in_ntask > 0) {
         k_up<T, RMAX, CHEB, KB><<<in_ntask, WG, 0, s>>>(in_task, in_cl, in_B, in_map, in_pts, a.x,
-                                                      a.incx, a.ldx, a.partial, p.r);
+                                                      a.incx, a.ldx, a.partial, a.w, p.chebA,
+                                                      p.r);
         mark(1);
-        k_reduce<T, CHEB, KB><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial,
-                                                                      p.chebA, a.w, p.r);
+        // one right-hand side with stored F: the translation sums the slots
+        // of clusters with several tasks itself
+        constexpr bool T1 = KB == 1 && !FREE;
+        if (!T1 && in_nmulti > 0)
+            k_reduce<T, CHEB, KB><<<nblk(in_nmulti), WG, 0, s>>>(in_cl, in_multi, in_nmulti,
+                                                                 a.partial, p.chebA, a.w, p.r);
         mark(2);
-        if (tr) {
+        if constexpr (T1) {
+            if (tr)
+                k_translate1<T, RMAX, true, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                    p, out_cl, out_ncl, a.partial, a.w, a.z);
+            else
+                k_translate1<T, RMAX, false, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                    p, out_cl, out_ncl, a.partial, a.w, a.z);
+        } else if (tr) {
             if constexpr (KB == 1)
                 k_translate<T, RMAX, FREE, KID, true, CHEB, 1>
                     <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl,
@@ -890,7 +1061,6 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
     const DTask *in_task = tr ? p.rtask : p.ctask;
     const int in_ntask = tr ? p.nrtask : p.nctask;
     const DCluster *in_cl = tr ? p.rcl : p.ccl;
-    const int in_ncl = tr ? p.nrc : p.ncc;
     const V2<T> *in_map = tr ? p.rmap : p.cmap;
     const T *in_pts = tr ? p.x : p.y;
     const DChunk *out_ch = tr ? p.cch : p.rch;
@@ -904,8 +1074,12 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
     const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
     if (p.nfar > 0 && in_ntask > 0) {
         k_up<T, RMAX, true, 1><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x,
-                                                     a.incx, 0, a.partial, p.r);
-        k_reduce<T, true, 1><<<nblk(in_ncl), WG, 0, s>>>(in_cl, in_ncl, a.partial, p.chebA, a.w, p.r);
+                                                     a.incx, 0, a.partial, a.w, p.chebA, p.r);
+        const int32_t *in_multi = tr ? p.multi_r : p.multi_c;
+        const int in_nmulti = tr ? p.nmulti_r : p.nmulti_c;
+        if (in_nmulti > 0)
+            k_reduce<T, true, 1><<<nblk(in_nmulti), WG, 0, s>>>(in_cl, in_multi, in_nmulti,
+                                                                a.partial, p.chebA, a.w, p.r);
         if (tr)
             k_translate<T, RMAX, true, KID, true, true, 1>
                 <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
@@ -939,9 +1113,13 @@ hipError_t grad_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
 
 
 // ------------------------------------------ multi-RHS passes on the MFMA units
-// Forward products of 16 right-hand sides per launch (nq <= 16 of them valid;
-// the others read and write zeros), far field in Chebyshev form, every
-// contraction on the matrix cores:
+// Forward products of up to 32 right-hand sides per launch, in G groups of 16
+// (the N of the 16x16xK shapes): every operand a pass generates or streams
+// (Chebyshev terms, regenerated or stored near-field entries, F) feeds the G
+// groups' products, so a launch of 32 costs one launch of 16 in vector-unit
+// work and in streamed bytes. nq <= 16 G right-hand sides are valid; the
+// others read and write zeros. Far field in Chebyshev form; every contraction
+// on the matrix cores:
 //   up         mu = T(points)^T X per task, one wavefront per task; a cluster
 //              with a single task applies A at once (w = A mu), the others
 //              leave partial sums for the reduce
@@ -959,19 +1137,24 @@ hipError_t grad_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
 //      lo*hi + mid*mid, the terms down to 2^-16 of hi*hi), FP32 accumulation:
 //      operands are carried to FP32's 24 bits, where two pieces carry 16
 // The FP32 accumulators of the reduced-precision products hold one step's
-// sum only (64 points upward, one cluster's series downward) and are added
-// into FP64 sums after every step, so FP32 rounding does not grow with the
-// number of points or clusters. Near field, reduce and translate run in FP64
-// (v_mfma_f64_16x16x4f64) for every PREC; point differences and the
-// Chebyshev recurrence are FP64.
+// sum only (64 points upward, one cluster's series of one row tile downward)
+// and are added into the FP64 accumulators after every step, so FP32
+// rounding does not grow with the number of points or clusters. Near field,
+// reduce and translate run in FP64 (v_mfma_f64_16x16x4f64) for every PREC;
+// point differences and the Chebyshev recurrence are FP64.
 // Operand layouts (lane l, j16 = l % 16, g = l / 16): A[i][k] comes from the
 // lane with i = j16 and k = g (K = 4 shapes) or k = 8g + v, v < 8 (K = 32);
 // B[k][j] from the lane with j = j16 and the same k. Results: the FP32 shapes
-// hold D[4g + v][j16] in register v, the FP64 shape holds D[g + 4v][j16].
+// hold D[4g + v][j16] in register v, the FP64 shape holds D[g + 4v][j16]. The
+// reduced-precision products take row rho(i) = i/4 + 4 (i % 4) into A-row slot
+// i, so their register v holds row g + 4v, the FP64 shape's row: they add onto
+// the FP64 accumulators lane for lane.
 typedef float mf4_t __attribute__((ext_vector_type(4)));
 typedef double md4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 mbf8_t __attribute__((ext_vector_type(8)));
-constexpr int MQ = 16;  // right-hand sides per launch
+constexpr int MQ = 16;     // right-hand sides per group (the MFMA's N)
+constexpr int MGMAX = 2;   // groups per launch at most
+__device__ __forceinline__ int rho(int s) { return (s >> 2) + 4 * (s & 3); }
 
 __device__ __forceinline__ __bf16 bf16_hi(float x) { return (__bf16)x; }
 __device__ __forceinline__ __bf16 bf16_lo(float x) { return (__bf16)(x - (float)(__bf16)x); }
@@ -1024,41 +1207,37 @@ __device__ __forceinline__ void split8(const float *v, mbf8_t &h, mbf8_t &m, mbf
 __device__ __forceinline__ md4_t mfma64(double a, double b, md4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-// wave-uniform copies of lane l's value (l uniform)
-__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ int64_t rl_l(int64_t v, int l) {
-    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, l);
-    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
-    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
-}
-__device__ __forceinline__ double rl_d(double v, int l) {
-    return __longlong_as_double(rl_l(__double_as_longlong(v), l));
+__device__ __forceinline__ md4_t add_f32_acc(md4_t d, mf4_t f) {
+    return md4_t{d[0] + (double)f[0], d[1] + (double)f[1], d[2] + (double)f[2],
+                 d[3] + (double)f[3]};
 }
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
-// xt[j][q] = x[j*incx + q*ldx] for q < nq, 0 beyond: the right-hand sides of
-// one launch packed row by row (one 128-byte row per input index), so every
-// MFMA B operand of the upward and downward passes is a coalesced load.
+// xt[j][q] = x[j*incx + q*ldx] for q < nq, 0 beyond (q < 16 G): the
+// right-hand sides of one launch packed row by row, so every MFMA B operand
+// of the upward and downward passes is a coalesced load.
+template <int G>
 __global__ __launch_bounds__(WG) void k_pack_x(const double *x, int64_t incx, int64_t ldx, int nq,
                                               int64_t n, double *xt) {
+    constexpr int MQT = MQ * G;
     const int64_t j = (int64_t)blockIdx.x * WG + threadIdx.x;
     if (j >= n) return;
-    double v[MQ];
+    double v[MQT];
 #pragma unroll
-    for (int q = 0; q < MQ; ++q) v[q] = q < nq ? x[j * incx + q * ldx] : 0.0;
-    double2 *o = reinterpret_cast<double2 *>(xt + j * MQ);
+    for (int q = 0; q < MQT; ++q) v[q] = q < nq ? x[j * incx + q * ldx] : 0.0;
+    double2 *o = reinterpret_cast<double2 *>(xt + j * MQT);
 #pragma unroll
-    for (int q = 0; q < MQ / 2; ++q) o[q] = make_double2(v[2 * q], v[2 * q + 1]);
+    for (int q = 0; q < MQT / 2; ++q) o[q] = make_double2(v[2 * q], v[2 * q + 1]);
 }
 
 // Upward pass, one wavefront (the whole workgroup) per task: the moments
 // mu[m][q] = sum_j T_m(xh_j) x[j, q] of the task's points. The wavefront forms
 // T for PTS points (one per lane, FP64 recurrence) in LDS, then contracts over
-// the points on MFMA.
-template <int RP, int PREC>
+// the points on MFMA, each T operand feeding the G groups.
+template <int RP, int PREC, int G>
 __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double *pts,
                                               const double *xt, int64_t ldxt, int nq,
                                               const double *chebA, double *partial, double *w,
@@ -1066,32 +1245,34 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
     using TL = typename std::conditional<PREC == 2, double, float>::type;
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int MT = RP / 16;
+    constexpr int MQT = MQ * G;
     constexpr int PTS = (PREC == 2 && RP > 32) ? 32 : 64;  // points per step
-    static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQ * sizeof(double), "mu fits in Tl");
+    static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQT * sizeof(double), "mu fits in Tl");
     __shared__ __align__(16) TL Tl[RP][PTS + 1];
     const DTask Tk = tasks[blockIdx.x];
     const int lane = threadIdx.x;
     const int j16 = lane % 16, g = lane / 16;
+    // the A-operand row (rank) of this lane: j16 for FP64, rho(j16) otherwise
+    const int arow = PREC == 2 ? j16 : rho(j16);
     const double2 cm = make_double2(Tk.cx, Tk.cy);
     const int64_t first = Tk.first;
-    md4_t D64[MT];
-    mf4_t D32[MT];
-    double DF[MT][4];  // FP64 sums of the reduced-precision steps
+    md4_t D64[G][MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) {
-        D64[t] = md4_t{0.0, 0.0, 0.0, 0.0};
-        D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) DF[t][v] = 0.0;
-    }
+        for (int t = 0; t < MT; ++t) D64[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
     for (int base = 0; base < Tk.len; base += PTS) {
         // this lane's B operands of the step, loaded before the recurrence:
-        // x[point][j16] for the points k0 + g (K = 4) or k0 + 8g + v (K = 32)
-        double bx[PTS / 4];
+        // x[point][16 gi + j16] for the points k0 + g (K = 4) or k0 + 8g + v (K = 32)
+        double bx[G][PTS / 4];
 #pragma unroll
         for (int u = 0; u < PTS / 4; ++u) {
             const int i = base + (BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g);
-            bx[u] = (i < Tk.len && j16 < nq) ? xt[(first + i) * ldxt + j16] : 0.0;
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) {
+                const int q = gi * MQ + j16;
+                bx[gi][u] = (i < Tk.len && q < nq) ? xt[(first + i) * ldxt + q] : 0.0;
+            }
         }
         if (lane < PTS) {
             const int i = base + lane;
@@ -1110,89 +1291,106 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
             }
         }
         wave_sync();
-        if constexpr (!BF) {
+        if constexpr (PREC == 2) {
 #pragma unroll
-            for (int k0 = 0; k0 < PTS; k0 += 4) {
-                const double b = bx[k0 / 4];
+            for (int k0 = 0; k0 < PTS; k0 += 4)
 #pragma unroll
                 for (int t = 0; t < MT; ++t) {
-                    if constexpr (PREC == 2) D64[t] = mfma64(Tl[t * 16 + j16][k0 + g], b, D64[t]);
-                    else D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(Tl[t * 16 + j16][k0 + g], (float)b, D32[t], 0, 0, 0);
+                    const double av = Tl[t * 16 + arow][k0 + g];
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) D64[gi][t] = mfma64(av, bx[gi][k0 / 4], D64[gi][t]);
                 }
+        } else if constexpr (PREC == 0) {
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                mf4_t D32[G];
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k0 = 0; k0 < PTS; k0 += 4) {
+                    const float av = Tl[t * 16 + arow][k0 + g];
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi)
+                        D32[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, (float)bx[gi][k0 / 4], D32[gi], 0, 0, 0);
+                }
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi) D64[gi][t] = add_f32_acc(D64[gi][t], D32[gi]);
             }
         } else {
 #pragma unroll
-            for (int k0 = 0; k0 < PTS; k0 += 32) {
-                float bv[8];
+            for (int t = 0; t < MT; ++t) {
+                mf4_t D32[G];
 #pragma unroll
-                for (int v = 0; v < 8; ++v) bv[v] = (float)bx[(k0 / 32) * 8 + v];
-                mbf8_t bh, bm, bl;
-                split8<PREC>(bv, bh, bm, bl);
+                for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int t = 0; t < MT; ++t) {
+                for (int k0 = 0; k0 < PTS; k0 += 32) {
                     float tv[8];
 #pragma unroll
-                    for (int v = 0; v < 8; ++v) tv[v] = Tl[t * 16 + j16][k0 + 8 * g + v];
+                    for (int v = 0; v < 8; ++v) tv[v] = Tl[t * 16 + arow][k0 + 8 * g + v];
                     mbf8_t ah, am, al;
                     split8<PREC>(tv, ah, am, al);
-                    D32[t] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[t]);
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) {
+                        float bv[8];
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) bv[v] = (float)bx[gi][(k0 / 32) * 8 + v];
+                        mbf8_t bh, bm, bl;
+                        split8<PREC>(bv, bh, bm, bl);
+                        D32[gi] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[gi]);
+                    }
                 }
-            }
-        }
-        if constexpr (PREC != 2) {  // this step's FP32 sums into the FP64 ones
 #pragma unroll
-            for (int t = 0; t < MT; ++t) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) DF[t][v] += (double)D32[t][v];
-                D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+                for (int gi = 0; gi < G; ++gi) D64[gi][t] = add_f32_acc(D64[gi][t], D32[gi]);
             }
         }
         wave_sync();
     }
-    // mu[m][q] through LDS (Tl is free again)
-    double (*mu)[MQ] = reinterpret_cast<double (*)[MQ]>(&Tl[0][0]);
+    // mu[m][q] through LDS (Tl is free again); register v holds rank g + 4v
+    double (*mu)[MQT] = reinterpret_cast<double (*)[MQT]>(&Tl[0][0]);
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+    for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if constexpr (PREC == 2) mu[t * 16 + g + 4 * v][j16] = D64[t][v];
-            else mu[t * 16 + 4 * g + v][j16] = DF[t][v];
-        }
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) mu[t * 16 + g + 4 * v][gi * MQ + j16] = D64[gi][t][v];
     wave_sync();
     if (Tk.single) {
         // single task: w = A mu (the reduce's Chebyshev step, same order)
-        for (int idx = lane; idx < r * MQ; idx += WAVE) {
+        for (int idx = lane; idx < r * MQT; idx += WAVE) {
             const int q = idx / r, m = idx - q * r;
             double v = 0.0;
             for (int k = 0; k < r; ++k) v = __builtin_fma(chebA[m * r + k], mu[k][q], v);
-            w[((int64_t)Tk.cl * MQ + q) * r + m] = v;
+            w[((int64_t)Tk.cl * MQT + q) * r + m] = v;
         }
     } else {
-        for (int idx = lane; idx < r * MQ; idx += WAVE) {
+        for (int idx = lane; idx < r * MQT; idx += WAVE) {
             const int q = idx / r, m = idx - q * r;
-            partial[(Tk.slot * MQ + q) * r + m] = mu[m][q];
+            partial[(Tk.slot * MQT + q) * r + m] = mu[m][q];
         }
     }
 }
 
 // Translation, one wavefront per row cluster: Z = sum over the cluster's far
-// blocks of F_b W_b (F stored, or regenerated from the mapped nodes), then
-// C = A^T Z. The FP64 result layout of Z is the B-operand layout of the
-// second product, so Z feeds it from registers.
-template <int RP, bool FREE, int KID>
+// blocks of F_b W_b (F stored, or regenerated from the mapped nodes; each F
+// operand feeds the G groups), then C = A^T Z. The FP64 result layout of Z is
+// the B-operand layout of the second product, so Z feeds it from registers.
+template <int RP, bool FREE, int KID, int G>
 __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *cl, int ncl,
                                                    const int32_t *far_list, const double *w,
                                                    double *z) {
     constexpr int MT = RP / 16;
+    constexpr int MQT = MQ * G;
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     if (wid >= ncl) return;
     const int r = p.r;
     const int j16 = lane % 16, g = lane / 16;
     const DCluster C = cl[wid];
-    md4_t Z[MT];
+    md4_t Z[G][MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) Z[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int t = 0; t < MT; ++t) Z[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
     for (int e0 = C.far_beg; e0 < C.far_end; e0 += WAVE) {
         // the far blocks' records, one per lane, read together
         const int ne = min(WAVE, C.far_end - e0);
@@ -1206,19 +1404,20 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
         }
         for (int e = 0; e < ne; ++e) {
             const int rc = rl_i(b_rc, e), cc = rl_i(b_cc, e);
-            const double *ws = w + (int64_t)cc * MQ * r;
+            const double *ws = w + (int64_t)cc * MQT * r;
             const double *F = FREE ? nullptr : p.F + rl_l(b_foff, e);
             const double *tx = p.rnodes + (int64_t)rc * r;
             const double *ty = p.cnodes + (int64_t)cc * r;
 #pragma unroll
             for (int k0 = 0; k0 < RP; k0 += 16) {
                 // four K-steps: operands first, then the products
-                double bv[4], av[4][MT];
+                double bv[G][4], av[4][MT];
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     const int l = k0 + 4 * h + g;
                     const bool lok = l < r;
-                    bv[h] = lok ? ws[j16 * r + l] : 0.0;
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) bv[gi][h] = lok ? ws[(gi * MQ + j16) * r + l] : 0.0;
                     double tyl = 0.0;
                     if constexpr (FREE) tyl = lok ? ty[l] : 0.0;
 #pragma unroll
@@ -1234,44 +1433,50 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 #pragma unroll
                 for (int h = 0; h < 4; ++h)
 #pragma unroll
-                    for (int t = 0; t < MT; ++t) Z[t] = mfma64(av[h][t], bv[h], Z[t]);
+                    for (int t = 0; t < MT; ++t)
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) Z[gi][t] = mfma64(av[h][t], bv[gi][h], Z[gi][t]);
             }
         }
     }
     // C[m][q] = sum_k A[k][m] Z[k][q]; this lane holds Z[16t + g + 4v][j16] in
     // Z[t][v], which is B[k0 + g][j16] of the K-step k0 = 16t + 4v
-    md4_t Cc[MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) Cc[t] = md4_t{0.0, 0.0, 0.0, 0.0};
+    for (int gi = 0; gi < G; ++gi) {
+        md4_t Cc[MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+        for (int t = 0; t < MT; ++t) Cc[t] = md4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int k = t * 16 + 4 * v + g;
+        for (int t = 0; t < MT; ++t)
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt) {
-                const int m = tt * 16 + j16;
-                const double av = (k < r && m < r) ? p.chebA[k * r + m] : 0.0;
-                Cc[tt] = mfma64(av, Z[t][v], Cc[tt]);
+            for (int v = 0; v < 4; ++v) {
+                const int k = t * 16 + 4 * v + g;
+#pragma unroll
+                for (int tt = 0; tt < MT; ++tt) {
+                    const int m = tt * 16 + j16;
+                    const double av = (k < r && m < r) ? p.chebA[k * r + m] : 0.0;
+                    Cc[tt] = mfma64(av, Z[gi][t][v], Cc[tt]);
+                }
             }
-        }
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+        for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = t * 16 + g + 4 * v;
-            if (m < r) z[((int64_t)wid * MQ + j16) * r + m] = Cc[t][v];
-        }
+            for (int v = 0; v < 4; ++v) {
+                const int m = t * 16 + g + 4 * v;
+                if (m < r) z[((int64_t)wid * MQT + gi * MQ + j16) * r + m] = Cc[t][v];
+            }
+    }
 }
 
 // Downward pass, one wavefront per 64-row chunk. Rows 16t + j16 (t < 4) are
-// this lane's A-operand rows. Near field: per 64-column tile of a dense block
-// the right-hand sides are staged in LDS (xs[col][q]) and the block entries
-// (stored, or regenerated from the points) enter the FP64 MFMA directly. Far
-// field: for each covering cluster, T_k(xh) of the lane's rows by the FP64
-// recurrence, times the cluster's coefficients C[k][q]. The 64 x 16 result
-// goes through LDS to a row-per-lane layout and is written once.
-template <int RP, int PREC, bool FREE, int KID>
+// this lane's A-operand rows of the FP64 products (rows 16t + rho(j16) of the
+// reduced-precision ones). Near field: per 8 columns of a dense block the
+// block entries (stored, or regenerated from the points) enter the FP64 MFMA
+// directly, each feeding the G groups. Far field: for each covering cluster,
+// T_k(xh) of the lane's rows by the FP64 recurrence, times the cluster's
+// coefficients C[k][q]. The 64 x 16G result goes through LDS to a
+// row-per-lane layout and is written once.
+template <int RP, int PREC, bool FREE, int KID, int G>
 __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
                                               const DNear *nl, const DFar *fl,
                                               const double2 *cmapv, const double *pts_out,
@@ -1279,7 +1484,8 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                                               int64_t ldxt, const double *z, MvArgs a, int nq) {
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
-    __shared__ double ds_all[WPB][16 * MQ];  // one 16-row output tile per wavefront
+    constexpr int MQT = MQ * G;
+    __shared__ double ds_all[WPB][16 * MQT];  // one 16-row output tile per wavefront
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     double *Dl = ds_all[(threadIdx.x / WAVE) % WPB];
@@ -1287,24 +1493,22 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     const int r = p.r;
     const DChunk c = ch[wid];
     const int j16 = lane % 16, g = lane / 16;
-    double po[4];
-    bool rok[4];
+    const int frow = PREC == 2 ? j16 : rho(j16);  // the far-field A-operand row
+    double po[4], pf[4];
+    bool rok[4], fok[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         rok[t] = t * 16 + j16 < c.np;
         po[t] = rok[t] ? pts_out[c.p0 + t * 16 + j16] : 0.0;
+        fok[t] = t * 16 + frow < c.np;
+        pf[t] = PREC == 2 ? po[t] : (fok[t] ? pts_out[c.p0 + t * 16 + frow] : 0.0);
     }
     const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
-    md4_t D[4];
-    mf4_t D32[4];
-    double DF[4][4];  // FP64 sums of the reduced-precision far field, per cluster
+    md4_t D[G][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        D[t] = md4_t{0.0, 0.0, 0.0, 0.0};
-        D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+    for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) DF[t][v] = 0.0;
-    }
+        for (int t = 0; t < 4; ++t) D[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
     // ---- near field, FP64: two K-steps (8 columns) per group, operands first;
     // the chunk's block records are read together, one per lane
     for (int e0 = c.near_beg; e0 < c.near_end; e0 += WAVE) {
@@ -1331,12 +1535,16 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                 rowoff[t] = aoff + (int64_t)(i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1));
             }
             for (int k0 = 0; k0 < bn; k0 += 8) {
-                double bv[2], av[2][4];
+                double bv[G][2], av[2][4];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int cc = k0 + 4 * h + g;
                     const bool cok = cc < bn;
-                    bv[h] = (cok && j16 < nq) ? xtb[(int64_t)cc * ldxt + j16] : 0.0;
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) {
+                        const int q = gi * MQ + j16;
+                        bv[gi][h] = (cok && q < nq) ? xtb[(int64_t)cc * ldxt + q] : 0.0;
+                    }
                     double yc = 0.0;
                     if constexpr (FREE) yc = cok ? pts_in[col0 + cc] : 0.0;
 #pragma unroll
@@ -1352,18 +1560,24 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        if (t < nt) D[t] = mfma64(av[h][t], bv[h], D[t]);
+                        if (t < nt)
+#pragma unroll
+                            for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(av[h][t], bv[gi][h], D[gi][t]);
             }
         }
     }
     // ---- far field: the covering clusters' records one per lane; the
     // coefficients of the next cluster load while this one is applied
-    auto load_bz = [&](double *bz, int cl) {
-        const double *zc = z + (int64_t)cl * MQ * r + (int64_t)j16 * r;  // C[k][q = j16]
+    auto load_bz = [&](double (*bz)[KSEL], int cl) {
 #pragma unroll
-        for (int u = 0; u < KSEL; ++u) {
-            const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
-            bz[u] = (k < r && j16 < nq) ? zc[min(k, r - 1)] : 0.0;
+        for (int gi = 0; gi < G; ++gi) {
+            const int q = gi * MQ + j16;
+            const double *zc = z + ((int64_t)cl * MQT + q) * r;  // C[k][q]
+#pragma unroll
+            for (int u = 0; u < KSEL; ++u) {
+                const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
+                bz[gi][u] = (k < r && q < nq) ? zc[min(k, r - 1)] : 0.0;
+            }
         }
     };
     for (int e0 = c.far_beg; e0 < c.far_end; e0 += WAVE) {
@@ -1376,7 +1590,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
             f_cx = cmv.x;
             f_cy = cmv.y;
         }
-        double bz[KSEL], bzn[KSEL] = {};
+        double bz[G][KSEL], bzn[G][KSEL] = {};
         load_bz(bz, rl_i(f_cl, 0));
         for (int e = 0; e < ne; ++e) {
             const double2 cm = make_double2(rl_d(f_cx, e), rl_d(f_cy, e));
@@ -1384,7 +1598,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 if (t >= nt) break;
-                const double xh = (po[t] - cm.x) * cm.y;
+                const double xh = (pf[t] - cm.x) * cm.y;
                 double sel[KSEL];
                 if constexpr (!BF) {
                     // this lane's T_k, k = g + 4u: T_g and T_{g+4} from the prefix
@@ -1398,7 +1612,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                     const double t4x2 = c4 + c4;
 #pragma unroll
                     for (int u = 0; u < KSEL; ++u) {
-                        sel[u] = (rok[t] && 4 * u + g < r) ? ta : 0.0;
+                        sel[u] = (fok[t] && 4 * u + g < r) ? ta : 0.0;
                         const double tn = __builtin_fma(t4x2, tb, -ta);  // T_{g+4u+8}
                         ta = tb;
                         tb = tn;
@@ -1417,39 +1631,55 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                         }
                         const bool mine = ((k & 31) >> 3) == g;
                         const int u = (k >> 5) * 8 + (k & 7);
-                        if (mine && rok[t] && k < r) sel[u] = tv;
+                        if (mine && fok[t] && k < r) sel[u] = tv;
                     }
                 }
                 if constexpr (PREC == 2) {
 #pragma unroll
-                    for (int u = 0; u < KSEL; ++u) D[t] = mfma64(sel[u], bz[u], D[t]);
+                    for (int u = 0; u < KSEL; ++u)
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(sel[u], bz[gi][u], D[gi][t]);
                 } else if constexpr (PREC == 0) {
+                    mf4_t D32[G];
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int u = 0; u < KSEL; ++u)
-                        D32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[u], D32[t], 0, 0, 0);
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi)
+                            D32[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[gi][u], D32[gi], 0, 0, 0);
+                    // this cluster's FP32 sums into the FP64 accumulators
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
                 } else {
+                    mf4_t D32[G];
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int s8 = 0; s8 < KSEL / 8; ++s8) {
-                        float av[8], bv[8];
+                        float av[8];
 #pragma unroll
-                        for (int v = 0; v < 8; ++v) {
-                            av[v] = (float)sel[s8 * 8 + v];
-                            bv[v] = (float)bz[s8 * 8 + v];
-                        }
-                        mbf8_t ah, am, al, bh, bm, bl;
+                        for (int v = 0; v < 8; ++v) av[v] = (float)sel[s8 * 8 + v];
+                        mbf8_t ah, am, al;
                         split8<PREC>(av, ah, am, al);
-                        split8<PREC>(bv, bh, bm, bl);
-                        D32[t] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[t]);
-                    }
-                }
-                if constexpr (PREC != 2) {  // this cluster's FP32 sums into the FP64 ones
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) DF[t][v] += (double)D32[t][v];
-                    D32[t] = mf4_t{0.f, 0.f, 0.f, 0.f};
+                        for (int gi = 0; gi < G; ++gi) {
+                            float bv[8];
+#pragma unroll
+                            for (int v = 0; v < 8; ++v) bv[v] = (float)bz[gi][s8 * 8 + v];
+                            mbf8_t bh, bm, bl;
+                            split8<PREC>(bv, bh, bm, bl);
+                            D32[gi] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[gi]);
+                        }
+                    }
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
                 }
             }
 #pragma unroll
-            for (int u = 0; u < KSEL; ++u) bz[u] = bzn[u];
+            for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                for (int u = 0; u < KSEL; ++u) bz[gi][u] = bzn[gi][u];
         }
     }
     // ---- result, one 16-row tile at a time through LDS; a lane then writes
@@ -1458,22 +1688,19 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     for (int t = 0; t < 4; ++t) {
         if (t >= nt) break;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Dl[(g + 4 * v) * MQ + j16] = D[t][v];
-        if constexpr (PREC != 2) {
-            wave_sync();
+        for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Dl[(4 * g + v) * MQ + j16] += DF[t][v];
-        }
+            for (int v = 0; v < 4; ++v) Dl[(g + 4 * v) * MQT + gi * MQ + j16] = D[gi][t][v];
         wave_sync();
         const int row = j16;
         if (t * 16 + row < c.np) {
             const int64_t i = c.p0 + t * 16 + row;
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
+            for (int s4 = 0; s4 < 4 * G; ++s4) {
                 const int q = g + 4 * s4;
                 if (q < nq) {
                     double *yo = a.y + i * a.incy + q * a.ldy;
-                    *yo = (a.beta ? *yo : 0.0) + Dl[row * MQ + q];
+                    *yo = (a.beta ? *yo : 0.0) + Dl[row * MQT + q];
                 }
             }
         }
@@ -1482,53 +1709,59 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
 }
 
 // MODE 1: stored dense blocks (stored and hybrid plans); MODE 2: matrix-free
-template <int RP, int PREC, int MODE, int KID>
-hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, const int32_t *multi,
-                      int nmulti, hipStream_t s) {
+template <int RP, int PREC, int MODE, int KID, int G>
+hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) {
     constexpr bool FREE = MODE == 2;
+    constexpr int MQT = MQ * G;
     // rows of x are read in place when the right-hand sides of a row are
     // contiguous (ldx = 1, as for a row-major n x k array), else packed
     const double *xr = a.x;
     int64_t ldxr = a.incx;
     if (!(a.ldx == 1 && a.incx >= nq)) {
         if (p.ny > 0)
-            k_pack_x<<<(int)((p.ny + WG - 1) / WG), WG, 0, s>>>(a.x, a.incx, a.ldx, nq, p.ny, a.xt);
+            k_pack_x<G><<<(int)((p.ny + WG - 1) / WG), WG, 0, s>>>(a.x, a.incx, a.ldx, nq, p.ny, a.xt);
         xr = a.xt;
-        ldxr = MQ;
+        ldxr = MQT;
     }
     if (p.nfar > 0 && p.nctask > 0) {
-        k_up_m<RP, PREC><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
-                                                    a.partial, a.w, p.r);
-        if (nmulti > 0)
-            k_reduce<double, true, MQ><<<nblk(nmulti), WG, 0, s>>>(p.ccl, nmulti, a.partial, p.chebA,
-                                                                  a.w, p.r, multi);
-        k_translate_m<RP, FREE, KID><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc, a.w, a.z);
+        k_up_m<RP, PREC, G><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
+                                                       a.partial, a.w, p.r);
+        if (p.nmulti_c > 0)
+            k_reduce<double, true, MQT><<<nblk(p.nmulti_c), WG, 0, s>>>(
+                p.ccl, p.multi_c, p.nmulti_c, a.partial, p.chebA, a.w, p.r);
+        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc,
+                                                                   a.w, a.z);
     }
     if (p.nrch > 0)
-        k_down_m<RP, PREC, FREE, KID><<<nblk(p.nrch), WG, 0, s>>>(
+        k_down_m<RP, PREC, FREE, KID, G><<<nblk(p.nrch), WG, 0, s>>>(
             p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
     return hipGetLastError();
 }
 
-template <int RP, int PREC>
-hipError_t matvec_m_p(const DevPlan &p, const MvArgs &a, int mode, int nq, const int32_t *multi,
-                      int nmulti, hipStream_t s) {
-    if (mode != 2) return matvec_m_t<RP, PREC, 1, 0>(p, a, nq, multi, nmulti, s);
+template <int RP, int PREC, int G>
+hipError_t matvec_m_p(const DevPlan &p, const MvArgs &a, int mode, int nq, hipStream_t s) {
+    if (mode != 2) return matvec_m_t<RP, PREC, 1, 0, G>(p, a, nq, s);
     switch (p.kid) {
-    case 0: return matvec_m_t<RP, PREC, 2, 0>(p, a, nq, multi, nmulti, s);
-    case 1: return matvec_m_t<RP, PREC, 2, 1>(p, a, nq, multi, nmulti, s);
-    case 2: return matvec_m_t<RP, PREC, 2, 2>(p, a, nq, multi, nmulti, s);
-    default: return matvec_m_t<RP, PREC, 2, 3>(p, a, nq, multi, nmulti, s);
+    case 0: return matvec_m_t<RP, PREC, 2, 0, G>(p, a, nq, s);
+    case 1: return matvec_m_t<RP, PREC, 2, 1, G>(p, a, nq, s);
+    case 2: return matvec_m_t<RP, PREC, 2, 2, G>(p, a, nq, s);
+    default: return matvec_m_t<RP, PREC, 2, 3, G>(p, a, nq, s);
     }
+}
+
+template <int RP, int PREC>
+hipError_t matvec_m_g(const DevPlan &p, const MvArgs &a, int mode, int nq, hipStream_t s) {
+    if (nq > MQ) return matvec_m_p<RP, PREC, 2>(p, a, mode, nq, s);
+    return matvec_m_p<RP, PREC, 1>(p, a, mode, nq, s);
 }
 
 template <int RP>
 hipError_t matvec_m_r(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,
-                      const int32_t *multi, int nmulti, hipStream_t s) {
-    if (prec == 0) return matvec_m_p<RP, 0>(p, a, mode, nq, multi, nmulti, s);
-    if (prec == 1) return matvec_m_p<RP, 1>(p, a, mode, nq, multi, nmulti, s);
-    if (prec == 3) return matvec_m_p<RP, 3>(p, a, mode, nq, multi, nmulti, s);
-    return matvec_m_p<RP, 2>(p, a, mode, nq, multi, nmulti, s);
+                      hipStream_t s) {
+    if (prec == 0) return matvec_m_g<RP, 0>(p, a, mode, nq, s);
+    if (prec == 1) return matvec_m_g<RP, 1>(p, a, mode, nq, s);
+    if (prec == 3) return matvec_m_g<RP, 3>(p, a, mode, nq, s);
+    return matvec_m_g<RP, 2>(p, a, mode, nq, s);
 }
 
 template <class T, int RMAX, int KB>
@@ -1565,7 +1798,7 @@ hipError_t matvec_r(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, int mode
                              int64_t incy2, const double *sv, int64_t incs, hipStream_t s);     \
     hipError_t hmx_gen_basis_r##R(const DevPlan &p, hipStream_t s);                              \
     hipError_t hmx_matvec_m_r##R(const DevPlan &p, const MvArgs &a, int mode, int prec, int nq,  \
-                                 const int32_t *multi, int nmulti, hipStream_t s);
+                                 hipStream_t s);
 #define HMX_DECLARE_BUCKET_F32(R)                                                               \
     hipError_t hmx_matvec_f32_r##R(const DevPlanF &p, const MvArgsF &a, bool tr, int mode,      \
                                    int kb, hipStream_t s, const hipEvent_t *ev);                \

@@ -104,6 +104,9 @@ struct Builder {
     std::vector<DFarBlk> far;
     std::vector<int> far_depth;
     std::vector<DDense> dense;
+    // every indsplit of the tree (row and column side): the tree stays the
+    // same for other points exactly when each split falls at the same index
+    std::vector<DSplit> xsplits, ysplits;
     int64_t nd = 0, nk1 = 0, nk2 = 0, far_rows = 0, far_cols = 0, dense_entries = 0;
     int maxdepth = 0;
     int err = 0;
@@ -188,6 +191,8 @@ struct Builder {
         const int64_t ni1 = split(x, nx, i0, ni, a, b), ni2 = ni - ni1;
         const int64_t nj1 = split(y, ny, j0, nj, c, d), nj2 = nj - nj1;
         if (err) return;
+        xsplits.push_back({i0, ni, i0 + ni1, mid(a, b)});
+        ysplits.push_back({j0, nj, j0 + nj1, mid(c, d)});
         const int64_t i2 = i0 + ni1, j2 = j0 + nj1;
         const double ab2 = mid(a, b), cd2 = mid(c, d);
         const bool leaf = ni1 < bs && ni2 < bs && nj1 < bs && nj2 < bs;
@@ -519,15 +524,16 @@ struct hm_plan {
     DCluster *d_rcl = nullptr, *d_ccl = nullptr;
     DFarBlk *d_far = nullptr;
     int32_t *d_far_by_rc = nullptr, *d_far_by_cc = nullptr;
-    DFarBlk *d_far_rc = nullptr, *d_far_cc = nullptr;  // records in by_rc / by_cc order
+    DTrRec *d_tr_rc = nullptr, *d_tr_cc = nullptr;  // translation records in by_rc / by_cc order
     DDense *d_dense = nullptr;
     DChunk *d_rch = nullptr, *d_cch = nullptr;
     DNear *d_rnear = nullptr, *d_cnear = nullptr;
     DFar *d_rfar = nullptr, *d_cfar = nullptr;
     DTask *d_rtask = nullptr, *d_ctask = nullptr;
     std::vector<int32_t> multi_c;     // column clusters with more than one upward task
-    int32_t *d_multi_c = nullptr;
-    void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_multi_c (Staging)
+    std::vector<int32_t> multi_r;     // row clusters with more than one upward task
+    int32_t *d_multi_c = nullptr, *d_multi_r = nullptr;
+    void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_multi_r (Staging)
     // side stream and fork / join events of products whose near field runs
     // beside the far-field chain; made at the first such product that is not
     // being captured into a graph
@@ -539,6 +545,10 @@ struct hm_plan {
     // empty leaf, or a rank-0 one), and per far block its active rank (the
     // columns of its U, V and F in use; the rest, up to r, are zero)
     std::vector<int32_t> leaf_blk, far_rank;
+    // plans from points: the splits of the tree, for hm_plan_update_points
+    std::vector<DSplit> xsplits, ysplits;
+    DSplit *d_xsplits = nullptr, *d_ysplits = nullptr;
+    int32_t *d_status = nullptr;  // the plan's own status word (in d_tables)
     int max_kb = HM_MAX_KB;  // right-hand sides per forward launch (1, 2, 4 or 8)
     int tuned_kb = HM_MAX_KB, tuned_mode = -1;  // hm_plan_tune's choice
     std::vector<hipEvent_t> prof_events;  // 5 per product
@@ -572,7 +582,7 @@ struct hm_plan {
         dp.nrc = (int32_t)rcl.size(); dp.ncc = (int32_t)ccl.size();
         dp.far = d_far; dp.nfar = (int32_t)far.size();
         dp.far_by_rc = d_far_by_rc; dp.far_by_cc = d_far_by_cc;
-        dp.far_rc = d_far_rc; dp.far_cc = d_far_cc;
+        dp.tr_rc = d_tr_rc; dp.tr_cc = d_tr_cc;
         dp.dense = d_dense; dp.ndense = (int32_t)dense.size();
         dp.rch = d_rch; dp.cch = d_cch;
         dp.nrch = (int32_t)rows.ch.size(); dp.ncch = (int32_t)cols.ch.size();
@@ -581,6 +591,35 @@ struct hm_plan {
         dp.rtask = d_rtask; dp.ctask = d_ctask;
         dp.nrtask = (int32_t)rows.tasks.size(); dp.nctask = (int32_t)cols.tasks.size();
         dp.nslots_r = rows.nslots; dp.nslots_c = cols.nslots;
+        dp.multi_r = d_multi_r; dp.multi_c = d_multi_c;
+        dp.nmulti_r = (int32_t)multi_r.size(); dp.nmulti_c = (int32_t)multi_c.size();
+    }
+
+    // the clusters with more than one upward task (their slots are summed
+    // before the translation reads them)
+    void list_multi() {
+        for (size_t k = 0; k < ccl.size(); ++k)
+            if (ccl[k].nslots > 1) multi_c.push_back((int32_t)k);
+        for (size_t k = 0; k < rcl.size(); ++k)
+            if (rcl[k].nslots > 1) multi_r.push_back((int32_t)k);
+    }
+    // translation records of the far blocks in by_rc order (forward: the
+    // source is the column cluster) and in by_cc order (transposed: the row
+    // cluster)
+    void tr_records(const std::vector<int32_t> &by_rc, const std::vector<int32_t> &by_cc,
+                    std::vector<DTrRec> &tr_rc, std::vector<DTrRec> &tr_cc) const {
+        tr_rc.resize(by_rc.size());
+        tr_cc.resize(by_cc.size());
+        for (size_t e = 0; e < by_rc.size(); ++e) {
+            const DFarBlk &b = far[by_rc[e]];
+            const DCluster &C = ccl[b.cc];
+            tr_rc[e] = DTrRec{b.foff, C.pbase, b.cc, C.nslots};
+        }
+        for (size_t e = 0; e < by_cc.size(); ++e) {
+            const DFarBlk &b = far[by_cc[e]];
+            const DCluster &C = rcl[b.rc];
+            tr_cc[e] = DTrRec{b.foff, C.pbase, b.rc, C.nslots};
+        }
     }
 
     int elem_bytes() const { return f32 ? 4 : 8; }
@@ -873,6 +912,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         for (HCluster h : Bt.rcl) { h.first += ro; B.rcl.push_back(h); }
         for (HCluster h : Bt.ccl) { h.first += co; B.ccl.push_back(h); }
         for (DFarBlk f : Bt.far) { f.rc += rb; f.cc += cb; B.far.push_back(f); }
+        for (DSplit q : Bt.xsplits) { q.first += ro; q.at += ro; B.xsplits.push_back(q); }
+        for (DSplit q : Bt.ysplits) { q.first += co; q.at += co; B.ysplits.push_back(q); }
         B.nd += Bt.nd; B.nk1 += Bt.nk1; B.nk2 += Bt.nk2;
         B.far_rows += Bt.far_rows; B.far_cols += Bt.far_cols; B.dense_entries += Bt.dense_entries;
         B.maxdepth = std::max(B.maxdepth, Bt.maxdepth);
@@ -885,6 +926,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     P->nd = B.nd; P->nk1 = B.nk1; P->nk2 = B.nk2; P->maxdepth = B.maxdepth;
     P->far_rows = B.far_rows; P->far_cols = B.far_cols; P->dense_entries = B.dense_entries;
     P->leaves = std::move(B.leaves);
+    P->xsplits = std::move(B.xsplits);
+    P->ysplits = std::move(B.ysplits);
     P->row_off = std::move(row_off);
     P->col_off = std::move(col_off);
     P->dense = std::move(B.dense);
@@ -932,8 +975,7 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
 
     build_side(nx, P->dense, true, P->rcl, P->rows);
     build_side(ny, P->dense, false, P->ccl, P->cols);
-    for (size_t k = 0; k < P->ccl.size(); ++k)
-        if (P->ccl[k].nslots > 1) P->multi_c.push_back((int32_t)k);
+    P->list_multi();
     // the cluster maps the Float64 MFMA upward pass reads from its task
     // record: (a+b)/2 and 1/((b-a)/2) as the Float64 cluster maps form them
     if (!f32) {
@@ -956,9 +998,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         *out = P;
         return HM_OK;
     }
-    std::vector<DFarBlk> far_rc(by_rc.size()), far_cc(by_cc.size());
-    for (size_t e = 0; e < by_rc.size(); ++e) far_rc[e] = P->far[by_rc[e]];
-    for (size_t e = 0; e < by_cc.size(); ++e) far_cc[e] = P->far[by_cc[e]];
+    std::vector<DTrRec> tr_rc, tr_cc;
+    P->tr_records(by_rc, by_cc, tr_rc, tr_cc);
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
     {
@@ -968,8 +1009,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         st.add(&P->d_far, P->far);
         st.add(&P->d_far_by_rc, by_rc);
         st.add(&P->d_far_by_cc, by_cc);
-        st.add(&P->d_far_rc, far_rc);
-        st.add(&P->d_far_cc, far_cc);
+        st.add(&P->d_tr_rc, tr_rc);
+        st.add(&P->d_tr_cc, tr_cc);
         st.add(&P->d_dense, P->dense);
         st.add(&P->d_rch, P->rows.ch);
         st.add(&P->d_cch, P->cols.ch);
@@ -980,6 +1021,10 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         st.add(&P->d_rtask, P->rows.tasks);
         st.add(&P->d_ctask, P->cols.tasks);
         st.add(&P->d_multi_c, P->multi_c);
+        st.add(&P->d_multi_r, P->multi_r);
+        st.add(&P->d_xsplits, P->xsplits);
+        st.add(&P->d_ysplits, P->ysplits);
+        st.reserve(&P->d_status, 1);
         HM_TRY(st.commit(&P->d_tables));
     }
     P->far_by_rc_h = std::move(by_rc);
@@ -1101,6 +1146,7 @@ static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const in
     for (size_t k = 0; k < P->far.size(); ++k) by_rc[k] = by_cc[k] = (int32_t)k;
     build_side(nrows, P->dense, true, P->rcl, P->rows);
     build_side(ncols, P->dense, false, P->ccl, P->cols);
+    P->list_multi();
     // host-side tiling of the data
     std::vector<T> hA(nA, T(0)), hU(nU, T(0)), hV(nV, T(0)), hF(nF, T(0));
     {
@@ -1124,9 +1170,8 @@ static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const in
             }
         }
     }
-    std::vector<DFarBlk> far_rc(by_rc.size()), far_cc(by_cc.size());
-    for (size_t e = 0; e < by_rc.size(); ++e) far_rc[e] = P->far[by_rc[e]];
-    for (size_t e = 0; e < by_cc.size(); ++e) far_cc[e] = P->far[by_cc[e]];
+    std::vector<DTrRec> tr_rc, tr_cc;
+    P->tr_records(by_rc, by_cc, tr_rc, tr_cc);
     int rc_;
 #define HM_TRY(expr) do { rc_ = (expr); if (rc_ != HM_OK) { delete P; return rc_; } } while (0)
     {
@@ -1136,8 +1181,8 @@ static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const in
         st.add(&P->d_far, P->far);
         st.add(&P->d_far_by_rc, by_rc);
         st.add(&P->d_far_by_cc, by_cc);
-        st.add(&P->d_far_rc, far_rc);
-        st.add(&P->d_far_cc, far_cc);
+        st.add(&P->d_tr_rc, tr_rc);
+        st.add(&P->d_tr_cc, tr_cc);
         st.add(&P->d_dense, P->dense);
         st.add(&P->d_rch, P->rows.ch);
         st.add(&P->d_cch, P->cols.ch);
@@ -1147,6 +1192,8 @@ static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const in
         st.add(&P->d_cfar, P->cols.farl);
         st.add(&P->d_rtask, P->rows.tasks);
         st.add(&P->d_ctask, P->cols.tasks);
+        st.add(&P->d_multi_c, P->multi_c);
+        st.add(&P->d_multi_r, P->multi_r);
         HM_TRY(st.commit(&P->d_tables));
     }
     P->far_by_rc_h = std::move(by_rc);
@@ -1247,6 +1294,39 @@ int hm_plan_set_leaf_data(hm_plan_t P, const double *A, int64_t nA, const double
 int hm_plan_set_leaf_data_f32(hm_plan_t P, const float *A, int64_t nA, const float *F,
                               int64_t nF, void *stream) {
     return set_leaf_data_t(P, A, nA, F, nF, stream);
+}
+
+extern "C++" {
+template <class T>
+static int update_points_t(hm_plan_t P, const T *x, const T *y, int32_t *status, void *stream) {
+    if (!P || !x || !y) return fail(HM_ERR_INVALID, "update_points: NULL argument");
+    if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "update_points: a plan built from blocks has no points");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "update_points: tree-only plan");
+    if (int rc = check_type<T>(P, "update_points")) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    PlanBufs<T> &b = P->bufs<T>();
+    int32_t *st = status ? status : P->d_status;
+    HM_HIP(hipMemcpyAsync(b.x, x, sizeof(T) * (size_t)P->nx, hipMemcpyDeviceToDevice, s));
+    HM_HIP(hipMemcpyAsync(b.y, y, sizeof(T) * (size_t)P->ny, hipMemcpyDeviceToDevice, s));
+    hipError_t e = hmx_launch_check_points(b.x, P->nx, P->d_xsplits, (int64_t)P->xsplits.size(),
+                                           b.y, P->ny, P->d_ysplits, (int64_t)P->ysplits.size(),
+                                           st, s);
+    // the factors that depend on the points (dense blocks; U and V of stored
+    // plans) are formed again; F depends on the intervals alone
+    if (e == hipSuccess && !(P->flags & HM_PLAN_MATRIX_FREE)) e = hmx_launch_refit(b.dp, s);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "update_points launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+}  // extern "C++"
+
+int hm_plan_update_points(hm_plan_t P, const double *x, const double *y, int32_t *status,
+                          void *stream) {
+    return update_points_t(P, x, y, status, stream);
+}
+
+int hm_plan_update_points_f32(hm_plan_t P, const float *x, const float *y, int32_t *status,
+                              void *stream) {
+    return update_points_t(P, x, y, status, stream);
 }
 
 int hm_plan_add_col(hm_plan_t P, const double *u, int64_t ioff, int64_t j, void *stream) {
@@ -1405,7 +1485,8 @@ int hm_plan_blocks(hm_plan_t P, int64_t *ints, double *dbls, int64_t cap, int64_
 
 // right-hand sides per launch of the MFMA passes, and the default number of
 // right-hand sides from which forward products use them
-constexpr int MFMA_KB = 16;
+constexpr int MFMA_KB = 16;      // the tuner's code for the MFMA passes
+constexpr int MFMA_LAUNCH = 32;  // right-hand sides per MFMA launch (two groups of 16)
 constexpr int64_t HM_MFMA_MIN_K = 8;
 
 // A product entry point for element type T may only be called on a plan of
@@ -1517,7 +1598,7 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
         }
         if (use_m) {
             const int pm = prec >= 0 ? prec : 2;
-            for (int64_t q = 0; q < k; q += MFMA_KB) {
+            for (int64_t q = 0; q < k; q += MFMA_LAUNCH) {
                 MvArgs a = ws_args<double>(P);
                 a.y = y + ioff + q * ldy;
                 a.x = x + joff + q * ldx;
@@ -1526,9 +1607,8 @@ static int matvec_common(hm_plan_t P, T *y, const T *x, int64_t ioff, int64_t jo
                 a.ldy = ldy;
                 a.ldx = ldx;
                 a.beta = beta;
-                const int nq = (int)std::min<int64_t>(MFMA_KB, k - q);
-                hipError_t e = hmx_launch_matvec_m(P->b64.dp, a, mode, pm, nq, P->d_multi_c,
-                                                   (int)P->multi_c.size(), s);
+                const int nq = (int)std::min<int64_t>(MFMA_LAUNCH, k - q);
+                hipError_t e = hmx_launch_matvec_m(P->b64.dp, a, mode, pm, nq, s);
                 if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec launch: %s", hipGetErrorString(e));
             }
             return HM_OK;

@@ -13,7 +13,8 @@ from .barycentric import (  # noqa: F401
     BarycentricFactors, BarycentricMatrix2D, BarycentricPoly2D, evaluate, update_,
 )
 from .hierarchical import (  # noqa: F401
-    HierarchicalMatrix, LowRankMatrix, add_col_, blocks_plan, lmul_, rmul_,
+    Diagonal, HierarchicalMatrix, LowRankMatrix, UpperTriangular, add_col_, blocks_plan,
+    hierarchical, lmul_, rmul_,
 )
 from .even import EvenBarycentricLeaves, EvenBarycentricMatrix, EvenLeaf, barycentricmatrix  # noqa: F401
 from . import workloads  # noqa: F401
@@ -29,5 +30,5 @@ __all__ = [
     "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch", "rmul_", "lmul_", "add_col_",
     "EvenBarycentricMatrix", "EvenBarycentricLeaves", "barycentricmatrix",
     "BarycentricPoly2D", "BarycentricMatrix2D", "BarycentricFactors", "evaluate", "update_",
-    "blocks_plan",
+    "blocks_plan", "hierarchical", "UpperTriangular", "Diagonal",
 ]

@@ -146,6 +146,10 @@ def _load():
     L.hm_plan_profile_read.argtypes = [_vp, _dp, _ip]
     L.hm_entries.restype = ctypes.c_int
     L.hm_entries.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp]
+    L.hm_plan_update_points.restype = ctypes.c_int
+    L.hm_plan_update_points.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.hm_plan_update_points_f32.restype = ctypes.c_int
+    L.hm_plan_update_points_f32.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.hm_plan_add_col.restype = ctypes.c_int
     L.hm_plan_add_col.argtypes = [_vp, _vp, _i64, _i64, _vp]
     L.hm_plan_scale.restype = ctypes.c_int

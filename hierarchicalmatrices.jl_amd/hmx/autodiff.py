@@ -13,20 +13,76 @@ depends on x and y only through comparisons) is held fixed, the near field is
 regenerated with df/dx, and the far field is differentiated through its
 Chebyshev-form basis. The forward plan is matrix-free, so nothing is stored
 between forward and backward beyond the points and b.
+
+Plans are reused from step to step: a forward takes a plan of the same shape
+(points per side, kernel, intervals, leaf threshold, rank, element type) from
+a small pool and gives it the new points on the device
+(hm_plan_update_points), which checks there that they keep the tree; only
+when they do not (or on the first step) is a plan built from host copies of
+the points. A plan stays with its autograd context until that context is
+freed (the graph and the output that holds it are gone), so two live graphs
+never share one and a retained graph can be differentiated again.
 """
+import weakref
+
 import torch
 
 from .kernelmatrix import KernelMatrix
 
 
+class _PlanPool:
+    def __init__(self, per_key=4):
+        self.free = {}
+        self.per_key = per_key
+
+    def take(self, key):
+        lst = self.free.get(key)
+        return lst.pop() if lst else None
+
+    def give(self, key, H):
+        lst = self.free.setdefault(key, [])
+        if len(lst) < self.per_key:
+            lst.append(H)
+        else:
+            H.close()
+
+    def clear(self):
+        for lst in self.free.values():
+            for H in lst:
+                H.close()
+        self.free.clear()
+
+
+_POOL = _PlanPool()
+
+
+def _forward_plan(x, y, b, kernel, intervals, blocksize, rank):
+    key = (str(kernel), x.numel(), y.numel(), tuple(float(v) for v in intervals), blocksize,
+           rank, x.dtype)
+    H = _POOL.take(key)
+    u = None
+    if H is not None:
+        H.update_points(x, y, check=False)
+        u = H.matvec(b)
+        if not H.tree_status():  # read after the product is queued
+            H.close()
+            H = u = None
+    if H is None:
+        H = KernelMatrix(kernel, x.cpu().numpy(), y.cpu().numpy(), *intervals,
+                         blocksize=blocksize, rank=rank, matrix_free=True)
+        u = H.matvec(b)
+    return key, H, u
+
+
 class _KernelMatVec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, y, b, kernel, intervals, blocksize, rank):
-        H = KernelMatrix(kernel, x.detach().cpu().numpy(), y.detach().cpu().numpy(), *intervals,
-                         blocksize=blocksize, rank=rank, matrix_free=True)
-        b = b.detach().contiguous()
-        u = H.matvec(b)
+        x, y, b = (v.detach().contiguous() for v in (x, y, b))
+        key, H, u = _forward_plan(x, y, b, kernel, intervals, blocksize, rank)
         ctx.H = H
+        # back to the pool once the graph is gone (stream-ordered: products
+        # that reuse it run after this graph's work)
+        weakref.finalize(ctx, _POOL.give, key, H)
         ctx.save_for_backward(b)
         return u
 

@@ -97,6 +97,109 @@ def _child_size(c, k):
     return c.shape[k - 1]
 
 
+class UpperTriangular:
+    """UpperTriangular(A): the upper triangle of a Matrix or LowRankMatrix
+    (LinearAlgebra's wrapper; a leaf type of the reference's test tree
+    @hierarchical UpperTriangularHierarchicalMatrix Matrix (UpperTriangular,
+    LowRankMatrix) Diagonal, test/runtests.jl:13)."""
+
+    def __init__(self, data):
+        self.data = data
+
+    @property
+    def shape(self):
+        return self.data.shape
+
+    @property
+    def dtype(self):
+        return _leaf_dtype(self.data)
+
+    def dense(self):
+        return np.triu(_dense(self.data))
+
+    def __getitem__(self, ij):
+        i, j = ij
+        return self.data[i, j] if i <= j else 0.0
+
+
+class Diagonal:
+    """Diagonal(d) (LinearAlgebra's), a leaf type of the reference's test
+    tree."""
+
+    def __init__(self, d):
+        self.diag = np.ascontiguousarray(d)
+
+    @property
+    def shape(self):
+        return (len(self.diag), len(self.diag))
+
+    @property
+    def dtype(self):
+        return self.diag.dtype.type
+
+    def dense(self):
+        return np.diag(self.diag)
+
+    def __getitem__(self, ij):
+        i, j = ij
+        return float(self.diag[i]) if i == j else 0.0
+
+
+def _dense(c):
+    if isinstance(c, np.ndarray):
+        return c
+    if isinstance(c, LowRankMatrix):
+        return (c.U * c.S[None, :]) @ c.V.T
+    if isinstance(c, (BarycentricMatrix2D, BarycentricFactors)):
+        return c.U @ c.F @ c.V.T
+    return c.dense()
+
+
+def _leaf_matches(A, t):
+    """isinstance for a leaf type: a class, a tuple of classes, or a
+    parametric wrapper (Wrapper, Inner) such as (UpperTriangular,
+    LowRankMatrix), the reference's UpperTriangular{T, LowRankMatrix{T}}."""
+    if isinstance(t, tuple) and len(t) == 2 and t[0] in (UpperTriangular,):
+        return isinstance(A, t[0]) and isinstance(A.data, t[1])
+    if t is np.ndarray:
+        return isinstance(A, np.ndarray) and A.ndim == 2 and A.dtype in (np.float64, np.float32)
+    return isinstance(A, t)
+
+
+def hierarchical(name, *leaf_types):
+    """@hierarchical Name Types... (src/hierarchical.jl:4-172): a tree class
+    whose children are trees of the same class (assigned code 1) or leaves
+    of the given types (codes 2, 3, ... in order; anything else stays
+    unassigned). A type is a class (np.ndarray standing for Matrix), or a
+    pair (Wrapper, Inner) for the reference's parametric Wrapper{T, Inner{T}}.
+    The fields <Type>blocks (M x N object arrays) mirror the reference's;
+    products run on the device like HierarchicalMatrix's, with leaves that are
+    neither Matrix nor LowRankMatrix formed densely."""
+    types = tuple(leaf_types)
+
+    def tname(t):
+        t = t[0] if isinstance(t, tuple) else t
+        return "Matrix" if t is np.ndarray else t.__name__
+
+    field_types = {f"{name}blocks": name}
+    field_types.update({f"{tname(t)}blocks": t for t in types})
+
+    def blocks_field(self, code):
+        M, N = self.assigned.shape
+        out = np.empty((M, N), dtype=object)
+        for m in range(M):
+            for n in range(N):
+                out[m, n] = self.blocks[m][n] if self.assigned[m, n] == code else None
+        return out
+
+    ns = {"leaf_types": types, "field_types": field_types,
+          "__doc__": f"@hierarchical {name} " + " ".join(tname(t) for t in types)}
+    cls = type(name, (HierarchicalMatrix,), ns)
+    for code, f in enumerate(field_types, start=1):
+        setattr(cls, f, property(lambda self, c=code: blocks_field(self, c)))
+    return cls
+
+
 class HierarchicalMatrix:
     """An M x N block tree whose children are HierarchicalMatrix,
     LowRankMatrix or dense (2-D float64 numpy) blocks."""
@@ -123,20 +226,28 @@ class HierarchicalMatrix:
     def shape(self):
         return self.size()
 
+    # the leaf types after the tree itself, in code order (2, 3, ...): the
+    # fields of @hierarchical HierarchicalMatrix LowRankMatrix Matrix
+    leaf_types = ((LowRankMatrix, BarycentricMatrix2D, BarycentricFactors), np.ndarray)
+
+    def _code_of(self, A):
+        """The `assigned` code of a child: 1 for a tree of this type, 2 + l
+        for the l-th leaf type, 0 (unassigned) for anything else."""
+        if type(A) is type(self):
+            return 1
+        for l, t in enumerate(self.leaf_types):
+            if _leaf_matches(A, t):
+                return 2 + l
+        return 0
+
     def __setitem__(self, key, A):
         b1, b2 = key
         m, n = (b1.K if isinstance(b1, Block) else int(b1)), (b2.K if isinstance(b2, Block) else int(b2))
-        if isinstance(A, HierarchicalMatrix):
-            code = 1
-        elif isinstance(A, LowRankMatrix):
-            code = 2
-        elif isinstance(A, (BarycentricMatrix2D, BarycentricFactors)):
-            code = 2  # a far leaf: the LowRank slot of the reference's tree types
-        elif isinstance(A, np.ndarray) and A.ndim == 2 and A.dtype in (np.float64, np.float32):
-            code = 3
-            A = np.asfortranarray(A)
-        else:  # no matching field: the reference leaves the block unassigned
+        code = self._code_of(A)
+        if code == 0:  # no matching field: the reference leaves the block unassigned
             return
+        if isinstance(A, np.ndarray):
+            A = np.asfortranarray(A)
         self.blocks[m - 1][n - 1] = A
         self.assigned[m - 1, n - 1] = code
         self._plan = None
@@ -258,9 +369,9 @@ class _BlocksPlan(KernelMatrix):
                 put(c.U, U, t)
                 put(c.F, S, t)
                 put(c.V, V, t)
-            else:
+            else:  # Matrix, or another leaf type formed densely
                 kind[t] = 3
-                put(c, A, t)
+                put(_dense(c), A, t)
         IP = ctypes.POINTER(ctypes.c_int64)
         h = ctypes.c_void_p()
         ctor = lib.hm_plan_from_blocks_f32 if dt == np.float32 else lib.hm_plan_from_blocks
@@ -280,7 +391,7 @@ class _BlocksPlan(KernelMatrix):
 def _leaf_dtype(c):
     if isinstance(c, (LowRankMatrix, BarycentricMatrix2D, BarycentricFactors)):
         return np.dtype(c.U.dtype).type
-    return np.dtype(c.dtype).type
+    return np.dtype(c.dtype).type if isinstance(c, np.ndarray) else c.dtype
 
 
 class _FlatLeaves:

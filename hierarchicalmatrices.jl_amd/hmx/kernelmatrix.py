@@ -366,6 +366,31 @@ class KernelMatrix:
             return out.cpu().numpy()
         return out
 
+    def update_points(self, x, y, check=True, stream=None):
+        """New point values x, y (device tensors of the plan's element type,
+        non-increasing) for the same tree (hm_plan_update_points): the dense
+        blocks and stored bases are formed again on the device, F stays. The
+        device checks that every split of the tree falls at the same index;
+        with check=True the answer is read back (one small synchronisation)
+        and returned, with check=False it stays on the device (read it later
+        through tree_status()), so the call can be captured into a graph."""
+        torch = _torch()
+        m, n = self.shape
+        x = self._grad_vec(x, m, "x")
+        y = self._grad_vec(y, n, "y")
+        if x.stride(0) != 1 or y.stride(0) != 1:
+            raise ValueError("x and y must be contiguous")
+        if getattr(self, "_status", None) is None:
+            self._status = torch.zeros(1, dtype=torch.int32, device=x.device)
+        fn = lib.hm_plan_update_points_f32 if self.f32 else lib.hm_plan_update_points
+        _lib.check(fn(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                      ctypes.c_void_p(self._status.data_ptr()), _stream_ptr(stream)))
+        return self.tree_status() if check else None
+
+    def tree_status(self):
+        """True when the points of the last update_points keep the tree."""
+        return int(self._status.item()) == 0
+
     def _grad_vec(self, t, n, name):
         """A 1-D device tensor of the plan's element type with at least n
         entries at its stride (the gradient entry points read n of them)."""

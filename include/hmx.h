@@ -72,9 +72,10 @@ enum hm_mv_flags {
     HM_MV_FAR_BF16X3 = 16,       /* far field on BF16 MFMA with operands split hi + lo
                                     (hi*hi + hi*lo + lo*hi), FP32 accumulate: operands
                                     carry 16 bits */
-    HM_MV_MFMA = 32,             /* forward: 16 right-hand sides per launch on the FP64
-                                    MFMA passes (far field in Chebyshev form); the
-                                    default when k >= 8 */
+    HM_MV_MFMA = 32,             /* forward: the MFMA passes, up to 32 right-hand sides
+                                    per launch in two groups of 16 that share every
+                                    generated or streamed operand (far field in
+                                    Chebyshev form); the default when k >= 8 */
     HM_MV_NO_MFMA = 64,          /* forward: never the MFMA passes (up to 8 right-hand
                                     sides per launch on the vector units) */
     HM_MV_FAR_BF16X6 = 128       /* far field on BF16 MFMA with operands split in three,
@@ -181,6 +182,25 @@ int hm_plan_from_blocks_f32(int64_t nrows, int64_t ncols, int64_t nleaves, const
  * defines no scale! for the KernelMatrix leaves (HM_ERR_UNSUPPORTED). A
  * barycentric leaf (kind 4) scales its U or V rows as a low-rank leaf does. */
 int hm_plan_scale(hm_plan_t plan, int side, const double *b, int64_t off, void *stream);
+
+/* New point values for a plan of hm_plan_from_points (Float64) or its _f32
+ * twin (no counterpart in the reference, which rebuilds KernelMatrix(f, x, y,
+ * ...) for other points): x (nx) and y (ny) are DEVICE arrays of the plan's
+ * element type. The block tree depends on the points only through the
+ * comparisons of indsplit (src/BarycentricMatrix.jl:299-307); the plan keeps
+ * its tree and checks on the device that the new points are finite,
+ * non-increasing and fall on the same side of every split, writing 0 to the
+ * DEVICE int *status when they do and 1 when they do not (status NULL: a
+ * word inside the plan). With 1 the plan's products are not those of the
+ * KernelMatrix of the new points: rebuild it. Dense blocks and the stored
+ * bases U, V are formed again from the new points; F (which depends on the
+ * intervals only) stays. Stream-ordered: no synchronization, no allocation,
+ * and no memset (the status word is cleared by a kernel), so the call can be
+ * captured into a graph. */
+int hm_plan_update_points(hm_plan_t plan, const double *x, const double *y, int32_t *status,
+                          void *stream);
+int hm_plan_update_points_f32(hm_plan_t plan, const float *x, const float *y, int32_t *status,
+                              void *stream);
 
 /* add_col!(H, u, istart, j) (src/HierarchicalMatrix.jl:104-146 with the leaf
  * forms of src/algebra.jl:324-380) in place on the device, for a Float64 plan

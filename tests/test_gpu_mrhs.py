@@ -26,7 +26,7 @@ def rhs(seed, n, k):
 
 @pytest.mark.parametrize("kernel", list(KIDS))
 @pytest.mark.parametrize("mode", ["stored", "hybrid", "free"])
-@pytest.mark.parametrize("k", [16, 20])
+@pytest.mark.parametrize("k", [16, 20, 32, 45])
 def test_mfma_vs_oracle_and_vector_units(cuda, kernel, mode, k):
     w = hmx.workloads.config_plain(n=4096, blocksize=64, rank=16)
     print(f"case: mfma kernel={kernel} mode={mode} k={k}", flush=True)
@@ -38,7 +38,8 @@ def test_mfma_vs_oracle_and_vector_units(cuda, kernel, mode, k):
     Yv = H.matvec(Bt, mfma=False).cpu().numpy()
     assert rel(Ym, Yv) <= 1e-13
     t = O.Tree(KIDS[kernel], w["x"], w["y"], *w["intervals"], 64, 16)
-    for q in (0, 7, 15, k - 1):
+    # launches of 32 right-hand sides (two groups of 16) and the rest
+    for q in sorted({0, 7, 15, min(16, k - 1), min(31, k - 1), k - 1}):
         ref = t.matvec(np.ascontiguousarray(B[:, q]))
         assert rel(Ym[:, q], ref) <= 1e-12, (q, rel(Ym[:, q], ref))
 
@@ -48,10 +49,10 @@ def test_mfma_rank_buckets(cuda, rank, bs):
     x, y, iv = hmx.workloads.sorted_uniform_points(12000, 2)
     print(f"case: mfma rank={rank}", flush=True)
     H = hmx.KernelMatrix("cauchykernel", x, y, *iv, blocksize=bs, rank=rank, hybrid=True)
-    B = rhs(4, 12000, 16)
+    B = rhs(4, 12000, 32)
     Ym = H.matvec(torch.from_numpy(B).to(cuda), mfma=True).cpu().numpy()
     t = O.Tree(O.CAUCHY, x, y, *iv, bs, rank)
-    for q in (0, 9, 15):
+    for q in (0, 9, 15, 16, 31):
         ref = t.matvec(np.ascontiguousarray(B[:, q]))
         assert rel(Ym[:, q], ref) <= 1e-12, (q, rel(Ym[:, q], ref))
 
