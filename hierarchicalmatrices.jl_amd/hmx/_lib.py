@@ -16,6 +16,7 @@ HM_ERR_DEGENERATE = 3
 HM_ERR_OOM = 4
 HM_ERR_HIP = 5
 HM_ERR_UNSUPPORTED = 6
+HM_DENSE_NOALLOC_MAX = 8192  # items per hm_dense_gemm_noalloc call
 
 HM_USER = 4  # kernel id of a host function f (hm_plan_set_leaf_data)
 
@@ -221,6 +222,8 @@ def _load():
     L.hm_dense_gemm.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _ip, ctypes.c_double,
                                 ctypes.POINTER(_vp), _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 _ip, ctypes.c_double, ctypes.POINTER(_vp), _ip, _vp]
+    L.hm_dense_gemm_noalloc.restype = ctypes.c_int
+    L.hm_dense_gemm_noalloc.argtypes = L.hm_dense_gemm.argtypes
     return L
 
 

@@ -525,6 +525,20 @@ int hm_dense_gemm(int transa, int transb, int64_t count, const int64_t *mnk, dou
                   const double *const *B, const int64_t *ldb, double beta, double *const *C,
                   const int64_t *ldc, void *stream);
 
+/* hm_dense_gemm without allocating: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, checks and result
+ * as hm_dense_gemm. The item table goes to a fixed device array of
+ * HM_DENSE_NOALLOC_MAX items (each GPU holds its own copy until the process
+ * ends); a larger count returns HM_ERR_UNSUPPORTED. Every call on one GPU
+ * shares that array, which keeps the last call's items: calls that overlap
+ * on one GPU (other host threads, other streams) race. */
+#define HM_DENSE_NOALLOC_MAX 8192
+int hm_dense_gemm_noalloc(int transa, int transb, int64_t count, const int64_t *mnk, double alpha,
+                          const double *const *A, const int64_t *lda, const double *const *D,
+                          const double *const *B, const int64_t *ldb, double beta,
+                          double *const *C, const int64_t *ldc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
