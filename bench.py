@@ -25,6 +25,8 @@ N = 1 << 20
 BS = 256
 RANK = 24
 HBM_PEAK_GBS = 8000.0
+# FP64 vector issue: 1024 SIMDs x 2.4 GHz, one wave64 FP64 instruction per SIMD per 4 cycles
+VALU_F64_PEAK_GINSTR = 614.4
 
 
 def parse():
@@ -97,6 +99,19 @@ def pmc_traffic():
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch k_down_stored:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_valu():
+    """FP64 VALU wave instructions per product of the matrix-free downward
+    pass, from the committed rocprofv3 SQ counter summary; None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_down_matrix_free_n2p20_valu_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("valu_f64_instr_per_product k_down_free:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -941,6 +956,20 @@ def main():
                 up_pass=dict(achieved=round(bytes_up / (passes[0] * 1e-3) / 1e9, 1),
                              avg_ms=round(float(passes[0]), 4)))
 
+    def valu_roof(down_ms):
+        """The matrix-free downward pass against the FP64 vector-unit issue
+        rate: FP64 VALU wave instructions per product (committed PMC counts)
+        over the live pass time."""
+        instr = pmc_valu()
+        if instr is None or not down_ms:
+            return None
+        ach = instr / (float(down_ms) * 1e-3) / 1e9
+        return dict(bound="valu", kernel="k_down matrix-free (near field regenerated + far field)",
+                    achieved=round(ach, 1), peak=VALU_F64_PEAK_GINSTR,
+                    unit="G FP64 wave-instructions/s", frac=round(ach / VALU_F64_PEAK_GINSTR, 4),
+                    instructions_per_product=instr, avg_ms=round(float(down_ms), 4),
+                    source="profiles/k_down_matrix_free_n2p20_valu_counters.md (rocprofv3 --pmc)")
+
     workloads = None
     if not args.no_workloads:
         wsteps, wwarm = max(3, min(args.steps, 10)), max(1, min(args.warmup, 2))
@@ -972,6 +1001,9 @@ def main():
         "bytes_per_matvec": info["bytes_stored"],
         "bytes_ref_layout": info["bytes_ref_layout"],
         "pass_ms": {k: round(float(v), 4) for k, v in zip(names, passes)},
+        "pass_ms_method": ("each pass timed with HIP events around its own launches, the "
+                           "passes back to back on one stream (no side-stream overlap); "
+                           "ms_per_step is the whole product as run"),
         "rel_err_vs_dense_64rows": err_dense,
         "build_s": round(build_s, 3),
         "variants": {k: {"value": (round(1000.0 / v["ms"], 3) if not v["nonfinite"] else None),
@@ -983,7 +1015,9 @@ def main():
                          "hbm_bytes_per_matvec": v["bytes"],
                          "pass_ms": {q: round(float(t), 4) for q, t in zip(names, v["passes"])},
                          "rel_diff_vs_stored": v["err"],
-                         "nonfinite_factors": v["nonfinite"]} for k, v in variants.items()},
+                         "nonfinite_factors": v["nonfinite"],
+                         **({"valu_roofline": valu_roof(v["passes"][3])}
+                            if k == "matrix_free" else {})} for k, v in variants.items()},
         "roofline": roof,
         "workloads": workloads,
         "measured_hbm_peaks": peaks,

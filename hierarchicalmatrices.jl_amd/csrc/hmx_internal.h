@@ -105,6 +105,22 @@ struct DTrRec {
     int32_t nslots;
 };
 
+// A node of one side's interval tree: a range that the tree rule splits (or
+// a half it leaves whole), with its two halves. The nested far field of the
+// MFMA passes sums Chebyshev moments from the halves into their parent and
+// pushes expansion coefficients from a parent into its halves, one fixed
+// re-expansion per half, so each point enters one leaf's moments and each row
+// evaluates one leaf's series.
+struct DTNode {
+    int64_t first;
+    int32_t len;
+    int32_t parent;    // -1 at a root
+    int32_t child[2];  // -1 at a leaf
+    int32_t cl;        // the row / column cluster with this range, or -1
+    int32_t half;      // which half of its parent (0: the half holding the upper interval end)
+    double cx, cy;     // (center, 1/halfwidth) of its interval, as the cluster maps form them
+};
+
 // One indsplit of the tree: the range [first, first + len) of one side split
 // at index `at` by the midpoint `mid` (points at or above it first). For other
 // points the split falls at the same index exactly when p[at - 1] >= mid (if

@@ -1216,6 +1216,58 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
+// Results of the FP64 shape held as Mo[gi][t][v] = X[16t + g + 4v][16 gi + j16]
+// (rank index m = 16t + g + 4v, right-hand side q = 16 gi + j16), stored as
+// dst[q r + m], the layout of partial slots, w, z, moments and coefficients.
+template <int RP, int G>
+__device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], double *dst, int r,
+                                            int j16, int g) {
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        double *dq = dst + (int64_t)(gi * MQ + j16) * r;
+#pragma unroll
+        for (int t = 0; t < RP / 16; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = t * 16 + g + 4 * v;
+                if (m < r) dq[m] = Mo[gi][t][v];
+            }
+    }
+}
+
+// Node values w = A mu of one cluster from its moments held as in
+// store_ranks: Mo[gi][t][v] is B[k0 + g][j16] of the K-step k0 = 16t + 4v, so
+// the product takes mu from registers; A's rows are loaded once for the
+// groups. Stored as dst[q r + m].
+template <int RP, int G>
+__device__ __forceinline__ void nodes_from_moments(const md4_t (&Mo)[G][RP / 16],
+                                                   const double *chebA, double *dst, int r,
+                                                   int j16, int g) {
+    constexpr int MT = RP / 16;
+    double av[MT][MT][4];
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                av[tt][t][v] = (m < r && k < r) ? chebA[m * r + k] : 0.0;
+            }
+    md4_t W[G][MT];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int tt = 0; tt < MT; ++tt) {
+            W[gi][tt] = md4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], Mo[gi][t][v], W[gi][tt]);
+        }
+    store_ranks<RP, G>(W, dst, r, j16, g);
+}
+
 // xt[j][q] = x[j*incx + q*ldx] for q < nq, 0 beyond (q < 16 G): the
 // right-hand sides of one launch packed row by row, so every MFMA B operand
 // of the upward and downward passes is a coalesced load.
@@ -1237,17 +1289,19 @@ __global__ __launch_bounds__(WG) void k_pack_x(const double *x, int64_t incx, in
 // mu[m][q] = sum_j T_m(xh_j) x[j, q] of the task's points. The wavefront forms
 // T for PTS points (one per lane, FP64 recurrence) in LDS, then contracts over
 // the points on MFMA, each T operand feeding the G groups.
-template <int RP, int PREC, int G>
+// NEST: the tasks are the column tree's leaves (task.cl the node, task.slot
+// its column cluster or -1); the moments go to mom[node] and, for a leaf that
+// is a cluster, w = A mu to w[cluster].
+template <int RP, int PREC, int G, bool NEST>
 __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double *pts,
                                               const double *xt, int64_t ldxt, int nq,
                                               const double *chebA, double *partial, double *w,
-                                              int r) {
+                                              int r, double *mom) {
     using TL = typename std::conditional<PREC == 2, double, float>::type;
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int MT = RP / 16;
     constexpr int MQT = MQ * G;
     constexpr int PTS = (PREC == 2 && RP > 32) ? 32 : 64;  // points per step
-    static_assert(RP * (PTS + 1) * sizeof(TL) >= RP * MQT * sizeof(double), "mu fits in Tl");
     __shared__ __align__(16) TL Tl[RP][PTS + 1];
     const DTask Tk = tasks[blockIdx.x];
     const int lane = threadIdx.x;
@@ -1271,7 +1325,10 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
 #pragma unroll
             for (int gi = 0; gi < G; ++gi) {
                 const int q = gi * MQ + j16;
-                bx[gi][u] = (i < Tk.len && q < nq) ? xt[(first + i) * ldxt + q] : 0.0;
+                // in bounds for every lane, issued unconditionally, zeroed after
+                const bool ok = i < Tk.len && q < nq;
+                const double v = xt[(first + (ok ? i : 0)) * ldxt + (ok ? q : 0)];
+                bx[gi][u] = ok ? v : 0.0;
             }
         }
         if (lane < PTS) {
@@ -1345,29 +1402,142 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
         }
         wave_sync();
     }
-    // mu[m][q] through LDS (Tl is free again); register v holds rank g + 4v
-    double (*mu)[MQT] = reinterpret_cast<double (*)[MQT]>(&Tl[0][0]);
+    // D64 holds mu as store_ranks and nodes_from_moments read it
+    if constexpr (NEST) {
+        store_ranks<RP, G>(D64, mom + (int64_t)Tk.cl * MQT * r, r, j16, g);
+        if (Tk.slot >= 0)
+            nodes_from_moments<RP, G>(D64, chebA, w + Tk.slot * MQT * r, r, j16, g);
+    } else if (Tk.single) {
+        nodes_from_moments<RP, G>(D64, chebA, w + (int64_t)Tk.cl * MQT * r, r, j16, g);
+    } else {  // a multi-task cluster's slot gets mu itself
+        store_ranks<RP, G>(D64, partial + Tk.slot * MQT * r, r, j16, g);
+    }
+}
+
+// Nested far field, upward: the moments of each internal column node of one
+// depth from its two halves' (mu_P = M_0 mu_0 + M_1 mu_1, the halves' series
+// re-expanded on the parent's interval), one wavefront per node; a node that
+// is a column cluster also gets w = A mu.
+template <int RP, int G>
+__global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids, int n,
+                                           const double *M, double *mom, const double *chebA,
+                                           double *w, int r) {
+    constexpr int MT = RP / 16;
+    constexpr int MQT = MQ * G;
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= n) return;
+    const int j16 = lane % 16, g = lane / 16;
+    const int P = ids[wid];
+    const DTNode N = tn[P];
+    md4_t W[G][MT];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int t = 0; t < MT; ++t) W[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int C = N.child[h];
+        if (C < 0 || tn[C].len == 0) continue;  // wave-uniform
+        const double *Mh = M + (int64_t)h * r * r;
+        const double *src = mom + (int64_t)C * MQT * r;
+        // A[m][k] = M_h[m][k], B[k][q] = mu_C[k][q]; loads in bounds, the
+        // terms past r zeroed after
+        double av[MT][MT][4], bv[G][MT][4];
+#pragma unroll
+        for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                    const double a = Mh[min(m, r - 1) * r + min(k, r - 1)];
+                    av[tt][t][v] = (m < r && k < r) ? a : 0.0;
+                }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = t * 16 + 4 * v + g;
+                    const double b = src[(int64_t)(gi * MQ + j16) * r + min(k, r - 1)];
+                    bv[gi][t][v] = k < r ? b : 0.0;
+                }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], bv[gi][t][v], W[gi][tt]);
+    }
+    store_ranks<RP, G>(W, mom + (int64_t)P * MQT * r, r, j16, g);
+    if (N.cl >= 0) nodes_from_moments<RP, G>(W, chebA, w + (int64_t)N.cl * MQT * r, r, j16, g);
+}
+
+// Nested far field, downward: the coefficients of each row node of one depth,
+// c_N = z[its row cluster] (if it is one) + M_h^T c_parent (the parent's
+// series re-expanded on this half's interval), one wavefront per node.
+template <int RP, int G>
+__global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids, int n,
+                                           const double *M, const double *z, double *cof, int r) {
+    constexpr int MT = RP / 16;
+    constexpr int MQT = MQ * G;
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= n) return;
+    const int j16 = lane % 16, g = lane / 16;
+    const int id = ids[wid];
+    const DTNode N = tn[id];
+    md4_t W[G][MT];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
 #pragma unroll
         for (int t = 0; t < MT; ++t)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) mu[t * 16 + g + 4 * v][gi * MQ + j16] = D64[gi][t][v];
-    wave_sync();
-    if (Tk.single) {
-        // single task: w = A mu (the reduce's Chebyshev step, same order)
-        for (int idx = lane; idx < r * MQT; idx += WAVE) {
-            const int q = idx / r, m = idx - q * r;
-            double v = 0.0;
-            for (int k = 0; k < r; ++k) v = __builtin_fma(chebA[m * r + k], mu[k][q], v);
-            w[((int64_t)Tk.cl * MQT + q) * r + m] = v;
-        }
-    } else {
-        for (int idx = lane; idx < r * MQT; idx += WAVE) {
-            const int q = idx / r, m = idx - q * r;
-            partial[(Tk.slot * MQT + q) * r + m] = mu[m][q];
-        }
+            for (int v = 0; v < 4; ++v) {
+                const int m = t * 16 + g + 4 * v;
+                double c = 0.0;
+                if (N.cl >= 0 && m < r) c = z[((int64_t)N.cl * MQT + gi * MQ + j16) * r + m];
+                W[gi][t][v] = c;
+            }
+    if (N.parent >= 0) {  // wave-uniform
+        const double *Mh = M + (int64_t)N.half * r * r;
+        const double *src = cof + (int64_t)N.parent * MQT * r;
+        // A[m][k] = M_h[k][m], B[k][q] = c_P[k][q]
+        double av[MT][MT][4], bv[G][MT][4];
+#pragma unroll
+        for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                    const double a = Mh[min(k, r - 1) * r + min(m, r - 1)];
+                    av[tt][t][v] = (m < r && k < r) ? a : 0.0;
+                }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = t * 16 + 4 * v + g;
+                    const double b = src[(int64_t)(gi * MQ + j16) * r + min(k, r - 1)];
+                    bv[gi][t][v] = k < r ? b : 0.0;
+                }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int tt = 0; tt < MT; ++tt)
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], bv[gi][t][v], W[gi][tt]);
     }
+    store_ranks<RP, G>(W, cof + (int64_t)id * MQT * r, r, j16, g);
 }
 
 // Translation, one wavefront per row cluster: Z = sum over the cluster's far
@@ -1410,24 +1580,28 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
             const double *ty = p.cnodes + (int64_t)cc * r;
 #pragma unroll
             for (int k0 = 0; k0 < RP; k0 += 16) {
-                // four K-steps: operands first, then the products
+                // four K-steps: operands first (every load in bounds and issued
+                // unconditionally, the terms past r zeroed after), then the products
                 double bv[G][4], av[4][MT];
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     const int l = k0 + 4 * h + g;
                     const bool lok = l < r;
+                    const int lc = min(l, r - 1);
 #pragma unroll
-                    for (int gi = 0; gi < G; ++gi) bv[gi][h] = lok ? ws[(gi * MQ + j16) * r + l] : 0.0;
-                    double tyl = 0.0;
-                    if constexpr (FREE) tyl = lok ? ty[l] : 0.0;
+                    for (int gi = 0; gi < G; ++gi) {
+                        const double v = ws[(gi * MQ + j16) * r + lc];
+                        bv[gi][h] = lok ? v : 0.0;
+                    }
+                    const double tyl = FREE ? ty[lc] : 0.0;
 #pragma unroll
                     for (int t = 0; t < MT; ++t) {
                         const int m = t * 16 + j16;
-                        av[h][t] = 0.0;
-                        if (lok && m < r) {
-                            if constexpr (FREE) av[h][t] = kern_fast<double, KID>(tx[m], tyl);
-                            else av[h][t] = F[m + (int64_t)r * l];
-                        }
+                        const bool ok = lok && m < r;
+                        double v;
+                        if constexpr (FREE) v = kern_fast<double, KID>(tx[min(m, r - 1)], tyl);
+                        else v = F[min(m, r - 1) + (int64_t)r * lc];
+                        av[h][t] = ok ? v : 0.0;
                     }
                 }
 #pragma unroll
@@ -1476,7 +1650,9 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 // T_k(xh) of the lane's rows by the FP64 recurrence, times the cluster's
 // coefficients C[k][q]. The 64 x 16G result goes through LDS to a
 // row-per-lane layout and is written once.
-template <int RP, int PREC, bool FREE, int KID, int G>
+// NEST: the far field of a chunk is its row leaf's series alone (the
+// coefficients of every covering cluster pushed down the row tree into p.mx).
+template <int RP, int PREC, bool FREE, int KID, int G, bool NEST>
 __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
                                               const DNear *nl, const DFar *fl,
                                               const double2 *cmapv, const double *pts_out,
@@ -1568,11 +1744,12 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     }
     // ---- far field: the covering clusters' records one per lane; the
     // coefficients of the next cluster load while this one is applied
+    const double *zs = NEST ? p.mx : z;
     auto load_bz = [&](double (*bz)[KSEL], int cl) {
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
             const int q = gi * MQ + j16;
-            const double *zc = z + ((int64_t)cl * MQT + q) * r;  // C[k][q]
+            const double *zc = zs + ((int64_t)cl * MQT + q) * r;  // C[k][q]
 #pragma unroll
             for (int u = 0; u < KSEL; ++u) {
                 const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
@@ -1580,15 +1757,23 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
             }
         }
     };
-    for (int e0 = c.far_beg; e0 < c.far_end; e0 += WAVE) {
-        const int ne = min(WAVE, c.far_end - e0);
+    const int fbeg = NEST ? 0 : c.far_beg, fend = NEST ? 1 : c.far_end;
+    for (int e0 = fbeg; e0 < fend; e0 += WAVE) {
+        const int ne = min(WAVE, fend - e0);
         int f_cl = 0;
         double f_cx = 0.0, f_cy = 0.0;
         if (lane < ne) {
-            f_cl = fl[e0 + lane].cl;
-            const double2 cmv = cmapv[f_cl];
-            f_cx = cmv.x;
-            f_cy = cmv.y;
+            if constexpr (NEST) {
+                f_cl = p.chunk_leaf[wid];
+                const DTNode L = p.xtn[f_cl];
+                f_cx = L.cx;
+                f_cy = L.cy;
+            } else {
+                f_cl = fl[e0 + lane].cl;
+                const double2 cmv = cmapv[f_cl];
+                f_cx = cmv.x;
+                f_cy = cmv.y;
+            }
         }
         double bz[G][KSEL], bzn[G][KSEL] = {};
         load_bz(bz, rl_i(f_cl, 0));
@@ -1708,7 +1893,11 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     }
 }
 
-// MODE 1: stored dense blocks (stored and hybrid plans); MODE 2: matrix-free
+// MODE 1: stored dense blocks (stored and hybrid plans); MODE 2: matrix-free.
+// With the plan's interval trees (p.nest) the far field is nested: column
+// leaves' moments, combined up the column tree, translated, and the
+// coefficients pushed down the row tree to each chunk's leaf; else every
+// covering cluster is applied per level.
 template <int RP, int PREC, int MODE, int KID, int G>
 hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) {
     constexpr bool FREE = MODE == 2;
@@ -1723,18 +1912,42 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
         xr = a.xt;
         ldxr = MQT;
     }
-    if (p.nfar > 0 && p.nctask > 0) {
-        k_up_m<RP, PREC, G><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
-                                                       a.partial, a.w, p.r);
+    const bool nest = p.nest && p.nfar > 0;
+    if (p.nfar > 0 && nest) {
+        if (p.nyleaf > 0)
+            k_up_m<RP, PREC, G, true><<<p.nyleaf, WAVE, 0, s>>>(p.yleaf, p.y, xr, ldxr, nq, p.chebA,
+                                                                a.partial, a.w, p.r, p.my);
+        for (int d = p.nylev - 1; d >= 0; --d) {
+            const int off = p.ylev_off_h[d], cnt = p.ylev_off_h[d + 1] - off;
+            if (cnt > 0)
+                k_m2m<RP, G><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
+                                                       p.chebA, a.w, p.r);
+        }
+        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc,
+                                                                   a.w, a.z);
+        for (int d = 0; d < p.nxlev; ++d) {
+            const int off = p.xlev_off_h[d], cnt = p.xlev_off_h[d + 1] - off;
+            if (cnt > 0)
+                k_l2l<RP, G><<<nblk(cnt), WG, 0, s>>>(p.xtn, p.xlev + off, cnt, p.nestM, a.z, p.mx,
+                                                       p.r);
+        }
+    } else if (p.nfar > 0 && p.nctask > 0) {
+        k_up_m<RP, PREC, G, false><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
+                                                              a.partial, a.w, p.r, nullptr);
         if (p.nmulti_c > 0)
             k_reduce<double, true, MQT><<<nblk(p.nmulti_c), WG, 0, s>>>(
                 p.ccl, p.multi_c, p.nmulti_c, a.partial, p.chebA, a.w, p.r);
         k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc,
                                                                    a.w, a.z);
     }
-    if (p.nrch > 0)
-        k_down_m<RP, PREC, FREE, KID, G><<<nblk(p.nrch), WG, 0, s>>>(
-            p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
+    if (p.nrch > 0) {
+        if (nest)
+            k_down_m<RP, PREC, FREE, KID, G, true><<<nblk(p.nrch), WG, 0, s>>>(
+                p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
+        else
+            k_down_m<RP, PREC, FREE, KID, G, false><<<nblk(p.nrch), WG, 0, s>>>(
+                p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
+    }
     return hipGetLastError();
 }
 

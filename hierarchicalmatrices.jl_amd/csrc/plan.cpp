@@ -111,6 +111,54 @@ struct Builder {
     int maxdepth = 0;
     int err = 0;
     std::string msg;
+    // the interval trees of both sides (every range node() splits, with its
+    // two halves); tok turns false if a lookup meets an inconsistent node, and
+    // the plan then keeps the per-level far field
+    struct TNode {
+        int depth;
+        int64_t first, len;
+        double lo, hi;
+        int32_t parent, child[2], half;
+        int32_t cl;  // the row / column cluster with this range, or -1
+    };
+    std::vector<TNode> xt, yt;
+    std::unordered_map<uint64_t, int32_t> xtm, ytm;
+    bool tok = true;
+
+    static uint64_t tkey(int depth, double lo, double hi) {
+        uint64_t a, b;
+        std::memcpy(&a, &lo, 8);
+        std::memcpy(&b, &hi, 8);
+        uint64_t h = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull + (h_mix(a) << 6));
+        return h_mix(h ^ (uint64_t)depth * 0xD1B54A32D192ED03ull);
+    }
+    static uint64_t h_mix(uint64_t v) {
+        v ^= v >> 30; v *= 0xBF58476D1CE4E5B9ull;
+        v ^= v >> 27; v *= 0x94D049BB133111EBull;
+        return v ^ (v >> 31);
+    }
+    int32_t tfind(const std::vector<TNode> &t, const std::unordered_map<uint64_t, int32_t> &m,
+                  int depth, double lo, double hi) const {
+        auto it = m.find(tkey(depth, lo, hi));
+        if (it == m.end()) return -1;
+        const TNode &q = t[it->second];
+        return (q.depth == depth && q.lo == lo && q.hi == hi) ? it->second : -2;
+    }
+    // the node (depth, interval), made on first sight as half `half` of parent
+    void tadd(std::vector<TNode> &t, std::unordered_map<uint64_t, int32_t> &m, int depth,
+              int64_t first, int64_t len, double lo, double hi, int32_t parent, int half) {
+        const int32_t f = tfind(t, m, depth, lo, hi);
+        if (f == -2) { tok = false; return; }
+        if (f >= 0) {
+            const TNode &q = t[f];
+            if (q.first != first || q.len != len || q.parent != parent) tok = false;
+            return;
+        }
+        const int32_t id = (int32_t)t.size();
+        t.push_back({depth, first, len, lo, hi, parent, {-1, -1}, half, -1});
+        m.emplace(tkey(depth, lo, hi), id);
+        if (parent >= 0) t[parent].child[half] = id;
+    }
 
     // indsplit (src/BarycentricMatrix.jl:299-307) on a validated non-increasing
     // range: the linear scan stops at the first point below the midpoint, which
@@ -174,6 +222,15 @@ struct Builder {
             const int32_t rc = cluster(rcl, rmap, depth, i0, m, a, b);
             const int32_t cc = cluster(ccl, cmap, depth, j0, n, c, d);
             far.push_back({rc, cc, 0});
+            if (tok) {
+                const int32_t xn = tfind(xt, xtm, depth, a, b), yn = tfind(yt, ytm, depth, c, d);
+                if (xn < 0 || yn < 0) {
+                    tok = false;
+                } else {
+                    xt[xn].cl = rc;
+                    yt[yn].cl = cc;
+                }
+            }
             far_depth.push_back(depth);
         }
     }
@@ -195,6 +252,17 @@ struct Builder {
         ysplits.push_back({j0, nj, j0 + nj1, mid(c, d)});
         const int64_t i2 = i0 + ni1, j2 = j0 + nj1;
         const double ab2 = mid(a, b), cd2 = mid(c, d);
+        if (tok) {  // this node's ranges and their halves in the interval trees
+            const int32_t px = tfind(xt, xtm, depth, a, b), py = tfind(yt, ytm, depth, c, d);
+            if (px < 0 || py < 0) {
+                tok = false;
+            } else {
+                tadd(xt, xtm, depth + 1, i0, ni1, a, ab2, px, 0);
+                tadd(xt, xtm, depth + 1, i2, ni2, ab2, b, px, 1);
+                tadd(yt, ytm, depth + 1, j0, nj1, c, cd2, py, 0);
+                tadd(yt, ytm, depth + 1, j2, nj2, cd2, d, py, 1);
+            }
+        }
         const bool leaf = ni1 < bs && ni2 < bs && nj1 < bs && nj2 < bs;
         const int cd_ = depth + 1;
         if (type == 0) {  // KernelMatrix, src/KernelMatrix.jl:49-70
@@ -481,6 +549,9 @@ struct PlanBufs {
     T *A = nullptr, *U = nullptr, *V = nullptr, *F = nullptr;
     T *partial = nullptr, *w = nullptr, *z = nullptr;
     T *xt = nullptr;  // packed right-hand sides of the MFMA passes (ny x 16)
+    // nested far field of the MFMA passes (Float64 plans): the re-expansion
+    // matrices, moments per column node, coefficients per row node
+    T *nestM = nullptr, *my = nullptr, *mx = nullptr;
     // x .. chebA and the workspace partial .. xt live in this one allocation
     // (Staging); A, U, V and F are allocated on their own
     void *arena = nullptr;
@@ -533,7 +604,18 @@ struct hm_plan {
     std::vector<int32_t> multi_c;     // column clusters with more than one upward task
     std::vector<int32_t> multi_r;     // row clusters with more than one upward task
     int32_t *d_multi_c = nullptr, *d_multi_r = nullptr;
-    void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_multi_r (Staging)
+    // nested far field of the MFMA passes (DTNode): the interval trees, the
+    // column tree's internal nodes and the row tree's nodes by depth (CSR
+    // over depth), one task per non-empty column leaf, each row chunk's leaf
+    bool nest = false;
+    std::vector<DTNode> xtn, ytn;
+    std::vector<int32_t> xlev, xlev_off, ylev, ylev_off;
+    std::vector<DTask> yleaf;
+    std::vector<int32_t> chunk_leaf;
+    DTNode *d_xtn = nullptr, *d_ytn = nullptr;
+    int32_t *d_xlev = nullptr, *d_ylev = nullptr, *d_chunk_leaf = nullptr;
+    DTask *d_yleaf = nullptr;
+    void *d_tables = nullptr;  // the one allocation holding d_rcl .. d_yleaf (Staging)
     // side stream and fork / join events of products whose near field runs
     // beside the far-field chain; made at the first such product that is not
     // being captured into a graph
@@ -593,6 +675,103 @@ struct hm_plan {
         dp.nslots_r = rows.nslots; dp.nslots_c = cols.nslots;
         dp.multi_r = d_multi_r; dp.multi_c = d_multi_c;
         dp.nmulti_r = (int32_t)multi_r.size(); dp.nmulti_c = (int32_t)multi_c.size();
+        dp.nest = nest ? 1 : 0;
+        dp.xtn = d_xtn; dp.ytn = d_ytn;
+        dp.xlev = d_xlev; dp.ylev = d_ylev;
+        dp.xlev_off_h = xlev_off.data(); dp.ylev_off_h = ylev_off.data();
+        dp.nxlev = xlev_off.empty() ? 0 : (int32_t)xlev_off.size() - 1;
+        dp.nylev = ylev_off.empty() ? 0 : (int32_t)ylev_off.size() - 1;
+        dp.chunk_leaf = d_chunk_leaf;
+        dp.yleaf = d_yleaf; dp.nyleaf = (int32_t)yleaf.size();
+    }
+
+    // The nested far field's tables from the builder's interval trees; nest
+    // stays false (the per-level far field is used) unless every half is
+    // exactly half of its parent's interval (to 1e-12), every column leaf
+    // fits one upward task and every row chunk lies in one row leaf.
+    template <class TN>
+    void build_nest(const std::vector<TN> &xt, const std::vector<TN> &yt, bool tok) {
+        nest = false;
+        if (!tok || f32 || xt.empty() || yt.empty()) return;
+        auto conv = [](const std::vector<TN> &t, std::vector<DTNode> &o) {
+            o.resize(t.size());
+            for (size_t k = 0; k < t.size(); ++k) {
+                const TN &q = t[k];
+                DTNode d{};
+                d.first = q.first;
+                d.len = (int32_t)q.len;
+                d.parent = q.parent;
+                d.child[0] = q.child[0];
+                d.child[1] = q.child[1];
+                d.cl = q.cl;
+                d.half = q.half;
+                d.cx = 0.5 * (q.lo + q.hi);
+                d.cy = 1.0 / (0.5 * (q.hi - q.lo));
+                o[k] = d;
+            }
+            for (size_t k = 0; k < t.size(); ++k) {
+                const DTNode &d = o[k];
+                if ((d.child[0] < 0) != (d.child[1] < 0) || !std::isfinite(d.cy)) return false;
+                if (d.parent < 0) continue;
+                const DTNode &P = o[d.parent];
+                const double alpha = P.cy / d.cy, beta = (d.cx - P.cx) * P.cy;
+                if (!(std::fabs(alpha - 0.5) <= 1e-12) ||
+                    !(std::fabs(beta - (d.half ? 0.5 : -0.5)) <= 1e-12))
+                    return false;
+            }
+            return true;
+        };
+        if (!conv(xt, xtn) || !conv(yt, ytn)) return;
+        auto by_depth = [](const std::vector<TN> &t, bool internal_only, std::vector<int32_t> &ids,
+                           std::vector<int32_t> &off) {
+            int maxd = 0;
+            for (const TN &q : t) maxd = std::max(maxd, q.depth);
+            off.assign((size_t)maxd + 2, 0);
+            ids.clear();
+            for (int dd = 0; dd <= maxd; ++dd) {
+                for (size_t k = 0; k < t.size(); ++k) {
+                    const TN &q = t[k];
+                    if (q.depth != dd || q.len == 0) continue;
+                    if (internal_only && q.child[0] < 0) continue;
+                    ids.push_back((int32_t)k);
+                }
+                off[dd + 1] = (int32_t)ids.size();
+            }
+        };
+        by_depth(yt, true, ylev, ylev_off);
+        by_depth(xt, false, xlev, xlev_off);
+        yleaf.clear();
+        for (size_t k = 0; k < ytn.size(); ++k) {
+            const DTNode &d = ytn[k];
+            if (d.child[0] >= 0 || d.len == 0) continue;
+            if (d.len > HM_TASK_POINTS) return;
+            DTask T{};
+            T.cl = (int32_t)k;
+            T.off = 0;
+            T.len = d.len;
+            T.single = 1;
+            T.slot = d.cl;
+            T.first = d.first;
+            T.cx = d.cx;
+            T.cy = d.cy;
+            yleaf.push_back(T);
+        }
+        const std::vector<DChunk> &ch = rows.ch;
+        chunk_leaf.assign(ch.size(), -1);
+        for (size_t k = 0; k < xtn.size(); ++k) {
+            const DTNode &d = xtn[k];
+            if (d.child[0] >= 0 || d.len == 0) continue;
+            size_t c = std::lower_bound(ch.begin(), ch.end(), d.first,
+                                        [](const DChunk &a, int64_t v) { return a.p0 < v; }) -
+                       ch.begin();
+            for (; c < ch.size() && ch[c].p0 < d.first + d.len; ++c) {
+                if (ch[c].p0 + ch[c].np > d.first + d.len) return;
+                chunk_leaf[c] = (int32_t)k;
+            }
+        }
+        for (int32_t v : chunk_leaf)
+            if (v < 0) return;
+        nest = true;
     }
 
     // the clusters with more than one upward task (their slots are summed
@@ -697,6 +876,43 @@ struct Inst {
 
 
 
+// The re-expansion of a half's Chebyshev series into its parent's: with the
+// half's variable s and the parent's s/2 + beta (beta = -1/2 for half 0, +1/2
+// for half 1), T_k(s/2 + beta) = sum_m M[h][k][m] T_m(s) exactly for k < r, by
+// T_{k+1}(u) = 2 u T_k(u) - T_{k-1}(u) and s T_m(s) = (T_{m+1}(s) + T_{|m-1|}(s)) / 2;
+// formed in long double and rounded once. Layout M[(h r + k) r + m].
+extern "C++" {
+template <class T>
+std::vector<T> nest_matrices(int r) {
+    std::vector<T> out((size_t)2 * r * r);
+    for (int h = 0; h < 2; ++h) {
+        const long double beta = h ? 0.5L : -0.5L, alpha = 0.5L;
+        std::vector<std::vector<long double>> P((size_t)r, std::vector<long double>((size_t)r, 0.0L));
+        P[0][0] = 1.0L;
+        if (r > 1) {
+            P[1][0] = beta;
+            P[1][1] = alpha;
+        }
+        for (int k = 1; k + 1 < r; ++k) {
+            std::vector<long double> sp((size_t)r, 0.0L);  // s * P[k]
+            for (int m = 0; m <= k; ++m) {
+                const long double c = P[k][m];
+                if (m == 0) {
+                    sp[1] += c;
+                } else {
+                    sp[m + 1] += 0.5L * c;
+                    sp[m - 1] += 0.5L * c;
+                }
+            }
+            for (int j = 0; j < r; ++j) P[k + 1][j] = 2.0L * alpha * sp[j] + 2.0L * beta * P[k][j] - P[k - 1][j];
+        }
+        for (int k = 0; k < r; ++k)
+            for (int m = 0; m < r; ++m) out[((size_t)h * r + k) * r + m] = (T)P[k][m];
+    }
+    return out;
+}
+}  // extern "C++"
+
 // The element-typed part of a plan: nodes, weights, cluster maps and the
 // Chebyshev-form matrix in T, the device arrays, the device view, and the
 // compression (dense blocks, F, U, V) for stored and hybrid plans.
@@ -741,6 +957,8 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
                     (T)((m ? 2.0L : 1.0L) * cosl((long double)m * (2 * k - 1) * PI_L / (2.0L * r)) / r);
     }
     const std::vector<T> xv(x, x + P->nx), yv(y, y + P->ny);
+    std::vector<T> nestM;
+    if (std::is_same_v<T, double> && P->nest) nestM = nest_matrices<T>(r);
     int rc;
 #define HM_TRY_T(expr) do { rc = (expr); if (rc != HM_OK) return rc; } while (0)
     const int64_t nslots = std::max(P->rows.nslots, P->cols.nslots);
@@ -755,6 +973,11 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         st.add(&b.rmap, rmap);
         st.add(&b.cmap, cmap);
         st.add(&b.chebA, chebA);
+        if (std::is_same_v<T, double> && P->nest) {
+            st.add(&b.nestM, nestM);
+            st.reserve(&b.my, (int64_t)P->ytn.size() * r * HM_WS_RHS);
+            st.reserve(&b.mx, (int64_t)P->xtn.size() * r * HM_WS_RHS);
+        }
         st.reserve(&b.partial, nslots * r * HM_WS_RHS);
         st.reserve(&b.w, ncl * r * HM_WS_RHS);
         st.reserve(&b.z, ncl * r * HM_WS_RHS);
@@ -777,6 +1000,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
     dp.x = b.x; dp.y = b.y; dp.lam = b.lam;
     dp.rnodes = b.rnodes; dp.cnodes = b.cnodes;
     dp.rmap = b.rmap; dp.cmap = b.cmap; dp.chebA = b.chebA;
+    dp.nestM = b.nestM; dp.my = b.my; dp.mx = b.mx;
     dp.A = b.A; dp.U = b.U; dp.V = b.V; dp.F = b.F;
 
     if (stored) {
@@ -894,6 +1118,8 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
             Bt.rmap.reserve(5 * q);
             Bt.cmap.reserve(5 * q);
         }
+        Bt.tadd(Bt.xt, Bt.xtm, 0, 0, I.nx, I.a, I.b, -1, 0);
+        Bt.tadd(Bt.yt, Bt.ytm, 0, 0, I.ny, I.c, I.d, -1, 0);
         Bt.node(0, 0, 0, I.nx, 0, I.ny, I.a, I.b, I.c, I.d);
         if (Bt.err) {
             return count == 1 ? fail(Bt.err, "%s", Bt.msg.c_str())
@@ -914,6 +1140,23 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         for (DFarBlk f : Bt.far) { f.rc += rb; f.cc += cb; B.far.push_back(f); }
         for (DSplit q : Bt.xsplits) { q.first += ro; q.at += ro; B.xsplits.push_back(q); }
         for (DSplit q : Bt.ysplits) { q.first += co; q.at += co; B.ysplits.push_back(q); }
+        {
+            auto merge = [](std::vector<Builder::TNode> &dst, const std::vector<Builder::TNode> &src,
+                            int64_t off, int32_t cbase) {
+                const int32_t base = (int32_t)dst.size();
+                for (Builder::TNode q : src) {
+                    q.first += off;
+                    if (q.cl >= 0) q.cl += cbase;
+                    if (q.parent >= 0) q.parent += base;
+                    for (int h = 0; h < 2; ++h)
+                        if (q.child[h] >= 0) q.child[h] += base;
+                    dst.push_back(q);
+                }
+            };
+            merge(B.xt, Bt.xt, ro, rb);
+            merge(B.yt, Bt.yt, co, cb);
+            B.tok = B.tok && Bt.tok;
+        }
         B.nd += Bt.nd; B.nk1 += Bt.nk1; B.nk2 += Bt.nk2;
         B.far_rows += Bt.far_rows; B.far_cols += Bt.far_cols; B.dense_entries += Bt.dense_entries;
         B.maxdepth = std::max(B.maxdepth, Bt.maxdepth);
@@ -976,6 +1219,7 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
     build_side(nx, P->dense, true, P->rcl, P->rows);
     build_side(ny, P->dense, false, P->ccl, P->cols);
     P->list_multi();
+    P->build_nest(B.xt, B.yt, B.tok);
     // the cluster maps the Float64 MFMA upward pass reads from its task
     // record: (a+b)/2 and 1/((b-a)/2) as the Float64 cluster maps form them
     if (!f32) {
@@ -1022,6 +1266,12 @@ static int make_plan(int kernel_id, const Inst *inst, int64_t count, int blocksi
         st.add(&P->d_ctask, P->cols.tasks);
         st.add(&P->d_multi_c, P->multi_c);
         st.add(&P->d_multi_r, P->multi_r);
+        st.add(&P->d_xtn, P->xtn);
+        st.add(&P->d_ytn, P->ytn);
+        st.add(&P->d_xlev, P->xlev);
+        st.add(&P->d_ylev, P->ylev);
+        st.add(&P->d_chunk_leaf, P->chunk_leaf);
+        st.add(&P->d_yleaf, P->yleaf);
         st.add(&P->d_xsplits, P->xsplits);
         st.add(&P->d_ysplits, P->ysplits);
         st.reserve(&P->d_status, 1);
@@ -1194,6 +1444,12 @@ static int from_blocks_t(int64_t nrows, int64_t ncols, int64_t nleaves, const in
         st.add(&P->d_ctask, P->cols.tasks);
         st.add(&P->d_multi_c, P->multi_c);
         st.add(&P->d_multi_r, P->multi_r);
+        st.add(&P->d_xtn, P->xtn);
+        st.add(&P->d_ytn, P->ytn);
+        st.add(&P->d_xlev, P->xlev);
+        st.add(&P->d_ylev, P->ylev);
+        st.add(&P->d_chunk_leaf, P->chunk_leaf);
+        st.add(&P->d_yleaf, P->yleaf);
         HM_TRY(st.commit(&P->d_tables));
     }
     P->far_by_rc_h = std::move(by_rc);
@@ -1461,6 +1717,7 @@ int hm_plan_info(hm_plan_t P, int64_t *info, int n) {
     v[HM_INFO_N_INSTANCES] = (int64_t)P->row_off.size() - 1;
     v[HM_INFO_FROM_BLOCKS] = P->from_blocks ? 1 : 0;
     v[HM_INFO_FLOAT32] = P->f32 ? 1 : 0;
+    v[HM_INFO_NESTED] = P->nest ? 1 : 0;
     for (int k = 0; k < n && k < HM_INFO_COUNT; ++k) info[k] = v[k];
     for (int k = HM_INFO_COUNT; k < n; ++k) info[k] = 0;
     return HM_OK;

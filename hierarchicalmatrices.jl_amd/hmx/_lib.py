@@ -38,7 +38,7 @@ INFO_KEYS = [
     "nrows", "ncols", "rank", "blocksize", "kernel", "n_d", "n_k1", "n_k2", "n_dense", "n_far",
     "n_rowclusters", "n_colclusters", "depth", "dense_entries", "basis_rows_u", "basis_rows_v",
     "bytes_stored", "bytes_ref_layout", "flags", "n_rowchunks", "n_colchunks", "nonfinite",
-    "n_instances", "from_blocks", "float32",
+    "n_instances", "from_blocks", "float32", "nested",
 ]
 
 _vp = ctypes.c_void_p

@@ -245,6 +245,9 @@ enum hm_info {
     HM_INFO_FLAGS, HM_INFO_N_ROWCHUNKS, HM_INFO_N_COLCHUNKS, HM_INFO_NONFINITE,
     HM_INFO_N_INSTANCES, HM_INFO_FROM_BLOCKS,
     HM_INFO_FLOAT32,           /* 1 for a plan built by hm_plan_from_points_f32 */
+    HM_INFO_NESTED,            /* 1 when the MFMA passes apply the far field through the
+                                  interval trees (each point in one leaf's moments, each
+                                  row in one leaf's series); 0: per covering cluster */
     HM_INFO_COUNT
 };
 int hm_plan_info(hm_plan_t plan, int64_t *info, int n);
