@@ -49,12 +49,13 @@ struct DevPlanT {
     int32_t nmulti_r, nmulti_c;
     // nested far field of the MFMA passes (Float64 plans from points; nest
     // is 0 when the plan's tree does not allow it): the interval trees of
-    // both sides, their node ids by depth (internal column nodes, all row
-    // nodes), one upward task per non-empty column leaf (task.cl = the leaf
-    // node, task.slot = its column cluster or -1), each row chunk's row leaf,
-    // the two re-expansion matrices T_k(s/2 -+ 1/2) = sum_m M[h][k][m] T_m(s),
-    // and the moments per column node and coefficients per row node (r x 32
-    // each). The level offsets are host arrays, read by the launching code.
+    // both sides, their internal node ids by depth, one upward task per
+    // non-empty column leaf (task.cl = the leaf node, task.slot = its column
+    // cluster or -1), each row chunk's row leaf, the two re-expansion matrices
+    // T_k(s/2 -+ 1/2) = sum_m M[h][k][m] T_m(s), and the moments per column
+    // node and coefficients per internal row node (r x 32 each; a row leaf's
+    // are formed by the downward pass). The level offsets are host arrays,
+    // read by the launching code.
     int nest;
     const DTNode *xtn, *ytn;
     const int32_t *xlev, *ylev;

@@ -1218,19 +1218,23 @@ __device__ __forceinline__ void wave_sync() {
 
 // Results of the FP64 shape held as Mo[gi][t][v] = X[16t + g + 4v][16 gi + j16]
 // (rank index m = 16t + g + 4v, right-hand side q = 16 gi + j16), stored as
-// dst[q r + m], the layout of partial slots, w, z, moments and coefficients.
-template <int RP, int G>
+// dst[q r + m], the layout of partial slots, w and z, or (KQ, the nested
+// moments and coefficients) as dst[m 16G + q], so that a lane's right-hand
+// sides are contiguous.
+template <int RP, int G, bool KQ = false>
 __device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], double *dst, int r,
                                             int j16, int g) {
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
-        double *dq = dst + (int64_t)(gi * MQ + j16) * r;
 #pragma unroll
         for (int t = 0; t < RP / 16; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
-                if (m < r) dq[m] = Mo[gi][t][v];
+                if (m < r) {
+                    if constexpr (KQ) dst[(int64_t)m * (MQ * G) + gi * MQ + j16] = Mo[gi][t][v];
+                    else dst[(int64_t)(gi * MQ + j16) * r + m] = Mo[gi][t][v];
+                }
             }
     }
 }
@@ -1404,7 +1408,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
     }
     // D64 holds mu as store_ranks and nodes_from_moments read it
     if constexpr (NEST) {
-        store_ranks<RP, G>(D64, mom + (int64_t)Tk.cl * MQT * r, r, j16, g);
+        store_ranks<RP, G, true>(D64, mom + (int64_t)Tk.cl * MQT * r, r, j16, g);
         if (Tk.slot >= 0)
             nodes_from_moments<RP, G>(D64, chebA, w + Tk.slot * MQT * r, r, j16, g);
     } else if (Tk.single) {
@@ -1437,23 +1441,12 @@ __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids
         for (int t = 0; t < MT; ++t) W[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int C = N.child[h];
-        if (C < 0 || tn[C].len == 0) continue;  // wave-uniform
+        const int C = N.child[h];  // an empty half's moments stay zero
         const double *Mh = M + (int64_t)h * r * r;
         const double *src = mom + (int64_t)C * MQT * r;
         // A[m][k] = M_h[m][k], B[k][q] = mu_C[k][q]; loads in bounds, the
-        // terms past r zeroed after
-        double av[MT][MT][4], bv[G][MT][4];
-#pragma unroll
-        for (int tt = 0; tt < MT; ++tt)
-#pragma unroll
-            for (int t = 0; t < MT; ++t)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
-                    const double a = Mh[min(m, r - 1) * r + min(k, r - 1)];
-                    av[tt][t][v] = (m < r && k < r) ? a : 0.0;
-                }
+        // terms past r zeroed after; one output tile's A operands at a time
+        double bv[G][MT][4];
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
 #pragma unroll
@@ -1461,37 +1454,42 @@ __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int k = t * 16 + 4 * v + g;
-                    const double b = src[(int64_t)(gi * MQ + j16) * r + min(k, r - 1)];
+                    const double b = src[(int64_t)min(k, r - 1) * MQT + gi * MQ + j16];
                     bv[gi][t][v] = k < r ? b : 0.0;
                 }
 #pragma unroll
-        for (int gi = 0; gi < G; ++gi)
+        for (int tt = 0; tt < MT; ++tt) {
+            double av[MT][4];
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt)
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                    const double a = Mh[min(m, r - 1) * r + min(k, r - 1)];
+                    av[t][v] = (m < r && k < r) ? a : 0.0;
+                }
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi)
 #pragma unroll
                 for (int t = 0; t < MT; ++t)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], bv[gi][t][v], W[gi][tt]);
+                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], bv[gi][t][v], W[gi][tt]);
+        }
     }
-    store_ranks<RP, G>(W, mom + (int64_t)P * MQT * r, r, j16, g);
+    store_ranks<RP, G, true>(W, mom + (int64_t)P * MQT * r, r, j16, g);
     if (N.cl >= 0) nodes_from_moments<RP, G>(W, chebA, w + (int64_t)N.cl * MQT * r, r, j16, g);
 }
 
-// Nested far field, downward: the coefficients of each row node of one depth,
-// c_N = z[its row cluster] (if it is one) + M_h^T c_parent (the parent's
-// series re-expanded on this half's interval), one wavefront per node.
+// Nested far field, downward: the coefficients of row node N, c_N = z[its
+// row cluster] (if it is one) + M_h^T c_parent (the parent's series
+// re-expanded on this half's interval), in the FP64 result layout (as
+// store_ranks reads it).
 template <int RP, int G>
-__global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids, int n,
-                                           const double *M, const double *z, double *cof, int r) {
+__device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const double *z,
+                                         const double *cof, int r, int j16, int g,
+                                         md4_t (&W)[G][RP / 16]) {
     constexpr int MT = RP / 16;
     constexpr int MQT = MQ * G;
-    const int wid = wave_id();
-    const int lane = threadIdx.x % WAVE;
-    if (wid >= n) return;
-    const int j16 = lane % 16, g = lane / 16;
-    const int id = ids[wid];
-    const DTNode N = tn[id];
-    md4_t W[G][MT];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
 #pragma unroll
@@ -1506,18 +1504,9 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
     if (N.parent >= 0) {  // wave-uniform
         const double *Mh = M + (int64_t)N.half * r * r;
         const double *src = cof + (int64_t)N.parent * MQT * r;
-        // A[m][k] = M_h[k][m], B[k][q] = c_P[k][q]
-        double av[MT][MT][4], bv[G][MT][4];
-#pragma unroll
-        for (int tt = 0; tt < MT; ++tt)
-#pragma unroll
-            for (int t = 0; t < MT; ++t)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
-                    const double a = Mh[min(k, r - 1) * r + min(m, r - 1)];
-                    av[tt][t][v] = (m < r && k < r) ? a : 0.0;
-                }
+        // A[m][k] = M_h[k][m], B[k][q] = c_P[k][q]; one output tile's A
+        // operands at a time
+        double bv[G][MT][4];
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
 #pragma unroll
@@ -1525,19 +1514,45 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int k = t * 16 + 4 * v + g;
-                    const double b = src[(int64_t)(gi * MQ + j16) * r + min(k, r - 1)];
+                    const double b = src[(int64_t)min(k, r - 1) * MQT + gi * MQ + j16];
                     bv[gi][t][v] = k < r ? b : 0.0;
                 }
 #pragma unroll
-        for (int gi = 0; gi < G; ++gi)
+        for (int tt = 0; tt < MT; ++tt) {
+            double av[MT][4];
 #pragma unroll
-            for (int tt = 0; tt < MT; ++tt)
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                    const double a = Mh[min(k, r - 1) * r + min(m, r - 1)];
+                    av[t][v] = (m < r && k < r) ? a : 0.0;
+                }
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi)
 #pragma unroll
                 for (int t = 0; t < MT; ++t)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], bv[gi][t][v], W[gi][tt]);
+                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], bv[gi][t][v], W[gi][tt]);
+        }
     }
-    store_ranks<RP, G>(W, cof + (int64_t)id * MQT * r, r, j16, g);
+}
+
+// The coefficients of the internal row nodes of one depth (l2l_node), one
+// wavefront per node; a chunk's leaf forms its own in the downward pass.
+template <int RP, int G>
+__global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids, int n,
+                                           const double *M, const double *z, double *cof, int r) {
+    constexpr int MQT = MQ * G;
+    const int wid = wave_id();
+    const int lane = threadIdx.x % WAVE;
+    if (wid >= n) return;
+    const int j16 = lane % 16, g = lane / 16;
+    const int id = ids[wid];
+    const DTNode N = tn[id];
+    md4_t W[G][RP / 16];
+    l2l_node<RP, G>(N, M, z, cof, r, j16, g, W);
+    store_ranks<RP, G, true>(W, cof + (int64_t)id * MQT * r, r, j16, g);
 }
 
 // Translation, one wavefront per row cluster: Z = sum over the cluster's far
@@ -1546,8 +1561,7 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
 // the B-operand layout of the second product, so Z feeds it from registers.
 template <int RP, bool FREE, int KID, int G>
 __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *cl, int ncl,
-                                                   const int32_t *far_list, const double *w,
-                                                   double *z) {
+                                                   const double *w, double *z) {
     constexpr int MT = RP / 16;
     constexpr int MQT = MQ * G;
     const int wid = wave_id();
@@ -1562,21 +1576,21 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 #pragma unroll
         for (int t = 0; t < MT; ++t) Z[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
     for (int e0 = C.far_beg; e0 < C.far_end; e0 += WAVE) {
-        // the far blocks' records, one per lane, read together
+        // the far blocks' translation records (in this cluster's list
+        // order), one per lane, read together
         const int ne = min(WAVE, C.far_end - e0);
-        int b_rc = 0, b_cc = 0;
+        int b_cc = 0;
         int64_t b_foff = 0;
         if (lane < ne) {
-            const DFarBlk b = p.far[far_list[e0 + lane]];
-            b_rc = b.rc;
-            b_cc = b.cc;
+            const DTrRec b = p.tr_rc[e0 + lane];
+            b_cc = b.src;
             b_foff = b.foff;
         }
         for (int e = 0; e < ne; ++e) {
-            const int rc = rl_i(b_rc, e), cc = rl_i(b_cc, e);
+            const int cc = rl_i(b_cc, e);
             const double *ws = w + (int64_t)cc * MQT * r;
             const double *F = FREE ? nullptr : p.F + rl_l(b_foff, e);
-            const double *tx = p.rnodes + (int64_t)rc * r;
+            const double *tx = p.rnodes + (int64_t)wid * r;
             const double *ty = p.cnodes + (int64_t)cc * r;
 #pragma unroll
             for (int k0 = 0; k0 < RP; k0 += 16) {
@@ -1661,7 +1675,10 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     constexpr int MQT = MQ * G;
-    __shared__ double ds_all[WPB][16 * MQT];  // one 16-row output tile per wavefront
+    // per wavefront: one 16-row output tile; with NEST first the leaf's
+    // coefficients, rows of r + 1 (padded against bank conflicts)
+    constexpr int LDSW = (NEST && (RP + 1) * MQT > 16 * MQT) ? (RP + 1) * MQT : 16 * MQT;
+    __shared__ double ds_all[WPB][LDSW];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
     double *Dl = ds_all[(threadIdx.x / WAVE) % WPB];
@@ -1744,12 +1761,30 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     }
     // ---- far field: the covering clusters' records one per lane; the
     // coefficients of the next cluster load while this one is applied
-    const double *zs = NEST ? p.mx : z;
+    // NEST: the chunk's row leaf forms its coefficients (l2l_node from its
+    // parent's) into LDS, and the far field reads them there
+    int leaf = 0;
+    if constexpr (NEST) {
+        leaf = p.chunk_leaf[wid];
+        const DTNode N = p.xtn[leaf];
+        md4_t W[G][RP / 16];
+        l2l_node<RP, G>(N, p.nestM, z, p.mx, r, j16, g, W);
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int t = 0; t < RP / 16; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = t * 16 + g + 4 * v;
+                    if (m < r) Dl[(gi * MQ + j16) * (r + 1) + m] = W[gi][t][v];
+                }
+        wave_sync();
+    }
     auto load_bz = [&](double (*bz)[KSEL], int cl) {
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
             const int q = gi * MQ + j16;
-            const double *zc = zs + ((int64_t)cl * MQT + q) * r;  // C[k][q]
+            const double *zc = NEST ? Dl + q * (r + 1) : z + ((int64_t)cl * MQT + q) * r;  // C[k][q]
 #pragma unroll
             for (int u = 0; u < KSEL; ++u) {
                 const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
@@ -1764,8 +1799,8 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         double f_cx = 0.0, f_cy = 0.0;
         if (lane < ne) {
             if constexpr (NEST) {
-                f_cl = p.chunk_leaf[wid];
-                const DTNode L = p.xtn[f_cl];
+                const DTNode L = p.xtn[leaf];
+                f_cl = leaf;
                 f_cx = L.cx;
                 f_cy = L.cy;
             } else {
@@ -1867,6 +1902,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                 for (int u = 0; u < KSEL; ++u) bz[gi][u] = bzn[gi][u];
         }
     }
+    if constexpr (NEST) wave_sync();  // the output tiles reuse the coefficients' LDS
     // ---- result, one 16-row tile at a time through LDS; a lane then writes
     // 16 consecutive rows of one right-hand side per store
 #pragma unroll
@@ -1923,8 +1959,7 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
                 k_m2m<RP, G><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
                                                        p.chebA, a.w, p.r);
         }
-        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc,
-                                                                   a.w, a.z);
+        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w, a.z);
         for (int d = 0; d < p.nxlev; ++d) {
             const int off = p.xlev_off_h[d], cnt = p.xlev_off_h[d + 1] - off;
             if (cnt > 0)
@@ -1937,12 +1972,14 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
         if (p.nmulti_c > 0)
             k_reduce<double, true, MQT><<<nblk(p.nmulti_c), WG, 0, s>>>(
                 p.ccl, p.multi_c, p.nmulti_c, a.partial, p.chebA, a.w, p.r);
-        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, p.far_by_rc,
-                                                                   a.w, a.z);
+        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w, a.z);
     }
     if (p.nrch > 0) {
+        // nested: one leaf series per row, evaluated in FP64 for every PREC
+        // (a reduced-precision form saves nothing on one series and costs the
+        // near field registers)
         if (nest)
-            k_down_m<RP, PREC, FREE, KID, G, true><<<nblk(p.nrch), WG, 0, s>>>(
+            k_down_m<RP, 2, FREE, KID, G, true><<<nblk(p.nrch), WG, 0, s>>>(
                 p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
         else
             k_down_m<RP, PREC, FREE, KID, G, false><<<nblk(p.nrch), WG, 0, s>>>(

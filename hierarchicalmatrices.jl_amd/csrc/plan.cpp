@@ -605,8 +605,8 @@ struct hm_plan {
     std::vector<int32_t> multi_r;     // row clusters with more than one upward task
     int32_t *d_multi_c = nullptr, *d_multi_r = nullptr;
     // nested far field of the MFMA passes (DTNode): the interval trees, the
-    // column tree's internal nodes and the row tree's nodes by depth (CSR
-    // over depth), one task per non-empty column leaf, each row chunk's leaf
+    // internal nodes of both trees by depth (CSR over depth), one task per
+    // non-empty column leaf, each row chunk's leaf
     bool nest = false;
     std::vector<DTNode> xtn, ytn;
     std::vector<int32_t> xlev, xlev_off, ylev, ylev_off;
@@ -739,7 +739,7 @@ struct hm_plan {
             }
         };
         by_depth(yt, true, ylev, ylev_off);
-        by_depth(xt, false, xlev, xlev_off);
+        by_depth(xt, true, xlev, xlev_off);  // leaves form theirs in the downward pass
         yleaf.clear();
         for (size_t k = 0; k < ytn.size(); ++k) {
             const DTNode &d = ytn[k];
@@ -983,6 +983,11 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         st.reserve(&b.z, ncl * r * HM_WS_RHS);
         if (std::is_same_v<T, double>) st.reserve(&b.xt, P->ny * HM_WS_RHS);
         HM_TRY_T(st.commit(&b.arena));
+    }
+    if (b.my) {  // the moments of empty column nodes are read as zero
+        const hipError_t e0 =
+            hipMemsetAsync(b.my, 0, sizeof(T) * P->ytn.size() * r * HM_WS_RHS, s);
+        if (e0 != hipSuccess) return fail(HM_ERR_HIP, "memset: %s", hipGetErrorString(e0));
     }
     const bool stored = !(flags & HM_PLAN_MATRIX_FREE);
     const bool hybrid = (flags & HM_PLAN_HYBRID) != 0;
