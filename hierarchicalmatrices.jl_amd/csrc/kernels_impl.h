@@ -1216,13 +1216,21 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
+// The nested far field's node values w (of column clusters) and series z (of
+// row clusters) are stored in FP64 for the FP64 form and rounded once to FP32
+// for the reduced-precision forms (whose moments already carry FP32 terms):
+// the passes that move them, the translation above all, read and write half
+// the bytes, and every sum over them still runs in FP64. The chains of
+// moments and coefficients between tree levels stay FP64.
+template <int PREC> using NodeT = std::conditional_t<PREC == 2, double, float>;
+
 // Results of the FP64 shape held as Mo[gi][t][v] = X[16t + g + 4v][16 gi + j16]
 // (rank index m = 16t + g + 4v, right-hand side q = 16 gi + j16), stored as
 // dst[q r + m], the layout of partial slots, w and z, or (KQ, the nested
 // moments and coefficients) as dst[m 16G + q], so that a lane's right-hand
 // sides are contiguous.
-template <int RP, int G, bool KQ = false>
-__device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], double *dst, int r,
+template <int RP, int G, bool KQ = false, class S = double>
+__device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], S *dst, int r,
                                             int j16, int g) {
 #pragma unroll
     for (int gi = 0; gi < G; ++gi) {
@@ -1232,8 +1240,8 @@ __device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], doubl
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
                 if (m < r) {
-                    if constexpr (KQ) dst[(int64_t)m * (MQ * G) + gi * MQ + j16] = Mo[gi][t][v];
-                    else dst[(int64_t)(gi * MQ + j16) * r + m] = Mo[gi][t][v];
+                    if constexpr (KQ) dst[(int64_t)m * (MQ * G) + gi * MQ + j16] = (S)Mo[gi][t][v];
+                    else dst[(int64_t)(gi * MQ + j16) * r + m] = (S)Mo[gi][t][v];
                 }
             }
     }
@@ -1242,10 +1250,10 @@ __device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], doubl
 // Node values w = A mu of one cluster from its moments held as in
 // store_ranks: Mo[gi][t][v] is B[k0 + g][j16] of the K-step k0 = 16t + 4v, so
 // the product takes mu from registers; A's rows are loaded once for the
-// groups. Stored as dst[q r + m].
-template <int RP, int G>
+// groups. Stored as dst[q r + m] (FP64, or rounded to FP32 once).
+template <int RP, int G, class S>
 __device__ __forceinline__ void nodes_from_moments(const md4_t (&Mo)[G][RP / 16],
-                                                   const double *chebA, double *dst, int r,
+                                                   const double *chebA, S *dst, int r,
                                                    int j16, int g) {
     constexpr int MT = RP / 16;
     double av[MT][MT][4];
@@ -1296,11 +1304,12 @@ __global__ __launch_bounds__(WG) void k_pack_x(const double *x, int64_t incx, in
 // NEST: the tasks are the column tree's leaves (task.cl the node, task.slot
 // its column cluster or -1); the moments go to mom[node] and, for a leaf that
 // is a cluster, w = A mu to w[cluster].
-template <int RP, int PREC, int G, bool NEST>
+template <int RP, int PREC, int G, bool NEST, class WS>
 __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double *pts,
                                               const double *xt, int64_t ldxt, int nq,
-                                              const double *chebA, double *partial, double *w,
+                                              const double *chebA, double *partial, WS *w,
                                               int r, double *mom) {
+    static_assert(NEST || std::is_same<WS, double>::value, "per-level w is FP64");
     using TL = typename std::conditional<PREC == 2, double, float>::type;
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int MT = RP / 16;
@@ -1422,10 +1431,10 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
 // depth from its two halves' (mu_P = M_0 mu_0 + M_1 mu_1, the halves' series
 // re-expanded on the parent's interval), one wavefront per node; a node that
 // is a column cluster also gets w = A mu.
-template <int RP, int G>
+template <int RP, int G, class WS>
 __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids, int n,
                                            const double *M, double *mom, const double *chebA,
-                                           double *w, int r) {
+                                           WS *w, int r) {
     constexpr int MT = RP / 16;
     constexpr int MQT = MQ * G;
     const int wid = wave_id();
@@ -1484,8 +1493,8 @@ __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids
 // row cluster] (if it is one) + M_h^T c_parent (the parent's series
 // re-expanded on this half's interval), in the FP64 result layout (as
 // store_ranks reads it).
-template <int RP, int G>
-__device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const double *z,
+template <int RP, int G, class WS>
+__device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const WS *z,
                                          const double *cof, int r, int j16, int g,
                                          md4_t (&W)[G][RP / 16]) {
     constexpr int MT = RP / 16;
@@ -1498,7 +1507,7 @@ __device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
                 double c = 0.0;
-                if (N.cl >= 0 && m < r) c = z[((int64_t)N.cl * MQT + gi * MQ + j16) * r + m];
+                if (N.cl >= 0 && m < r) c = (double)z[((int64_t)N.cl * MQT + gi * MQ + j16) * r + m];
                 W[gi][t][v] = c;
             }
     if (N.parent >= 0) {  // wave-uniform
@@ -1540,9 +1549,9 @@ __device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const
 
 // The coefficients of the internal row nodes of one depth (l2l_node), one
 // wavefront per node; a chunk's leaf forms its own in the downward pass.
-template <int RP, int G>
+template <int RP, int G, class WS>
 __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids, int n,
-                                           const double *M, const double *z, double *cof, int r) {
+                                           const double *M, const WS *z, double *cof, int r) {
     constexpr int MQT = MQ * G;
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
@@ -1551,7 +1560,7 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
     const int id = ids[wid];
     const DTNode N = tn[id];
     md4_t W[G][RP / 16];
-    l2l_node<RP, G>(N, M, z, cof, r, j16, g, W);
+    l2l_node<RP, G, WS>(N, M, z, cof, r, j16, g, W);
     store_ranks<RP, G, true>(W, cof + (int64_t)id * MQT * r, r, j16, g);
 }
 
@@ -1559,9 +1568,9 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
 // blocks of F_b W_b (F stored, or regenerated from the mapped nodes; each F
 // operand feeds the G groups), then C = A^T Z. The FP64 result layout of Z is
 // the B-operand layout of the second product, so Z feeds it from registers.
-template <int RP, bool FREE, int KID, int G>
+template <int RP, bool FREE, int KID, int G, class WS>
 __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *cl, int ncl,
-                                                   const double *w, double *z) {
+                                                   const WS *w, WS *z) {
     constexpr int MT = RP / 16;
     constexpr int MQT = MQ * G;
     const int wid = wave_id();
@@ -1588,7 +1597,7 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
         }
         for (int e = 0; e < ne; ++e) {
             const int cc = rl_i(b_cc, e);
-            const double *ws = w + (int64_t)cc * MQT * r;
+            const WS *ws = w + (int64_t)cc * MQT * r;
             const double *F = FREE ? nullptr : p.F + rl_l(b_foff, e);
             const double *tx = p.rnodes + (int64_t)wid * r;
             const double *ty = p.cnodes + (int64_t)cc * r;
@@ -1604,7 +1613,7 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
                     const int lc = min(l, r - 1);
 #pragma unroll
                     for (int gi = 0; gi < G; ++gi) {
-                        const double v = ws[(gi * MQ + j16) * r + lc];
+                        const double v = (double)ws[(gi * MQ + j16) * r + lc];
                         bv[gi][h] = lok ? v : 0.0;
                     }
                     const double tyl = FREE ? ty[lc] : 0.0;
@@ -1614,7 +1623,7 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
                         const bool ok = lok && m < r;
                         double v;
                         if constexpr (FREE) v = kern_fast<double, KID>(tx[min(m, r - 1)], tyl);
-                        else v = F[min(m, r - 1) + (int64_t)r * lc];
+                        else v = __builtin_nontemporal_load(F + min(m, r - 1) + (int64_t)r * lc);
                         av[h][t] = ok ? v : 0.0;
                     }
                 }
@@ -1651,7 +1660,7 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
-                if (m < r) z[((int64_t)wid * MQT + gi * MQ + j16) * r + m] = Cc[t][v];
+                if (m < r) z[((int64_t)wid * MQT + gi * MQ + j16) * r + m] = (WS)Cc[t][v];
             }
     }
 }
@@ -1662,16 +1671,19 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 // block entries (stored, or regenerated from the points) enter the FP64 MFMA
 // directly, each feeding the G groups. Far field: for each covering cluster,
 // T_k(xh) of the lane's rows by the FP64 recurrence, times the cluster's
-// coefficients C[k][q]. The 64 x 16G result goes through LDS to a
-// row-per-lane layout and is written once.
+// coefficients C[k][q]. The 64 x 16G result goes through LDS, 16 rows at a
+// time, and is written once: whole rows per store when a row's right-hand
+// sides are contiguous (ldy = 1), else 16 rows of one right-hand side per
+// store.
 // NEST: the far field of a chunk is its row leaf's series alone (the
 // coefficients of every covering cluster pushed down the row tree into p.mx).
-template <int RP, int PREC, bool FREE, int KID, int G, bool NEST>
+template <int RP, int PREC, bool FREE, int KID, int G, bool NEST, class WS>
 __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
                                               const DNear *nl, const DFar *fl,
                                               const double2 *cmapv, const double *pts_out,
                                               const double *pts_in, const double *xt,
-                                              int64_t ldxt, const double *z, MvArgs a, int nq) {
+                                              int64_t ldxt, const WS *z, MvArgs a, int nq) {
+    static_assert(NEST || std::is_same<WS, double>::value, "per-level z is FP64");
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     constexpr int MQT = MQ * G;
@@ -1697,6 +1709,26 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         pf[t] = PREC == 2 ? po[t] : (fok[t] ? pts_out[c.p0 + t * 16 + frow] : 0.0);
     }
     const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
+    // NEST: the chunk's row leaf forms its coefficients (l2l_node from its
+    // parent's) into LDS first, while no accumulators are live; the far field
+    // reads them there
+    int leaf = 0;
+    if constexpr (NEST) {
+        leaf = p.chunk_leaf[wid];
+        const DTNode N = p.xtn[leaf];
+        md4_t W[G][RP / 16];
+        l2l_node<RP, G, WS>(N, p.nestM, z, p.mx, r, j16, g, W);
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int t = 0; t < RP / 16; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = t * 16 + g + 4 * v;
+                    if (m < r) Dl[(gi * MQ + j16) * (r + 1) + m] = W[gi][t][v];
+                }
+        wave_sync();
+    }
     md4_t D[G][4];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
@@ -1761,30 +1793,14 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     }
     // ---- far field: the covering clusters' records one per lane; the
     // coefficients of the next cluster load while this one is applied
-    // NEST: the chunk's row leaf forms its coefficients (l2l_node from its
-    // parent's) into LDS, and the far field reads them there
-    int leaf = 0;
-    if constexpr (NEST) {
-        leaf = p.chunk_leaf[wid];
-        const DTNode N = p.xtn[leaf];
-        md4_t W[G][RP / 16];
-        l2l_node<RP, G>(N, p.nestM, z, p.mx, r, j16, g, W);
-#pragma unroll
-        for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-            for (int t = 0; t < RP / 16; ++t)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = t * 16 + g + 4 * v;
-                    if (m < r) Dl[(gi * MQ + j16) * (r + 1) + m] = W[gi][t][v];
-                }
-        wave_sync();
-    }
+    // (NEST: the leaf's coefficients, from LDS)
     auto load_bz = [&](double (*bz)[KSEL], int cl) {
 #pragma unroll
         for (int gi = 0; gi < G; ++gi) {
             const int q = gi * MQ + j16;
-            const double *zc = NEST ? Dl + q * (r + 1) : z + ((int64_t)cl * MQT + q) * r;  // C[k][q]
+            const double *zc;  // C[k][q]
+            if constexpr (NEST) zc = Dl + q * (r + 1);
+            else zc = z + ((int64_t)cl * MQT + q) * r;
 #pragma unroll
             for (int u = 0; u < KSEL; ++u) {
                 const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
@@ -1903,8 +1919,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
         }
     }
     if constexpr (NEST) wave_sync();  // the output tiles reuse the coefficients' LDS
-    // ---- result, one 16-row tile at a time through LDS; a lane then writes
-    // 16 consecutive rows of one right-hand side per store
+    // ---- result, one 16-row tile at a time through LDS
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         if (t >= nt) break;
@@ -1913,15 +1928,30 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
 #pragma unroll
             for (int v = 0; v < 4; ++v) Dl[(g + 4 * v) * MQT + gi * MQ + j16] = D[gi][t][v];
         wave_sync();
-        const int row = j16;
-        if (t * 16 + row < c.np) {
-            const int64_t i = c.p0 + t * 16 + row;
+        if (a.ldy == 1) {
+            // a row's right-hand sides are contiguous: each store writes
+            // WAVE / MQT whole rows
+            constexpr int RPS = WAVE / MQT;
+            const int q = lane % MQT;
 #pragma unroll
-            for (int s4 = 0; s4 < 4 * G; ++s4) {
-                const int q = g + 4 * s4;
-                if (q < nq) {
-                    double *yo = a.y + i * a.incy + q * a.ldy;
+            for (int s = 0; s < 16 / RPS; ++s) {
+                const int row = s * RPS + lane / MQT;
+                if (t * 16 + row < c.np && q < nq) {
+                    double *yo = a.y + (c.p0 + t * 16 + row) * a.incy + q;
                     *yo = (a.beta ? *yo : 0.0) + Dl[row * MQT + q];
+                }
+            }
+        } else {
+            const int row = j16;
+            if (t * 16 + row < c.np) {
+                const int64_t i = c.p0 + t * 16 + row;
+#pragma unroll
+                for (int s4 = 0; s4 < 4 * G; ++s4) {
+                    const int q = g + 4 * s4;
+                    if (q < nq) {
+                        double *yo = a.y + i * a.incy + q * a.ldy;
+                        *yo = (a.beta ? *yo : 0.0) + Dl[row * MQT + q];
+                    }
                 }
             }
         }
@@ -1949,40 +1979,43 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
         ldxr = MQT;
     }
     const bool nest = p.nest && p.nfar > 0;
+    using WS = NodeT<PREC>;  // nested w and z (the workspace holds them in its first bytes)
+    WS *wn = reinterpret_cast<WS *>(a.w), *zn = reinterpret_cast<WS *>(a.z);
     if (p.nfar > 0 && nest) {
         if (p.nyleaf > 0)
-            k_up_m<RP, PREC, G, true><<<p.nyleaf, WAVE, 0, s>>>(p.yleaf, p.y, xr, ldxr, nq, p.chebA,
-                                                                a.partial, a.w, p.r, p.my);
+            k_up_m<RP, PREC, G, true, WS><<<p.nyleaf, WAVE, 0, s>>>(
+                p.yleaf, p.y, xr, ldxr, nq, p.chebA, a.partial, wn, p.r, p.my);
         for (int d = p.nylev - 1; d >= 0; --d) {
             const int off = p.ylev_off_h[d], cnt = p.ylev_off_h[d + 1] - off;
             if (cnt > 0)
-                k_m2m<RP, G><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
-                                                       p.chebA, a.w, p.r);
+                k_m2m<RP, G, WS><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
+                                                           p.chebA, wn, p.r);
         }
-        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w, a.z);
+        k_translate_m<RP, FREE, KID, G, WS><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, wn, zn);
         for (int d = 0; d < p.nxlev; ++d) {
             const int off = p.xlev_off_h[d], cnt = p.xlev_off_h[d + 1] - off;
             if (cnt > 0)
-                k_l2l<RP, G><<<nblk(cnt), WG, 0, s>>>(p.xtn, p.xlev + off, cnt, p.nestM, a.z, p.mx,
-                                                       p.r);
+                k_l2l<RP, G, WS><<<nblk(cnt), WG, 0, s>>>(p.xtn, p.xlev + off, cnt, p.nestM, zn,
+                                                           p.mx, p.r);
         }
     } else if (p.nfar > 0 && p.nctask > 0) {
-        k_up_m<RP, PREC, G, false><<<p.nctask, WAVE, 0, s>>>(p.ctask, p.y, xr, ldxr, nq, p.chebA,
-                                                              a.partial, a.w, p.r, nullptr);
+        k_up_m<RP, PREC, G, false, double><<<p.nctask, WAVE, 0, s>>>(
+            p.ctask, p.y, xr, ldxr, nq, p.chebA, a.partial, a.w, p.r, nullptr);
         if (p.nmulti_c > 0)
             k_reduce<double, true, MQT><<<nblk(p.nmulti_c), WG, 0, s>>>(
                 p.ccl, p.multi_c, p.nmulti_c, a.partial, p.chebA, a.w, p.r);
-        k_translate_m<RP, FREE, KID, G><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w, a.z);
+        k_translate_m<RP, FREE, KID, G, double><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w,
+                                                                           a.z);
     }
     if (p.nrch > 0) {
         // nested: one leaf series per row, evaluated in FP64 for every PREC
         // (a reduced-precision form saves nothing on one series and costs the
         // near field registers)
         if (nest)
-            k_down_m<RP, 2, FREE, KID, G, true><<<nblk(p.nrch), WG, 0, s>>>(
-                p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
+            k_down_m<RP, 2, FREE, KID, G, true, WS><<<nblk(p.nrch), WG, 0, s>>>(
+                p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, zn, a, nq);
         else
-            k_down_m<RP, PREC, FREE, KID, G, false><<<nblk(p.nrch), WG, 0, s>>>(
+            k_down_m<RP, PREC, FREE, KID, G, false, double><<<nblk(p.nrch), WG, 0, s>>>(
                 p, p.rch, p.nrch, p.rnear, p.rfar, p.rmap, p.x, p.y, xr, ldxr, a.z, a, nq);
     }
     return hipGetLastError();

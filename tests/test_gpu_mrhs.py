@@ -103,6 +103,14 @@ def test_mfma_strides_and_accumulate(cuda):
     H._mv(Y, X, 1, 1, 1, 1, 1, False, False, None, mfma=True)
     want = Y0 + H.matvec(torch.from_numpy(B).to(cuda), mfma=False)
     assert rel(Y.cpu().numpy(), want.cpu().numpy()) <= 1e-13
+    # column-major output (each right-hand side contiguous, ldy = 4099)
+    Yc = torch.zeros(17, 4099, dtype=torch.float64, device=cuda)
+    Yt = Yc[:, :4096].t()
+    Yt.copy_(Y0)
+    H._mv(Yt, X, 1, 1, 1, 1, 1, False, False, None, mfma=True)
+    assert Yt.stride() == (1, 4099)
+    assert rel(Yt.cpu().numpy(), want.cpu().numpy()) <= 1e-13
+    assert torch.count_nonzero(Yc[:, 4096:]) == 0
 
 
 def test_mfma_graph_and_tuner(cuda):
