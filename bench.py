@@ -344,7 +344,58 @@ def summary_gradient(steps, warmup, stream):
                      "rel_err_vs_dense_fp64": {"grad_x": rel_err(x.grad[idx], gx),
                                                "grad_y": rel_err(y.grad[idx], gy),
                                                "grad_b": rel_err(b.grad[idx], gb)}}
+        out[name].update(held_plan_steps(x, y, b, t, w["intervals"], steps, warmup, stream))
     return out
+
+
+def held_plan_steps(x0, y0, b0, t, intervals, steps, warmup, stream):
+    """The same step through hmx.autodiff.KernelProduct (one held plan, the
+    tree check left on the device): eager, and captured into a CUDA graph and
+    replayed; the replay's values and gradients against kernel_matvec's eager
+    ones. Each phase differentiates its own leaf tensors, and no autograd
+    graph of an eager phase is alive during the capture (its gradient
+    accumulators would run on the stream they were made on)."""
+    import torch
+    from hmx.autodiff import KernelProduct, kernel_matvec
+
+    def leaves():
+        return [v.detach().clone().requires_grad_(True) for v in (x0, y0, b0)]
+
+    def make_step(fn, x, y, b):
+        def step():
+            x.grad = y.grad = b.grad = None
+            u = fn(x, y, b)
+            (0.5 * torch.sum((u - t) ** 2)).backward()
+            return u.detach()
+        return step
+
+    x, y, b = leaves()
+    u = make_step(lambda *a: kernel_matvec("cauchykernel", *a, intervals, 64, 16), x, y, b)()
+    want = [u, x.grad, y.grad, b.grad]
+    del x, y, b
+    kp = KernelProduct("cauchykernel", x0.detach(), y0.detach(), intervals, 64, 16)
+    res = {"held_plan_eager_ms_per_step": round(timed(make_step(kp, *leaves()), steps, warmup,
+                                                      stream), 4)}
+    x, y, b = leaves()
+    kstep = make_step(kp, x, y, b)
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            kstep()
+    stream.wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        u_static = kstep()
+    ms_g = timed(g.replay, steps, warmup, stream)
+    got = [u_static, x.grad, y.grad, b.grad]
+    res["graph_ms_per_step"] = round(ms_g, 4)
+    res["graph_value"] = round(1000.0 / ms_g, 3)
+    res["graph_equals_eager"] = all(bool(torch.equal(a, c)) for a, c in zip(got, want))
+    res["graph_tree_kept"] = kp.tree_ok()
+    del g
+    kp.close()
+    return res
 
 
 def summary_scale(steps, warmup, stream):
@@ -495,6 +546,7 @@ def run_gradient(args):
             L.backward()
 
         ms = timed(step, args.steps, args.warmup, stream)
+        held = held_plan_steps(x, y, b, t, w["intervals"], args.steps, args.warmup, stream)
         # dense FP64 gradient of the same (possibly Float32) points on 256
         # sampled rows / columns (torch, GPU)
         n = x.numel()
@@ -510,17 +562,17 @@ def run_gradient(args):
         e = lambda a, c: float(torch.linalg.norm(a.double() - c) / torch.linalg.norm(c))
         step()
         return n, ms, {"grad_x": e(x.grad[idx], gx), "grad_y": e(y.grad[idx], gy),
-                       "grad_b": e(b.grad[idx], gb)}
+                       "grad_b": e(b.grad[idx], gb)}, held
 
-    n, ms, err = measure(torch.float64)
-    _, ms32, err32 = measure(torch.float32)
+    n, ms, err, held = measure(torch.float64)
+    _, ms32, err32, held32 = measure(torch.float32)
     return {"metric": "gradient steps/s (forward + pullback to x, y, b), n=2^16",
             "unit": "steps/s", "value": round(1000 / ms, 3), "ms_per_step": round(ms, 4),
             "dtype": "f64", "steps": args.steps,
             "config": {"workload": "gradient", "n": n, "rank": 16, "leaf_blocksize": 64},
-            "rel_err_vs_dense_fp64": err,
+            "rel_err_vs_dense_fp64": err, **held,
             "variants": {"f32": {"value": round(1000 / ms32, 3), "ms_per_step": round(ms32, 4),
-                                 "dtype": "f32", "rel_err_vs_dense_fp64": err32}}}
+                                 "dtype": "f32", "rel_err_vs_dense_fp64": err32, **held32}}}
 
 
 def run_scale(args):

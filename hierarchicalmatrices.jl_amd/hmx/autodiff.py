@@ -22,6 +22,11 @@ when they do not (or on the first step) is a plan built from host copies of
 the points. A plan stays with its autograd context until that context is
 freed (the graph and the output that holds it are gone), so two live graphs
 never share one and a retained graph can be differentiated again.
+
+kernel_matvec reads the device's tree check back on every forward (one
+synchronisation). KernelProduct holds one plan and leaves the check on the
+device until asked (tree_ok), so that a whole step can be captured into a
+CUDA graph and replayed.
 """
 import weakref
 
@@ -95,6 +100,73 @@ class _KernelMatVec(torch.autograd.Function):
         gb, gy = H.tgrad(r, b)
         return (gx, gy if ctx.needs_input_grad[1] else None,
                 gb if ctx.needs_input_grad[2] else None, None, None, None, None)
+
+
+class _KernelProductFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, y, b, kp):
+        x, y, b = (v.detach().contiguous() for v in (x, y, b))
+        H = kp.H
+        H.update_points(x, y, check=False)
+        torch.bitwise_or(kp._bad, H._status, out=kp._bad)
+        u = H.matvec(b)
+        ctx.H = H
+        ctx.save_for_backward(b)
+        return u
+
+    @staticmethod
+    def backward(ctx, r):
+        (b,) = ctx.saved_tensors
+        H = ctx.H
+        r = r.contiguous()
+        gx = H.grad_x(b, r) if ctx.needs_input_grad[0] else None
+        gb, gy = H.tgrad(r, b)
+        return (gx, gy if ctx.needs_input_grad[1] else None,
+                gb if ctx.needs_input_grad[2] else None, None)
+
+
+class KernelProduct:
+    """u = H(x, y) b, differentiable in x, y and b, with one matrix-free plan
+    held by the object (built from the first points on the host).
+
+    Every call gives the plan the new points on the device
+    (hm_plan_update_points) and records there whether they keep the plan's
+    tree; nothing is read back and nothing is allocated outside torch, so a
+    training step that calls it (forward and backward) can be captured into a
+    CUDA graph (torch.cuda.graph) and replayed. tree_ok() reads, with one
+    synchronisation, whether every call since the previous tree_ok() kept the
+    tree: when one did not, the products of those calls followed the old tree
+    (not the reference's for the new points), and rebuild(x, y) gives the
+    object a plan built for the new points (a graph captured before must be
+    captured again). kernel_matvec makes that check on every call instead."""
+
+    def __init__(self, kernel, x, y, intervals, blocksize, rank):
+        self.kernel, self.intervals = kernel, tuple(float(v) for v in intervals)
+        self.blocksize, self.rank = blocksize, rank
+        self.H = None
+        self.rebuild(x, y)
+
+    def rebuild(self, x, y):
+        if self.H is not None:
+            self.H.close()
+        self.H = KernelMatrix(self.kernel, x.detach().cpu().numpy(), y.detach().cpu().numpy(),
+                              *self.intervals, blocksize=self.blocksize, rank=self.rank,
+                              matrix_free=True)
+        self.H._status = torch.zeros(1, dtype=torch.int32, device=x.device)
+        self._bad = torch.zeros(1, dtype=torch.int32, device=x.device)
+
+    def __call__(self, x, y, b):
+        return _KernelProductFn.apply(x, y, b, self)
+
+    def tree_ok(self):
+        ok = int(self._bad.item()) == 0
+        self._bad.zero_()
+        return ok
+
+    def close(self):
+        if self.H is not None:
+            self.H.close()
+            self.H = None
 
 
 def kernel_matvec(kernel, x, y, b, intervals, blocksize, rank):
