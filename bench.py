@@ -1025,9 +1025,16 @@ def main():
     workloads = None
     if not args.no_workloads:
         wsteps, wwarm = max(3, min(args.steps, 10)), max(1, min(args.warmup, 2))
-        workloads = {"batched": summary_batched(wsteps, wwarm, stream),
-                     "gradient": summary_gradient(wsteps, wwarm, stream),
-                     "scale": summary_scale(max(2, wsteps // 2), 1, stream)}
+        workloads = {}
+        for wname, fn, st, wu in (("batched", summary_batched, wsteps, wwarm),
+                                  ("gradient", summary_gradient, wsteps, wwarm),
+                                  ("scale", summary_scale, max(2, wsteps // 2), 1)):
+            try:  # an error in one workload is reported in its place, not lost with the line
+                workloads[wname] = fn(st, wu, stream)
+            except Exception as e:  # noqa: BLE001
+                workloads[wname] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
     peaks = None if args.no_peaks else stream_peaks(stream)
     cpu = None
     if not args.no_cpu:
