@@ -290,7 +290,26 @@ def summary_batched(steps, warmup, stream):
                      "rel_err_vs_fp64_gpu": rel_err(Y, ref),
                      "rel_err_instance0_vs_dense_fp64": rel_err(Y[:n0], d0)}
     H.close()
+    del Y
+    out["leaf_tuning"] = leaf_tuning([(i["x"], i["y"], i["intervals"]) for i in inst])
     return out
+
+
+def leaf_tuning(spec, reps=3):
+    """hmx.tune_blocksize over leaf thresholds 32/64/128 and the hybrid and
+    matrix-free forms for the batch, 32 right-hand sides, FP64: each
+    candidate's tuned time and its error on sampled rows of four instances
+    against the dense FP64 product."""
+    import hmx
+    best, res = hmx.tune_blocksize("cauchykernel", spec, 16, k=32, candidates=(32, 64, 128),
+                                   forms=("hybrid", "matrix_free"), reps=reps, tol=1e-10)
+    return {"best": None if best is None else {
+                "blocksize": best["blocksize"], "form": best["form"],
+                "ms_per_step": round(best["ms"], 4), "rel_err_vs_dense_fp64": best["rel_err"],
+                "rhs_per_launch": best["rhs_per_launch"]},
+            "candidates": [{"blocksize": d["blocksize"], "form": d["form"],
+                            "ms_per_step": round(d["ms"], 4), "rel_err": d["rel_err"]}
+                           for d in res]}
 
 
 def summary_gradient(steps, warmup, stream):
@@ -517,6 +536,7 @@ def run_batched(args):
     best = min(v["ms_per_step"] for v in out["variants"].values())
     out["ms_per_step"] = best
     out["value"] = round(1000 / best, 3)
+    out["leaf_tuning"] = leaf_tuning(spec)
     return out
 
 

@@ -22,6 +22,7 @@ from . import lowrank  # noqa: F401
 from . import cholesky  # noqa: F401
 from .cholesky import hierarchicalcholesky, solvetransposed  # noqa: F401
 from . import autodiff  # noqa: F401
+from .tuning import tune_blocksize  # noqa: F401
 
 __all__ = [
     "BLOCKRANK", "BLOCKSIZE", "Block", "KernelMatrix", "HMError", "blocksize", "size",
@@ -30,5 +31,5 @@ __all__ = [
     "HierarchicalMatrix", "LowRankMatrix", "KernelMatrixBatch", "rmul_", "lmul_", "add_col_",
     "EvenBarycentricMatrix", "EvenBarycentricLeaves", "barycentricmatrix",
     "BarycentricPoly2D", "BarycentricMatrix2D", "BarycentricFactors", "evaluate", "update_",
-    "blocks_plan", "hierarchical", "UpperTriangular", "Diagonal",
+    "blocks_plan", "hierarchical", "UpperTriangular", "Diagonal", "tune_blocksize",
 ]

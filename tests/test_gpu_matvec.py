@@ -76,24 +76,37 @@ def test_log_entries_close(cuda):
     assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)) <= 4e-15
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_headline_size_sampled_rows(cuda, mode):
-    """n = 2^20, leaf 256, rank 24: sampled rows against the oracle's
-    traversal mul! (same per-row summation order as the full product)."""
+@pytest.fixture(scope="module")
+def headline_ref():
+    """The oracle's traversal mul! of the whole n = 2^20 product (leaf 256,
+    rank 24): a stored-factor oracle tree (about 17 GB of host memory, built
+    in about 15 s), its OpenMP product, and the tree freed."""
     w = hmx.workloads.config_headline()
+    t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], 256, 24)
+    ref = t.matvec_omp(w["b"])
+    del t
+    return w, ref
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_headline_size_full_vector(cuda, headline_ref, mode):
+    """n = 2^20, leaf 256, rank 24: all 2^20 entries against the oracle's
+    traversal mul! (relative 2-norm and entry by entry), and sampled rows
+    against the dense product."""
+    w, ref = headline_ref
     print(f"case: headline n=2^20 mode={mode}", flush=True)
     H = plan("cauchykernel", w, 256, 24, mode)
     u = (H @ torch.from_numpy(w["b"]).to(cuda)).cpu().numpy()
     H.close()
-    t = O.Tree(O.CAUCHY, w["x"], w["y"], *w["intervals"], 256, 24, store=False)
-    rng = np.random.default_rng(2)
-    rows = np.unique(np.concatenate([rng.integers(0, 1 << 20, 192), [0, 1, 127, 128, (1 << 20) - 1]]))
-    ref = t.matvec_rows(w["b"], rows)
-    assert rel(u[rows], ref) <= TOL
-    # properties over the whole vector: finite, and matches the dense product on a sample
     assert np.all(np.isfinite(u))
-    dense = O.dense_matvec(O.CAUCHY, w["x"], w["y"], w["b"], rows=rows[:32])
-    assert rel(u[rows[:32]], dense) <= 1e-11
+    e = rel(u, ref)
+    worst = float(np.max(np.abs(u - ref)) / np.max(np.abs(ref)))
+    print(f"headline {mode}: rel. 2-norm {e:.3e}, max |diff| / max |ref| {worst:.3e}", flush=True)
+    assert e <= TOL and worst <= TOL
+    rng = np.random.default_rng(2)
+    rows = np.unique(np.concatenate([rng.integers(0, 1 << 20, 32), [0, 1, 127, 128, (1 << 20) - 1]]))
+    dense = O.dense_matvec(O.CAUCHY, w["x"], w["y"], w["b"], rows=rows)
+    assert rel(u[rows], dense) <= 1e-11
 
 
 def test_stored_equals_free_full_vector(cuda):
