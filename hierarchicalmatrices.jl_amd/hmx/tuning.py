@@ -41,17 +41,17 @@ def _dense_rows(kid, x, y, B, rows, chunk=1 << 15):
 def tune_blocksize(kernel, spec, rank, *, k=1, candidates=(32, 64, 128, 256),
                    forms=("stored", "hybrid", "matrix_free"), reps=5, check_rows=64,
                    check_instances=4, tol=1e-10, seed=0):
-    """spec: (x, y, intervals) for one KernelMatrix, or a list of them for a
-    KernelMatrixBatch. Returns (best, results): results is a list of dicts
+    """spec: (x, y, intervals) for one KernelMatrix, or a sequence of them for
+    a KernelMatrixBatch. Returns (best, results): results is a list of dicts
     (blocksize, form, rhs_per_launch, launch_mode, ms, rel_err), fastest
     first; best is the fastest entry with rel_err <= tol (None if none)."""
     import torch
     name = kernel if isinstance(kernel, str) else kernel.name
     kid = KERNELS[name].kid
-    batch = isinstance(spec, list)
-    insts = spec if batch else [spec]
+    # a batch is a sequence of (x, y, intervals); one matrix is (x, y, intervals)
+    batch = isinstance(spec, (list, tuple)) and len(spec) > 0 and isinstance(spec[0], (list, tuple))
+    insts = list(spec) if batch else [spec]
     dev = torch.device("cuda")
-    n_rows = sum(len(i[0]) for i in insts)
     n_cols = sum(len(i[1]) for i in insts)
     rng = np.random.default_rng(seed)
     B = torch.from_numpy(rng.standard_normal((n_cols, k))).to(dev)
@@ -95,7 +95,6 @@ def tune_blocksize(kernel, spec, rank, *, k=1, candidates=(32, 64, 128, 256),
                                 launch_mode=int(mode), ms=ms, rel_err=(num / den) ** 0.5))
             H.close()
             del Y
-    assert n_rows == r0
     results.sort(key=lambda d: d["ms"])
     best = next((d for d in results if d["rel_err"] <= tol), None)
     return best, results
