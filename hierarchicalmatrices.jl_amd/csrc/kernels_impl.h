@@ -1665,6 +1665,16 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
     }
 }
 
+// Wavefronts per SIMD that the downward pass asks the register allocator
+// for. Left alone, it holds the 64 FP64 accumulators of a 64-row x 32-RHS
+// chunk in AGPRs beside up to 130 VGPRs and runs three wavefronts (two for the
+// per-level forms); the nested forms fit four (G = 2) or five (G = 1, ranks up
+// to 16) without spilling. The matrix-free log kernel and the larger ranks
+// keep the allocator's choice (they would spill).
+constexpr int down_m_waves(int rp, bool free, int kid, int g, bool nest) {
+    return (free && kid == 3) || !nest ? 1 : g == 1 ? (rp <= 16 ? 5 : 1) : (rp <= 32 ? 4 : 1);
+}
+
 // Downward pass, one wavefront per 64-row chunk. Rows 16t + j16 (t < 4) are
 // this lane's A-operand rows of the FP64 products (rows 16t + rho(j16) of the
 // reduced-precision ones). Near field: per 8 columns of a dense block the
@@ -1678,7 +1688,7 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 // NEST: the far field of a chunk is its row leaf's series alone (the
 // coefficients of every covering cluster pushed down the row tree into p.mx).
 template <int RP, int PREC, bool FREE, int KID, int G, bool NEST, class WS>
-__global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int nch,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves(RP, FREE, KID, G, NEST)))) void k_down_m(DevPlan p, const DChunk *ch, int nch,
                                               const DNear *nl, const DFar *fl,
                                               const double2 *cmapv, const double *pts_out,
                                               const double *pts_in, const double *xt,
@@ -1699,15 +1709,22 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
     const DChunk c = ch[wid];
     const int j16 = lane % 16, g = lane / 16;
     const int frow = PREC == 2 ? j16 : rho(j16);  // the far-field A-operand row
+    // the lane's row points: the regenerated near field needs them (po), the
+    // far field its A-operand rows' (pf, loaded after the near field when
+    // the near field is streamed, so no registers hold them meanwhile)
     double po[4], pf[4];
     bool rok[4], fok[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         rok[t] = t * 16 + j16 < c.np;
-        po[t] = rok[t] ? pts_out[c.p0 + t * 16 + j16] : 0.0;
         fok[t] = t * 16 + frow < c.np;
-        pf[t] = PREC == 2 ? po[t] : (fok[t] ? pts_out[c.p0 + t * 16 + frow] : 0.0);
+        po[t] = (FREE && rok[t]) ? pts_out[c.p0 + t * 16 + j16] : 0.0;
     }
+    auto load_pf = [&]() {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            pf[t] = (FREE && PREC == 2) ? po[t] : (fok[t] ? pts_out[c.p0 + t * 16 + frow] : 0.0);
+    };
     const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
     // NEST: the chunk's row leaf forms its coefficients (l2l_node from its
     // parent's) into LDS first, while no accumulators are live; the far field
@@ -1752,12 +1769,13 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
             const int64_t aoff = rl_l(d_aoff, e), col0 = rl_l(d_col0, e);
             const int bn = rl_i(d_n, e), boff = rl_i(d_off, e);
             const double *xtb = xt + col0 * ldxt;
-            // element (i, cc) of the block sits at rowoff(i) + 64 cc
-            int64_t rowoff[4];
+            const double *ab = FREE ? nullptr : p.A + aoff;
+            // element (i, cc) of the block sits at ab[rowoff(i) + 64 cc]
+            uint32_t rowoff[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int i = boff + t * 16 + j16;
-                rowoff[t] = aoff + (int64_t)(i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1));
+                rowoff[t] = (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1)));
             }
             for (int k0 = 0; k0 < bn; k0 += 8) {
                 double bv[G][2], av[2][4];
@@ -1777,7 +1795,7 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
                         av[h][t] = 0.0;
                         if (rok[t] && cok) {
                             if constexpr (FREE) av[h][t] = kern_fast<double, KID>(po[t], yc);
-                            else av[h][t] = __builtin_nontemporal_load(p.A + rowoff[t] + (int64_t)cc * HM_TILE);
+                            else av[h][t] = __builtin_nontemporal_load(ab + rowoff[t] + (uint32_t)(cc * HM_TILE));
                         }
                     }
                 }
@@ -1791,131 +1809,157 @@ __global__ __launch_bounds__(WG) void k_down_m(DevPlan p, const DChunk *ch, int 
             }
         }
     }
-    // ---- far field: the covering clusters' records one per lane; the
-    // coefficients of the next cluster load while this one is applied
-    // (NEST: the leaf's coefficients, from LDS)
-    auto load_bz = [&](double (*bz)[KSEL], int cl) {
+    // ---- far field
+    load_pf();
+    if constexpr (NEST) {
+        // one FP64 series per row: this lane's T_k (k = g + 4u; T_g and
+        // T_{g+4} from the prefix T_0..T_7, then T_{k+8} = 2 T_4 T_{k+4} - T_k)
+        // times the leaf's coefficients, read from LDS at each use rather than
+        // held across the four row tiles
+        const DTNode L = p.xtn[leaf];
 #pragma unroll
-        for (int gi = 0; gi < G; ++gi) {
-            const int q = gi * MQ + j16;
-            const double *zc;  // C[k][q]
-            if constexpr (NEST) zc = Dl + q * (r + 1);
-            else zc = z + ((int64_t)cl * MQT + q) * r;
+        for (int t = 0; t < 4; ++t) {
+            if (t >= nt) break;
+            const double xh = (pf[t] - L.cx) * L.cy;
+            const double x2 = xh + xh;
+            const double c2 = __builtin_fma(x2, xh, -1.0), c3 = __builtin_fma(x2, c2, -xh);
+            const double c4 = __builtin_fma(x2, c3, -c2), c5 = __builtin_fma(x2, c4, -c3);
+            const double c6 = __builtin_fma(x2, c5, -c4), c7 = __builtin_fma(x2, c6, -c5);
+            double ta = g == 0 ? 1.0 : g == 1 ? xh : g == 2 ? c2 : c3;  // T_g
+            double tb = g == 0 ? c4 : g == 1 ? c5 : g == 2 ? c6 : c7;   // T_{g+4}
+            const double t4x2 = c4 + c4;
 #pragma unroll
-            for (int u = 0; u < KSEL; ++u) {
-                const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
-                bz[gi][u] = (k < r && q < nq) ? zc[min(k, r - 1)] : 0.0;
+            for (int u = 0; u < RP / 4; ++u) {
+                const int k = 4 * u + g;
+                const double sel = (fok[t] && k < r) ? ta : 0.0;
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi) {
+                    const int q = gi * MQ + j16;
+                    const double v = Dl[q * (r + 1) + min(k, r - 1)];
+                    D[gi][t] = mfma64(sel, (k < r && q < nq) ? v : 0.0, D[gi][t]);
+                }
+                const double tn = __builtin_fma(t4x2, tb, -ta);  // T_{k+8}
+                ta = tb;
+                tb = tn;
             }
         }
-    };
-    const int fbeg = NEST ? 0 : c.far_beg, fend = NEST ? 1 : c.far_end;
-    for (int e0 = fbeg; e0 < fend; e0 += WAVE) {
-        const int ne = min(WAVE, fend - e0);
-        int f_cl = 0;
-        double f_cx = 0.0, f_cy = 0.0;
-        if (lane < ne) {
-            if constexpr (NEST) {
-                const DTNode L = p.xtn[leaf];
-                f_cl = leaf;
-                f_cx = L.cx;
-                f_cy = L.cy;
-            } else {
+    }
+    if constexpr (!NEST) {
+        // every covering cluster, its records one per lane; the coefficients
+        // of the next cluster load while this one is applied
+        auto load_bz = [&](double (*bz)[KSEL], int cl) {
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) {
+                const int q = gi * MQ + j16;
+                const double *zc = z + ((int64_t)cl * MQT + q) * r;  // C[k][q]
+#pragma unroll
+                for (int u = 0; u < KSEL; ++u) {
+                    const int k = BF ? (u / 8) * 32 + 8 * g + (u % 8) : 4 * u + g;
+                    bz[gi][u] = (k < r && q < nq) ? zc[min(k, r - 1)] : 0.0;
+                }
+            }
+        };
+        for (int e0 = c.far_beg; e0 < c.far_end; e0 += WAVE) {
+            const int ne = min(WAVE, c.far_end - e0);
+            int f_cl = 0;
+            double f_cx = 0.0, f_cy = 0.0;
+            if (lane < ne) {
                 f_cl = fl[e0 + lane].cl;
                 const double2 cmv = cmapv[f_cl];
                 f_cx = cmv.x;
                 f_cy = cmv.y;
             }
-        }
-        double bz[G][KSEL], bzn[G][KSEL] = {};
-        load_bz(bz, rl_i(f_cl, 0));
-        for (int e = 0; e < ne; ++e) {
-            const double2 cm = make_double2(rl_d(f_cx, e), rl_d(f_cy, e));
-            if (e + 1 < ne) load_bz(bzn, rl_i(f_cl, e + 1));
+            double bz[G][KSEL], bzn[G][KSEL] = {};
+            load_bz(bz, rl_i(f_cl, 0));
+            for (int e = 0; e < ne; ++e) {
+                const double2 cm = make_double2(rl_d(f_cx, e), rl_d(f_cy, e));
+                if (e + 1 < ne) load_bz(bzn, rl_i(f_cl, e + 1));
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (t >= nt) break;
-                const double xh = (pf[t] - cm.x) * cm.y;
-                double sel[KSEL];
-                if constexpr (!BF) {
-                    // this lane's T_k, k = g + 4u: T_g and T_{g+4} from the prefix
-                    // T_0..T_7, then T_{k+4} = 2 T_4 T_k - T_{k-4} (stride-4 recurrence)
-                    const double x2 = xh + xh;
-                    const double c2 = __builtin_fma(x2, xh, -1.0), c3 = __builtin_fma(x2, c2, -xh);
-                    const double c4 = __builtin_fma(x2, c3, -c2), c5 = __builtin_fma(x2, c4, -c3);
-                    const double c6 = __builtin_fma(x2, c5, -c4), c7 = __builtin_fma(x2, c6, -c5);
-                    double ta = g == 0 ? 1.0 : g == 1 ? xh : g == 2 ? c2 : c3;  // T_g
-                    double tb = g == 0 ? c4 : g == 1 ? c5 : g == 2 ? c6 : c7;   // T_{g+4}
-                    const double t4x2 = c4 + c4;
+                for (int t = 0; t < 4; ++t) {
+                    if (t >= nt) break;
+                    const double xh = (pf[t] - cm.x) * cm.y;
+                    double sel[KSEL];
+                    if constexpr (!BF) {
+                        // this lane's T_k, k = g + 4u: T_g and T_{g+4} from the prefix
+                        // T_0..T_7, then T_{k+4} = 2 T_4 T_k - T_{k-4} (stride-4 recurrence)
+                        const double x2 = xh + xh;
+                        const double c2 = __builtin_fma(x2, xh, -1.0), c3 = __builtin_fma(x2, c2, -xh);
+                        const double c4 = __builtin_fma(x2, c3, -c2), c5 = __builtin_fma(x2, c4, -c3);
+                        const double c6 = __builtin_fma(x2, c5, -c4), c7 = __builtin_fma(x2, c6, -c5);
+                        double ta = g == 0 ? 1.0 : g == 1 ? xh : g == 2 ? c2 : c3;  // T_g
+                        double tb = g == 0 ? c4 : g == 1 ? c5 : g == 2 ? c6 : c7;   // T_{g+4}
+                        const double t4x2 = c4 + c4;
 #pragma unroll
-                    for (int u = 0; u < KSEL; ++u) {
-                        sel[u] = (fok[t] && 4 * u + g < r) ? ta : 0.0;
-                        const double tn = __builtin_fma(t4x2, tb, -ta);  // T_{g+4u+8}
-                        ta = tb;
-                        tb = tn;
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < KSEL; ++u) sel[u] = 0.0;
-                    double t0 = 1.0, t1 = xh;
-#pragma unroll
-                    for (int k = 0; k < RP; ++k) {
-                        double tv = k == 0 ? t0 : t1;
-                        if (k >= 2) {
-                            tv = __builtin_fma(xh + xh, t1, -t0);
-                            t0 = t1;
-                            t1 = tv;
+                        for (int u = 0; u < KSEL; ++u) {
+                            sel[u] = (fok[t] && 4 * u + g < r) ? ta : 0.0;
+                            const double tn = __builtin_fma(t4x2, tb, -ta);  // T_{g+4u+8}
+                            ta = tb;
+                            tb = tn;
                         }
-                        const bool mine = ((k & 31) >> 3) == g;
-                        const int u = (k >> 5) * 8 + (k & 7);
-                        if (mine && fok[t] && k < r) sel[u] = tv;
-                    }
-                }
-                if constexpr (PREC == 2) {
+                    } else {
 #pragma unroll
-                    for (int u = 0; u < KSEL; ++u)
+                        for (int u = 0; u < KSEL; ++u) sel[u] = 0.0;
+                        double t0 = 1.0, t1 = xh;
 #pragma unroll
-                        for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(sel[u], bz[gi][u], D[gi][t]);
-                } else if constexpr (PREC == 0) {
-                    mf4_t D32[G];
-#pragma unroll
-                    for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int u = 0; u < KSEL; ++u)
-#pragma unroll
-                        for (int gi = 0; gi < G; ++gi)
-                            D32[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[gi][u], D32[gi], 0, 0, 0);
-                    // this cluster's FP32 sums into the FP64 accumulators
-#pragma unroll
-                    for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
-                } else {
-                    mf4_t D32[G];
-#pragma unroll
-                    for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int s8 = 0; s8 < KSEL / 8; ++s8) {
-                        float av[8];
-#pragma unroll
-                        for (int v = 0; v < 8; ++v) av[v] = (float)sel[s8 * 8 + v];
-                        mbf8_t ah, am, al;
-                        split8<PREC>(av, ah, am, al);
-#pragma unroll
-                        for (int gi = 0; gi < G; ++gi) {
-                            float bv[8];
-#pragma unroll
-                            for (int v = 0; v < 8; ++v) bv[v] = (float)bz[gi][s8 * 8 + v];
-                            mbf8_t bh, bm, bl;
-                            split8<PREC>(bv, bh, bm, bl);
-                            D32[gi] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[gi]);
+                        for (int k = 0; k < RP; ++k) {
+                            double tv = k == 0 ? t0 : t1;
+                            if (k >= 2) {
+                                tv = __builtin_fma(xh + xh, t1, -t0);
+                                t0 = t1;
+                                t1 = tv;
+                            }
+                            const bool mine = ((k & 31) >> 3) == g;
+                            const int u = (k >> 5) * 8 + (k & 7);
+                            if (mine && fok[t] && k < r) sel[u] = tv;
                         }
                     }
+                    if constexpr (PREC == 2) {
 #pragma unroll
-                    for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
+                        for (int u = 0; u < KSEL; ++u)
+#pragma unroll
+                            for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(sel[u], bz[gi][u], D[gi][t]);
+                    } else if constexpr (PREC == 0) {
+                        mf4_t D32[G];
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int u = 0; u < KSEL; ++u)
+#pragma unroll
+                            for (int gi = 0; gi < G; ++gi)
+                                D32[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)sel[u], (float)bz[gi][u], D32[gi], 0, 0, 0);
+                        // this cluster's FP32 sums into the FP64 accumulators
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
+                    } else {
+                        mf4_t D32[G];
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) D32[gi] = mf4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int s8 = 0; s8 < KSEL / 8; ++s8) {
+                            float av[8];
+#pragma unroll
+                            for (int v = 0; v < 8; ++v) av[v] = (float)sel[s8 * 8 + v];
+                            mbf8_t ah, am, al;
+                            split8<PREC>(av, ah, am, al);
+#pragma unroll
+                            for (int gi = 0; gi < G; ++gi) {
+                                float bv[8];
+#pragma unroll
+                                for (int v = 0; v < 8; ++v) bv[v] = (float)bz[gi][s8 * 8 + v];
+                                mbf8_t bh, bm, bl;
+                                split8<PREC>(bv, bh, bm, bl);
+                                D32[gi] = mfma_split<PREC>(ah, am, al, bh, bm, bl, D32[gi]);
+                            }
+                        }
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) D[gi][t] = add_f32_acc(D[gi][t], D32[gi]);
+                    }
                 }
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                    for (int u = 0; u < KSEL; ++u) bz[gi][u] = bzn[gi][u];
             }
-#pragma unroll
-            for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-                for (int u = 0; u < KSEL; ++u) bz[gi][u] = bzn[gi][u];
         }
     }
     if constexpr (NEST) wave_sync();  // the output tiles reuse the coefficients' LDS
