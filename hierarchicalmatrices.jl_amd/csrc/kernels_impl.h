@@ -1250,8 +1250,9 @@ __device__ __forceinline__ void store_ranks(const md4_t (&Mo)[G][RP / 16], S *ds
 // Node values w = A mu of one cluster from its moments held as in
 // store_ranks: Mo[gi][t][v] is B[k0 + g][j16] of the K-step k0 = 16t + 4v, so
 // the product takes mu from registers; A's rows are loaded once for the
-// groups. Stored as dst[q r + m] (FP64, or rounded to FP32 once).
-template <int RP, int G, class S>
+// groups. Stored as dst[q r + m], or (KQ, the nested far field's) as
+// dst[m 16G + q] (FP64, or rounded to FP32 once).
+template <int RP, int G, class S, bool KQ = false>
 __device__ __forceinline__ void nodes_from_moments(const md4_t (&Mo)[G][RP / 16],
                                                    const double *chebA, S *dst, int r,
                                                    int j16, int g) {
@@ -1277,7 +1278,7 @@ __device__ __forceinline__ void nodes_from_moments(const md4_t (&Mo)[G][RP / 16]
 #pragma unroll
                 for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[tt][t][v], Mo[gi][t][v], W[gi][tt]);
         }
-    store_ranks<RP, G>(W, dst, r, j16, g);
+    store_ranks<RP, G, KQ>(W, dst, r, j16, g);
 }
 
 // xt[j][q] = x[j*incx + q*ldx] for q < nq, 0 beyond (q < 16 G): the
@@ -1419,7 +1420,7 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
     if constexpr (NEST) {
         store_ranks<RP, G, true>(D64, mom + (int64_t)Tk.cl * MQT * r, r, j16, g);
         if (Tk.slot >= 0)
-            nodes_from_moments<RP, G>(D64, chebA, w + Tk.slot * MQT * r, r, j16, g);
+            nodes_from_moments<RP, G, WS, true>(D64, chebA, w + Tk.slot * MQT * r, r, j16, g);
     } else if (Tk.single) {
         nodes_from_moments<RP, G>(D64, chebA, w + (int64_t)Tk.cl * MQT * r, r, j16, g);
     } else {  // a multi-task cluster's slot gets mu itself
@@ -1486,7 +1487,7 @@ __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids
         }
     }
     store_ranks<RP, G, true>(W, mom + (int64_t)P * MQT * r, r, j16, g);
-    if (N.cl >= 0) nodes_from_moments<RP, G>(W, chebA, w + (int64_t)N.cl * MQT * r, r, j16, g);
+    if (N.cl >= 0) nodes_from_moments<RP, G, WS, true>(W, chebA, w + (int64_t)N.cl * MQT * r, r, j16, g);
 }
 
 // Nested far field, downward: the coefficients of row node N, c_N = z[its
@@ -1507,7 +1508,7 @@ __device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
                 double c = 0.0;
-                if (N.cl >= 0 && m < r) c = (double)z[((int64_t)N.cl * MQT + gi * MQ + j16) * r + m];
+                if (N.cl >= 0 && m < r) c = (double)z[((int64_t)N.cl * r + m) * MQT + gi * MQ + j16];
                 W[gi][t][v] = c;
             }
     if (N.parent >= 0) {  // wave-uniform
@@ -1568,7 +1569,7 @@ __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids
 // blocks of F_b W_b (F stored, or regenerated from the mapped nodes; each F
 // operand feeds the G groups), then C = A^T Z. The FP64 result layout of Z is
 // the B-operand layout of the second product, so Z feeds it from registers.
-template <int RP, bool FREE, int KID, int G, class WS>
+template <int RP, bool FREE, int KID, int G, class WS, bool KQ>
 __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *cl, int ncl,
                                                    const WS *w, WS *z) {
     constexpr int MT = RP / 16;
@@ -1613,7 +1614,8 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
                     const int lc = min(l, r - 1);
 #pragma unroll
                     for (int gi = 0; gi < G; ++gi) {
-                        const double v = (double)ws[(gi * MQ + j16) * r + lc];
+                        const double v = KQ ? (double)ws[lc * MQT + gi * MQ + j16]
+                                            : (double)ws[(gi * MQ + j16) * r + lc];
                         bv[gi][h] = lok ? v : 0.0;
                     }
                     const double tyl = FREE ? ty[lc] : 0.0;
@@ -1660,7 +1662,10 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
-                if (m < r) z[((int64_t)wid * MQT + gi * MQ + j16) * r + m] = (WS)Cc[t][v];
+                if (m < r) {
+                    if constexpr (KQ) z[((int64_t)wid * r + m) * MQT + gi * MQ + j16] = (WS)Cc[t][v];
+                    else z[((int64_t)wid * MQT + gi * MQ + j16) * r + m] = (WS)Cc[t][v];
+                }
             }
     }
 }
@@ -2035,7 +2040,7 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
                 k_m2m<RP, G, WS><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
                                                            p.chebA, wn, p.r);
         }
-        k_translate_m<RP, FREE, KID, G, WS><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, wn, zn);
+        k_translate_m<RP, FREE, KID, G, WS, true><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, wn, zn);
         for (int d = 0; d < p.nxlev; ++d) {
             const int off = p.xlev_off_h[d], cnt = p.xlev_off_h[d + 1] - off;
             if (cnt > 0)
@@ -2048,7 +2053,7 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
         if (p.nmulti_c > 0)
             k_reduce<double, true, MQT><<<nblk(p.nmulti_c), WG, 0, s>>>(
                 p.ccl, p.multi_c, p.nmulti_c, a.partial, p.chebA, a.w, p.r);
-        k_translate_m<RP, FREE, KID, G, double><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w,
+        k_translate_m<RP, FREE, KID, G, double, false><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, a.w,
                                                                            a.z);
     }
     if (p.nrch > 0) {
