@@ -1548,6 +1548,27 @@ __device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const
     }
 }
 
+// A chunk's row leaf's coefficients (l2l_node from its parent's) into the
+// wavefront's LDS, rows of r + 1 (padded against bank conflicts), where the
+// downward pass's far field reads them.
+template <int RP, int G, class WS>
+__device__ __forceinline__ void leaf_series_to_lds(const DevPlan &p, int leaf, const WS *z, int r,
+                                                   int j16, int g, double *Dl) {
+    const DTNode N = p.xtn[leaf];
+    md4_t W[G][RP / 16];
+    l2l_node<RP, G, WS>(N, p.nestM, z, p.mx, r, j16, g, W);
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int t = 0; t < RP / 16; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = t * 16 + g + 4 * v;
+                if (m < r) Dl[(gi * MQ + j16) * (r + 1) + m] = W[gi][t][v];
+            }
+    wave_sync();
+}
+
 // The coefficients of the internal row nodes of one depth (l2l_node), one
 // wavefront per node; a chunk's leaf forms its own in the downward pass.
 template <int RP, int G, class WS>
@@ -1704,11 +1725,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
     constexpr int MQT = MQ * G;
     // per wavefront: one 16-row output tile; with NEST first the leaf's
     // coefficients, rows of r + 1 (padded against bank conflicts)
-    constexpr int LDSW = (NEST && (RP + 1) * MQT > 16 * MQT) ? (RP + 1) * MQT : 16 * MQT;
+    constexpr int LDSW0 = (NEST && (RP + 1) * MQT > 16 * MQT) ? (RP + 1) * MQT : 16 * MQT;
+    // stored blocks: an 8-column slab of A and 8 rows of x first
+    constexpr int LDSW = (!FREE && LDSW0 < 8 * HM_TILE + 8 * MQT) ? 8 * HM_TILE + 8 * MQT : LDSW0;
     __shared__ double ds_all[WPB][LDSW];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
-    double *Dl = ds_all[(threadIdx.x / WAVE) % WPB];
+    double *Dl = ds_all[uniform((int)(threadIdx.x / WAVE)) % WPB];  // a scalar address
     if (wid >= nch) return;
     const int r = p.r;
     const DChunk c = ch[wid];
@@ -1732,88 +1755,166 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
     };
     const int nt = (c.np + 15) / 16;  // row tiles holding rows of this chunk (wave-uniform)
     // NEST: the chunk's row leaf forms its coefficients (l2l_node from its
-    // parent's) into LDS first, while no accumulators are live; the far field
-    // reads them there
-    int leaf = 0;
-    if constexpr (NEST) {
-        leaf = p.chunk_leaf[wid];
-        const DTNode N = p.xtn[leaf];
-        md4_t W[G][RP / 16];
-        l2l_node<RP, G, WS>(N, p.nestM, z, p.mx, r, j16, g, W);
-#pragma unroll
-        for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-            for (int t = 0; t < RP / 16; ++t)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = t * 16 + g + 4 * v;
-                    if (m < r) Dl[(gi * MQ + j16) * (r + 1) + m] = W[gi][t][v];
-                }
-        wave_sync();
-    }
+    // parent's) into LDS, where the far field reads them: before the near
+    // field while no accumulators are live (matrix-free), or after it (stored
+    // blocks pass through the same LDS)
+    const int leaf = NEST ? p.chunk_leaf[wid] : 0;
+    if constexpr (NEST && FREE) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
     md4_t D[G][4];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
 #pragma unroll
         for (int t = 0; t < 4; ++t) D[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
-    // ---- near field, FP64: two K-steps (8 columns) per group, operands first;
-    // the chunk's block records are read together, one per lane
+    // ---- near field, FP64: two K-steps (8 columns) per step; the chunk's
+    // block records are read together, one per lane
     for (int e0 = c.near_beg; e0 < c.near_end; e0 += WAVE) {
         const int ne = min(WAVE, c.near_end - e0);
         int64_t d_aoff = 0, d_col0 = 0;
-        int d_n = 0, d_off = 0;
+        int d_n = 0, d_off = 0, d_m = 1;
         if (lane < ne) {
             const DNear sn = nl[e0 + lane];
             const DDense Dd = p.dense[sn.blk];
             d_aoff = Dd.aoff;
             d_col0 = Dd.col0;
             d_n = Dd.n;
+            if constexpr (!FREE) d_m = Dd.m;
             d_off = sn.off;
         }
-        for (int e = 0; e < ne; ++e) {
-            const int64_t aoff = rl_l(d_aoff, e), col0 = rl_l(d_col0, e);
-            const int bn = rl_i(d_n, e), boff = rl_i(d_off, e);
-            const double *xtb = xt + col0 * ldxt;
-            const double *ab = FREE ? nullptr : p.A + aoff;
-            // element (i, cc) of the block sits at ab[rowoff(i) + 64 cc]
-            uint32_t rowoff[4];
+        if constexpr (FREE) {
+            // entries regenerated from the points straight into the A operand
+            for (int e = 0; e < ne; ++e) {
+                const int64_t col0 = rl_l(d_col0, e);
+                const int bn = rl_i(d_n, e);
+                const double *xtb = xt + col0 * ldxt;
+                for (int k0 = 0; k0 < bn; k0 += 8) {
+                    double bv[G][2], av[2][4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int i = boff + t * 16 + j16;
-                rowoff[t] = (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1)));
+                    for (int h = 0; h < 2; ++h) {
+                        const int cc = k0 + 4 * h + g;
+                        const bool cok = cc < bn;
+#pragma unroll
+                        for (int gi = 0; gi < G; ++gi) {
+                            const int q = gi * MQ + j16;
+                            bv[gi][h] = (cok && q < nq) ? xtb[(int64_t)cc * ldxt + q] : 0.0;
+                        }
+                        const double yc = cok ? pts_in[col0 + cc] : 0.0;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            av[h][t] = 0.0;
+                            if (rok[t] && cok) av[h][t] = kern_fast<double, KID>(po[t], yc);
+                        }
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (t < nt)
+#pragma unroll
+                                for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(av[h][t], bv[gi][h], D[gi][t]);
+                }
             }
-            for (int k0 = 0; k0 < bn; k0 += 8) {
-                double bv[G][2], av[2][4];
+        } else {
+            // stored blocks: one slab of 8 columns (the chunk's 64 rows of the
+            // block, 4 KB) and the matching 8 rows of x (8 x 16G values) at a
+            // time through this wavefront's LDS, copied by global_load_lds
+            // (one 4-byte word per lane and instruction, no registers held);
+            // the next slab's copies are issued as soon as this slab's
+            // operands are in registers, and run while it multiplies. Row
+            // indices past the block's last are clamped to it and columns past
+            // its last to that one (in-bounds copies, zeroed at the operand);
+            // element (i, cc) of a block sits at rowoff(i) + 64 cc.
+            using lds_ptr = __attribute__((address_space(3))) void *;
+            constexpr int WR = 2 * MQT;     // 4-byte words per row of x
+            constexpr int RPI = WAVE / WR;  // rows of x per copy instruction
+            double *Sa = Dl;                // [8][64]: column-major slab of A
+            double *Sx = Dl + 8 * HM_TILE;  // [8][16G]: rows of x
+            uint32_t voa[2] = {0, 0};
+            // this lane's word of x: row lane / WR of an instruction, value
+            // min(q, nq - 1), half lane % 2
+            const int xq = min((lane % WR) / 2, nq - 1);
+            const uint32_t vox = 8u * (uint32_t)xq + 4u * (uint32_t)(lane & 1);
+            auto setup = [&](int e) {  // this lane's two rows of block e, as byte offsets
+                const int bn = rl_i(d_n, e), boff = rl_i(d_off, e), bm = rl_i(d_m, e);
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const int i = min(boff + 32 * hh + lane / 2, bm - 1);
+                    voa[hh] = 8u * (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1))) +
+                              4u * (uint32_t)(lane & 1);
+                }
+            };
+            auto issue = [&](int e, int k0) {
+                const int bn = rl_i(d_n, e);
+                if (bn == 0) return;  // an empty block (wave-uniform): nothing to copy
+                const char *ab = reinterpret_cast<const char *>(p.A + rl_l(d_aoff, e));
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int cc = min(k0 + j / 2, bn - 1);
+                    __builtin_amdgcn_global_load_lds(ab + (int64_t)cc * (8 * HM_TILE) + voa[j % 2],
+                                                     (lds_ptr)(Sa + 32 * j), 4, 0, 0);
+                }
+                const char *xb = reinterpret_cast<const char *>(xt + rl_l(d_col0, e) * ldxt);
+#pragma unroll
+                for (int jx = 0; jx < 8 / RPI; ++jx) {
+                    const int cc = min(k0 + jx * RPI + lane / WR, bn - 1);
+                    __builtin_amdgcn_global_load_lds(xb + (int64_t)cc * ldxt * 8 + vox,
+                                                     (lds_ptr)(Sx + jx * RPI * MQT), 4, 0, 0);
+                }
+            };
+            int e = 0, k0 = 0;
+            if (ne > 0) {
+                setup(0);
+                issue(0, 0);
+            }
+            while (e < ne) {
+                const int bn = rl_i(d_n, e);
+                // the slab's copies complete before these reads: the compiler
+                // does not count global_load_lds among the writes a LDS read
+                // waits for, so the wait is explicit (and a barrier to moving
+                // the reads above it)
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                double av[2][4], bv[2][G];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const int cc = k0 + 4 * h + g;
-                    const bool cok = cc < bn;
+                    const bool cok = k0 + 4 * h + g < bn;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const double v = Sa[(4 * h + g) * HM_TILE + 16 * t + j16];
+                        av[h][t] = (rok[t] && cok) ? v : 0.0;
+                    }
 #pragma unroll
                     for (int gi = 0; gi < G; ++gi) {
                         const int q = gi * MQ + j16;
-                        bv[gi][h] = (cok && q < nq) ? xtb[(int64_t)cc * ldxt + q] : 0.0;
-                    }
-                    double yc = 0.0;
-                    if constexpr (FREE) yc = cok ? pts_in[col0 + cc] : 0.0;
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        av[h][t] = 0.0;
-                        if (rok[t] && cok) {
-                            if constexpr (FREE) av[h][t] = kern_fast<double, KID>(po[t], yc);
-                            else av[h][t] = __builtin_nontemporal_load(ab + rowoff[t] + (uint32_t)(cc * HM_TILE));
-                        }
+                        const double v = Sx[(4 * h + g) * MQT + q];
+                        bv[h][gi] = (cok && q < nq) ? v : 0.0;
                     }
                 }
+                int e2 = e, k2 = k0 + 8;
+                if (k2 >= bn) {
+                    ++e2;
+                    k2 = 0;
+                }
+                // the next slab's copies overwrite this one after the reads
+                // have returned their data
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                if (e2 < ne) {
+                    if (e2 != e) setup(e2);
+                    issue(e2, k2);
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
                         if (t < nt)
 #pragma unroll
-                            for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(av[h][t], bv[gi][h], D[gi][t]);
+                            for (int gi = 0; gi < G; ++gi) D[gi][t] = mfma64(av[h][t], bv[h][gi], D[gi][t]);
+                e = e2;
+                k0 = k2;
             }
+            wave_sync();  // every lane's slab reads done before LDS is reused
         }
     }
+    if constexpr (NEST && !FREE) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
     // ---- far field
     load_pf();
     if constexpr (NEST) {
