@@ -54,6 +54,9 @@ def parse():
                     help="headline = the n=2^20 product (the bench line); the others time "
                          "the batched, gradient, scale, EvenBarycentricMatrix, LowRankMatrix "
                          "and hierarchical Cholesky workloads")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="batched workload: time the products only (no autotuner, graph "
+                         "replays or leaf-size tuning), for kernel traces and counters")
     args = ap.parse_args()
     args.variants_given = args.variants is not None
     if args.variants is None:
@@ -515,7 +518,7 @@ def run_batched(args):
                 "ms_per_step": round(ms_m, 4), "value": round(1000 / ms_m, 3),
                 "rel_err_vs_fp64": float(torch.linalg.norm(Ym - ref) / torch.linalg.norm(ref))}
             del Ym
-        if name == "matrix_free" or name == "hybrid":
+        if (name == "matrix_free" or name == "hybrid") and not args.no_tuning:
             kb, mode, us = H.tune(k=32)
             ms_t = timed(lambda: H.matvec(B, tuned=True), args.steps, args.warmup, stream)
             g = H.graph(Y, B, tuned=True)
@@ -536,7 +539,8 @@ def run_batched(args):
     best = min(v["ms_per_step"] for v in out["variants"].values())
     out["ms_per_step"] = best
     out["value"] = round(1000 / best, 3)
-    out["leaf_tuning"] = leaf_tuning(spec)
+    if not args.no_tuning:
+        out["leaf_tuning"] = leaf_tuning(spec)
     return out
 
 
