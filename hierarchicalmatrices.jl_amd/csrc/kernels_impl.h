@@ -1694,11 +1694,11 @@ __global__ __launch_bounds__(WG) void k_translate_m(DevPlan p, const DCluster *c
 // Wavefronts per SIMD that the downward pass asks the register allocator
 // for. Left alone, it holds the 64 FP64 accumulators of a 64-row x 32-RHS
 // chunk in AGPRs beside up to 130 VGPRs and runs three wavefronts (two for the
-// per-level forms); the nested forms fit four (G = 2) or five (G = 1, ranks up
-// to 16) without spilling. The matrix-free log kernel and the larger ranks
-// keep the allocator's choice (they would spill).
+// per-level forms); the nested forms up to rank 16 fit four (G = 2) or five
+// (G = 1). Above rank 16 (where a forced count was measured slower or would
+// spill) and for the matrix-free log kernel the allocator keeps its choice.
 constexpr int down_m_waves(int rp, bool free, int kid, int g, bool nest) {
-    return (free && kid == 3) || !nest ? 1 : g == 1 ? (rp <= 16 ? 5 : 1) : (rp <= 32 ? 4 : 1);
+    return (free && kid == 3) || !nest || rp > 16 ? 1 : g == 1 ? 5 : 4;
 }
 
 // Downward pass, one wavefront per 64-row chunk. Rows 16t + j16 (t < 4) are
@@ -1723,11 +1723,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
     constexpr bool BF = PREC == 1 || PREC == 3;  // split-BF16 operands (K = 32 shape)
     constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     constexpr int MQT = MQ * G;
+    // stored blocks stream through LDS slabs, except where the slab path's
+    // registers would cost the occupancy (two groups above rank 16): there
+    // the block entries load straight into the A operand
+    constexpr bool SLAB = !FREE && RP <= 16;
     // per wavefront: one 16-row output tile; with NEST first the leaf's
     // coefficients, rows of r + 1 (padded against bank conflicts)
     constexpr int LDSW0 = (NEST && (RP + 1) * MQT > 16 * MQT) ? (RP + 1) * MQT : 16 * MQT;
     // stored blocks: an 8-column slab of A and 8 rows of x first
-    constexpr int LDSW = (!FREE && LDSW0 < 8 * HM_TILE + 8 * MQT) ? 8 * HM_TILE + 8 * MQT : LDSW0;
+    constexpr int LDSW = (SLAB && LDSW0 < 8 * HM_TILE + 8 * MQT) ? 8 * HM_TILE + 8 * MQT : LDSW0;
     __shared__ double ds_all[WPB][LDSW];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
@@ -1759,7 +1763,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
     // field while no accumulators are live (matrix-free), or after it (stored
     // blocks pass through the same LDS)
     const int leaf = NEST ? p.chunk_leaf[wid] : 0;
-    if constexpr (NEST && FREE) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
+    if constexpr (NEST && !SLAB) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
     md4_t D[G][4];
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
@@ -1777,15 +1781,24 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
             d_aoff = Dd.aoff;
             d_col0 = Dd.col0;
             d_n = Dd.n;
-            if constexpr (!FREE) d_m = Dd.m;
+            if constexpr (SLAB) d_m = Dd.m;
             d_off = sn.off;
         }
-        if constexpr (FREE) {
-            // entries regenerated from the points straight into the A operand
+        if constexpr (!SLAB) {
+            // entries regenerated from the points (or loaded) straight into
+            // the A operand
             for (int e = 0; e < ne; ++e) {
                 const int64_t col0 = rl_l(d_col0, e);
                 const int bn = rl_i(d_n, e);
                 const double *xtb = xt + col0 * ldxt;
+                const double *ab = FREE ? nullptr : p.A + rl_l(d_aoff, e);
+                // element (i, cc) of a stored block sits at ab[rowoff(i) + 64 cc]
+                uint32_t rowoff[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = rl_i(d_off, e) + t * 16 + j16;
+                    rowoff[t] = FREE ? 0u : (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1)));
+                }
                 for (int k0 = 0; k0 < bn; k0 += 8) {
                     double bv[G][2], av[2][4];
 #pragma unroll
@@ -1797,11 +1810,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
                             const int q = gi * MQ + j16;
                             bv[gi][h] = (cok && q < nq) ? xtb[(int64_t)cc * ldxt + q] : 0.0;
                         }
-                        const double yc = cok ? pts_in[col0 + cc] : 0.0;
+                        double yc = 0.0;
+                        if constexpr (FREE) yc = cok ? pts_in[col0 + cc] : 0.0;
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             av[h][t] = 0.0;
-                            if (rok[t] && cok) av[h][t] = kern_fast<double, KID>(po[t], yc);
+                            if (rok[t] && cok) {
+                                if constexpr (FREE) av[h][t] = kern_fast<double, KID>(po[t], yc);
+                                else av[h][t] = __builtin_nontemporal_load(ab + rowoff[t] + (uint32_t)(cc * HM_TILE));
+                            }
                         }
                     }
 #pragma unroll
@@ -1914,7 +1931,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
             wave_sync();  // every lane's slab reads done before LDS is reused
         }
     }
-    if constexpr (NEST && !FREE) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
+    if constexpr (NEST && SLAB) leaf_series_to_lds<RP, G, WS>(p, leaf, z, r, j16, g, Dl);
     // ---- far field
     load_pf();
     if constexpr (NEST) {
