@@ -1834,47 +1834,81 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
             // stored blocks: one slab of 8 columns (the chunk's 64 rows of the
             // block, 4 KB) and the matching 8 rows of x (8 x 16G values) at a
             // time through this wavefront's LDS, copied by global_load_lds
-            // (one 4-byte word per lane and instruction, no registers held);
-            // the next slab's copies are issued as soon as this slab's
-            // operands are in registers, and run while it multiplies. Row
-            // indices past the block's last are clamped to it and columns past
-            // its last to that one (in-bounds copies, zeroed at the operand);
-            // element (i, cc) of a block sits at rowoff(i) + 64 cc.
+            // (no registers held); the next slab's copies are issued as soon
+            // as this slab's operands are in registers, and run while it
+            // multiplies. A lane copies 16 bytes per instruction (two rows of
+            // a column: the chunk starts on an even row of the block, and a
+            // 64-row tile, padded in storage, never splits such a pair; two
+            // values of a row of x: nq even, x 16-byte aligned) or else one
+            // 4-byte word. Rows past the block's last are clamped to it (or,
+            // in pairs, to the chunk's first) and columns past its last to
+            // that one: in-bounds copies, zeroed at the operand. Element
+            // (i, cc) of a block sits at rowoff(i) + 64 cc.
             using lds_ptr = __attribute__((address_space(3))) void *;
             constexpr int WR = 2 * MQT;     // 4-byte words per row of x
-            constexpr int RPI = WAVE / WR;  // rows of x per copy instruction
+            constexpr int RPI = WAVE / WR;  // rows of x per 4-byte copy instruction
+            constexpr int LR = MQT / 2;     // lanes per row of x, 16-byte copies
+            constexpr int RPI16 = WAVE / LR;
             double *Sa = Dl;                // [8][64]: column-major slab of A
             double *Sx = Dl + 8 * HM_TILE;  // [8][16G]: rows of x
+            const bool x16 = nq % 2 == 0 && ldxt % 2 == 0 &&
+                             (reinterpret_cast<uintptr_t>(xt) & 15) == 0;  // wave-uniform
             uint32_t voa[2] = {0, 0};
-            // this lane's word of x: row lane / WR of an instruction, value
-            // min(q, nq - 1), half lane % 2
-            const int xq = min((lane % WR) / 2, nq - 1);
-            const uint32_t vox = 8u * (uint32_t)xq + 4u * (uint32_t)(lane & 1);
-            auto setup = [&](int e) {  // this lane's two rows of block e, as byte offsets
+            // this lane's part of a row of x: row lane / WR (lane / LR) of an
+            // instruction; value min(q, nq - 1), half lane % 2 (values
+            // min(2 (lane % LR), nq - 2) and the next)
+            const uint32_t vox =
+                x16 ? 8u * (uint32_t)min(2 * (lane % LR), nq - 2)
+                    : 8u * (uint32_t)min((lane % WR) / 2, nq - 1) + 4u * (uint32_t)(lane & 1);
+            auto setup = [&](int e) {  // this lane's rows of block e, as byte offsets
                 const int bn = rl_i(d_n, e), boff = rl_i(d_off, e), bm = rl_i(d_m, e);
+                if ((boff & 1) == 0) {
+                    int i = boff + 2 * (lane % 32);
+                    if (i > bm - 1) i = boff;
+                    voa[0] = 8u * (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1)));
+                } else {
 #pragma unroll
-                for (int hh = 0; hh < 2; ++hh) {
-                    const int i = min(boff + 32 * hh + lane / 2, bm - 1);
-                    voa[hh] = 8u * (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1))) +
-                              4u * (uint32_t)(lane & 1);
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const int i = min(boff + 32 * hh + lane / 2, bm - 1);
+                        voa[hh] = 8u * (uint32_t)((i >> 6) * HM_TILE * bn + (i & (HM_TILE - 1))) +
+                                  4u * (uint32_t)(lane & 1);
+                    }
                 }
             };
             auto issue = [&](int e, int k0) {
                 const int bn = rl_i(d_n, e);
                 if (bn == 0) return;  // an empty block (wave-uniform): nothing to copy
                 const char *ab = reinterpret_cast<const char *>(p.A + rl_l(d_aoff, e));
+                if ((rl_i(d_off, e) & 1) == 0) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int cc = min(k0 + j / 2, bn - 1);
-                    __builtin_amdgcn_global_load_lds(ab + (int64_t)cc * (8 * HM_TILE) + voa[j % 2],
-                                                     (lds_ptr)(Sa + 32 * j), 4, 0, 0);
+                    for (int j = 0; j < 4; ++j) {
+                        const int cc = min(k0 + 2 * j + lane / 32, bn - 1);
+                        __builtin_amdgcn_global_load_lds(ab + (int64_t)cc * (8 * HM_TILE) + voa[0],
+                                                         (lds_ptr)(Sa + 128 * j), 16, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int cc = min(k0 + j / 2, bn - 1);
+                        __builtin_amdgcn_global_load_lds(ab + (int64_t)cc * (8 * HM_TILE) + voa[j % 2],
+                                                         (lds_ptr)(Sa + 32 * j), 4, 0, 0);
+                    }
                 }
                 const char *xb = reinterpret_cast<const char *>(xt + rl_l(d_col0, e) * ldxt);
+                if (x16) {
 #pragma unroll
-                for (int jx = 0; jx < 8 / RPI; ++jx) {
-                    const int cc = min(k0 + jx * RPI + lane / WR, bn - 1);
-                    __builtin_amdgcn_global_load_lds(xb + (int64_t)cc * ldxt * 8 + vox,
-                                                     (lds_ptr)(Sx + jx * RPI * MQT), 4, 0, 0);
+                    for (int jx = 0; jx < 8 / RPI16; ++jx) {
+                        const int cc = min(k0 + jx * RPI16 + lane / LR, bn - 1);
+                        __builtin_amdgcn_global_load_lds(xb + (int64_t)cc * ldxt * 8 + vox,
+                                                         (lds_ptr)(Sx + jx * RPI16 * MQT), 16, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int jx = 0; jx < 8 / RPI; ++jx) {
+                        const int cc = min(k0 + jx * RPI + lane / WR, bn - 1);
+                        __builtin_amdgcn_global_load_lds(xb + (int64_t)cc * ldxt * 8 + vox,
+                                                         (lds_ptr)(Sx + jx * RPI * MQT), 4, 0, 0);
+                    }
                 }
             };
             int e = 0, k0 = 0;

@@ -117,6 +117,13 @@ def test_gpu_strided_product_vs_oracle(cuda, seed):
     B = rng.standard_normal((cols, 5))
     Y = H.plan().matvec(torch.from_numpy(B).to(cuda)).cpu().numpy()
     assert np.linalg.norm(Y - D @ B) <= 1e-13 * np.linalg.norm(D @ B)
+    # on the matrix cores: the leaves' odd sizes start chunks at odd rows of
+    # the dense blocks around them; 32 right-hand sides (copies of row pairs
+    # and of value pairs of x) and 17 (single values)
+    for k in (32, 17):
+        B = rng.standard_normal((cols, k))
+        Y = H.plan().matvec(torch.from_numpy(B).to(cuda), mfma=True).cpu().numpy()
+        assert np.linalg.norm(Y - D @ B) <= 1e-13 * np.linalg.norm(D @ B), k
 
 
 # ------------------------------------- scale! and add_col! (src/HierarchicalMatrix.jl)
