@@ -1428,11 +1428,80 @@ __global__ __launch_bounds__(WAVE) void k_up_m(const DTask *tasks, const double 
     }
 }
 
-// Nested far field, upward: the moments of each internal column node of one
-// depth from its two halves' (mu_P = M_0 mu_0 + M_1 mu_1, the halves' series
-// re-expanded on the parent's interval), one wavefront per node; a node that
-// is a column cluster also gets w = A mu.
-template <int RP, int G, class WS>
+// Nested far field, level kernels. Moments combine up the column tree
+// (mu_P = M_0 mu_0 + M_1 mu_1: the halves' series re-expanded on the parent's
+// interval) and coefficients go down the row tree (c_N = z[its row cluster] +
+// M_h^T c_parent), one wavefront per node. A node's result, held as the MFMA
+// result layout (value v of lane (j16, g) is row 16t + 4v + g), is the B
+// operand of the K-step 16t + 4v as it stands, so a wavefront can go on to the
+// next depth from registers: the upward launches cover two depths each, and
+// the moments of the lower one never reach memory.
+
+// W += M_h mu, mu in the result layout (rows past r zero)
+template <int RP, int G>
+__device__ __forceinline__ void m2m_acc(const double *Mh, const md4_t (&mu)[G][RP / 16], int r,
+                                        int j16, int g, md4_t (&W)[G][RP / 16]) {
+    constexpr int MT = RP / 16;
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        double av[MT][4];
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                const double a = Mh[min(m, r - 1) * r + min(k, r - 1)];
+                av[t][v] = (m < r && k < r) ? a : 0.0;
+            }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], mu[gi][t][v], W[gi][tt]);
+    }
+}
+
+// The stored moments of node C in the result layout (zero past r)
+template <int RP, int G>
+__device__ __forceinline__ void load_moments(const double *mom, int C, int r, int j16, int g,
+                                             md4_t (&mu)[G][RP / 16]) {
+    constexpr int MQT = MQ * G;
+    const double *src = mom + (int64_t)C * MQT * r;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int t = 0; t < RP / 16; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int k = t * 16 + 4 * v + g;
+                const double b = src[(int64_t)min(k, r - 1) * MQT + gi * MQ + j16];
+                mu[gi][t][v] = k < r ? b : 0.0;
+            }
+}
+
+// mu_N from its two halves' stored moments (an empty half's stay zero)
+template <int RP, int G>
+__device__ __forceinline__ void m2m_node(const DTNode &N, const double *M, const double *mom, int r,
+                                         int j16, int g, md4_t (&W)[G][RP / 16]) {
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+        for (int t = 0; t < RP / 16; ++t) W[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        md4_t mu[G][RP / 16];
+        load_moments<RP, G>(mom, N.child[h], r, j16, g, mu);
+        m2m_acc<RP, G>(M + (int64_t)h * r * r, mu, r, j16, g, W);
+    }
+}
+
+// Upward, one depth: the moments of its internal column nodes (stored), and
+// w = A mu of those that are column clusters. TWO: also the depth below, each
+// internal half formed from its own halves in registers (its w stored, its
+// moments not: only this node reads them); every load is issued before the
+// first product, so the halves' chains overlap.
+template <int RP, int G, class WS, bool TWO>
 __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids, int n,
                                            const double *M, double *mom, const double *chebA,
                                            WS *w, int r) {
@@ -1445,65 +1514,62 @@ __global__ __launch_bounds__(WG) void k_m2m(const DTNode *tn, const int32_t *ids
     const int P = ids[wid];
     const DTNode N = tn[P];
     md4_t W[G][MT];
+    if constexpr (!TWO) {
+        m2m_node<RP, G>(N, M, mom, r, j16, g, W);
+    } else {
+        const DTNode Cn[2] = {tn[N.child[0]], tn[N.child[1]]};
+        // internal halves (wave-uniform); a leaf's moments come from the
+        // upward pass, an empty half's are zero
+        bool in[2];
+        md4_t qm[2][2][G][MT];  // per half: its halves' moments, or its own in [0]
 #pragma unroll
-    for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-        for (int t = 0; t < MT; ++t) W[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int C = N.child[h];  // an empty half's moments stay zero
-        const double *Mh = M + (int64_t)h * r * r;
-        const double *src = mom + (int64_t)C * MQT * r;
-        // A[m][k] = M_h[m][k], B[k][q] = mu_C[k][q]; loads in bounds, the
-        // terms past r zeroed after; one output tile's A operands at a time
-        double bv[G][MT][4];
+        for (int h = 0; h < 2; ++h) {
+            in[h] = Cn[h].child[0] >= 0 && Cn[h].len > 0;
+            load_moments<RP, G>(mom, in[h] ? Cn[h].child[0] : N.child[h], r, j16, g, qm[h][0]);
+            if (in[h]) load_moments<RP, G>(mom, Cn[h].child[1], r, j16, g, qm[h][1]);
+        }
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-            for (int t = 0; t < MT; ++t)
+            for (int t = 0; t < MT; ++t) W[gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
+        md4_t mu[2][G][MT];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int k = t * 16 + 4 * v + g;
-                    const double b = src[(int64_t)min(k, r - 1) * MQT + gi * MQ + j16];
-                    bv[gi][t][v] = k < r ? b : 0.0;
-                }
+        for (int h = 0; h < 2; ++h) {
+            if (in[h]) {
 #pragma unroll
-        for (int tt = 0; tt < MT; ++tt) {
-            double av[MT][4];
+                for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-            for (int t = 0; t < MT; ++t)
+                    for (int t = 0; t < MT; ++t) mu[h][gi][t] = md4_t{0.0, 0.0, 0.0, 0.0};
+                m2m_acc<RP, G>(M, qm[h][0], r, j16, g, mu[h]);
+                m2m_acc<RP, G>(M + (int64_t)r * r, qm[h][1], r, j16, g, mu[h]);
+            } else {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
-                    const double a = Mh[min(m, r - 1) * r + min(k, r - 1)];
-                    av[t][v] = (m < r && k < r) ? a : 0.0;
-                }
+                for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-            for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-                for (int t = 0; t < MT; ++t)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], bv[gi][t][v], W[gi][tt]);
+                    for (int t = 0; t < MT; ++t) mu[h][gi][t] = qm[h][0][gi][t];
+            }
+            m2m_acc<RP, G>(M + (int64_t)h * r * r, mu[h], r, j16, g, W);
         }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (in[h] && Cn[h].cl >= 0)
+                nodes_from_moments<RP, G, WS, true>(mu[h], chebA, w + (int64_t)Cn[h].cl * MQT * r, r,
+                                                    j16, g);
     }
     store_ranks<RP, G, true>(W, mom + (int64_t)P * MQT * r, r, j16, g);
     if (N.cl >= 0) nodes_from_moments<RP, G, WS, true>(W, chebA, w + (int64_t)N.cl * MQT * r, r, j16, g);
 }
 
-// Nested far field, downward: the coefficients of row node N, c_N = z[its
-// row cluster] (if it is one) + M_h^T c_parent (the parent's series
-// re-expanded on this half's interval), in the FP64 result layout (as
-// store_ranks reads it).
+// z[N's row cluster] (zero if it is none) in the result layout (the FP64
+// result layout, as store_ranks reads it)
 template <int RP, int G, class WS>
-__device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const WS *z,
-                                         const double *cof, int r, int j16, int g,
-                                         md4_t (&W)[G][RP / 16]) {
-    constexpr int MT = RP / 16;
+__device__ __forceinline__ void load_z(const DTNode &N, const WS *z, int r, int j16, int g,
+                                       md4_t (&W)[G][RP / 16]) {
     constexpr int MQT = MQ * G;
 #pragma unroll
     for (int gi = 0; gi < G; ++gi)
 #pragma unroll
-        for (int t = 0; t < MT; ++t)
+        for (int t = 0; t < RP / 16; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = t * 16 + g + 4 * v;
@@ -1511,41 +1577,44 @@ __device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const
                 if (N.cl >= 0 && m < r) c = (double)z[((int64_t)N.cl * r + m) * MQT + gi * MQ + j16];
                 W[gi][t][v] = c;
             }
-    if (N.parent >= 0) {  // wave-uniform
-        const double *Mh = M + (int64_t)N.half * r * r;
-        const double *src = cof + (int64_t)N.parent * MQT * r;
-        // A[m][k] = M_h[k][m], B[k][q] = c_P[k][q]; one output tile's A
-        // operands at a time
-        double bv[G][MT][4];
+}
+
+// W += M_h^T c_P, c_P in the result layout
+template <int RP, int G>
+__device__ __forceinline__ void l2l_acc(const double *Mh, const md4_t (&cp)[G][RP / 16], int r,
+                                        int j16, int g, md4_t (&W)[G][RP / 16]) {
+    constexpr int MT = RP / 16;
+    // A[m][k] = M_h[k][m], B[k][q] = c_P[k][q]
+#pragma unroll
+    for (int tt = 0; tt < MT; ++tt) {
+        double av[MT][4];
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
+                const double a = Mh[min(k, r - 1) * r + min(m, r - 1)];
+                av[t][v] = (m < r && k < r) ? a : 0.0;
+            }
 #pragma unroll
         for (int gi = 0; gi < G; ++gi)
 #pragma unroll
             for (int t = 0; t < MT; ++t)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int k = t * 16 + 4 * v + g;
-                    const double b = src[(int64_t)min(k, r - 1) * MQT + gi * MQ + j16];
-                    bv[gi][t][v] = k < r ? b : 0.0;
-                }
-#pragma unroll
-        for (int tt = 0; tt < MT; ++tt) {
-            double av[MT][4];
-#pragma unroll
-            for (int t = 0; t < MT; ++t)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int m = tt * 16 + j16, k = t * 16 + 4 * v + g;
-                    const double a = Mh[min(k, r - 1) * r + min(m, r - 1)];
-                    av[t][v] = (m < r && k < r) ? a : 0.0;
-                }
-#pragma unroll
-            for (int gi = 0; gi < G; ++gi)
-#pragma unroll
-                for (int t = 0; t < MT; ++t)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], bv[gi][t][v], W[gi][tt]);
-        }
+                for (int v = 0; v < 4; ++v) W[gi][tt] = mfma64(av[t][v], cp[gi][t][v], W[gi][tt]);
     }
+}
+
+// Downward, row node N: c_N = z[N's row cluster] (if it is one) + M_h^T c_P
+// from its parent's stored coefficients
+template <int RP, int G, class WS>
+__device__ __forceinline__ void l2l_node(const DTNode &N, const double *M, const WS *z,
+                                         const double *cof, int r, int j16, int g,
+                                         md4_t (&W)[G][RP / 16]) {
+    md4_t cp[G][RP / 16];
+    if (N.parent >= 0) load_moments<RP, G>(cof, N.parent, r, j16, g, cp);
+    load_z<RP, G, WS>(N, z, r, j16, g, W);
+    if (N.parent >= 0) l2l_acc<RP, G>(M + (int64_t)N.half * r * r, cp, r, j16, g, W);
 }
 
 // A chunk's row leaf's coefficients (l2l_node from its parent's) into the
@@ -1569,8 +1638,10 @@ __device__ __forceinline__ void leaf_series_to_lds(const DevPlan &p, int leaf, c
     wave_sync();
 }
 
-// The coefficients of the internal row nodes of one depth (l2l_node), one
-// wavefront per node; a chunk's leaf forms its own in the downward pass.
+// Downward, one depth: the coefficients of its internal row nodes (a chunk's
+// leaf forms its own in the downward pass). Two depths per launch measured
+// slower here: the halves' products wait on this node's, and their registers
+// cost the occupancy.
 template <int RP, int G, class WS>
 __global__ __launch_bounds__(WG) void k_l2l(const DTNode *tn, const int32_t *ids, int n,
                                            const double *M, const WS *z, double *cof, int r) {
@@ -2186,11 +2257,14 @@ hipError_t matvec_m_t(const DevPlan &p, const MvArgs &a, int nq, hipStream_t s) 
         if (p.nyleaf > 0)
             k_up_m<RP, PREC, G, true, WS><<<p.nyleaf, WAVE, 0, s>>>(
                 p.yleaf, p.y, xr, ldxr, nq, p.chebA, a.partial, wn, p.r, p.my);
-        for (int d = p.nylev - 1; d >= 0; --d) {
+        // moments: two depths per launch (the even depths, each with the one
+        // below it) where the second node's registers fit, else one
+        constexpr bool TWO = RP <= 32;
+        for (int d = TWO ? (p.nylev - 1) & ~1 : p.nylev - 1; d >= 0; d -= TWO ? 2 : 1) {
             const int off = p.ylev_off_h[d], cnt = p.ylev_off_h[d + 1] - off;
             if (cnt > 0)
-                k_m2m<RP, G, WS><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM, p.my,
-                                                           p.chebA, wn, p.r);
+                k_m2m<RP, G, WS, TWO><<<nblk(cnt), WG, 0, s>>>(p.ytn, p.ylev + off, cnt, p.nestM,
+                                                                p.my, p.chebA, wn, p.r);
         }
         k_translate_m<RP, FREE, KID, G, WS, true><<<nblk(p.nrc), WG, 0, s>>>(p, p.rcl, p.nrc, wn, zn);
         for (int d = 0; d < p.nxlev; ++d) {
