@@ -632,7 +632,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
     for (int q = 0; q < KB; ++q) acc[q] = T(0);
 
     // ---- near field
-    const int near_end = a.part == 2 ? c.near_beg : c.near_end;
+    const int near_end = a.part >= 2 ? c.near_beg : c.near_end;
     for (int e = c.near_beg; e < near_end; ++e) {
         const DNear s = nl[e];
         const DDense D = p.dense[s.blk];
@@ -956,6 +956,17 @@ hipError_t gen_basis_r(const DevPlanT<T> &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// y[i incy + q ldy] += t[q n + i] for i < n, q < KB: a split product's far
+// field, added once the near field has written y
+template <class T, int KB>
+__global__ __launch_bounds__(WG) void k_add_far(T *y, int64_t incy, int64_t ldy, const T *t,
+                                               int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int q = 0; q < KB; ++q) y[i * incy + q * ldy] += t[q * n + i];
+}
+
 // MODE: 0 stored, 1 hybrid, 2 free
 template <class T, int RMAX, int MODE, int KID, int KB>
 hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStream_t s, const hipEvent_t *ev) {
@@ -995,8 +1006,10 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         }
     };
     // the near field depends on x alone: with a side stream it runs beside the
-    // far-field chain (up, reduce, translate) and the far field adds onto it
-    const bool split = a.side && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
+    // far-field chain (up, reduce, translate, and the downward far field into
+    // yt), which is added onto it after the join
+    const bool split = a.side && a.yt && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
+    const int64_t out_rows = tr ? p.ny : p.nx;
     if (split) {
         hipError_t e = hipEventRecord(a.fork, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(a.side, a.fork, 0);
@@ -1043,11 +1056,17 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
     }
     mark(3);
     if (split) {
+        MvArgsT<T> af = a;
+        af.part = 3;
+        af.y = a.yt;
+        af.incy = 1;
+        af.ldy = out_rows;
+        af.beta = 0;
+        down(af, s);
         const hipError_t e = hipStreamWaitEvent(s, a.join, 0);
         if (e != hipSuccess) return e;
-        MvArgsT<T> af = a;
-        af.part = 2;
-        down(af, s);
+        k_add_far<T, KB><<<(int)((out_rows + WG - 1) / WG), WG, 0, s>>>(a.y, a.incy, a.ldy, a.yt,
+                                                                         out_rows);
     } else if (out_n > 0) {
         down(a, s);
     }

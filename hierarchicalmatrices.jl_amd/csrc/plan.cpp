@@ -549,6 +549,7 @@ struct PlanBufs {
     T *A = nullptr, *U = nullptr, *V = nullptr, *F = nullptr;
     T *partial = nullptr, *w = nullptr, *z = nullptr;
     T *xt = nullptr;  // packed right-hand sides of the MFMA passes (ny x 16)
+    T *yt = nullptr;  // a split product's far field (hybrid and matrix-free plans)
     // nested far field of the MFMA passes (Float64 plans): the re-expansion
     // matrices, moments per column node, coefficients per row node
     T *nestM = nullptr, *my = nullptr, *mx = nullptr;
@@ -982,6 +983,8 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         st.reserve(&b.w, ncl * r * HM_WS_RHS);
         st.reserve(&b.z, ncl * r * HM_WS_RHS);
         if (std::is_same_v<T, double>) st.reserve(&b.xt, P->ny * HM_WS_RHS);
+        if (flags & (HM_PLAN_HYBRID | HM_PLAN_MATRIX_FREE))
+            st.reserve(&b.yt, std::max(P->nx, P->ny) * HM_MAX_KB);
         HM_TRY_T(st.commit(&b.arena));
     }
     if (b.my) {  // the moments of empty column nodes are read as zero
@@ -1808,6 +1811,7 @@ static MvArgsT<T> ws_args(hm_plan_t P) {
     a.w = b.w;
     a.z = b.z;
     a.xt = b.xt;
+    a.yt = b.yt;
     return a;
 }
 }  // extern "C++"
