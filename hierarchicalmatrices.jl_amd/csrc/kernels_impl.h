@@ -956,15 +956,12 @@ hipError_t gen_basis_r(const DevPlanT<T> &p, hipStream_t s) {
     return hipGetLastError();
 }
 
-// y[i incy + q ldy] += t[q n + i] for i < n, q < KB: a split product's far
-// field, added once the near field has written y
-template <class T, int KB>
-__global__ __launch_bounds__(WG) void k_add_far(T *y, int64_t incy, int64_t ldy, const T *t,
-                                               int64_t n) {
+// y[i incy] += t[i] for i < n: a split product's far field, added once the
+// near field has written y
+template <class T>
+__global__ __launch_bounds__(WG) void k_add_far(T *y, int64_t incy, const T *t, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
-#pragma unroll
-    for (int q = 0; q < KB; ++q) y[i * incy + q * ldy] += t[q * n + i];
+    if (i < n) y[i * incy] += t[i];
 }
 
 // MODE: 0 stored, 1 hybrid, 2 free
@@ -1006,9 +1003,11 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         }
     };
     // the near field depends on x alone: with a side stream it runs beside the
-    // far-field chain (up, reduce, translate, and the downward far field into
-    // yt), which is added onto it after the join
-    const bool split = a.side && a.yt && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
+    // far-field chain (up, reduce, translate); one right-hand side: the
+    // downward far field too, into yt, added onto y after the join (with
+    // several, the add costs more than the overlap gains: their far field
+    // adds onto y after the join)
+    const bool split = a.side && (KB > 1 || a.yt) && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
     const int64_t out_rows = tr ? p.ny : p.nx;
     if (split) {
         hipError_t e = hipEventRecord(a.fork, s);
@@ -1055,18 +1054,22 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         mark(2);
     }
     mark(3);
-    if (split) {
+    if (split && KB == 1) {
         MvArgsT<T> af = a;
         af.part = 3;
         af.y = a.yt;
         af.incy = 1;
-        af.ldy = out_rows;
         af.beta = 0;
         down(af, s);
         const hipError_t e = hipStreamWaitEvent(s, a.join, 0);
         if (e != hipSuccess) return e;
-        k_add_far<T, KB><<<(int)((out_rows + WG - 1) / WG), WG, 0, s>>>(a.y, a.incy, a.ldy, a.yt,
-                                                                         out_rows);
+        k_add_far<T><<<(int)((out_rows + WG - 1) / WG), WG, 0, s>>>(a.y, a.incy, a.yt, out_rows);
+    } else if (split) {
+        const hipError_t e = hipStreamWaitEvent(s, a.join, 0);
+        if (e != hipSuccess) return e;
+        MvArgsT<T> af = a;
+        af.part = 2;
+        down(af, s);
     } else if (out_n > 0) {
         down(a, s);
     }

@@ -984,7 +984,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         st.reserve(&b.z, ncl * r * HM_WS_RHS);
         if (std::is_same_v<T, double>) st.reserve(&b.xt, P->ny * HM_WS_RHS);
         if (flags & (HM_PLAN_HYBRID | HM_PLAN_MATRIX_FREE))
-            st.reserve(&b.yt, std::max(P->nx, P->ny) * HM_MAX_KB);
+            st.reserve(&b.yt, std::max(P->nx, P->ny));
         HM_TRY_T(st.commit(&b.arena));
     }
     if (b.my) {  // the moments of empty column nodes are read as zero
