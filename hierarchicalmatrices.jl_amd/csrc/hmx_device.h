@@ -79,14 +79,15 @@ struct MvArgsT {
     T *w;        // >= max(nrc, ncc) * r * kb
     T *z;        // >= max(nrc, ncc) * r * kb
     T *xt;       // >= ny * 16: right-hand sides packed row by row (MFMA passes)
-    T *yt;       // >= max(nx, ny): a split single-RHS product's far field
+    T *yt;       // >= max(nx, ny): a split product's far field (one RHS, matrix-free or Float32)
     // downward pass: 0 near and far field; 1 near field only (y = beta*y +
     // near); 2 far field only (y += far); 3 far field only (y = far)
     int part;
     // when set and no pass events are requested, the near field runs on `side`
     // (forked from the launch stream through `fork`) beside the upward,
     // reduce and translate passes; after `join` the far field is added onto
-    // y (one right-hand side: written into yt beside the near field first)
+    // y (one right-hand side of a matrix-free or Float32 plan: written into
+    // yt beside the near field first)
     hipStream_t side;
     hipEvent_t fork, join;
 };

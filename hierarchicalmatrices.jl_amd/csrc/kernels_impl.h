@@ -1003,11 +1003,15 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         }
     };
     // the near field depends on x alone: with a side stream it runs beside the
-    // far-field chain (up, reduce, translate); one right-hand side: the
-    // downward far field too, into yt, added onto y after the join (with
-    // several, the add costs more than the overlap gains: their far field
-    // adds onto y after the join)
-    const bool split = a.side && (KB > 1 || a.yt) && !ev && out_n > 0 && p.nfar > 0 && in_ntask > 0;
+    // far-field chain (up, reduce, translate), and the far field adds onto y
+    // after the join. One right-hand side, matrix-free or Float32: the
+    // downward far field also runs beside the near field, into yt, added onto
+    // y after the join (measured faster there; slower for an FP64 hybrid
+    // product, whose streamed near field it slows, and with several
+    // right-hand sides, whose add costs more than the overlap gains)
+    constexpr bool FAR_AHEAD = KB == 1 && (FREE || std::is_same<T, float>::value);
+    const bool split = a.side && (!FAR_AHEAD || a.yt) && !ev && out_n > 0 && p.nfar > 0 &&
+                       in_ntask > 0;
     const int64_t out_rows = tr ? p.ny : p.nx;
     if (split) {
         hipError_t e = hipEventRecord(a.fork, s);
@@ -1054,7 +1058,7 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
         mark(2);
     }
     mark(3);
-    if (split && KB == 1) {
+    if (split && FAR_AHEAD) {
         MvArgsT<T> af = a;
         af.part = 3;
         af.y = a.yt;

@@ -983,7 +983,7 @@ int setup_typed(hm_plan *P, const double *x, const double *y, const std::vector<
         st.reserve(&b.w, ncl * r * HM_WS_RHS);
         st.reserve(&b.z, ncl * r * HM_WS_RHS);
         if (std::is_same_v<T, double>) st.reserve(&b.xt, P->ny * HM_WS_RHS);
-        if (flags & (HM_PLAN_HYBRID | HM_PLAN_MATRIX_FREE))
+        if ((flags & HM_PLAN_MATRIX_FREE) || (std::is_same_v<T, float> && (flags & HM_PLAN_HYBRID)))
             st.reserve(&b.yt, std::max(P->nx, P->ny));
         HM_TRY_T(st.commit(&b.arena));
     }

@@ -68,6 +68,53 @@ def test_f32_products_vs_oracle(cuda, kernel, mode):
     assert rel(u, dense) <= 3e-5
 
 
+@pytest.mark.parametrize("mode", ["hybrid", "free"])
+def test_f32_side_stream_split(cuda, mode):
+    """Float32 hybrid and matrix-free plans with at least 2^24 near-field
+    entries: one right-hand side runs the near field on a side stream and the
+    far field beside it into a plan buffer, added onto y after the join.
+    Against the one-stream product of the same plan (pass profiling keeps one
+    stream; <= 1e-6, the two add near and far field in different orders) for
+    forward, transpose, accumulate and repeated products (identical), and
+    against the dense FP64 product on sampled rows (<= 3e-5; on this grid the
+    Float32 nodes coincide with points, so the oracle's stored Float32
+    traversal holds Inf factors, as the reference's Float32 update! does)."""
+    from hmx._lib import lib
+    n = 1 << 16
+    x, y, iv = grid32(n)
+    print(f"case: f32 side-stream split n=2^16 leaf 256 mode={mode}", flush=True)
+    H = hmx.KernelMatrix("cauchykernel", x, y, *iv, blocksize=256, rank=16,
+                         matrix_free=mode == "free", hybrid=mode == "hybrid")
+    assert H.info["dense_entries"] >= 1 << 24
+    b = hmx.workloads.normal(0, n).astype(f32)
+    bd = torch.from_numpy(b).to(cuda)
+    u0 = torch.from_numpy(hmx.workloads.normal(9, n).astype(f32)).to(cuda)
+    s = torch.cuda.Stream()
+
+    def products():
+        with torch.cuda.stream(s):
+            out = [H.matvec(bd, stream=s), H.matvec(bd, transpose=True, stream=s),
+                   H.mul(u0.clone(), bd, stream=s)]
+            out += [H.matvec(bd, stream=s) for _ in range(2)]
+        s.synchronize()
+        return [o.cpu().numpy() for o in out]
+
+    split = products()
+    assert lib.hm_plan_profile(H._h, 1) == 0
+    one = products()
+    assert lib.hm_plan_profile(H._h, 0) == 0
+    H.close()
+    for a, e in zip(split, one):
+        assert np.all(np.isfinite(a))
+        assert rel(a, e) <= 1e-6
+    for q in (3, 4):
+        assert np.array_equal(split[q], split[0])
+    rows = np.arange(0, n, 61)
+    ref = O.dense_matvec(O.CAUCHY, x.astype(np.float64), y.astype(np.float64),
+                         b.astype(np.float64), rows=rows)
+    assert rel(split[0][rows], ref) <= 3e-5
+
+
 def test_f32_default_rank(cuda):
     """BLOCKRANK(Float32) = 10, BLOCKSIZE(Float32) = 40 at n = 20000 (shifted
     grid: random Float32 points with y = x + 0.5/n would coincide with other
