@@ -826,7 +826,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
 // where S_c is the Chebyshev series of the cluster's coefficients, so der_p is
 // the exact derivative of val_p with respect to the point p. The near field is
 // regenerated from the points. DM = 1 writes y = beta*y + s .* der; DM = 2
-// writes y = beta*y + val and y2 = s .* der.
+// writes y = beta*y + val and y2 = beta*y2 + s .* der (s null: ones).
 template <class T, int RMAX, int KID, bool TRANS, int DM>
 __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *ch, int nch,
                                                  const DNear *nl, const DFar *fl,
@@ -878,7 +878,7 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
     }
     if (act) {
         const int64_t i = c.p0 + lane;
-        const T sc = sv[i * incs];
+        const T sc = sv ? sv[i * incs] : T(1);  // wave-uniform: no s means ones
         T *yo = a.y + i * a.incy;
         if constexpr (DM == 1) {
             *yo = (a.beta ? *yo : T(0)) + sc * accd;
@@ -1118,6 +1118,9 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
     if (out_n > 0) {
         if (tr)
             k_down_grad<T, RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
+                p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
+        else if (y2)  // the product and the x-derivative sums together
+            k_down_grad<T, RMAX, KID, false, 2><<<nblk(out_n), WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
         else
             k_down_grad<T, RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(

@@ -1993,9 +1993,43 @@ static int t_grad_common(hm_plan_t P, T *gb, T *gy, const T *r, const T *s, int6
 }
 }  // extern "C++"
 
+extern "C++" {
+template <class T>
+static int mv_grad_common(hm_plan_t P, T *y, T *g, const T *b, const T *s, int64_t incy,
+                          int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
+    if (!P || !y || !g || !b) return fail(HM_ERR_INVALID, "matvec_grad: NULL argument");
+    if (P->flags & HM_PLAN_TREE_ONLY) return fail(HM_ERR_INVALID, "matvec_grad: tree-only plan");
+    if (P->from_blocks) return fail(HM_ERR_UNSUPPORTED, "matvec_grad: a plan built from blocks has no points");
+    if (P->kid == HM_USER) return fail(HM_ERR_UNSUPPORTED, "matvec_grad: f is a host function (no derivative on the device)");
+    if (int rc = check_type<T>(P, "matvec_grad")) return rc;
+    if (incy < 1 || incg < 1 || incb < 1 || (s && incs < 1) || (beta != 0 && beta != 1))
+        return fail(HM_ERR_INVALID, "matvec_grad: bad strides or beta");
+    MvArgsT<T> a = ws_args<T>(P);
+    a.y = y;
+    a.x = b;
+    a.incy = incy;
+    a.incx = incb;
+    a.beta = beta;
+    hipError_t e = hmx_launch_grad(P->bufs<T>().dp, a, false, g, incg, s, s ? incs : 0, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(HM_ERR_HIP, "matvec_grad launch: %s", hipGetErrorString(e));
+    return HM_OK;
+}
+}  // extern "C++"
+
 int hm_matvec_grad_x(hm_plan_t P, double *g, const double *b, const double *s, int64_t ioff,
                      int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
     return grad_x_common(P, g, b, s, ioff, joff, incg, incb, incs, beta, stream);
+}
+
+int hm_matvec_grad(hm_plan_t P, double *y, double *g, const double *b, const double *s,
+                   int64_t incy, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream) {
+    return mv_grad_common(P, y, g, b, s, incy, incg, incb, incs, beta, stream);
+}
+
+int hm_matvec_grad_f32(hm_plan_t P, float *y, float *g, const float *b, const float *s,
+                       int64_t incy, int64_t incg, int64_t incb, int64_t incs, int beta,
+                       void *stream) {
+    return mv_grad_common(P, y, g, b, s, incy, incg, incb, incs, beta, stream);
 }
 
 int hm_matvec_t_grad(hm_plan_t P, double *gb, double *gy, const double *r, const double *s,

@@ -129,6 +129,9 @@ def _load():
     L.hm_matvec_grad_x.restype = ctypes.c_int
     L.hm_matvec_grad_x.argtypes = [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
                                    ctypes.c_int, _vp]
+    L.hm_matvec_grad.restype = ctypes.c_int
+    L.hm_matvec_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int,
+                                 _vp]
     L.hm_matvec_t_grad.restype = ctypes.c_int
     L.hm_matvec_t_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int,
                                    _vp]
@@ -168,6 +171,8 @@ def _load():
                        ctypes.c_uint, _vp]
     L.hm_matvec_grad_x_f32.restype = ctypes.c_int
     L.hm_matvec_grad_x_f32.argtypes = L.hm_matvec_grad_x.argtypes
+    L.hm_matvec_grad_f32.restype = ctypes.c_int
+    L.hm_matvec_grad_f32.argtypes = L.hm_matvec_grad.argtypes
     L.hm_matvec_t_grad_f32.restype = ctypes.c_int
     L.hm_matvec_t_grad_f32.argtypes = L.hm_matvec_t_grad.argtypes
     L.hm_graph_matvec_f32.restype = ctypes.c_int

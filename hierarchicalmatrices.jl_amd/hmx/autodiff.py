@@ -5,14 +5,18 @@
 cotangent r, returns
 
   grad_b = H^T r                          (transpose product)
-  grad_x = r .* d/dx_i (H b)_i            (hm_matvec_grad_x)
+  grad_x = r .* d/dx_i (H b)_i            (the sums d/dx_i (H b)_i come out of
+                                           the forward pass, hm_matvec_grad)
   grad_y = b .* d/dy_j (H^T r)_j          (hm_matvec_t_grad, same pass as grad_b)
 
 These are the exact derivatives of the computed product: the block tree (which
 depends on x and y only through comparisons) is held fixed, the near field is
 regenerated with df/dx, and the far field is differentiated through its
 Chebyshev-form basis. The forward plan is matrix-free, so nothing is stored
-between forward and backward beyond the points and b.
+between forward and backward beyond the points, b and (when x needs a
+gradient) the derivative sums: the forward then runs hm_matvec_grad, one pass
+for H b and d/dx_i (H b)_i, and the backward's x-gradient is r .* those sums
+with no pass of its own.
 
 Plans are reused from step to step: a forward takes a plan of the same shape
 (points per side, kernel, intervals, leaf threshold, rank, element type) from
@@ -61,42 +65,48 @@ class _PlanPool:
 _POOL = _PlanPool()
 
 
-def _forward_plan(x, y, b, kernel, intervals, blocksize, rank):
+def _product(H, b, dx):
+    """u = H b, and with dx the sums d/dx_i (H b)_i from the same pass."""
+    return H.matvec_grad(b) if dx else (H.matvec(b), None)
+
+
+def _forward_plan(x, y, b, kernel, intervals, blocksize, rank, dx):
     key = (str(kernel), x.numel(), y.numel(), tuple(float(v) for v in intervals), blocksize,
            rank, x.dtype)
     H = _POOL.take(key)
-    u = None
+    u = d = None
     if H is not None:
         H.update_points(x, y, check=False)
-        u = H.matvec(b)
+        u, d = _product(H, b, dx)
         if not H.tree_status():  # read after the product is queued
             H.close()
-            H = u = None
+            H = u = d = None
     if H is None:
         H = KernelMatrix(kernel, x.cpu().numpy(), y.cpu().numpy(), *intervals,
                          blocksize=blocksize, rank=rank, matrix_free=True)
-        u = H.matvec(b)
-    return key, H, u
+        u, d = _product(H, b, dx)
+    return key, H, u, d
 
 
 class _KernelMatVec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, y, b, kernel, intervals, blocksize, rank):
         x, y, b = (v.detach().contiguous() for v in (x, y, b))
-        key, H, u = _forward_plan(x, y, b, kernel, intervals, blocksize, rank)
+        key, H, u, d = _forward_plan(x, y, b, kernel, intervals, blocksize, rank,
+                                     ctx.needs_input_grad[0])
         ctx.H = H
         # back to the pool once the graph is gone (stream-ordered: products
         # that reuse it run after this graph's work)
         weakref.finalize(ctx, _POOL.give, key, H)
-        ctx.save_for_backward(b)
+        ctx.save_for_backward(b, d)
         return u
 
     @staticmethod
     def backward(ctx, r):
-        (b,) = ctx.saved_tensors
+        b, d = ctx.saved_tensors
         H = ctx.H
         r = r.contiguous()
-        gx = H.grad_x(b, r) if ctx.needs_input_grad[0] else None
+        gx = r * d if ctx.needs_input_grad[0] else None
         gb, gy = H.tgrad(r, b)
         return (gx, gy if ctx.needs_input_grad[1] else None,
                 gb if ctx.needs_input_grad[2] else None, None, None, None, None)
@@ -109,17 +119,17 @@ class _KernelProductFn(torch.autograd.Function):
         H = kp.H
         H.update_points(x, y, check=False)
         torch.bitwise_or(kp._bad, H._status, out=kp._bad)
-        u = H.matvec(b)
+        u, d = _product(H, b, ctx.needs_input_grad[0])
         ctx.H = H
-        ctx.save_for_backward(b)
+        ctx.save_for_backward(b, d)
         return u
 
     @staticmethod
     def backward(ctx, r):
-        (b,) = ctx.saved_tensors
+        b, d = ctx.saved_tensors
         H = ctx.H
         r = r.contiguous()
-        gx = H.grad_x(b, r) if ctx.needs_input_grad[0] else None
+        gx = r * d if ctx.needs_input_grad[0] else None
         gb, gy = H.tgrad(r, b)
         return (gx, gy if ctx.needs_input_grad[1] else None,
                 gb if ctx.needs_input_grad[2] else None, None)

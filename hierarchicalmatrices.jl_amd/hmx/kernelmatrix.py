@@ -417,6 +417,23 @@ class KernelMatrix:
                                    0, 0, 1, b.stride(0), s.stride(0), 0, _stream_ptr(stream)))
         return g
 
+    def matvec_grad(self, b, s=None, stream=None):
+        """(H b, s .* d/dx_i (H b)_i) in one pass (device tensors, 1-D: b of
+        length size(H, 2), s of length size(H, 1) or None for ones)."""
+        torch = _torch()
+        m, n = self.shape
+        b = self._grad_vec(b, n, "b")
+        if s is not None:
+            s = self._grad_vec(s, m, "s")
+        u = torch.empty(m, dtype=self._tdtype(), device=b.device)
+        g = torch.empty_like(u)
+        fn = lib.hm_matvec_grad_f32 if self.f32 else lib.hm_matvec_grad
+        check(fn(self._h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                 ctypes.c_void_p(b.data_ptr()),
+                 ctypes.c_void_p(s.data_ptr() if s is not None else None), 1, 1, b.stride(0),
+                 s.stride(0) if s is not None else 1, 0, _stream_ptr(stream)))
+        return u, g
+
     def tgrad(self, r, s, stream=None):
         """(H^T r, s .* d/dy_j (H^T r)_j) for device tensors r (rows) and s (cols)."""
         torch = _torch()

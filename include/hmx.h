@@ -286,6 +286,17 @@ int hm_matvec_t(hm_plan_t plan, double *y, const double *x, int64_t ioff, int64_
 int hm_matvec_grad_x(hm_plan_t plan, double *g, const double *b, const double *s, int64_t ioff,
                      int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream);
 
+/* The product and its x-derivative sums in one pass (no counterpart in the
+ * reference; the forward counterpart of hm_matvec_t_grad):
+ *   y[i*incy] = beta*y[...] + (H b)_i,
+ *   g[i*incg] = beta*g[...] + s[i*incs] * d/dx_i (H b)_i   (s NULL: ones)
+ * with H b computed as hm_matvec_grad_x differentiates it (near field
+ * regenerated, far field in Chebyshev form). A rrule that keeps g from the
+ * forward (s NULL) forms the x-gradient as r .* g, with no pass of its own.
+ * One right-hand side. */
+int hm_matvec_grad(hm_plan_t plan, double *y, double *g, const double *b, const double *s,
+                   int64_t incy, int64_t incg, int64_t incb, int64_t incs, int beta, void *stream);
+
 /* Adjoint pass for the b- and y-gradients (no counterpart in the reference):
  *   gb = beta*gb + H^T r,   gy[j] = beta*gy[j] + s[j] * d/dy_j (H^T r)_j
  * with r the output cotangent; s = b gives the y-gradient. gb and gy are
@@ -357,6 +368,9 @@ int hm_matvec_t_f32(hm_plan_t plan, float *y, const float *x, int64_t ioff, int6
 int hm_matvec_grad_x_f32(hm_plan_t plan, float *g, const float *b, const float *s, int64_t ioff,
                          int64_t joff, int64_t incg, int64_t incb, int64_t incs, int beta,
                          void *stream);
+int hm_matvec_grad_f32(hm_plan_t plan, float *y, float *g, const float *b, const float *s,
+                       int64_t incy, int64_t incg, int64_t incb, int64_t incs, int beta,
+                       void *stream);
 int hm_matvec_t_grad_f32(hm_plan_t plan, float *gb, float *gy, const float *r, const float *s,
                          int64_t incgb, int64_t incgy, int64_t incr, int64_t incs, int beta,
                          void *stream);

@@ -145,3 +145,45 @@ def test_gradcheck_small(cuda):
     inputs = (x.clone().requires_grad_(True), y.clone().requires_grad_(True),
               b.clone().requires_grad_(True))
     assert torch.autograd.gradcheck(f, inputs, eps=h, atol=1e-3, rtol=1e-5)
+
+
+@pytest.mark.parametrize("kernel", ["cauchykernel", "logkernel"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_matvec_grad_one_pass(cuda, kernel, dtype):
+    """hm_matvec_grad: H b and s .* d/dx_i (H b)_i from one pass. The sums are
+    the ones hm_matvec_grad_x forms (bit for bit: the same arithmetic), the
+    product agrees with hm_matvec's (<= 1e-13 FP64, 1e-5 FP32: the near field
+    through the kernel-and-derivative routine) and with the dense product;
+    beta = 1 accumulates into both outputs; s NULL means ones."""
+    import ctypes
+    from hmx._lib import lib
+    n = 4096
+    w, x, y, b, _ = setup(n, kernel)
+    print(f"case: matvec_grad kernel={kernel} dtype={dtype}", flush=True)
+    np_dt = np.float32 if dtype == torch.float32 else np.float64
+    H = hmx.KernelMatrix(kernel, w["x"].astype(np_dt), w["y"].astype(np_dt), *w["intervals"],
+                         blocksize=64, rank=16, matrix_free=True)
+    bd = b.to(dtype)
+    s = torch.from_numpy(np.random.default_rng(5).standard_normal(n)).to(cuda).to(dtype)
+    u, g = H.matvec_grad(bd, s)
+    assert torch.equal(g, H.grad_x(bd, s))
+    u1, g1 = H.matvec_grad(bd)
+    assert torch.equal(u1, u)
+    assert torch.equal(s * g1, H.grad_x(bd, s))
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    assert rel(u.double(), H.matvec(bd).double()) <= tol
+    # the dense product of the plan's own points (Float32-rounded for Float32)
+    xp, yp, bp = (v.to(dtype).double() for v in (x, y, b))
+    Kb = dense(kernel, xp, yp, bp, torch.zeros_like(xp))[0]
+    assert rel(u.double(), Kb) <= (1e-6 if dtype == torch.float64 else 3e-5)
+    # beta = 1: y += H b, g += s .* d/dx (H b)
+    u0 = torch.ones_like(u)
+    g0 = torch.full_like(u, 2.0)
+    fn = lib.hm_matvec_grad_f32 if dtype == torch.float32 else lib.hm_matvec_grad
+    assert fn(H._h, ctypes.c_void_p(u0.data_ptr()), ctypes.c_void_p(g0.data_ptr()),
+              ctypes.c_void_p(bd.data_ptr()), ctypes.c_void_p(s.data_ptr()), 1, 1, 1, 1, 1,
+              None) == 0
+    torch.cuda.synchronize()
+    # (the kernel may fuse the add into a multiply-add: equal to rounding)
+    assert rel(u0.double(), (1 + u).double()) <= 1e-7 and rel(g0.double(), (2 + g).double()) <= 1e-7
+    H.close()
