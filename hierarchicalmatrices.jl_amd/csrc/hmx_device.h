@@ -131,13 +131,15 @@ hipError_t hmx_launch_count_nonfinite(const double *v, int64_t n, unsigned long 
 hipError_t hmx_launch_scale(const DevPlan &p, int side, const double *b, hipStream_t s);
 
 // hm_plan_update_points: *status = 0 when the points x (nx) and y (ny) are
-// finite, non-increasing and split as the tree's recorded splits do, else 1
-// (the status word is cleared by a kernel first, so the launch is capturable);
-// refit re-forms the dense blocks and the stored bases from the plan's points
+// new points x, y copied into the plan's arrays dx, dy and checked on the way:
+// status 0 if finite, non-increasing and split as the tree's recorded splits
+// do, else 1 (the status word is cleared by a kernel first, so the launch is
+// capturable); refit re-forms the dense blocks and the stored bases from the
+// plan's points
 template <class T>
-hipError_t hmx_launch_check_points(const T *x, int64_t nx, const DSplit *xs, int64_t nxs,
-                                   const T *y, int64_t ny, const DSplit *ys, int64_t nys,
-                                   int32_t *status, hipStream_t s);
+hipError_t hmx_launch_update_points(const T *x, T *dx, int64_t nx, const DSplit *xs, int64_t nxs,
+                                    const T *y, T *dy, int64_t ny, const DSplit *ys, int64_t nys,
+                                    int32_t *status, hipStream_t s);
 hipError_t hmx_launch_refit(const DevPlanT<double> &p, hipStream_t s);
 hipError_t hmx_launch_refit(const DevPlanT<float> &p, hipStream_t s);
 

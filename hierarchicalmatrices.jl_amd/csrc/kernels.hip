@@ -143,15 +143,26 @@ __global__ __launch_bounds__(hmxk::WG) void k_scale_basis(const DCluster *cl, do
 
 __global__ void k_zero_status(int32_t *status) { *status = 0; }
 
-// monotone and finite points (t < n - 1 compares t and t + 1), then the
-// splits; any failing thread stores 1
+// New point values: each side's points are copied into the plan's array (d)
+// and checked as they are read: monotone and finite (t < n - 1 compares t and
+// t + 1), then every split of its tree. Workgroups [0, bx) take x's points and
+// splits, the rest y's; any failing thread stores 1 (the status is zeroed
+// before the launch).
 template <class T>
-__global__ __launch_bounds__(hmxk::WG) void k_check_points(const T *p, int64_t n, const DSplit *sp,
-                                                          int64_t nsp, int32_t *status) {
-    const int64_t t = (int64_t)blockIdx.x * hmxk::WG + threadIdx.x;
+__global__ __launch_bounds__(hmxk::WG) void k_copy_check_points(
+    const T *x, T *dx, int64_t nx, const DSplit *xs, int64_t nxs, const T *y, T *dy, int64_t ny,
+    const DSplit *ys, int64_t nys, int64_t bx, int32_t *status) {
+    const bool sx = (int64_t)blockIdx.x < bx;  // workgroup-uniform
+    const T *p = sx ? x : y;
+    T *d = sx ? dx : dy;
+    const int64_t n = sx ? nx : ny, nsp = sx ? nxs : nys;
+    const DSplit *sp = sx ? xs : ys;
+    const int64_t t = ((int64_t)blockIdx.x - (sx ? 0 : bx)) * hmxk::WG + threadIdx.x;
     bool bad = false;
     if (t < n) {
-        const double v = (double)p[t];
+        const T v0 = p[t];
+        d[t] = v0;
+        const double v = (double)v0;
         bad = !isfinite(v) || (t + 1 < n && !(v >= (double)p[t + 1]));
     } else if (t < n + nsp) {
         const DSplit q = sp[t - n];
@@ -186,21 +197,24 @@ __global__ __launch_bounds__(hmxk::WG) void k_add_col(DevPlan p, AddColJobs J, c
 }  // namespace
 
 template <class T>
-hipError_t hmx_launch_check_points(const T *x, int64_t nx, const DSplit *xs, int64_t nxs,
-                                   const T *y, int64_t ny, const DSplit *ys, int64_t nys,
-                                   int32_t *status, hipStream_t s) {
+hipError_t hmx_launch_update_points(const T *x, T *dx, int64_t nx, const DSplit *xs, int64_t nxs,
+                                    const T *y, T *dy, int64_t ny, const DSplit *ys, int64_t nys,
+                                    int32_t *status, hipStream_t s) {
     k_zero_status<<<1, 1, 0, s>>>(status);
     const int64_t bx = (nx + nxs + hmxk::WG - 1) / hmxk::WG, by = (ny + nys + hmxk::WG - 1) / hmxk::WG;
-    if (bx > 0) k_check_points<T><<<(unsigned)bx, hmxk::WG, 0, s>>>(x, nx, xs, nxs, status);
-    if (by > 0) k_check_points<T><<<(unsigned)by, hmxk::WG, 0, s>>>(y, ny, ys, nys, status);
+    if (bx + by > 0)
+        k_copy_check_points<T><<<(unsigned)(bx + by), hmxk::WG, 0, s>>>(x, dx, nx, xs, nxs, y, dy,
+                                                                        ny, ys, nys, bx, status);
     return hipGetLastError();
 }
-template hipError_t hmx_launch_check_points<double>(const double *, int64_t, const DSplit *,
-                                                    int64_t, const double *, int64_t,
-                                                    const DSplit *, int64_t, int32_t *, hipStream_t);
-template hipError_t hmx_launch_check_points<float>(const float *, int64_t, const DSplit *, int64_t,
-                                                   const float *, int64_t, const DSplit *, int64_t,
-                                                   int32_t *, hipStream_t);
+template hipError_t hmx_launch_update_points<double>(const double *, double *, int64_t,
+                                                     const DSplit *, int64_t, const double *,
+                                                     double *, int64_t, const DSplit *, int64_t,
+                                                     int32_t *, hipStream_t);
+template hipError_t hmx_launch_update_points<float>(const float *, float *, int64_t,
+                                                    const DSplit *, int64_t, const float *, float *,
+                                                    int64_t, const DSplit *, int64_t, int32_t *,
+                                                    hipStream_t);
 
 hipError_t hmx_launch_refit(const DevPlan &p, hipStream_t s) {
     if (p.kid != HM_USER && p.ndense > 0 && p.A) hmxk::k_gen_dense<double><<<p.ndense, hmxk::WG, 0, s>>>(p);

@@ -1570,11 +1570,9 @@ static int update_points_t(hm_plan_t P, const T *x, const T *y, int32_t *status,
     hipStream_t s = (hipStream_t)stream;
     PlanBufs<T> &b = P->bufs<T>();
     int32_t *st = status ? status : P->d_status;
-    HM_HIP(hipMemcpyAsync(b.x, x, sizeof(T) * (size_t)P->nx, hipMemcpyDeviceToDevice, s));
-    HM_HIP(hipMemcpyAsync(b.y, y, sizeof(T) * (size_t)P->ny, hipMemcpyDeviceToDevice, s));
-    hipError_t e = hmx_launch_check_points(b.x, P->nx, P->d_xsplits, (int64_t)P->xsplits.size(),
-                                           b.y, P->ny, P->d_ysplits, (int64_t)P->ysplits.size(),
-                                           st, s);
+    hipError_t e = hmx_launch_update_points(x, b.x, P->nx, P->d_xsplits, (int64_t)P->xsplits.size(),
+                                            y, b.y, P->ny, P->d_ysplits, (int64_t)P->ysplits.size(),
+                                            st, s);
     // the factors that depend on the points (dense blocks; U and V of stored
     // plans) are formed again; F depends on the intervals alone
     if (e == hipSuccess && !(P->flags & HM_PLAN_MATRIX_FREE)) e = hmx_launch_refit(b.dp, s);
