@@ -483,8 +483,9 @@ __global__ __launch_bounds__(WG) void k_translate(DevPlanT<T> p, const DCluster 
 // its partial slots in slot order (and, in Chebyshev form, mapped by A), the
 // reduce's arithmetic in the reduce's order, so no reduce runs before this
 // pass. Few registers keep many wavefronts resident to cover the memory
-// latency.
-template <class T, int RMAX, bool TRANS, bool CHEB>
+// latency. FREE: F is regenerated from the two clusters' mapped nodes
+// (F[m, l] = f(tx_m, ty_l) forward) instead of loaded.
+template <class T, int RMAX, bool TRANS, bool CHEB, bool FREE = false, int KID = 0>
 __global__ __launch_bounds__(WG) void k_translate1(DevPlanT<T> p, const DCluster *cl, int ncl,
                                                   const T *partial, const T *w, T *z) {
     constexpr int NH = RMAX <= 32 ? 2 : 1;  // column parts
@@ -515,7 +516,6 @@ __global__ __launch_bounds__(WG) void k_translate1(DevPlanT<T> p, const DCluster
             b_ns = b.nslots;
         }
         auto load = [&](int e, T *fv, T &wl) {
-            const T *F = p.F + rl_l(b_foff, e);
             const int ns = rl_i(b_ns, e);
             if (ns > 1) {  // a source with several tasks (wave-uniform branch)
                 wl = sum_slots(partial + rl_l(b_pbase, e) * r + lw, (int64_t)r, ns);
@@ -523,10 +523,22 @@ __global__ __launch_bounds__(WG) void k_translate1(DevPlanT<T> p, const DCluster
             } else {
                 wl = w[(int64_t)rl_i(b_src, e) * r + lw];
             }
+            if constexpr (FREE) {
+                // forward: F[m, l] = f(tx_m, ty_l); transpose: lane row m is F's column
+                const T to = (TRANS ? p.cnodes : p.rnodes)[(int64_t)wid * r + mm];
+                const T *oth = (TRANS ? p.rnodes : p.cnodes) + (int64_t)rl_i(b_src, e) * r;
 #pragma unroll
-            for (int j = 0; j < LHMAX; ++j) {
-                const int l = min(l0 + j, r - 1);  // in bounds; the extra terms are dropped
-                fv[j] = __builtin_nontemporal_load(TRANS ? F + (l + r * mm) : F + (mm + r * l));
+                for (int j = 0; j < LHMAX; ++j) {
+                    const T t = oth[min(l0 + j, r - 1)];
+                    fv[j] = TRANS ? kern_fast<T, KID>(t, to) : kern_fast<T, KID>(to, t);
+                }
+            } else {
+                const T *F = p.F + rl_l(b_foff, e);
+#pragma unroll
+                for (int j = 0; j < LHMAX; ++j) {
+                    const int l = min(l0 + j, r - 1);  // in bounds; the extra terms are dropped
+                    fv[j] = __builtin_nontemporal_load(TRANS ? F + (l + r * mm) : F + (mm + r * l));
+                }
             }
         };
         auto apply = [&](const T *fv, T wl) {
@@ -1029,26 +1041,21 @@ hipError_t matvec_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, hipStrea
                                                       a.incx, a.ldx, a.partial, a.w, p.chebA,
                                                       p.r);
         mark(1);
-        // one right-hand side with stored F: the translation sums the slots
-        // of clusters with several tasks itself
-        constexpr bool T1 = KB == 1 && !FREE;
+        // one right-hand side: the translation sums the slots of clusters with
+        // several tasks itself (stored or regenerated F)
+        constexpr bool T1 = KB == 1;
         if (!T1 && in_nmulti > 0)
             k_reduce<T, CHEB, KB><<<nblk(in_nmulti), WG, 0, s>>>(in_cl, in_multi, in_nmulti,
                                                                  a.partial, p.chebA, a.w, p.r);
         mark(2);
         if constexpr (T1) {
             if (tr)
-                k_translate1<T, RMAX, true, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                k_translate1<T, RMAX, true, CHEB, FREE, KID><<<nblk(out_ncl), WG, 0, s>>>(
                     p, out_cl, out_ncl, a.partial, a.w, a.z);
             else
-                k_translate1<T, RMAX, false, CHEB><<<nblk(out_ncl), WG, 0, s>>>(
+                k_translate1<T, RMAX, false, CHEB, FREE, KID><<<nblk(out_ncl), WG, 0, s>>>(
                     p, out_cl, out_ncl, a.partial, a.w, a.z);
-        } else if (tr) {
-            if constexpr (KB == 1)
-                k_translate<T, RMAX, FREE, KID, true, CHEB, 1>
-                    <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl,
-                                                                   out_far_list, a.w, a.z);
-        } else {
+        } else {  // several right-hand sides (forward only)
             k_translate<T, RMAX, FREE, KID, false, CHEB, KB>
                 <<<nblk(tr_waves(out_ncl, p.r, KB)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
                                                                 a.w, a.z);
@@ -1097,23 +1104,17 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
     const int out_ncl = tr ? p.ncc : p.nrc;
     const V2<T> *out_map = tr ? p.cmap : p.rmap;
     const T *out_pts = tr ? p.y : p.x;
-    const int32_t *out_far_list = tr ? p.far_by_cc : p.far_by_rc;
     if (p.nfar > 0 && in_ntask > 0) {
         k_up<T, RMAX, true, 1><<<in_ntask, WG, 0, s>>>(in_task, in_cl, nullptr, in_map, in_pts, a.x,
                                                      a.incx, 0, a.partial, a.w, p.chebA, p.r);
-        const int32_t *in_multi = tr ? p.multi_r : p.multi_c;
-        const int in_nmulti = tr ? p.nmulti_r : p.nmulti_c;
-        if (in_nmulti > 0)
-            k_reduce<T, true, 1><<<nblk(in_nmulti), WG, 0, s>>>(in_cl, in_multi, in_nmulti,
-                                                                a.partial, p.chebA, a.w, p.r);
+        // the translation sums the slots of clusters with several tasks
+        // itself, F regenerated from the nodes
         if (tr)
-            k_translate<T, RMAX, true, KID, true, true, 1>
-                <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
-                                                               a.w, a.z);
+            k_translate1<T, RMAX, true, true, true, KID><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, a.partial, a.w, a.z);
         else
-            k_translate<T, RMAX, true, KID, false, true, 1>
-                <<<nblk(tr_waves(out_ncl, p.r, 1)), WG, 0, s>>>(p, out_cl, out_ncl, out_far_list,
-                                                               a.w, a.z);
+            k_translate1<T, RMAX, false, true, true, KID><<<nblk(out_ncl), WG, 0, s>>>(
+                p, out_cl, out_ncl, a.partial, a.w, a.z);
     }
     if (out_n > 0) {
         if (tr)
