@@ -839,6 +839,10 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
 // the exact derivative of val_p with respect to the point p. The near field is
 // regenerated from the points. DM = 1 writes y = beta*y + s .* der; DM = 2
 // writes y = beta*y + val and y2 = beta*y2 + s .* der (s null: ones).
+// One chunk per workgroup: its WPB wavefronts take every WPB-th column of each
+// near block and every WPB-th covering cluster (WPB times the wavefronts of a
+// wavefront per chunk, whose column loop waits on its loads), and their sums
+// are added in LDS in wavefront order.
 template <class T, int RMAX, int KID, bool TRANS, int DM>
 __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *ch, int nch,
                                                  const DNear *nl, const DFar *fl,
@@ -846,11 +850,12 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
                                                  const T *pts_in, const T *z, MvArgsT<T> a,
                                                  T *y2, int64_t incy2, const T *sv,
                                                  int64_t incs) {
-    const int wid = wave_id();
+    __shared__ T red[2][WPB][WAVE];
+    const int wv = uniform((int)(threadIdx.x / WAVE));
     const int lane = threadIdx.x % WAVE;
-    if (wid >= nch) return;
+    if ((int)blockIdx.x >= nch) return;  // workgroup-uniform
     const int r = p.r;
-    const DChunk c = ch[wid];
+    const DChunk c = ch[blockIdx.x];
     const bool act = lane < c.np;
     const T po = act ? pts_out[c.p0 + lane] : T(0);
     T accv = T(0), accd = T(0);
@@ -861,7 +866,7 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
         const int nq = TRANS ? D.m : D.n;
         const T *xin = a.x + q0 * a.incx;
         const T *qin = pts_in + q0;
-        for (int j = 0; j < nq; ++j) {
+        for (int j = wv; j < nq; j += WPB) {
             const T d = TRANS ? qin[j] - po : po - qin[j];
             T f, fx;
             kern_val_dx<T, KID>(d, f, fx);
@@ -870,7 +875,7 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
             accd = fmaT(TRANS ? -fx : fx, v, accd);
         }
     }
-    for (int e = c.far_beg; e < c.far_end; ++e) {
+    for (int e = c.far_beg + wv; e < c.far_end; e += WPB) {
         const DFar f = fl[e];
         const T *zc = z + (int64_t)f.cl * r;
         const V2<T> cm = cmapv[f.cl];
@@ -887,6 +892,17 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
         }
         if constexpr (DM == 2) accv += fmaT(xh, b1, zc[0] - b2);
         accd = fmaT(fmaT(xh, d1, b1 - d2), cm.y, accd);
+    }
+    red[0][wv][lane] = accv;
+    red[1][wv][lane] = accd;
+    __syncthreads();
+    if (wv != 0) return;
+    accv = red[0][0][lane];
+    accd = red[1][0][lane];
+#pragma unroll
+    for (int u = 1; u < WPB; ++u) {
+        accv += red[0][u][lane];
+        accd += red[1][u][lane];
     }
     if (act) {
         const int64_t i = c.p0 + lane;
@@ -1118,13 +1134,13 @@ hipError_t grad_t(const DevPlanT<T> &p, const MvArgsT<T> &a, bool tr, T *y2, int
     }
     if (out_n > 0) {
         if (tr)
-            k_down_grad<T, RMAX, KID, true, 2><<<nblk(out_n), WG, 0, s>>>(
+            k_down_grad<T, RMAX, KID, true, 2><<<out_n, WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
         else if (y2)  // the product and the x-derivative sums together
-            k_down_grad<T, RMAX, KID, false, 2><<<nblk(out_n), WG, 0, s>>>(
+            k_down_grad<T, RMAX, KID, false, 2><<<out_n, WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
         else
-            k_down_grad<T, RMAX, KID, false, 1><<<nblk(out_n), WG, 0, s>>>(
+            k_down_grad<T, RMAX, KID, false, 1><<<out_n, WG, 0, s>>>(
                 p, out_ch, out_n, out_near, out_far, out_map, out_pts, in_pts, a.z, a, y2, incy2, sv, incs);
     }
     return hipGetLastError();
