@@ -1841,9 +1841,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(down_m_waves
     constexpr int KSEL = BF ? ((RP + 31) / 32) * 8 : RP / 4;  // T values per lane and row tile
     constexpr int MQT = MQ * G;
     // stored blocks stream through LDS slabs, except where the slab path's
-    // registers would cost the occupancy (two groups above rank 16): there
-    // the block entries load straight into the A operand
-    constexpr bool SLAB = !FREE && RP <= 16;
+    // registers would cost the occupancy (two groups above rank 16, or one
+    // above rank 32): there the block entries load straight into the A operand
+    constexpr bool SLAB = !FREE && (RP <= 16 || (RP <= 32 && G == 1));
     // per wavefront: one 16-row output tile; with NEST first the leaf's
     // coefficients, rows of r + 1 (padded against bank conflicts)
     constexpr int LDSW0 = (NEST && (RP + 1) * MQT > 16 * MQT) ? (RP + 1) * MQT : 16 * MQT;
