@@ -142,6 +142,15 @@ def test_error_statuses():
     H = tree("cauchykernel", x, y, iv, 16, 8)
     assert _lib.lib.hm_matvec(H._h, None, None, 0, 0, 1, 1, 1, 0, 0, 0, 0, None) == \
         _lib.HM_ERR_INVALID
+    # the one-pass product and x-derivative sums: NULL outputs, then a tree-only
+    # plan (rejected on the host before any pointer is used)
+    assert _lib.lib.hm_matvec_grad(H._h, None, None, None, None, 1, 1, 1, 1, 0, None) == \
+        _lib.HM_ERR_INVALID
+    buf = np.zeros(256)
+    p = ctypes.c_void_p(buf.ctypes.data)
+    assert _lib.lib.hm_matvec_grad(H._h, p, p, p, None, 1, 1, 1, 1, 0, None) == \
+        _lib.HM_ERR_INVALID
+    assert b"tree-only" in _lib.lib.hm_last_error()
 
 
 def test_empty_cluster_is_reported():
