@@ -583,33 +583,49 @@ __global__ __launch_bounds__(WG) void k_translate1(DevPlanT<T> p, const DCluster
 
 // Matrix-free near field of one lane and one right-hand side: sum_j f(p, q_j)
 // v_j (or f(q_j, p) v_j when TR). For the power kernels 1/d^m two terms share
-// one reciprocal: v1/d1^m + v2/d2^m = (v1 d2^m + v2 d1^m) / (d1 d2)^m.
+// one reciprocal: v1/d1^m + v2/d2^m = (v1 d2^m + v2 d1^m) / (d1 d2)^m. The
+// block's columns are staged through LDS 64 at a time (one coalesced load per
+// lane of the points and of v, then broadcast reads; the pairs start at even
+// columns, as over the whole block). sq and sx: this wavefront's 64-entry LDS
+// tiles.
 template <class T, int KID, bool TR>
-__device__ __forceinline__ T near_free(T p, const T *qs, const T *v,
-                                            int64_t inc, int n, bool act) {
+__device__ __forceinline__ T near_free_staged(T p, const T *qs, const T *v, int64_t inc, int n,
+                                              bool act, T *sq, T *sx, int lane) {
     T acc = T(0);
-    if constexpr (KID <= 2) {
-        int j = 0;
-        for (; j + 2 <= n; j += 2) {
-            T d1 = TR ? qs[j] - p : p - qs[j];
-            T d2 = TR ? qs[j + 1] - p : p - qs[j + 1];
-            if constexpr (KID == 1) { d1 *= d1; d2 *= d2; }
-            if constexpr (KID == 2) { d1 *= d1 * d1; d2 *= d2 * d2; }
-            const T num = fmaT(v[(int64_t)j * inc], d2, v[(int64_t)(j + 1) * inc] * d1);
-            acc = fmaT(num, rcpT(d1 * d2), acc);
+    for (int jt = 0; jt < n; jt += WAVE) {
+        const int nj = min(WAVE, n - jt);
+        if (lane < nj) {
+            sq[lane] = qs[jt + lane];
+            sx[lane] = v[(int64_t)(jt + lane) * inc];
         }
-        if (j < n) {
-            const T f = TR ? kern_fast<T, KID>(qs[j], p) : kern_fast<T, KID>(p, qs[j]);
-            acc = fmaT(f, v[(int64_t)j * inc], acc);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if constexpr (KID <= 2) {
+            int j = 0;
+            for (; j + 2 <= nj; j += 2) {
+                T d1 = TR ? sq[j] - p : p - sq[j];
+                T d2 = TR ? sq[j + 1] - p : p - sq[j + 1];
+                if constexpr (KID == 1) { d1 *= d1; d2 *= d2; }
+                if constexpr (KID == 2) { d1 *= d1 * d1; d2 *= d2 * d2; }
+                const T num = fmaT(sx[j], d2, sx[j + 1] * d1);
+                acc = fmaT(num, rcpT(d1 * d2), acc);
+            }
+            if (j < nj) {
+                const T f = TR ? kern_fast<T, KID>(sq[j], p) : kern_fast<T, KID>(p, sq[j]);
+                acc = fmaT(f, sx[j], acc);
+            }
+        } else {
+            for (int j = 0; j < nj; ++j) {
+                const T f = TR ? kern_fast<T, KID>(sq[j], p) : kern_fast<T, KID>(p, sq[j]);
+                acc = fmaT(f, sx[j], acc);
+            }
         }
-    } else {
-        for (int j = 0; j < n; ++j) {
-            const T f = TR ? kern_fast<T, KID>(qs[j], p) : kern_fast<T, KID>(p, qs[j]);
-            acc = fmaT(f, v[(int64_t)j * inc], acc);
-        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
     return act ? acc : T(0);
 }
+
 
 // Downward pass. Forward: output = rows; near blocks read by rows; far basis U
 // (or its Chebyshev form). Transpose (KB = 1 only): output = columns; near
@@ -627,7 +643,7 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
     static_assert(!TRANS || KB == 1, "the transpose handles one right-hand side per pass");
     constexpr int UN = KB == 1 ? 16 : 8;
     // per-wavefront staging of right-hand-side tiles and far coefficients
-    constexpr int XS = KB == 1 ? 1 : (WAVE * KB > KB * RMAX ? WAVE * KB : KB * RMAX);
+    constexpr int XS = KB == 1 ? (FREE ? 2 * WAVE : 1) : (WAVE * KB > KB * RMAX ? WAVE * KB : KB * RMAX);
     __shared__ T xs_all[WPB][XS];
     const int wid = wave_id();
     const int lane = threadIdx.x % WAVE;
@@ -697,7 +713,8 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
                     }
                 }
             } else if constexpr (KB == 1) {
-                acc[0] += near_free<T, KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act);
+                acc[0] += near_free_staged<T, KID, false>(po, pts_in + D.col0, xin, a.incx, D.n, act,
+                                                          xs, xs + WAVE, lane);
             } else {
                 // right-hand sides of a 64-column tile staged in LDS, read as
                 // broadcasts; one kernel value per (row, column)
@@ -740,7 +757,8 @@ __global__ __launch_bounds__(WG) void k_down(DevPlanT<T> p, const DChunk *ch, in
                 Halve<WAVE, WAVE / 2>::run(accT, lane, base);
                 acc[0] += accT[0];
             } else {
-                acc[0] += near_free<T, KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act);
+                acc[0] += near_free_staged<T, KID, true>(po, pts_in + D.row0, xin, a.incx, D.m, act,
+                                                         xs, xs + WAVE, lane);
             }
         }
     }
@@ -851,6 +869,7 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
                                                  T *y2, int64_t incy2, const T *sv,
                                                  int64_t incs) {
     __shared__ T red[2][WPB][WAVE];
+    __shared__ T stq[WPB][2][WAVE];  // a wavefront's columns: points, then x
     const int wv = uniform((int)(threadIdx.x / WAVE));
     const int lane = threadIdx.x % WAVE;
     if ((int)blockIdx.x >= nch) return;  // workgroup-uniform
@@ -866,13 +885,29 @@ __global__ __launch_bounds__(WG) void k_down_grad(DevPlanT<T> p, const DChunk *c
         const int nq = TRANS ? D.m : D.n;
         const T *xin = a.x + q0 * a.incx;
         const T *qin = pts_in + q0;
-        for (int j = wv; j < nq; j += WPB) {
-            const T d = TRANS ? qin[j] - po : po - qin[j];
-            T f, fx;
-            kern_val_dx<T, KID>(d, f, fx);
-            const T v = xin[(int64_t)j * a.incx];
-            if constexpr (DM == 2) accv = fmaT(f, v, accv);
-            accd = fmaT(TRANS ? -fx : fx, v, accd);
+        // this wavefront's columns wv, wv + WPB, ..., staged 64 at a time
+        // (one coalesced load per lane, then broadcast reads), in order
+        const int nw = nq > wv ? (nq - wv + WPB - 1) / WPB : 0;
+        for (int jt = 0; jt < nw; jt += WAVE) {
+            const int nj = min(WAVE, nw - jt);
+            if (lane < nj) {
+                const int j = wv + (jt + lane) * WPB;
+                stq[wv][0][lane] = qin[j];
+                stq[wv][1][lane] = xin[(int64_t)j * a.incx];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            for (int u = 0; u < nj; ++u) {
+                const T qj = stq[wv][0][u];
+                const T d = TRANS ? qj - po : po - qj;
+                T f, fx;
+                kern_val_dx<T, KID>(d, f, fx);
+                const T v = stq[wv][1][u];
+                if constexpr (DM == 2) accv = fmaT(f, v, accv);
+                accd = fmaT(TRANS ? -fx : fx, v, accd);
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         }
     }
     for (int e = c.far_beg + wv; e < c.far_end; e += WPB) {
